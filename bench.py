@@ -1,0 +1,258 @@
+#!/usr/bin/env python3
+"""Headline benchmark: split-field FDTD cell-updates/s on one MI355X.
+
+Workload (one "step" = one full leapfrog step, H and E, over the whole grid): a
+curvilinear 160 x 160 x 320 Yee grid (scale factors h_j = 1 + 0.3 g_j from a seeded
+sum of 8 low-order Fourier modes), a Si-like core eps = 12.1 in eps = 2.1 cladding,
+PEC on ~5% of the scatterer's E edges as seeded slabs, 10-cell PML on the z faces
+and PEC walls on x/y (the layout the reference's monitors accept), a modal sheet
+source with the reference's sin^2/CW envelope, and the reference's running DFT
+monitors on two port planes.  Inputs are synthetic and resident in HBM before the
+timed region.
+
+Prints one JSON line: FP32 fused-path Gcell-updates/s (value), the roofline of its
+sweep kernels against MI355X HBM peak, the plain FP64 path on the same run, the
+FP32-vs-FP64 |S| error from the monitor projections, and the reference's own
+main_calc timed on this host's cores (oracle/_ref; the restatement in oracle/ where
+that is not built).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Gcell-updates/s + % HBM peak, FP32 vs FP64 GPU path at <=1e-3 |S| error"
+
+
+def bytes_per_cell(phi, field_bytes, coef_bytes):
+    """Algorithmic bytes of one cell-update: every field component read and written
+    once (12 words), split/aux slots in a fraction phi of cells (12*phi words), one
+    coefficient word per component (6 words)."""
+    return field_bytes * (12 + 12 * phi) + 6 * coef_bytes
+
+
+def overlap(prep, H, E):
+    """Projection of monitor phasors (H, E: complex[2*L0*L1], (x, y) interleaved) on the
+    source profile, normalised by the profile norm: the modal overlap S_calc forms
+    (F:437-445) with the incident profile as the mode vector."""
+    import numpy as np
+    c = prep.case
+    w = np.concatenate([c.Hinc, c.Einc])
+    v = np.concatenate([H, E])
+    return complex(np.vdot(w, v) / np.vdot(w, w).real)
+
+
+def build_workload(small=False, box=False):
+    import numpy as np
+    from smatranem_amd import casegen
+    if small:
+        dims, pml = (32, 32, 48), 4
+    else:
+        dims, pml = (160, 160, 320), 10
+    L0, L1, L2 = dims
+    case = casegen.waveguide_case(
+        L0, L1, L2, outer=L2 // 8, pml_xy=pml if box else 0, pml_z=pml, omega=2 * np.pi / 20.0,
+        eps_core=12.1, eps_clad=2.1, core=(L0 // 8, L1 // 10), scale_amp=0.3, n_modes=8,
+        pec_fraction=0.05, seed=2024, iBc=(0, 0, 0), time_period=(5, 100000, 100010))
+    return case
+
+
+def time_plan(plan, steps, warmup, torch):
+    stream = torch.cuda.ExternalStream(plan.stream)
+    plan.run_steps(0, warmup)
+    plan.sync()
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    t0 = time.perf_counter()
+    plan.run_steps(warmup, steps)
+    ev1.record(stream)
+    plan.sync()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    return (t1 - t0) * 1e3, ev0.elapsed_time(ev1)
+
+
+def cpu_baseline(prep, seconds):
+    """The reference's main_calc (compiled from its source, oracle/_ref) or, where that
+    is not built, the oracle's restatement, single-threaded on this host, over whole
+    steps of the same grid until ~`seconds` have elapsed."""
+    import numpy as np
+    import oracle
+    c = prep.case
+    N, iR, iI, _ = oracle.set_tau(c.omega, c.min_dt / c.dt_factor)
+    st = oracle.setup_from_media(c.L, c.mu, c.eps, iR=iR, iI=iI)
+    ref = oracle.load_ref()
+    kind = "reference" if ref is not None else "port"
+    fn = ref.main_calc if ref is not None else oracle.load_oracle().orc_main_calc
+    n, plane = c.n, c.plane
+    flen = 3 * (n + st["Lnotn"] + 1)
+    H, E = np.zeros(flen), np.zeros(flen)
+    mons = [np.zeros(4 * plane) for _ in range(4)]
+    ExpN = oracle.expn_table(N)
+    tp = c.time_period
+    Hinc = np.ascontiguousarray(c.Hinc).view(np.float64)
+    Einc = np.ascontiguousarray(c.Einc).view(np.float64)
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        t = steps % N
+        P = steps // N
+        e = ExpN[4 * t:4 * t + 4]
+        tn = (t + 1) % N
+        fE = 2.0 * oracle.set_env(N, tp, 1, t, P)
+        fH = 2.0 * oracle.set_env(N, tp, 0, t + 1, P)
+        Ei = fE * (e[2] * Einc[0::2] + e[3] * Einc[1::2])
+        Hi = fH * (ExpN[4 * tn] * Hinc[0::2] + ExpN[4 * tn + 1] * Hinc[1::2])
+        oracle.main_calc(fn, c.L, st["Ah"], st["Bh"], st["Ae"], st["Be"], c.iBc, H, E,
+                         (c.iB, c.iT, c.iI, c.inc_mode), np.ascontiguousarray(Hi),
+                         np.ascontiguousarray(Ei), ExpN[4 * tn:4 * tn + 4].copy(), *mons,
+                         st["Cont"], st["addr"])
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            break
+    return dict(value=round(n * steps / el / 1e9, 6), unit="Gcell-updates/s", cores=1, kind=kind,
+                sample=f"{steps} full steps of the same {c.L[0]}x{c.L[1]}x{c.L[2]} grid, "
+                       f"single-threaded, {el:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 leg")
+    ap.add_argument("--no-box", action="store_true", help="skip the all-faces-PML variant")
+    ap.add_argument("--small", action="store_true", help="32x32x48 grid (quick check)")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py: this build runs on one GPU only; the multi-GPU design (z-slab halo "
+              "exchange) is described in DESIGN.md and not run.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from smatranem_amd import casegen, fdtd
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    K, W = args.steps, args.warmup
+
+    t_setup = time.perf_counter()
+    case = build_workload(args.small)
+    prep = casegen.prepare(case)
+    total = W + K
+    assert total <= (case.time_period[2]) * prep.N
+    n = case.n
+    phi = prep.pml_fraction
+    setup_s = time.perf_counter() - t_setup
+    print(f"[bench] grid {case.L[:3]} n={n} N_2pi={prep.N} phi={phi:.4f} setup {setup_s:.1f}s",
+          file=sys.stderr, flush=True)
+
+    out = {}
+    # FP32 fused path
+    with prep.plan(fdtd.F32) as p32:
+        wall_ms, ev_ms = time_plan(p32, K, W, torch)
+        H32, E32 = p32.read_fields()
+        m32 = p32.read_monitors()
+    ms_step = wall_ms / K
+    gcells = n / (ms_step * 1e-3) / 1e9
+    b32 = bytes_per_cell(phi, 4, 4)
+    kern_ms = ev_ms / K
+    achieved = b32 * n / (kern_ms * 1e-3) / 1e9
+    print(f"[bench] fp32: {ms_step:.4f} ms/step, {gcells:.2f} Gcell/s, {achieved:.0f} GB/s "
+          f"algorithmic", file=sys.stderr, flush=True)
+
+    fp64 = None
+    err = None
+    if not args.no_fp64:
+        with prep.plan(fdtd.F64) as p64:
+            wall64, ev64 = time_plan(p64, K, W, torch)
+            H64, E64 = p64.read_fields()
+            m64 = p64.read_monitors()
+        ms64 = wall64 / K
+        g64 = n / (ms64 * 1e-3) / 1e9
+        b64 = bytes_per_cell(phi, 8, 8)
+        ach64 = b64 * n / (ev64 / K * 1e-3) / 1e9
+        fp64 = dict(value=round(g64, 4), ms_per_step=round(ms64, 5),
+                    roofline=dict(bound="hbm", achieved=round(ach64, 1), peak=HBM_PEAK_GBS,
+                                  unit="GB/s", frac=round(ach64 / HBM_PEAK_GBS, 4),
+                                  bytes_per_cell=round(b64, 2)),
+                    speedup_fp32_over_fp64=round(gcells / g64, 3))
+        S = {}
+        for name, (h32, e32, h64, e64) in {"bottom": (m32[0], m32[1], m64[0], m64[1]),
+                                           "top": (m32[2], m32[3], m64[2], m64[3])}.items():
+            s32, s64 = overlap(prep, h32, e32), overlap(prep, h64, e64)
+            S[name] = dict(abs_fp32=abs(s32), abs_fp64=abs(s64), abs_err=abs(abs(s32) - abs(s64)),
+                           rel_err=abs(abs(s32) - abs(s64)) / max(abs(s64), 1e-300))
+        err = dict(S_abs_err_max=max(v["abs_err"] for v in S.values()), S=S,
+                   fields_rel_l2=float(np.linalg.norm(np.concatenate([H32 - H64, E32 - E64])) /
+                                       np.linalg.norm(np.concatenate([H64, E64]))))
+        print(f"[bench] fp64: {ms64:.4f} ms/step, {g64:.2f} Gcell/s; |S| err "
+              f"{err['S_abs_err_max']:.2e}", file=sys.stderr, flush=True)
+
+    box = None
+    if not args.no_box:
+        bcase = build_workload(args.small, box=True)
+        bprep = casegen.prepare(bcase)
+        with bprep.plan(fdtd.F32) as pb:
+            kb = max(K // 5, 100)
+            wb, evb = time_plan(pb, kb, min(W, 100), torch)
+        bphi = bprep.pml_fraction
+        bb = bytes_per_cell(bphi, 4, 4)
+        box = dict(workload="same grid, 10-cell PML on all six faces, reference monitors off",
+                   steps=kb, value=round(bcase.n / (wb / kb * 1e-3) / 1e9, 4), pml_fraction=round(bphi, 4),
+                   roofline_frac=round(bb * bcase.n / (evb / kb * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
+        print(f"[bench] box-PML fp32: {box['value']:.2f} Gcell/s", file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(prep, args.cpu_seconds)
+        print(f"[bench] cpu baseline: {cpu['value']} Gcell/s ({cpu['kind']})", file=sys.stderr, flush=True)
+
+    line = {
+        "metric": METRIC,
+        "value": round(gcells, 4),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(ms_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "curvilinear split-field FDTD 160x160x320 (configs[1]): Si core "
+                               "eps 12.1 in eps 2.1, scale factors 1+0.3*g (8 Fourier modes), "
+                               "5% PEC edges, 10-cell z PML, PEC x/y walls, modal sheet source, "
+                               "reference DFT monitors on 2 port planes",
+                   "grid": list(case.L[:3]), "cells": n, "N_2pi": prep.N,
+                   "pml_fraction": round(phi, 4), "pec_edges": case.meta.get("pec_edges"),
+                   "small": bool(args.small)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "h_sweep_f32 + e_sweep_f32 (one step)",
+                     "bytes_per_cell": round(b32, 2), "kernel_ms_per_step": round(kern_ms, 5)},
+        "fp64": fp64,
+        "err_vs_fp64": err,
+        "pml_all_faces": box,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

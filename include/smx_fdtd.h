@@ -1,0 +1,179 @@
+/*
+ * smx_fdtd.h -- C ABI of the MI355X split-field FDTD library (libsmx_fdtd.so).
+ *
+ * Drop-in for the time-stepper of SMatrAnEM's FDTD executable,
+ * FDTDcalc/FDTDsplitfieldMetal.c (cited as F:<line>).  Plain C types only: pointers,
+ * sizes and int status codes; complex arrays are interleaved (re, im) doubles, the
+ * memory layout of C99 `double complex`, so a caller holding `double complex *` passes
+ * it cast to `double *`.  All pointers are host pointers unless a name says `_dev`.
+ *
+ * Storage contract shared with the reference (F:151-152, F:188-233):
+ *   n = L0*L1*L2; cell l = j0 + L0*j1 + L0*L1*j2.
+ *   Fields H_t, E_t: double[3*(n + Lnotn + 1)]; main slots X[c + 3l], then split/aux
+ *   slots X[3n + c + 3*address_notn[l]]; index Lnotn is the never-written zero slot.
+ *   Coefficients Ah, Bh, Ae, Be: double[3*(n + Lnotn)], same indexing.
+ *   Cont[l] in {'n','a','e','h','b'}; address_notn[l] = running index of non-'n' cells,
+ *   Lnotn for 'n' cells.
+ */
+#ifndef SMX_FDTD_H
+#define SMX_FDTD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  Codes >= 100 are the reference's own fatal conditions; the drop-in
+ * executable prints the reference's message for them and exits with EXIT_FAILURE. */
+enum {
+    SMX_OK = 0,
+    SMX_ERR_ARG = 1,           /* invalid argument or inconsistent sizes */
+    SMX_ERR_HIP = 2,           /* HIP runtime error (see smx_last_error) */
+    SMX_ERR_NO_DEVICE = 3,     /* no HIP device visible */
+    SMX_ERR_OOM = 4,           /* device allocation failed */
+    SMX_ERR_STATE = 5,         /* call out of order (e.g. run before set_source) */
+    SMX_ERR_MONITOR_PML = 100, /* a monitor plane holds a PML cell: "iBTIin[m] error!" F:320 */
+    SMX_ERR_ENVELOPE = 101     /* step beyond stop-period: "f_env error!" F:471 */
+};
+
+/* Human-readable text of a status code, and the last HIP error text of this thread. */
+const char *smx_status_string(int status);
+const char *smx_last_error(void);
+
+/* ------------------------------------------------------------------------------------
+ * (i) Parity entry point: exactly main_calc (F:5-12, F:242-249).
+ *
+ * One leapfrog step on host arrays with the reference's argument meanings and layouts:
+ * H sweep, E sweep, then the running DFT on the monitor planes iBTIin[0..1].  Runs the
+ * plain FP64 HIP kernels; the arrays are copied to the device and back on every call.
+ * Hi, Ei: double[2*L0*L1] source planes, (x, y) interleaved.  ExpN2: complex[2].
+ * HB, EB, HT, ET: complex[2*L0*L1], accumulated in place.
+ * Returns SMX_OK, or SMX_ERR_MONITOR_PML where the reference exits (F:320).
+ * Replaces: main_calc, FDTDcalc/FDTDsplitfieldMetal.c:242.
+ * ---------------------------------------------------------------------------------- */
+int smx_main_calc(const int L[4], const double *Ah, const double *Bh, const double *Ae,
+                  const double *Be, const int iBc[3], double *H_t, double *E_t,
+                  const int iBTIin[4], const double *Hi, const double *Ei, const double *ExpN2,
+                  double *HB, double *EB, double *HT, double *ET, const char *Cont,
+                  const int *address_notn);
+
+/* ------------------------------------------------------------------------------------
+ * (ii) Production plan: device-resident state, many steps per call.
+ *
+ * A plan owns the device copies of fields, coefficients and monitors of one grid.
+ * Precision SMX_F64 runs the plain FP64 kernels on the reference storage contract and
+ * is bit-identical to main_calc.  SMX_F32 runs the fused FP32 kernels on a packed
+ * layout (total fields plus split slots, per-cell FP32 coefficient words, per-axis PML
+ * tables) with FP64 monitor accumulators.
+ * ---------------------------------------------------------------------------------- */
+typedef struct smx_plan smx_plan;
+
+enum { SMX_F64 = 0, SMX_F32 = 1 };
+
+/* Grid and run description: the scalars main() holds before its time loop (F:57-104). */
+typedef struct {
+    int L[4];       /* L0, L1, L2 (= scatterer slices + 2*outer), outer (F:57) */
+    int iBc[3];     /* BoundaryCondition B0, B1, B2 in {-1, 0, 1} (F:46) */
+    int iBTIin[4];  /* iB, iT (monitor planes), iI (source plane), incident mode (F:84) */
+    int precision;  /* SMX_F64 or SMX_F32 */
+    int Lnotn;      /* number of non-'n' cells (F:102-104) */
+    double invtau_R, invtau_I; /* time constants from set_tau (F:92) */
+} smx_grid;
+
+/*
+ * Create a plan from the setup state main() builds before its loop:
+ *   mu, eps: complex[3n] after remove_sigma (F:76), per component (x, y, z), scale
+ *            factor ratios included;
+ *   sigma:   double[2*(L0+L1+L2)] PML profiles from set_sigma (F:75);
+ *   Cont, address_notn: cell classes (F:104).
+ * The F64 plan builds the reference's Ah/Bh/Ae/Be (F:106-143) on the host and uploads
+ * them; the F32 plan packs its own layout.  Replaces: the setup of main, F:100-152.
+ */
+int smx_plan_create(const smx_grid *grid, const double *mu, const double *eps,
+                    const double *sigma, const char *Cont, const int *address_notn,
+                    smx_plan **out);
+
+/*
+ * Create an F64 plan directly from the four coefficient arrays main_calc receives
+ * (F:106-107), e.g. from a caller that builds them itself.  grid->precision must be
+ * SMX_F64; invtau_R/I are unused.
+ */
+int smx_plan_create_coeffs(const smx_grid *grid, const double *Ah, const double *Bh,
+                           const double *Ae, const double *Be, const char *Cont,
+                           const int *address_notn, smx_plan **out);
+
+int smx_plan_destroy(smx_plan *plan);
+
+/* Size in doubles of one field array in the reference layout: 3*(n + Lnotn + 1). */
+size_t smx_plan_field_len(const smx_plan *plan);
+
+/* Copy fields in/out in the reference layout (double[3*(n+Lnotn+1)]).  For F32 plans
+ * the main slot is reconstructed as total minus split, in FP64. */
+int smx_plan_write_fields(smx_plan *plan, const double *H_t, const double *E_t);
+int smx_plan_read_fields(const smx_plan *plan, double *H_t, double *E_t);
+
+/* Monitors HB, EB, HT, ET: complex[2*L0*L1] each, read, written or zeroed. */
+int smx_plan_read_monitors(const smx_plan *plan, double *HB, double *EB, double *HT, double *ET);
+int smx_plan_write_monitors(smx_plan *plan, const double *HB, const double *EB,
+                            const double *HT, const double *ET);
+
+/* One step with explicit host source planes and phases (main_calc's inputs). */
+int smx_plan_step(smx_plan *plan, const double *Hi, const double *Ei, const double *ExpN2);
+
+/*
+ * Modal source (F:479-511 output): Hinc, Einc complex[2*L0*L1], (x, y) interleaved;
+ * N_2pi steps per period and the envelope periods (ramp, cw, stop) (F:60, F:92).
+ */
+int smx_plan_set_source(smx_plan *plan, const double *Hinc, const double *Einc, int N_2pi,
+                        const int time_period[3]);
+
+/*
+ * Run envelope periods [P0, P1) of the loop at F:153-167, synthesising the source on
+ * the device.  sums[2] (may be NULL) accumulates the envelope sums sum00, sum05
+ * printed at F:168.  Returns SMX_ERR_ENVELOPE where the reference exits (F:471).
+ */
+int smx_plan_run(smx_plan *plan, int P0, int P1, double sums[2]);
+
+/*
+ * Run global steps [step0, step0 + nsteps) of the same loop, step k being period
+ * k / N_2pi, phase step k % N_2pi.  smx_plan_run(P0, P1) is
+ * smx_plan_run_steps(P0*N_2pi, (P1-P0)*N_2pi).
+ */
+int smx_plan_run_steps(smx_plan *plan, long long step0, long long nsteps, double sums[2]);
+
+/* Wait for all queued work of the plan. */
+int smx_plan_sync(smx_plan *plan);
+
+/* The plan's HIP stream (a hipStream_t), for callers that order or time their own
+ * work against the plan's kernels.  Valid until smx_plan_destroy. */
+void *smx_plan_stream(const smx_plan *plan);
+
+/* ------------------------------------------------------------------------------------
+ * Host setup helpers: the reference's setup functions, FP64, same results.
+ * ---------------------------------------------------------------------------------- */
+
+/* set_tau (F:515-524): returns N_2pi; writes 1/tau_R, 1/tau_I and Dt. */
+int smx_set_tau(double omega, double min_Dt, double *invtau_R, double *invtau_I, double *Dt);
+/* set_env (F:452-475); NAN where the reference exits. */
+double smx_set_env(int N_2pi, const int time_period[3], int j_half, int j_time, int j_period);
+/* ExpN table (F:93-99): complex[2*N_2pi]. */
+void smx_expn_table(int N_2pi, double *ExpN);
+/* set_sigma (F:858-926) into a zeroed sigma[2*(L0+L1+L2)]. */
+void smx_set_sigma(const int L[4], const double *mu, const double *eps, double *sigma);
+/* remove_sigma (F:822-853): 0, or 1/2 where the reference exits on mu/eps. */
+int smx_remove_sigma(const int L[4], double *mu, double *eps, const double *sigma);
+/* set_address (F:188-233): returns Lnotn. */
+int smx_set_address(const int L[4], const double *sigma, const double *mu, const double *eps,
+                    char *Cont, int *address_notn);
+/* Coefficient build (F:106-143) into zeroed arrays of 3*(n+Lnotn) doubles. */
+void smx_build_coeffs(const int L[4], const double *mu, const double *eps, const double *sigma,
+                      const char *Cont, const int *address_notn, int Lnotn, double invtau_R,
+                      double invtau_I, double *Ah, double *Bh, double *Ae, double *Be);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SMX_FDTD_H */

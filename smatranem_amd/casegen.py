@@ -1,0 +1,284 @@
+"""Synthetic FDTD cases with the content of the reference's input files.
+
+A case is what ``main`` (FDTDcalc/FDTDsplitfieldMetal.c:32-152) has read before its
+time loop: per-component mu and eps with the scale-factor ratios fx folded in
+(input_data, F:676-685) and the PML stretch of PMLset/PMLcalc.c:230 applied, the
+"min Dt^2" CFL value of PMLcalc.c:279-313, omega, the plane indices, the boundary
+factors and a modal source profile.  ``prepare`` then runs the reference's setup
+(set_sigma, remove_sigma, set_address, set_tau; F:63-104) through libsmx_fdtd's host
+functions.
+
+Staggered positions (Medium3DSi/Med3DSi.c:226-227): mu_x (1/2,0,0), mu_y (0,1/2,0),
+mu_z (0,0,1/2); eps_x (0,1/2,1/2), eps_y (1/2,0,1/2), eps_z (1/2,1/2,0), in cell units.
+Stretch of a component along its own axis d uses the sample at its own position
+along d: half for mu, integer for eps (PMLcalc.c:215-232).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import fdtd
+
+# position offsets (along x, y, z) of each component, in cells
+MU_POS = np.array([[0.5, 0.0, 0.0], [0.0, 0.5, 0.0], [0.0, 0.0, 0.5]])
+EPS_POS = np.array([[0.0, 0.5, 0.5], [0.5, 0.0, 0.5], [0.5, 0.5, 0.0]])
+
+PEC_EPS = 1.0e30  # a perfect electric conductor edge: B_e -> 0, E stays zero
+
+
+@dataclass
+class Case:
+    """Inputs of the time loop, as the reference holds them after reading its files."""
+    L: tuple                  # (L0, L1, L2, outer)
+    mu: np.ndarray            # complex128[3n], stretched, fx included
+    eps: np.ndarray           # complex128[3n]
+    omega: float
+    min_dt: float             # sqrt of the Med header's "min Dt^2" (F:70)
+    dt_factor: float = 1.0
+    iBc: tuple = (0, 0, 0)
+    iB: int = 0
+    iT: int = 1
+    iI: int = 0
+    inc_mode: int = 1
+    time_period: tuple = (2, 1, 8)
+    Hinc: np.ndarray = None   # complex128[2*L0*L1], (x, y) interleaved
+    Einc: np.ndarray = None
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def n(self):
+        return int(self.L[0]) * int(self.L[1]) * int(self.L[2])
+
+    @property
+    def plane(self):
+        return int(self.L[0]) * int(self.L[1])
+
+
+def pml_profile(length, cells_lo, cells_hi, omega, power=3, R=1e-8):
+    """sigma(j, h) of one axis: PMLcalc.c:246-260, sigma = smax*|(x - x0)/d|^p inside
+    the layer with smax = -(p+1) ln R / (4 omega d); x = j + h/2 in cell units, the
+    layer spanning [0, cells_lo] and [length - cells_hi, length]."""
+    sig = np.zeros((length, 2))
+    for h in range(2):
+        x = np.arange(length) + 0.5 * h
+        if cells_lo > 0:
+            d = float(cells_lo)
+            smax = (-0.25 * (power + 1) / (omega * d)) * np.log(R)
+            inside = x <= d
+            sig[inside, h] += smax * np.abs((x[inside] - d) / d) ** power
+        if cells_hi > 0:
+            d = float(cells_hi)
+            x0 = length - d
+            smax = (-0.25 * (power + 1) / (omega * d)) * np.log(R)
+            inside = x >= x0
+            sig[inside, h] += smax * np.abs((x[inside] - x0) / d) ** power
+    return sig
+
+
+def apply_stretch(L, mu, eps, sig_axes):
+    """Multiply mu, eps by the PML stretch (PMLcalc.c:230):
+    mu_c  *= s_{c+1}(int) s_{c+2}(int) / s_c(half),
+    eps_c *= s_{c+1}(half) s_{c+2}(half) / s_c(int), s = 1 + i*sigma.
+    sig_axes: three arrays (L_d, 2) of sigma at integer (h=0) and half (h=1) samples."""
+    L0, L1, L2 = (int(v) for v in L[:3])
+    s = [1.0 + 1j * np.asarray(sa) for sa in sig_axes]
+    sx = s[0][None, None, :, :]   # (1,1,L0,2)
+    sy = s[1][None, :, None, :]
+    sz = s[2][:, None, None, :]
+    grid = lambda a, h: np.broadcast_to(a[..., h], (L2, L1, L0))
+    S = [[grid(sx, h), grid(sy, h), grid(sz, h)] for h in range(2)]  # S[h][d]
+    mu3 = mu.reshape(L2, L1, L0, 3).copy()
+    eps3 = eps.reshape(L2, L1, L0, 3).copy()
+    for c in range(3):
+        d1, d2 = (c + 1) % 3, (c + 2) % 3
+        mu3[..., c] *= S[0][d1] * S[0][d2] / S[1][c]
+        eps3[..., c] *= S[1][d1] * S[1][d2] / S[0][c]
+    return mu3.reshape(-1), eps3.reshape(-1)
+
+
+def cfl_min_dt(mu_real, eps_real):
+    """sqrt of the "min Dt^2" header value of PMLcalc.c:279-313:
+    min over cells of sqrt(e0 e1 e2 m0 m1 m2 / ((e0+e1+e2)(m0+m1+m2))), from the
+    unstretched real parts (fx included), then the square root FDTD takes (F:70)."""
+    m = mu_real.reshape(-1, 3)
+    e = eps_real.reshape(-1, 3)
+    val = np.prod(e, axis=1) * np.prod(m, axis=1) / (e.sum(axis=1) * m.sum(axis=1))
+    return float(np.sqrt(np.sqrt(val.min())))
+
+
+def gaussian_profile(L0, L1, cx, cy, wx, wy):
+    x = np.arange(L0)[None, :]
+    y = np.arange(L1)[:, None]
+    return np.exp(-((x - cx) / wx) ** 2 - ((y - cy) / wy) ** 2)
+
+
+def sheet_source(L0, L1, profile, n_eff=1.0):
+    """A modal-like sheet: E along x with the given real profile and H along y
+    (Hinc = (Xi_r, Xi_{r+L0L1}), Einc = (Xi_{r+3L0L1}, -Xi_{r+2L0L1}), F:504-509)."""
+    p = np.asarray(profile, dtype=np.float64).reshape(-1)
+    Hinc = np.zeros(2 * L0 * L1, dtype=np.complex128)
+    Einc = np.zeros(2 * L0 * L1, dtype=np.complex128)
+    Einc[0::2] = p
+    Hinc[1::2] = n_eff * p
+    return Hinc, Einc
+
+
+def waveguide_case(L0, L1, L2, outer, pml_xy, pml_z, *, omega=2 * np.pi / 20.0, eps_core=12.1,
+                   eps_clad=2.1, core=(8, 6), scale_amp=0.0, n_modes=8, pec_fraction=0.0,
+                   seed=0, iBc=(0, 0, 0), dt_factor=1.0, time_period=(2, 1, 8), monitor_gap=2,
+                   metal_cells=0, lossy_cells=0):
+    """A straight dielectric core along z in cladding, on an L0 x L1 x L2 grid with
+    ``outer`` port slices at each end (B0.., T..), PML layers of ``pml_xy`` cells on the
+    x/y faces and ``pml_z`` on the z faces (PMLcalc.c:188-260), optional curvilinear scale
+    factors h_j = 1 + scale_amp*g_j (g_j a seeded sum of low-order Fourier modes,
+    normalised to max|g| = 1; fx_c = h_{c+1} h_{c+2} / h_c at each component's
+    position, MixedData/BindData.c:223-228), seeded PEC slabs covering about
+    ``pec_fraction`` of the E edges of the scatterer, and optional metal ('e'/'h'/'b')
+    and lossy cells.  Monitor planes sit between the z PML and the scatterer, the
+    source plane just above the bottom monitor."""
+    rng = np.random.default_rng(seed)
+    n = L0 * L1 * L2
+    z = np.arange(L2)[:, None, None]
+    y = np.arange(L1)[None, :, None]
+    x = np.arange(L0)[None, None, :]
+
+    # per-component material (before fx and stretch)
+    cx, cy = (L0 - 1) / 2.0, (L1 - 1) / 2.0
+    eps3 = np.empty((L2, L1, L0, 3))
+    mu3 = np.ones((L2, L1, L0, 3))
+    for c in range(3):
+        px, py, _ = EPS_POS[c]
+        inside = (np.abs(x + px - cx - 0.5) <= core[0] / 2) & (np.abs(y + py - cy - 0.5) <= core[1] / 2)
+        eps3[..., c] = np.where(np.broadcast_to(inside, (L2, L1, L0)), eps_core, eps_clad)
+
+    # curvilinear scale factors
+    if scale_amp:
+        # one smooth field per axis, evaluated at every staggered position
+        modes = [[(rng.integers(0, 3, 3), rng.uniform(0, 2 * np.pi), rng.normal())
+                  for _ in range(n_modes)] for _ in range(3)]
+
+        def hfield(d, pos):
+            X, Y, Z = x + pos[0], y + pos[1], z + pos[2]
+            g = np.zeros((L2, L1, L0))
+            for (k, ph, a) in modes[d]:
+                g = g + a * np.cos(2 * np.pi * (k[0] * X / L0 + k[1] * Y / L1 + k[2] * Z / L2) + ph)
+            return g
+        gmax = [max(np.abs(hfield(d, (0, 0, 0))).max(), 1e-12) for d in range(3)]
+
+        def fx_at(pos, c):
+            h = [1.0 + scale_amp * hfield(d, pos) / gmax[d] for d in range(3)]
+            return h[(c + 1) % 3] * h[(c + 2) % 3] / h[c]
+        for c in range(3):
+            mu3[..., c] *= fx_at(MU_POS[c], c)
+            eps3[..., c] *= fx_at(EPS_POS[c], c)
+
+    # PEC slabs in the scatterer region
+    pec_mask = np.zeros((L2, L1, L0), dtype=bool)
+    if pec_fraction > 0:
+        target = pec_fraction * (L2 - 2 * outer) * L1 * L0
+        zlo, zhi = outer + 2, L2 - outer - 2
+        tries = 0
+        while pec_mask.sum() < target and tries < 10000:
+            tries += 1
+            sz = rng.integers(1, 4)
+            sy = rng.integers(2, max(3, L1 // 6))
+            sx = rng.integers(2, max(3, L0 // 6))
+            z0 = rng.integers(zlo, max(zlo + 1, zhi - sz))
+            y0 = rng.integers(pml_xy, max(pml_xy + 1, L1 - pml_xy - sy))
+            x0 = rng.integers(pml_xy, max(pml_xy + 1, L0 - pml_xy - sx))
+            pec_mask[z0:z0 + sz, y0:y0 + sy, x0:x0 + sx] = True
+        eps3[pec_mask, :] = PEC_EPS
+
+    # optional metal / lossy cells (scatterer interior, away from PML and planes)
+    def pick(count):
+        zs = rng.integers(outer + 3, L2 - outer - 3, size=count)
+        ys = rng.integers(pml_xy + 1, L1 - pml_xy - 1, size=count)
+        xs = rng.integers(pml_xy + 1, L0 - pml_xy - 1, size=count)
+        return zs, ys, xs
+    mu_c = mu3.astype(np.complex128)
+    eps_c = eps3.astype(np.complex128)
+    if metal_cells:
+        zs, ys, xs = pick(metal_cells)
+        kinds = rng.integers(0, 3, size=metal_cells)
+        for zz, yy, xx, k in zip(zs, ys, xs, kinds):
+            if k in (0, 2):
+                eps_c[zz, yy, xx, :] = -10.0 + 1.5j
+            if k in (1, 2):
+                mu_c[zz, yy, xx, :] = -2.0 + 0.5j
+    if lossy_cells:
+        zs, ys, xs = pick(lossy_cells)
+        for zz, yy, xx in zip(zs, ys, xs):
+            eps_c[zz, yy, xx, :] = eps_c[zz, yy, xx, :].real + 0.3j
+
+    min_dt = cfl_min_dt(mu3.reshape(-1)[np.isfinite(mu3.reshape(-1))],
+                        np.where(eps3 >= PEC_EPS, eps_clad, eps3).reshape(-1))
+
+    # PML: separable profiles over the whole grid.  The reference divides the stretch
+    # out of every slice with one profile per axis (remove_sigma, F:822-853), so x/y
+    # layers must span all z; a layer only in the scatterer slices makes it exit.
+    sig_x = pml_profile(L0, pml_xy, pml_xy, omega)
+    sig_y = pml_profile(L1, pml_xy, pml_xy, omega)
+    sig_z = pml_profile(L2, pml_z, pml_z, omega)
+    mu_s, eps_s = apply_stretch((L0, L1, L2), mu_c.reshape(-1), eps_c.reshape(-1),
+                                (sig_x, sig_y, sig_z))
+
+    # Monitor planes must hold no PML cell (F:320): with x/y layers every plane does,
+    # so those cases run without the reference monitors (plane -1).
+    if pml_xy > 0:
+        iB = iT = -1
+        iI = pml_z + monitor_gap + 1
+    else:
+        iB = pml_z + monitor_gap
+        iT = L2 - 1 - pml_z - monitor_gap
+        iI = iB + 1
+    prof = gaussian_profile(L0, L1, cx, cy, max(core[0], 2), max(core[1], 2))
+    Hinc, Einc = sheet_source(L0, L1, prof, n_eff=np.sqrt(eps_core))
+    return Case(L=(L0, L1, L2, outer), mu=mu_s, eps=eps_s, omega=omega, min_dt=min_dt,
+                dt_factor=dt_factor, iBc=tuple(iBc), iB=iB, iT=iT, iI=iI, inc_mode=1,
+                time_period=tuple(time_period), Hinc=Hinc, Einc=Einc,
+                meta=dict(pec_edges=int(3 * pec_mask.sum()), seed=seed))
+
+
+@dataclass
+class Prepared:
+    """Setup state main() holds before its loop (F:57-152)."""
+    case: Case
+    sigma: np.ndarray
+    mu: np.ndarray     # complex128[3n], stretch removed
+    eps: np.ndarray
+    Cont: np.ndarray   # int8[n]
+    addr: np.ndarray   # int32[n]
+    Lnotn: int
+    N: int
+    invtau_R: float
+    invtau_I: float
+    dt: float
+
+    def grid(self, precision=fdtd.F64):
+        c = self.case
+        return fdtd.Grid(L=c.L, iBc=c.iBc, iBTIin=(c.iB, c.iT, c.iI, c.inc_mode),
+                         precision=precision, Lnotn=self.Lnotn, invtau_R=self.invtau_R,
+                         invtau_I=self.invtau_I)
+
+    def plan(self, precision=fdtd.F64):
+        p = fdtd.Plan(self.grid(precision), mu=self.mu, eps=self.eps, sigma=self.sigma,
+                      Cont=self.Cont, address_notn=self.addr)
+        p.set_source(self.case.Hinc, self.case.Einc, self.N, self.case.time_period)
+        return p
+
+    @property
+    def pml_fraction(self):
+        return float(np.count_nonzero(self.Cont != ord("n"))) / self.Cont.size
+
+
+def prepare(case: Case) -> Prepared:
+    """set_sigma, remove_sigma, set_address, set_tau (F:63-104) on the case."""
+    L = case.L
+    sigma = fdtd.set_sigma(L, case.mu, case.eps)
+    mu, eps = fdtd.remove_sigma(L, case.mu.copy(), case.eps.copy(), sigma)
+    cont, addr, lnotn = fdtd.set_address(L, sigma, mu, eps)
+    N, iR, iI, dt = fdtd.set_tau(case.omega, case.min_dt / case.dt_factor)
+    return Prepared(case=case, sigma=sigma, mu=mu, eps=eps, Cont=cont, addr=addr, Lnotn=lnotn,
+                    N=N, invtau_R=iR, invtau_I=iI, dt=dt)

@@ -1,0 +1,621 @@
+/*
+ * kern_f32.hip -- FP32 split-field step on the packed layout of smx_f32.h.
+ *
+ * Physics of FDTDcalc/FDTDsplitfieldMetal.c:242-329 (main_calc) restated on total
+ * fields T = main + split, which every neighbour's curl uses (the reference adds the
+ * split curl through address_notn, :266, :298, :274, :305).  Per cell kind:
+ *   regular 'n' (lossless):  X += -/+ b*(curl T)                (A = 1)
+ *   regular 'a' (PML):       main = T - S; main' = aM*main -/+ b*gM*T+;
+ *                            S' = aS*S -/+ b*gS*T-; T' = main' + S'
+ *   special (lossy/metal):   table coefficients, the reference's branch per class,
+ *                            including its backward curl of H main in metal E cells
+ *                            (:310) and frozen fields (:259-314).
+ * Monitors accumulate in FP64 on the monitor planes (:318-328).
+ *
+ * Launch: 256-thread blocks of BX x (256/BX) cells over (j0, j1), one z plane per
+ * block, blocks renumbered so that each XCD works through a contiguous run of z
+ * planes (its L2 then holds the z-neighbour plane of the next block it runs).
+ */
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+#include "smx_f32.h"
+#include "smx_fdtd.h"
+#include "smx_setup.h"
+
+namespace {
+
+thread_local char f32_err[512] = "";
+
+#define F32_HIP(call)                                                                     \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess) {                                                           \
+            snprintf(f32_err, sizeof f32_err, "%s: %s", #call, hipGetErrorString(e_));    \
+            return e_ == hipErrorOutOfMemory ? SMX_ERR_OOM : SMX_ERR_HIP;                 \
+        }                                                                                 \
+    } while (0)
+
+constexpr int kThreads = 256;
+constexpr int kXCD = 8; /* MI355X: 8 accelerator complex dies, blocks dealt round-robin */
+
+struct f32_args {
+    int L0, L1, L2;
+    int gx, gy, nblocks;
+    int64_t sy, sz;
+    float w[3];
+    int iB, iT, iI;
+    float sgn;
+    float *T[6];
+    float *S[6];
+    const float *b[6];
+    const float *pml;
+    const unsigned char *pmlflag;
+    const smx_special_f32 *special;
+    /* source: explicit plane (double[2*L0*L1]) or synthesis from the profile */
+    const double *src_plane;
+    const double *inc;
+    double src_f2, src_re, src_im;
+    /* monitors */
+    double *HB, *EB, *HT, *ET;
+    double monH_re, monH_im, monE_re, monE_im;
+};
+
+/* Hardware block h runs on XCD h % 8; give each XCD a contiguous run of logical ids. */
+__device__ inline int xcd_logical(int h, int nblocks)
+{
+    const int q = nblocks / kXCD, r = nblocks % kXCD;
+    const int x = h % kXCD, s = h / kXCD;
+    return x * q + (x < r ? x : r) + s;
+}
+
+struct cellpos {
+    int j0, j1, j2;
+    int64_t l, q;
+};
+
+template <int BX>
+__device__ inline bool decode(const f32_args &a, cellpos &p)
+{
+    constexpr int BY = kThreads / BX;
+    const int id = xcd_logical(blockIdx.x, a.nblocks);
+    const int bx = id % a.gx, rest = id / a.gx;
+    const int by = rest % a.gy;
+    p.j2 = rest / a.gy;
+    p.j0 = bx * BX + (int)(threadIdx.x % BX);
+    p.j1 = by * BY + (int)(threadIdx.x / BX);
+    if (p.j0 >= a.L0 || p.j1 >= a.L1) return false;
+    p.q = p.j0 + (int64_t)a.L0 * p.j1;
+    p.l = p.q + a.sz * p.j2;
+    return true;
+}
+
+/* PML tables of axis d: aH, gH, aE, gE, each L_d long. */
+__device__ inline const float *pml_axis(const f32_args &a, int d)
+{
+    return d == 0 ? a.pml : d == 1 ? a.pml + 4 * a.L0 : a.pml + 4 * (a.L0 + a.L1);
+}
+
+__device__ inline double source_value(const f32_args &a, int64_t q, int v)
+{
+    if (a.src_plane) return a.src_plane[v + 2 * q];
+    const double *z = a.inc + 2 * (v + 2 * q);
+    return a.src_f2 * (a.src_re * z[0] + a.src_im * z[1]);
+}
+
+/* Offsets and weights of the backward (j-1) or forward (j+1) neighbours. */
+struct nb3 {
+    int64_t o[3];
+    float w[3];
+};
+
+__device__ inline nb3 nb_backward(const f32_args &a, const cellpos &p)
+{
+    nb3 n;
+    n.o[0] = p.j0 == 0 ? (int64_t)(a.L0 - 1) : -1;
+    n.o[1] = p.j1 == 0 ? (int64_t)(a.L1 - 1) * a.sy : -a.sy;
+    n.o[2] = p.j2 == 0 ? (int64_t)(a.L2 - 1) * a.sz : -a.sz;
+    n.w[0] = p.j0 == 0 ? a.w[0] : 1.f;
+    n.w[1] = p.j1 == 0 ? a.w[1] : 1.f;
+    n.w[2] = p.j2 == 0 ? a.w[2] : 1.f;
+    return n;
+}
+
+__device__ inline nb3 nb_forward(const f32_args &a, const cellpos &p)
+{
+    nb3 n;
+    n.o[0] = p.j0 + 1 == a.L0 ? -(int64_t)(a.L0 - 1) : 1;
+    n.o[1] = p.j1 + 1 == a.L1 ? -(int64_t)(a.L1 - 1) * a.sy : a.sy;
+    n.o[2] = p.j2 + 1 == a.L2 ? -(int64_t)(a.L2 - 1) * a.sz : a.sz;
+    n.w[0] = p.j0 + 1 == a.L0 ? a.w[0] : 1.f;
+    n.w[1] = p.j1 + 1 == a.L1 ? a.w[1] : 1.f;
+    n.w[2] = p.j2 + 1 == a.L2 ? a.w[2] : 1.f;
+    return n;
+}
+
+/*
+ * Curl halves of field F (F[0..2] = x, y, z arrays, value = F - G when G is given,
+ * i.e. a main slot rebuilt from total and split) at cell l.
+ * Backward: Pc = F[c2](l) - w[c1] F[c2](l-e_c1); Mc = -F[c1](l) + w[c2] F[c1](l-e_c2).
+ * Forward:  Pc = w[c1] F[c2](l+e_c1) - F[c2](l); Mc = -w[c2] F[c1](l+e_c2) + F[c1](l).
+ */
+template <bool FWD>
+__device__ inline void curl_halves(float *const *F, float *const *G, const nb3 &n, int64_t l, float P[3],
+                                   float M[3])
+{
+    auto v = [&](int c, int64_t i) { return G ? F[c][i] - G[c][i] : F[c][i]; };
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const int c1 = c == 2 ? 0 : c + 1, c2 = c == 0 ? 2 : c - 1;
+        if (FWD) {
+            P[c] = n.w[c1] * v(c2, l + n.o[c1]) - v(c2, l);
+            M[c] = -n.w[c2] * v(c1, l + n.o[c2]) + v(c1, l);
+        } else {
+            P[c] = v(c2, l) - n.w[c1] * v(c2, l + n.o[c1]);
+            M[c] = -v(c1, l) + n.w[c2] * v(c1, l + n.o[c2]);
+        }
+    }
+}
+
+/* ---------------------------------------------------------------------------------- */
+/* Special (lossy / metal) cells: the reference branch of each class, FP32             */
+/* ---------------------------------------------------------------------------------- */
+
+/* H update of a special cell; sign = -1 (H takes -curl). */
+__device__ void special_h(const f32_args &a, const cellpos &p, const smx_special_f32 &sp)
+{
+    float *const *H = a.T;
+    float *const *Hs = a.S;
+    float *const *E = a.T + 3;
+    float *const *Es = a.S + 3;
+    const int k = sp.cls;
+    if (k == 'e') return;
+    const nb3 n = nb_backward(a, p);
+    float P[3], M[3];
+    if (k == 'n' || k == 'a') {
+        curl_halves<false>(E, nullptr, n, p.l, P, M);
+        if (k == 'n') {
+            float W[3] = {P[0] + M[0], P[1] + M[1], P[2] + M[2]};
+            if (p.j2 == a.iI) {
+                W[0] += a.sgn * (float)source_value(a, p.q, 1);
+                W[1] -= a.sgn * (float)source_value(a, p.q, 0);
+            }
+            for (int c = 0; c < 3; c++) H[c][p.l] = sp.hA[c] * H[c][p.l] - sp.hB[c] * W[c];
+        } else {
+            for (int c = 0; c < 3; c++) {
+                const float s = Hs[c][p.l], m = H[c][p.l] - s;
+                const float m2 = sp.hA[c] * m - sp.hB[c] * P[c];
+                const float s2 = sp.hAs[c] * s - sp.hBs[c] * M[c];
+                H[c][p.l] = m2 + s2;
+                Hs[c][p.l] = s2;
+            }
+        }
+        return;
+    }
+    /* 'h', 'b': curl of the E main slots (F:279), aux in the split slot */
+    curl_halves<false>(E, Es, n, p.l, P, M);
+    for (int c = 0; c < 3; c++) {
+        const float s = Hs[c][p.l];
+        float m = H[c][p.l] - s;
+        m += 0.5f * s;
+        const float s2 = sp.hAs[c] * s - sp.hBs[c] * (P[c] + M[c]);
+        m += 0.5f * s2;
+        H[c][p.l] = m + s2;
+        Hs[c][p.l] = s2;
+    }
+}
+
+__device__ void special_e(const f32_args &a, const cellpos &p, const smx_special_f32 &sp)
+{
+    float *const *H = a.T;
+    float *const *Hs = a.S;
+    float *const *E = a.T + 3;
+    float *const *Es = a.S + 3;
+    const int k = sp.cls;
+    if (k == 'h') return;
+    float P[3], M[3];
+    if (k == 'n' || k == 'a') {
+        const nb3 n = nb_forward(a, p);
+        curl_halves<true>(H, nullptr, n, p.l, P, M);
+        if (k == 'n') {
+            float W[3] = {P[0] + M[0], P[1] + M[1], P[2] + M[2]};
+            if (p.j2 == a.iI) {
+                W[0] += a.sgn * (float)source_value(a, p.q, 1);
+                W[1] -= a.sgn * (float)source_value(a, p.q, 0);
+            }
+            for (int c = 0; c < 3; c++) E[c][p.l] = sp.eA[c] * E[c][p.l] + sp.eB[c] * W[c];
+        } else {
+            for (int c = 0; c < 3; c++) {
+                const float s = Es[c][p.l], m = E[c][p.l] - s;
+                const float m2 = sp.eA[c] * m + sp.eB[c] * P[c];
+                const float s2 = sp.eAs[c] * s + sp.eBs[c] * M[c];
+                E[c][p.l] = m2 + s2;
+                Es[c][p.l] = s2;
+            }
+        }
+        return;
+    }
+    /* 'e', 'b': BACKWARD curl of the H main slots (F:310) */
+    const nb3 n = nb_backward(a, p);
+    curl_halves<false>(H, Hs, n, p.l, P, M);
+    for (int c = 0; c < 3; c++) {
+        const float s = Es[c][p.l];
+        float m = E[c][p.l] - s;
+        m += 0.5f * s;
+        const float s2 = sp.eAs[c] * s + sp.eBs[c] * (P[c] + M[c]);
+        m += 0.5f * s2;
+        E[c][p.l] = m + s2;
+        Es[c][p.l] = s2;
+    }
+}
+
+/* ---------------------------------------------------------------------------------- */
+/* Sweeps                                                                             */
+/* ---------------------------------------------------------------------------------- */
+
+template <int BX>
+__global__ __launch_bounds__(kThreads) void h_sweep_f32(f32_args a)
+{
+    cellpos p;
+    if (!decode<BX>(a, p)) return;
+    const float b0 = a.b[0][p.l];
+    if (__builtin_isnan(b0)) {
+        special_h(a, p, a.special[__float_as_uint(a.b[1][p.l])]);
+        return;
+    }
+    const float bc[3] = {b0, a.b[1][p.l], a.b[2][p.l]};
+    const nb3 n = nb_backward(a, p);
+    float P[3], M[3];
+    curl_halves<false>(a.T + 3, nullptr, n, p.l, P, M);
+    const bool is_pml = a.pmlflag[p.j0] | a.pmlflag[a.L0 + p.j1] | a.pmlflag[a.L0 + a.L1 + p.j2];
+    if (!is_pml) {
+        if (p.j2 == a.iI) { /* sheet source, 'n' cells only */
+            P[0] += a.sgn * (float)source_value(a, p.q, 1);
+            P[1] -= a.sgn * (float)source_value(a, p.q, 0);
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) a.T[c][p.l] -= bc[c] * (P[c] + M[c]);
+        return;
+    }
+    const int jj[3] = {p.j0, p.j1, p.j2};
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const int dm = c == 2 ? 0 : c + 1, ds = c == 0 ? 2 : c - 1; /* main axis c+1, split c+2 */
+        const float *tm = pml_axis(a, dm), *ts = pml_axis(a, ds);
+        const int Lm = dm == 0 ? a.L0 : dm == 1 ? a.L1 : a.L2;
+        const int Ls = ds == 0 ? a.L0 : ds == 1 ? a.L1 : a.L2;
+        const float aM = tm[jj[dm]], gM = tm[Lm + jj[dm]];
+        const float aS = ts[jj[ds]], gS = ts[Ls + jj[ds]];
+        const float s = a.S[c][p.l], m = a.T[c][p.l] - s;
+        const float m2 = aM * m - bc[c] * gM * P[c];
+        const float s2 = aS * s - bc[c] * gS * M[c];
+        a.T[c][p.l] = m2 + s2;
+        a.S[c][p.l] = s2;
+    }
+}
+
+template <int BX>
+__global__ __launch_bounds__(kThreads) void e_sweep_f32(f32_args a)
+{
+    cellpos p;
+    if (!decode<BX>(a, p)) return;
+    float *const *E = a.T + 3;
+    float *const *Es = a.S + 3;
+    const float b0 = a.b[3][p.l];
+    if (__builtin_isnan(b0)) {
+        special_e(a, p, a.special[__float_as_uint(a.b[4][p.l])]);
+    } else {
+        const float bc[3] = {b0, a.b[4][p.l], a.b[5][p.l]};
+        const nb3 n = nb_forward(a, p);
+        float P[3], M[3];
+        curl_halves<true>(a.T, nullptr, n, p.l, P, M);
+        const bool is_pml = a.pmlflag[p.j0] | a.pmlflag[a.L0 + p.j1] | a.pmlflag[a.L0 + a.L1 + p.j2];
+        if (!is_pml) {
+            if (p.j2 == a.iI) {
+                P[0] += a.sgn * (float)source_value(a, p.q, 1);
+                P[1] -= a.sgn * (float)source_value(a, p.q, 0);
+            }
+#pragma unroll
+            for (int c = 0; c < 3; c++) E[c][p.l] += bc[c] * (P[c] + M[c]);
+        } else {
+            const int jj[3] = {p.j0, p.j1, p.j2};
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const int dm = c == 2 ? 0 : c + 1, ds = c == 0 ? 2 : c - 1;
+                const float *tm = pml_axis(a, dm), *ts = pml_axis(a, ds);
+                const int Lm = dm == 0 ? a.L0 : dm == 1 ? a.L1 : a.L2;
+                const int Ls = ds == 0 ? a.L0 : ds == 1 ? a.L1 : a.L2;
+                const float aM = tm[2 * Lm + jj[dm]], gM = tm[3 * Lm + jj[dm]];
+                const float aS = ts[2 * Ls + jj[ds]], gS = ts[3 * Ls + jj[ds]];
+                const float s = Es[c][p.l], m = E[c][p.l] - s;
+                const float m2 = aM * m + bc[c] * gM * P[c];
+                const float s2 = aS * s + bc[c] * gS * M[c];
+                E[c][p.l] = m2 + s2;
+                Es[c][p.l] = s2;
+            }
+        }
+    }
+    /* monitors: reference main slots, main = T - S (F:318-328) */
+    if (p.j2 == a.iB || p.j2 == a.iT) {
+        double hv[2], ev[2];
+        for (int v = 0; v < 2; v++) {
+            hv[v] = (double)a.T[v][p.l] - (double)a.S[v][p.l];
+            ev[v] = (double)E[v][p.l] - (double)Es[v][p.l];
+        }
+        for (int m = 0; m < 2; m++) {
+            if (p.j2 != (m == 0 ? a.iB : a.iT)) continue;
+            double *HM = m == 0 ? a.HB : a.HT, *EM = m == 0 ? a.EB : a.ET;
+            for (int v = 0; v < 2; v++) {
+                const int64_t o = 2 * (v + 2 * p.q);
+                HM[o] += a.monH_re * hv[v];
+                HM[o + 1] += a.monH_im * hv[v];
+                EM[o] += a.monE_re * ev[v];
+                EM[o + 1] += a.monE_im * ev[v];
+            }
+        }
+    }
+}
+
+/* ---------------------------------------------------------------------------------- */
+/* Host side                                                                          */
+/* ---------------------------------------------------------------------------------- */
+
+const double kTol = 10. * 2.220446049250313e-16; /* constant.h:7 */
+
+f32_args base_args(const smx_f32_state *st, const smx_geom &g)
+{
+    f32_args a;
+    std::memset(&a, 0, sizeof a);
+    a.L0 = g.L0;
+    a.L1 = g.L1;
+    a.L2 = g.L2;
+    const int BY = kThreads / st->bx;
+    a.gx = (g.L0 + st->bx - 1) / st->bx;
+    a.gy = (g.L1 + BY - 1) / BY;
+    a.nblocks = a.gx * a.gy * g.L2;
+    a.sy = g.L0;
+    a.sz = (int64_t)g.L0 * g.L1;
+    a.w[0] = (float)g.iBc0;
+    a.w[1] = (float)g.iBc1;
+    a.w[2] = (float)g.iBc2;
+    a.iB = g.iB;
+    a.iT = g.iT;
+    a.iI = g.iI;
+    a.sgn = (float)g.sgn;
+    for (int i = 0; i < 6; i++) {
+        a.T[i] = st->T[i];
+        a.S[i] = st->S[i];
+        a.b[i] = st->b[i];
+    }
+    a.pml = st->pml;
+    a.pmlflag = st->pmlflag;
+    a.special = st->special;
+    return a;
+}
+
+} // namespace
+
+const char *smx_f32_last_error(void) { return f32_err; }
+
+void smx_f32_free(smx_f32_state *st)
+{
+    for (int i = 0; i < 6; i++) {
+        if (st->T[i]) (void)hipFree(st->T[i]);
+        if (st->S[i]) (void)hipFree(st->S[i]);
+        if (st->b[i]) (void)hipFree(st->b[i]);
+        st->T[i] = st->S[i] = st->b[i] = nullptr;
+    }
+    if (st->pml) (void)hipFree(st->pml);
+    if (st->pmlflag) (void)hipFree(st->pmlflag);
+    if (st->special) (void)hipFree(st->special);
+    st->pml = nullptr;
+    st->pmlflag = nullptr;
+    st->special = nullptr;
+    delete[] st->cont_host;
+    delete[] st->addr_host;
+    st->cont_host = nullptr;
+    st->addr_host = nullptr;
+}
+
+int smx_f32_create(smx_f32_state *st, const smx_geom *g, double iR, double iI, const double *mu,
+                   const double *eps, const double *sigma, const char *Cont, const int *addr,
+                   hipStream_t stream)
+{
+    const int L[4] = {g->L0, g->L1, g->L2, 0};
+    const int64_t L0 = g->L0, L1 = g->L1, L2 = g->L2, n = L0 * L1 * L2;
+    std::memset(st, 0, sizeof *st);
+    st->L[0] = g->L0;
+    st->L[1] = g->L1;
+    st->L[2] = g->L2;
+    st->n = n;
+    st->Lnotn = g->Lnotn;
+    st->bx = (g->L0 % 64 == 0 || (g->L0 > 32 && g->L0 % 32 != 0)) ? 64 : 32;
+
+    /* per-axis PML tables and flags */
+    const int64_t off[3] = {0, 2 * L0, 2 * L0 + 2 * L1};
+    const int Ld[3] = {g->L0, g->L1, g->L2};
+    std::vector<float> pml(4 * (size_t)(L0 + L1 + L2));
+    std::vector<unsigned char> flag((size_t)(L0 + L1 + L2));
+    {
+        size_t base = 0, fbase = 0;
+        for (int d = 0; d < 3; d++) {
+            for (int j = 0; j < Ld[d]; j++) {
+                const double sh = sigma[off[d] + 2 * j], se = sigma[off[d] + 2 * j + 1];
+                pml[base + j] = (float)((iR - iI * sh) / (iR + iI * sh));
+                pml[base + Ld[d] + j] = (float)(iR / (iR + iI * sh));
+                pml[base + 2 * Ld[d] + j] = (float)((iR - iI * se) / (iR + iI * se));
+                pml[base + 3 * Ld[d] + j] = (float)(iR / (iR + iI * se));
+                flag[fbase + j] = std::fabs(sh) > kTol || std::fabs(se) > kTol;
+            }
+            base += 4 * (size_t)Ld[d];
+            fbase += Ld[d];
+        }
+    }
+
+    /* coefficient words; lossy and metal cells go to the special table */
+    std::vector<float> bw(6 * (size_t)n);
+    std::vector<smx_special_f32> special;
+    for (int64_t l = 0; l < n; l++) {
+        const int64_t x = l % L0, y = (l / L0) % L1, z = l / (L0 * L1);
+        const bool is_a = flag[x] || flag[L0 + y] || flag[L0 + L1 + z];
+        const char k = Cont[l];
+        bool lossy = false;
+        for (int c = 0; c < 3; c++) lossy |= mu[6 * l + 2 * c + 1] != 0. || eps[6 * l + 2 * c + 1] != 0.;
+        const bool regular = !lossy && (k == 'n' || k == 'a');
+        if (regular && (k == 'a') != is_a) {
+            snprintf(f32_err, sizeof f32_err,
+                     "cell %lld: class '%c' disagrees with the separable PML profile", (long long)l, k);
+            return SMX_ERR_ARG;
+        }
+        if (regular) {
+            for (int c = 0; c < 3; c++) {
+                bw[(size_t)c * n + l] = (float)(1. / (iR * mu[6 * l + 2 * c]));
+                bw[(size_t)(3 + c) * n + l] = (float)(1. / (iR * eps[6 * l + 2 * c]));
+            }
+            continue;
+        }
+        double sg[6], w[24] = {0};
+        smx_cell_sigma(L, sigma, x, y, z, sg);
+        smx_cell_coeffs(k, mu + 6 * l, eps + 6 * l, sg, iR, iI, w);
+        smx_special_f32 sp;
+        std::memset(&sp, 0, sizeof sp);
+        sp.cls = k;
+        for (int c = 0; c < 3; c++) {
+            sp.hA[c] = (float)w[c];
+            sp.hB[c] = (float)w[3 + c];
+            sp.eA[c] = (float)w[6 + c];
+            sp.eB[c] = (float)w[9 + c];
+            sp.hAs[c] = (float)w[12 + c];
+            sp.hBs[c] = (float)w[15 + c];
+            sp.eAs[c] = (float)w[18 + c];
+            sp.eBs[c] = (float)w[21 + c];
+        }
+        const uint32_t idx = (uint32_t)special.size();
+        special.push_back(sp);
+        float fidx;
+        std::memcpy(&fidx, &idx, sizeof fidx);
+        const float nan = std::numeric_limits<float>::quiet_NaN();
+        bw[0 * (size_t)n + l] = nan;
+        bw[3 * (size_t)n + l] = nan;
+        bw[1 * (size_t)n + l] = fidx;
+        bw[4 * (size_t)n + l] = fidx;
+        bw[2 * (size_t)n + l] = 0.f;
+        bw[5 * (size_t)n + l] = 0.f;
+    }
+    st->nspecial = (int)special.size();
+
+    for (int i = 0; i < 6; i++) {
+        F32_HIP(hipMalloc(&st->T[i], sizeof(float) * n));
+        F32_HIP(hipMalloc(&st->S[i], sizeof(float) * n));
+        F32_HIP(hipMalloc(&st->b[i], sizeof(float) * n));
+        F32_HIP(hipMemsetAsync(st->T[i], 0, sizeof(float) * n, stream));
+        F32_HIP(hipMemsetAsync(st->S[i], 0, sizeof(float) * n, stream));
+        F32_HIP(hipMemcpyAsync(st->b[i], bw.data() + (size_t)i * n, sizeof(float) * n, hipMemcpyHostToDevice,
+                               stream));
+    }
+    F32_HIP(hipMalloc(&st->pml, sizeof(float) * pml.size()));
+    F32_HIP(hipMemcpyAsync(st->pml, pml.data(), sizeof(float) * pml.size(), hipMemcpyHostToDevice, stream));
+    F32_HIP(hipMalloc(&st->pmlflag, flag.size()));
+    F32_HIP(hipMemcpyAsync(st->pmlflag, flag.data(), flag.size(), hipMemcpyHostToDevice, stream));
+    if (!special.empty()) {
+        F32_HIP(hipMalloc(&st->special, sizeof(smx_special_f32) * special.size()));
+        F32_HIP(hipMemcpyAsync(st->special, special.data(), sizeof(smx_special_f32) * special.size(),
+                               hipMemcpyHostToDevice, stream));
+    }
+    st->cont_host = new char[n];
+    st->addr_host = new int[n];
+    std::memcpy(st->cont_host, Cont, n);
+    std::memcpy(st->addr_host, addr, sizeof(int) * n);
+    F32_HIP(hipStreamSynchronize(stream));
+    return SMX_OK;
+}
+
+hipError_t smx_f32_step(smx_f32_state *st, const smx_f32_launch_args *la, hipStream_t stream)
+{
+    f32_args a = base_args(st, la->g);
+    a.HB = la->HB;
+    a.EB = la->EB;
+    a.HT = la->HT;
+    a.ET = la->ET;
+    a.monH_re = la->s.monH_re;
+    a.monH_im = la->s.monH_im;
+    a.monE_re = la->s.monE_re;
+    a.monE_im = la->s.monE_im;
+    const dim3 grid(a.nblocks), block(kThreads);
+    /* H sweep: source from Ei */
+    a.src_plane = la->srcE_plane;
+    a.inc = la->Einc;
+    a.src_f2 = la->s.srcE_f2;
+    a.src_re = la->s.srcE_re;
+    a.src_im = la->s.srcE_im;
+    if (st->bx == 64) hipLaunchKernelGGL(h_sweep_f32<64>, grid, block, 0, stream, a);
+    else hipLaunchKernelGGL(h_sweep_f32<32>, grid, block, 0, stream, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    /* E sweep: source from Hi */
+    a.src_plane = la->srcH_plane;
+    a.inc = la->Hinc;
+    a.src_f2 = la->s.srcH_f2;
+    a.src_re = la->s.srcH_re;
+    a.src_im = la->s.srcH_im;
+    if (st->bx == 64) hipLaunchKernelGGL(e_sweep_f32<64>, grid, block, 0, stream, a);
+    else hipLaunchKernelGGL(e_sweep_f32<32>, grid, block, 0, stream, a);
+    return hipGetLastError();
+}
+
+int smx_f32_write_fields(smx_f32_state *st, const double *H_t, const double *E_t, hipStream_t stream)
+{
+    const int64_t n = st->n;
+    std::vector<float> T(6 * (size_t)n), S(6 * (size_t)n);
+    for (int64_t l = 0; l < n; l++) {
+        const bool special = st->cont_host[l] != 'n';
+        const int64_t s = 3 * n + 3 * (int64_t)st->addr_host[l];
+        for (int c = 0; c < 3; c++) {
+            const double hs = special ? H_t[s + c] : 0., es = special ? E_t[s + c] : 0.;
+            T[(size_t)c * n + l] = (float)(H_t[3 * l + c] + hs);
+            T[(size_t)(3 + c) * n + l] = (float)(E_t[3 * l + c] + es);
+            S[(size_t)c * n + l] = (float)hs;
+            S[(size_t)(3 + c) * n + l] = (float)es;
+        }
+    }
+    for (int i = 0; i < 6; i++) {
+        F32_HIP(hipMemcpyAsync(st->T[i], T.data() + (size_t)i * n, sizeof(float) * n, hipMemcpyHostToDevice, stream));
+        F32_HIP(hipMemcpyAsync(st->S[i], S.data() + (size_t)i * n, sizeof(float) * n, hipMemcpyHostToDevice, stream));
+    }
+    F32_HIP(hipStreamSynchronize(stream));
+    return SMX_OK;
+}
+
+int smx_f32_read_fields(const smx_f32_state *st, double *H_t, double *E_t, hipStream_t stream)
+{
+    const int64_t n = st->n;
+    std::vector<float> T(6 * (size_t)n), S(6 * (size_t)n);
+    for (int i = 0; i < 6; i++) {
+        F32_HIP(hipMemcpyAsync(T.data() + (size_t)i * n, st->T[i], sizeof(float) * n, hipMemcpyDeviceToHost, stream));
+        F32_HIP(hipMemcpyAsync(S.data() + (size_t)i * n, st->S[i], sizeof(float) * n, hipMemcpyDeviceToHost, stream));
+    }
+    F32_HIP(hipStreamSynchronize(stream));
+    const size_t flen = 3 * (size_t)(n + st->Lnotn + 1);
+    std::memset(H_t, 0, sizeof(double) * flen);
+    std::memset(E_t, 0, sizeof(double) * flen);
+    for (int64_t l = 0; l < n; l++) {
+        const bool special = st->cont_host[l] != 'n';
+        const int64_t s = 3 * n + 3 * (int64_t)st->addr_host[l];
+        for (int c = 0; c < 3; c++) {
+            const double th = T[(size_t)c * n + l], sh = S[(size_t)c * n + l];
+            const double te = T[(size_t)(3 + c) * n + l], se = S[(size_t)(3 + c) * n + l];
+            H_t[3 * l + c] = th - sh;
+            E_t[3 * l + c] = te - se;
+            if (special) {
+                H_t[s + c] = sh;
+                E_t[s + c] = se;
+            }
+        }
+    }
+    return SMX_OK;
+}

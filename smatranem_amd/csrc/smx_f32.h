@@ -1,0 +1,64 @@
+/*
+ * smx_f32.h -- the FP32 plan: packed device layout and its kernels (kern_f32.hip).
+ *
+ * Layout (all planar, j0 fastest, n = L0*L1*L2 floats each):
+ *   T[6]  total fields Hx Hy Hz Ex Ey Ez.  Total = reference main + split slot, the
+ *         field every neighbour's curl sees (main_calc adds the split curl through
+ *         address_notn, FDTDcalc/FDTDsplitfieldMetal.c:266, :298).
+ *   S[6]  split ('a' cells: the '-' part) or stable-form aux (metal cells); zero in
+ *         every other cell.  Reference main slot = T - S.
+ *   b[6]  per-cell coefficient words bH_c = 1/(iR*mu_c), bE_c = 1/(iR*eps_c).  A
+ *         lossy or metal cell is flagged by bH_x = bE_x = NaN, with bH_y/bE_y holding
+ *         (as raw bits) its index in the special table.
+ *   pml   per-axis tables for 'a' cells: aH, gH (integer samples), aE, gE (half
+ *         samples) with A = (iR - iI*s)/(iR + iI*s), B = b*iR/(iR + iI*s), plus the
+ *         per-axis 'a' flag.
+ *   special  full per-component coefficients and class of lossy/metal cells.
+ */
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "smx_internal.h"
+
+struct smx_special_f32 {
+    int cls; /* reference class char: 'n', 'a', 'e', 'h', 'b' */
+    int pad[3];
+    float hA[3], hB[3], hAs[3], hBs[3]; /* H: main A, B; split/aux A, B */
+    float eA[3], eB[3], eAs[3], eBs[3]; /* E likewise */
+};
+
+struct smx_f32_state {
+    int L[3];
+    int64_t n;
+    float *T[6];
+    float *S[6];
+    float *b[6];
+    float *pml;             /* 4*(L0+L1+L2) floats: per axis aH[L], gH[L], aE[L], gE[L] */
+    unsigned char *pmlflag; /* L0+L1+L2 */
+    smx_special_f32 *special;
+    int nspecial;
+    int bx; /* x width of a block (32 or 64) */
+    /* host copies for the reference-layout field import/export */
+    char *cont_host;
+    int *addr_host;
+    int Lnotn;
+};
+
+struct smx_f32_launch_args {
+    smx_geom g;
+    smx_step_scalars s;
+    const double *srcH_plane, *srcE_plane;
+    const double *Hinc, *Einc;
+    double *HB, *EB, *HT, *ET;
+};
+
+int smx_f32_create(smx_f32_state *st, const smx_geom *g, double iR, double iI, const double *mu,
+                   const double *eps, const double *sigma, const char *Cont, const int *addr,
+                   hipStream_t stream);
+void smx_f32_free(smx_f32_state *st);
+hipError_t smx_f32_step(smx_f32_state *st, const smx_f32_launch_args *a, hipStream_t stream);
+int smx_f32_write_fields(smx_f32_state *st, const double *H_t, const double *E_t, hipStream_t stream);
+int smx_f32_read_fields(const smx_f32_state *st, double *H_t, double *E_t, hipStream_t stream);
+const char *smx_f32_last_error(void);

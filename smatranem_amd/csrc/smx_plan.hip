@@ -1,0 +1,513 @@
+/*
+ * smx_plan.hip -- plan lifetime, device memory, the per-step driver and the C ABI
+ * entry points of libsmx_fdtd.so (declared in include/smx_fdtd.h).
+ *
+ * A plan owns one grid's device state.  The host computes the per-step scalars of
+ * the reference's time loop (envelope, phases; FDTDcalc/FDTDsplitfieldMetal.c:153-167)
+ * and launches the sweeps on the plan's stream; no host/device copy happens inside
+ * smx_plan_run.
+ */
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "smx_fdtd.h"
+#include "smx_internal.h"
+#include "smx_f32.h"
+
+namespace {
+
+thread_local char g_last_error[512] = "";
+
+int hip_fail(hipError_t e, const char *what)
+{
+    snprintf(g_last_error, sizeof g_last_error, "%s: %s", what, hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? SMX_ERR_OOM : SMX_ERR_HIP;
+}
+
+#define SMX_HIP(call)                                                                    \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess) return hip_fail(e_, #call);                                \
+    } while (0)
+
+int arg_fail(const char *what)
+{
+    snprintf(g_last_error, sizeof g_last_error, "invalid argument: %s", what);
+    return SMX_ERR_ARG;
+}
+
+} // namespace
+
+struct smx_plan {
+    smx_grid g{};
+    int64_t n = 0;        /* cells */
+    int64_t plane = 0;    /* L0*L1 */
+    int64_t flen = 0;     /* 3*(n+Lnotn+1), one reference field array */
+    int device = 0;
+    hipStream_t stream = nullptr;
+
+    /* F64: reference storage contract */
+    double *Ah = nullptr, *Bh = nullptr, *Ae = nullptr, *Be = nullptr;
+    double *H = nullptr, *E = nullptr;
+    char *Cont = nullptr;
+    int *addr = nullptr;
+
+    /* F32: packed layout (kern_f32.hip) */
+    smx_f32_state f32{};
+
+    /* shared: monitors (4 x complex[2*plane]), source profiles and explicit planes */
+    double *mon = nullptr;
+    double *inc = nullptr; /* Hinc then Einc, complex[2*plane] each */
+    double *src = nullptr; /* Hi then Ei, double[2*plane] each */
+
+    bool have_source = false;
+    int N = 0;
+    int tp[3] = {0, 0, 0};
+    std::vector<double> expn; /* complex[2N] */
+};
+
+namespace {
+
+smx_geom make_geom(const smx_plan *p)
+{
+    smx_geom g;
+    g.L0 = p->g.L[0];
+    g.L1 = p->g.L[1];
+    g.L2 = p->g.L[2];
+    g.iBc0 = p->g.iBc[0];
+    g.iBc1 = p->g.iBc[1];
+    g.iBc2 = p->g.iBc[2];
+    g.iB = p->g.iBTIin[0];
+    g.iT = p->g.iBTIin[1];
+    g.iI = p->g.iBTIin[2];
+    g.sgn = (double)(p->g.iBTIin[3] / std::abs(p->g.iBTIin[3])); /* F:253 */
+    g.n = p->n;
+    g.Lnotn = p->g.Lnotn;
+    return g;
+}
+
+int validate_grid(const smx_grid *g, const char *Cont)
+{
+    for (int d = 0; d < 3; d++)
+        if (g->L[d] < 1) return arg_fail("L[0..2] must be >= 1");
+    for (int d = 0; d < 3; d++)
+        if (g->iBc[d] < -1 || g->iBc[d] > 1) return arg_fail("iBc must be -1, 0 or 1");
+    if (g->iBTIin[3] == 0) return arg_fail("incident mode must be nonzero");
+    /* a monitor plane of -1 switches that monitor off (an extension: with PML on the
+     * x/y faces every plane holds PML cells, which the reference refuses, F:320) */
+    for (int m = 0; m < 2; m++)
+        if (g->iBTIin[m] < -1 || g->iBTIin[m] >= g->L[2]) return arg_fail("monitor plane outside the grid");
+    if (g->precision != SMX_F64 && g->precision != SMX_F32) return arg_fail("precision");
+    const int64_t n = (int64_t)g->L[0] * g->L[1] * g->L[2];
+    int64_t count = 0;
+    for (int64_t l = 0; l < n; l++) {
+        const char k = Cont[l];
+        if (k != 'n' && k != 'a' && k != 'e' && k != 'h' && k != 'b') return arg_fail("Cont holds an unknown class");
+        count += k != 'n';
+    }
+    if (count != g->Lnotn) return arg_fail("Lnotn differs from the count of non-'n' cells");
+    /* The reference exits at the first step when a monitor plane holds a PML cell
+     * (F:320); the class map never changes, so the plan refuses it up front. */
+    const int64_t plane = (int64_t)g->L[0] * g->L[1];
+    for (int m = 0; m < 2; m++)
+        for (int64_t q = 0; q < plane && g->iBTIin[m] >= 0; q++)
+            if (Cont[q + plane * g->iBTIin[m]] == 'a') {
+                snprintf(g_last_error, sizeof g_last_error, "iBTIin[%d] error!", m);
+                return SMX_ERR_MONITOR_PML;
+            }
+    return SMX_OK;
+}
+
+int alloc_common(smx_plan *p)
+{
+    SMX_HIP(hipGetDevice(&p->device));
+    SMX_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+    const size_t plane = (size_t)p->plane;
+    SMX_HIP(hipMalloc(&p->mon, sizeof(double) * 16 * plane));
+    SMX_HIP(hipMemsetAsync(p->mon, 0, sizeof(double) * 16 * plane, p->stream));
+    SMX_HIP(hipMalloc(&p->inc, sizeof(double) * 8 * plane));
+    SMX_HIP(hipMemsetAsync(p->inc, 0, sizeof(double) * 8 * plane, p->stream));
+    SMX_HIP(hipMalloc(&p->src, sizeof(double) * 4 * plane));
+    return SMX_OK;
+}
+
+int alloc_f64(smx_plan *p, const double *Ah, const double *Bh, const double *Ae, const double *Be,
+              const char *Cont, const int *addr)
+{
+    const size_t clen = 3 * (size_t)(p->n + p->g.Lnotn), flen = (size_t)p->flen, n = (size_t)p->n;
+    double **coef[4] = {&p->Ah, &p->Bh, &p->Ae, &p->Be};
+    const double *host[4] = {Ah, Bh, Ae, Be};
+    for (int i = 0; i < 4; i++) {
+        SMX_HIP(hipMalloc(coef[i], sizeof(double) * clen));
+        SMX_HIP(hipMemcpyAsync(*coef[i], host[i], sizeof(double) * clen, hipMemcpyHostToDevice, p->stream));
+    }
+    SMX_HIP(hipMalloc(&p->H, sizeof(double) * flen));
+    SMX_HIP(hipMalloc(&p->E, sizeof(double) * flen));
+    SMX_HIP(hipMemsetAsync(p->H, 0, sizeof(double) * flen, p->stream));
+    SMX_HIP(hipMemsetAsync(p->E, 0, sizeof(double) * flen, p->stream));
+    SMX_HIP(hipMalloc(&p->Cont, n));
+    SMX_HIP(hipMemcpyAsync(p->Cont, Cont, n, hipMemcpyHostToDevice, p->stream));
+    SMX_HIP(hipMalloc(&p->addr, sizeof(int) * n));
+    SMX_HIP(hipMemcpyAsync(p->addr, addr, sizeof(int) * n, hipMemcpyHostToDevice, p->stream));
+    SMX_HIP(hipStreamSynchronize(p->stream));
+    return SMX_OK;
+}
+
+smx_plan *new_plan(const smx_grid *g)
+{
+    smx_plan *p = new (std::nothrow) smx_plan();
+    if (!p) return nullptr;
+    p->g = *g;
+    p->n = (int64_t)g->L[0] * g->L[1] * g->L[2];
+    p->plane = (int64_t)g->L[0] * g->L[1];
+    p->flen = 3 * (p->n + g->Lnotn + 1);
+    return p;
+}
+
+void free_plan(smx_plan *p)
+{
+    if (!p) return;
+    if (p->stream) (void)hipStreamSynchronize(p->stream);
+    void *bufs[] = {p->Ah, p->Bh, p->Ae, p->Be, p->H, p->E, p->Cont, p->addr, p->mon, p->inc, p->src};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    smx_f32_free(&p->f32);
+    if (p->stream) (void)hipStreamDestroy(p->stream);
+    delete p;
+}
+
+int check_device()
+{
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count < 1) {
+        snprintf(g_last_error, sizeof g_last_error, "no HIP device: %s", hipGetErrorString(e));
+        return SMX_ERR_NO_DEVICE;
+    }
+    return SMX_OK;
+}
+
+/* Scalars of step (P, t) of the loop at F:153-167. */
+int step_scalars(const smx_plan *p, int P, int t, smx_step_scalars *s, double *envH, double *envE)
+{
+    const int N = p->N;
+    const double fE = smx_set_env(N, p->tp, 1, t, P);
+    const double fH = smx_set_env(N, p->tp, 0, t + 1, P);
+    if (std::isnan(fE) || std::isnan(fH)) {
+        snprintf(g_last_error, sizeof g_last_error, "f_env error!");
+        return SMX_ERR_ENVELOPE;
+    }
+    const double *e = p->expn.data();
+    const int tn = (t + 1) % N;
+    s->srcE_f2 = (2.) * fE;
+    s->srcE_re = e[4 * t + 2];
+    s->srcE_im = e[4 * t + 3];
+    s->srcH_f2 = (2.) * fH;
+    s->srcH_re = e[4 * tn + 0];
+    s->srcH_im = e[4 * tn + 1];
+    s->monH_re = e[4 * tn + 0];
+    s->monH_im = e[4 * tn + 1];
+    s->monE_re = e[4 * tn + 2];
+    s->monE_im = e[4 * tn + 3];
+    *envH = fH;
+    *envE = fE;
+    return SMX_OK;
+}
+
+smx_f64_args f64_args(smx_plan *p)
+{
+    smx_f64_args a;
+    a.g = make_geom(p);
+    a.Ah = p->Ah;
+    a.Bh = p->Bh;
+    a.Ae = p->Ae;
+    a.Be = p->Be;
+    a.H = p->H;
+    a.E = p->E;
+    a.Cont = p->Cont;
+    a.addr = p->addr;
+    a.srcH_plane = nullptr;
+    a.srcE_plane = nullptr;
+    a.Hinc = p->inc;
+    a.Einc = p->inc + 4 * p->plane;
+    a.HB = p->mon;
+    a.EB = p->mon + 4 * p->plane;
+    a.HT = p->mon + 8 * p->plane;
+    a.ET = p->mon + 12 * p->plane;
+    std::memset(&a.s, 0, sizeof a.s);
+    return a;
+}
+
+int launch_step(smx_plan *p, const smx_step_scalars &s, bool explicit_src)
+{
+    if (p->g.precision == SMX_F64) {
+        smx_f64_args a = f64_args(p);
+        a.s = s;
+        if (explicit_src) {
+            a.srcH_plane = p->src;
+            a.srcE_plane = p->src + 2 * p->plane;
+        }
+        SMX_HIP(smx_launch_f64_step(&a, p->stream));
+        return SMX_OK;
+    }
+    smx_f32_launch_args a;
+    a.g = make_geom(p);
+    a.s = s;
+    a.srcH_plane = explicit_src ? p->src : nullptr;
+    a.srcE_plane = explicit_src ? p->src + 2 * p->plane : nullptr;
+    a.Hinc = p->inc;
+    a.Einc = p->inc + 4 * p->plane;
+    a.HB = p->mon;
+    a.EB = p->mon + 4 * p->plane;
+    a.HT = p->mon + 8 * p->plane;
+    a.ET = p->mon + 12 * p->plane;
+    SMX_HIP(smx_f32_step(&p->f32, &a, p->stream));
+    return SMX_OK;
+}
+
+} // namespace
+
+/* ==================================================================================== */
+/* C ABI                                                                                */
+/* ==================================================================================== */
+
+extern "C" {
+
+const char *smx_status_string(int status)
+{
+    switch (status) {
+    case SMX_OK: return "ok";
+    case SMX_ERR_ARG: return "invalid argument";
+    case SMX_ERR_HIP: return "HIP runtime error";
+    case SMX_ERR_NO_DEVICE: return "no HIP device";
+    case SMX_ERR_OOM: return "device out of memory";
+    case SMX_ERR_STATE: return "call out of order";
+    case SMX_ERR_MONITOR_PML: return "monitor plane holds a PML cell";
+    case SMX_ERR_ENVELOPE: return "step beyond the stop period";
+    default: return "unknown status";
+    }
+}
+
+const char *smx_last_error(void) { return g_last_error; }
+
+int smx_plan_create_coeffs(const smx_grid *grid, const double *Ah, const double *Bh,
+                           const double *Ae, const double *Be, const char *Cont,
+                           const int *address_notn, smx_plan **out)
+{
+    if (!grid || !Ah || !Bh || !Ae || !Be || !Cont || !address_notn || !out) return arg_fail("null pointer");
+    *out = nullptr;
+    if (grid->precision != SMX_F64) return arg_fail("coefficient plans are FP64 only");
+    int rc = validate_grid(grid, Cont);
+    if (rc) return rc;
+    if ((rc = check_device())) return rc;
+    smx_plan *p = new_plan(grid);
+    if (!p) return SMX_ERR_OOM;
+    if ((rc = alloc_common(p)) || (rc = alloc_f64(p, Ah, Bh, Ae, Be, Cont, address_notn))) {
+        free_plan(p);
+        return rc;
+    }
+    *out = p;
+    return SMX_OK;
+}
+
+int smx_plan_create(const smx_grid *grid, const double *mu, const double *eps, const double *sigma,
+                    const char *Cont, const int *address_notn, smx_plan **out)
+{
+    if (!grid || !mu || !eps || !sigma || !Cont || !address_notn || !out) return arg_fail("null pointer");
+    *out = nullptr;
+    int rc = validate_grid(grid, Cont);
+    if (rc) return rc;
+    if ((rc = check_device())) return rc;
+    smx_plan *p = new_plan(grid);
+    if (!p) return SMX_ERR_OOM;
+    if ((rc = alloc_common(p))) {
+        free_plan(p);
+        return rc;
+    }
+    if (grid->precision == SMX_F64) {
+        const size_t clen = 3 * (size_t)(p->n + grid->Lnotn);
+        std::vector<double> A_h(clen, 0.), B_h(clen, 0.), A_e(clen, 0.), B_e(clen, 0.);
+        smx_build_coeffs(grid->L, mu, eps, sigma, Cont, address_notn, grid->Lnotn, grid->invtau_R,
+                         grid->invtau_I, A_h.data(), B_h.data(), A_e.data(), B_e.data());
+        rc = alloc_f64(p, A_h.data(), B_h.data(), A_e.data(), B_e.data(), Cont, address_notn);
+    } else {
+        smx_geom g = make_geom(p);
+        rc = smx_f32_create(&p->f32, &g, grid->invtau_R, grid->invtau_I, mu, eps, sigma, Cont,
+                            address_notn, p->stream);
+        if (rc == SMX_ERR_HIP || rc == SMX_ERR_OOM)
+            snprintf(g_last_error, sizeof g_last_error, "%s", smx_f32_last_error());
+    }
+    if (rc) {
+        free_plan(p);
+        return rc;
+    }
+    *out = p;
+    return SMX_OK;
+}
+
+int smx_plan_destroy(smx_plan *plan)
+{
+    free_plan(plan);
+    return SMX_OK;
+}
+
+size_t smx_plan_field_len(const smx_plan *plan) { return plan ? (size_t)plan->flen : 0; }
+
+int smx_plan_write_fields(smx_plan *p, const double *H_t, const double *E_t)
+{
+    if (!p || !H_t || !E_t) return arg_fail("null pointer");
+    if (p->g.precision == SMX_F64) {
+        SMX_HIP(hipMemcpyAsync(p->H, H_t, sizeof(double) * p->flen, hipMemcpyHostToDevice, p->stream));
+        SMX_HIP(hipMemcpyAsync(p->E, E_t, sizeof(double) * p->flen, hipMemcpyHostToDevice, p->stream));
+        SMX_HIP(hipStreamSynchronize(p->stream));
+        return SMX_OK;
+    }
+    return smx_f32_write_fields(&p->f32, H_t, E_t, p->stream);
+}
+
+int smx_plan_read_fields(const smx_plan *p, double *H_t, double *E_t)
+{
+    if (!p || !H_t || !E_t) return arg_fail("null pointer");
+    if (p->g.precision == SMX_F64) {
+        SMX_HIP(hipMemcpyAsync(H_t, p->H, sizeof(double) * p->flen, hipMemcpyDeviceToHost, p->stream));
+        SMX_HIP(hipMemcpyAsync(E_t, p->E, sizeof(double) * p->flen, hipMemcpyDeviceToHost, p->stream));
+        SMX_HIP(hipStreamSynchronize(p->stream));
+        return SMX_OK;
+    }
+    return smx_f32_read_fields(&p->f32, H_t, E_t, p->stream);
+}
+
+int smx_plan_read_monitors(const smx_plan *p, double *HB, double *EB, double *HT, double *ET)
+{
+    if (!p || !HB || !EB || !HT || !ET) return arg_fail("null pointer");
+    double *dst[4] = {HB, EB, HT, ET};
+    for (int i = 0; i < 4; i++)
+        SMX_HIP(hipMemcpyAsync(dst[i], p->mon + 4 * p->plane * i, sizeof(double) * 4 * p->plane,
+                               hipMemcpyDeviceToHost, p->stream));
+    SMX_HIP(hipStreamSynchronize(p->stream));
+    return SMX_OK;
+}
+
+int smx_plan_write_monitors(smx_plan *p, const double *HB, const double *EB, const double *HT,
+                            const double *ET)
+{
+    if (!p) return arg_fail("null pointer");
+    const double *src[4] = {HB, EB, HT, ET};
+    for (int i = 0; i < 4; i++) {
+        double *d = p->mon + 4 * p->plane * i;
+        if (src[i]) SMX_HIP(hipMemcpyAsync(d, src[i], sizeof(double) * 4 * p->plane, hipMemcpyHostToDevice, p->stream));
+        else SMX_HIP(hipMemsetAsync(d, 0, sizeof(double) * 4 * p->plane, p->stream));
+    }
+    SMX_HIP(hipStreamSynchronize(p->stream));
+    return SMX_OK;
+}
+
+int smx_plan_step(smx_plan *p, const double *Hi, const double *Ei, const double *ExpN2)
+{
+    if (!p || !Hi || !Ei || !ExpN2) return arg_fail("null pointer");
+    SMX_HIP(hipMemcpyAsync(p->src, Hi, sizeof(double) * 2 * p->plane, hipMemcpyHostToDevice, p->stream));
+    SMX_HIP(hipMemcpyAsync(p->src + 2 * p->plane, Ei, sizeof(double) * 2 * p->plane, hipMemcpyHostToDevice,
+                           p->stream));
+    smx_step_scalars s;
+    std::memset(&s, 0, sizeof s);
+    s.monH_re = ExpN2[0];
+    s.monH_im = ExpN2[1];
+    s.monE_re = ExpN2[2];
+    s.monE_im = ExpN2[3];
+    int rc = launch_step(p, s, true);
+    if (rc) return rc;
+    SMX_HIP(hipStreamSynchronize(p->stream));
+    return SMX_OK;
+}
+
+int smx_plan_set_source(smx_plan *p, const double *Hinc, const double *Einc, int N_2pi,
+                        const int time_period[3])
+{
+    if (!p || !Hinc || !Einc || !time_period) return arg_fail("null pointer");
+    if (N_2pi < 1) return arg_fail("N_2pi must be >= 1");
+    SMX_HIP(hipMemcpyAsync(p->inc, Hinc, sizeof(double) * 4 * p->plane, hipMemcpyHostToDevice, p->stream));
+    SMX_HIP(hipMemcpyAsync(p->inc + 4 * p->plane, Einc, sizeof(double) * 4 * p->plane, hipMemcpyHostToDevice,
+                           p->stream));
+    SMX_HIP(hipStreamSynchronize(p->stream));
+    p->N = N_2pi;
+    std::memcpy(p->tp, time_period, sizeof p->tp);
+    p->expn.assign(4 * (size_t)N_2pi, 0.);
+    smx_expn_table(N_2pi, p->expn.data());
+    p->have_source = true;
+    return SMX_OK;
+}
+
+int smx_plan_run_steps(smx_plan *p, long long step0, long long nsteps, double sums[2])
+{
+    if (!p) return arg_fail("null pointer");
+    if (!p->have_source) {
+        snprintf(g_last_error, sizeof g_last_error, "smx_plan_run before smx_plan_set_source");
+        return SMX_ERR_STATE;
+    }
+    if (step0 < 0 || nsteps < 0) return arg_fail("negative step");
+    for (long long k = step0; k < step0 + nsteps; k++) {
+        const int P = (int)(k / p->N), t = (int)(k % p->N);
+        smx_step_scalars s;
+        double fH, fE;
+        int rc = step_scalars(p, P, t, &s, &fH, &fE);
+        if (rc) return rc;
+        if (sums) {
+            sums[0] += fH;
+            sums[1] += fE;
+        }
+        if ((rc = launch_step(p, s, false))) return rc;
+    }
+    return SMX_OK;
+}
+
+int smx_plan_run(smx_plan *p, int P0, int P1, double sums[2])
+{
+    if (!p) return arg_fail("null pointer");
+    if (!p->have_source) {
+        snprintf(g_last_error, sizeof g_last_error, "smx_plan_run before smx_plan_set_source");
+        return SMX_ERR_STATE;
+    }
+    if (P1 < P0) return arg_fail("P1 < P0");
+    return smx_plan_run_steps(p, (long long)P0 * p->N, (long long)(P1 - P0) * p->N, sums);
+}
+
+void *smx_plan_stream(const smx_plan *p) { return p ? (void *)p->stream : nullptr; }
+
+int smx_plan_sync(smx_plan *p)
+{
+    if (!p) return arg_fail("null pointer");
+    SMX_HIP(hipStreamSynchronize(p->stream));
+    return SMX_OK;
+}
+
+int smx_main_calc(const int L[4], const double *Ah, const double *Bh, const double *Ae,
+                  const double *Be, const int iBc[3], double *H_t, double *E_t,
+                  const int iBTIin[4], const double *Hi, const double *Ei, const double *ExpN2,
+                  double *HB, double *EB, double *HT, double *ET, const char *Cont,
+                  const int *address_notn)
+{
+    if (!L || !iBc || !iBTIin || !Cont) return arg_fail("null pointer");
+    smx_grid g;
+    std::memset(&g, 0, sizeof g);
+    std::memcpy(g.L, L, sizeof g.L);
+    std::memcpy(g.iBc, iBc, sizeof g.iBc);
+    std::memcpy(g.iBTIin, iBTIin, sizeof g.iBTIin);
+    g.precision = SMX_F64;
+    const int64_t n = (int64_t)L[0] * L[1] * L[2];
+    for (int64_t l = 0; l < n; l++) g.Lnotn += Cont[l] != 'n';
+    smx_plan *p = nullptr;
+    int rc = smx_plan_create_coeffs(&g, Ah, Bh, Ae, Be, Cont, address_notn, &p);
+    if (rc) return rc;
+    if (!(rc = smx_plan_write_fields(p, H_t, E_t)) && !(rc = smx_plan_write_monitors(p, HB, EB, HT, ET)) &&
+        !(rc = smx_plan_step(p, Hi, Ei, ExpN2)) && !(rc = smx_plan_read_fields(p, H_t, E_t)))
+        rc = smx_plan_read_monitors(p, HB, EB, HT, ET);
+    free_plan(p);
+    return rc;
+}
+
+} // extern "C"

@@ -1,0 +1,146 @@
+"""Parity of the HIP library against the oracle on the GPU.
+
+FP64 (plain path): bit-identical to the oracle, which is itself bit-identical to the
+reference (tests/test_oracle_vs_ref.py).  FP32 (fused path): relative L2 against the
+FP64 oracle within the tolerances written in each test."""
+import numpy as np
+import pytest
+
+import oracle
+from smatranem_amd import casegen, fdtd
+from tests.cases import call_main_calc, random_state, rel_l2, small_waveguide
+
+pytestmark = pytest.mark.gpu
+
+
+def _eq(a, b):
+    return np.array_equal(np.asarray(a), np.asarray(b))
+
+
+@pytest.mark.parametrize("seed,L", [(0, (7, 6, 9)), (1, (65, 3, 4)), (2, (1, 5, 7)), (3, (6, 1, 5)),
+                                    (4, (4, 5, 1)), (5, (20, 17, 6)), (6, (3, 3, 2))])
+def test_main_calc_bitwise(seed, L):
+    st = random_state(seed, L=L)
+    Hg, Eg, mg, _ = call_main_calc(lambda *a: fdtd.main_calc(*a), st)
+    Ho, Eo, mo, rc = call_main_calc(oracle.load_oracle().orc_main_calc, st)
+    assert rc == 0
+    assert _eq(Hg, Ho) and _eq(Eg, Eo)
+    for a, b in zip(mg, mo):
+        assert _eq(a, b)
+
+
+def test_plan_steps_bitwise():
+    st = random_state(7, L=(9, 8, 10), iBc=(1, -1, 0))
+    g = fdtd.Grid(L=st["L"], iBc=st["iBc"], iBTIin=st["iBTI"], precision=fdtd.F64, Lnotn=st["Lnotn"])
+    with fdtd.Plan(g, coeffs=(st["Ah"], st["Bh"], st["Ae"], st["Be"]), Cont=st["Cont"],
+                   address_notn=st["addr"]) as p:
+        p.write_fields(st["H"], st["E"])
+        p.write_monitors(*[m.view(np.complex128) for m in st["mons"]])
+        Ho, Eo, mo = st["H"].copy(), st["E"].copy(), [m.copy() for m in st["mons"]]
+        for _ in range(6):
+            p.step(st["Hi"], st["Ei"], st["ExpN2"])
+            call_main_calc(oracle.load_oracle().orc_main_calc, st, Ho, Eo, mo)
+        H, E = p.read_fields()
+        assert _eq(H, Ho) and _eq(E, Eo)
+        for a, b in zip(p.read_monitors(), mo):
+            assert _eq(a.view(np.float64), b)
+
+
+def _oracle_run(prep, P0, P1, fields=None, monitors=None):
+    c = prep.case
+    N, iR, iI, _ = oracle.set_tau(c.omega, c.min_dt / c.dt_factor)
+    ref = oracle.setup_from_media(c.L, c.mu, c.eps, iR=iR, iI=iI)
+    return oracle.run(c.L, ref, c.iBc, (c.iB, c.iT, c.iI, c.inc_mode), c.Hinc, c.Einc, N,
+                      c.time_period, P0, P1, fields=fields, monitors=monitors)
+
+
+CASES = {
+    # z PML only, periodic/antiperiodic/PEC walls: the reference monitors are on
+    "ports": dict(pml_xy=0, pml_z=3, iBc=(1, -1, 0), metal_cells=6, lossy_cells=4),
+    # PML on every face: the reference monitors are off (plane -1)
+    "box": dict(pml_xy=2, pml_z=3, iBc=(0, 0, 0), metal_cells=6, lossy_cells=4),
+    "curvilinear": dict(pml_xy=0, pml_z=3, iBc=(1, 1, 0), scale_amp=0.3, pec_fraction=0.03,
+                        metal_cells=0, lossy_cells=0),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_plan_run_f64_bitwise(name):
+    prep = casegen.prepare(small_waveguide(seed=3, **CASES[name]))
+    with prep.plan(fdtd.F64) as p:
+        sums = p.run(0, 3)
+        sums = p.run(3, prep.case.time_period[2], sums)
+        H, E = p.read_fields()
+        mons = p.read_monitors()
+    Ho, Eo, mo, so = _oracle_run(prep, 0, prep.case.time_period[2])
+    assert np.abs(Ho).max() > 0
+    assert _eq(H, Ho) and _eq(E, Eo)
+    for a, b in zip(mons, mo):
+        assert _eq(a.view(np.float64), b)
+    assert _eq(sums, so)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_plan_run_f32_close(name):
+    """FP32 fields vs the FP64 oracle: relative L2 <= 2e-4 on fields, monitors."""
+    prep = casegen.prepare(small_waveguide(seed=3, **CASES[name]))
+    with prep.plan(fdtd.F32) as p:
+        p.run(0, prep.case.time_period[2])
+        H, E = p.read_fields()
+        mons = p.read_monitors()
+    Ho, Eo, mo, _ = _oracle_run(prep, 0, prep.case.time_period[2])
+    assert rel_l2(H, Ho) < 2e-4, rel_l2(H, Ho)
+    assert rel_l2(E, Eo) < 2e-4, rel_l2(E, Eo)
+    for a, b in zip(mons, mo):
+        if np.linalg.norm(b) > 0:
+            assert rel_l2(a.view(np.float64), b) < 2e-4
+
+
+def test_f32_round_trip_fields():
+    """write_fields/read_fields of the FP32 plan: total/split packing is lossless up to
+    FP32 rounding of the main and split slots."""
+    prep = casegen.prepare(small_waveguide(seed=5, **CASES["box"]))
+    rng = np.random.default_rng(0)
+    with prep.plan(fdtd.F32) as p:
+        H = rng.normal(size=p.flen)
+        E = rng.normal(size=p.flen)
+        H[-3:] = 0
+        E[-3:] = 0
+        p.write_fields(H, E)
+        H2, E2 = p.read_fields()
+    assert rel_l2(H2, H) < 1e-6 and rel_l2(E2, E) < 1e-6
+
+
+def test_monitor_plane_in_pml_is_refused():
+    prep = casegen.prepare(small_waveguide(seed=1, **CASES["box"]))
+    g = prep.grid(fdtd.F64)
+    g.iBTIin = (5, 20, 6, 1)
+    with pytest.raises(fdtd.SmxError) as e:
+        fdtd.Plan(g, mu=prep.mu, eps=prep.eps, sigma=prep.sigma, Cont=prep.Cont,
+                  address_notn=prep.addr)
+    assert e.value.status == 100
+
+
+def test_run_errors():
+    prep = casegen.prepare(small_waveguide(seed=1, **CASES["ports"]))
+    g = prep.grid(fdtd.F64)
+    with fdtd.Plan(g, mu=prep.mu, eps=prep.eps, sigma=prep.sigma, Cont=prep.Cont,
+                   address_notn=prep.addr) as p:
+        with pytest.raises(fdtd.SmxError) as e:
+            p.run(0, 1)
+        assert e.value.status == 5
+        p.set_source(prep.case.Hinc, prep.case.Einc, prep.N, prep.case.time_period)
+        with pytest.raises(fdtd.SmxError) as e:
+            p.run(prep.case.time_period[2] - 1, prep.case.time_period[2] + 1)
+        assert e.value.status == 101
+
+
+def test_zero_source_keeps_fields_zero():
+    prep = casegen.prepare(small_waveguide(seed=2, **CASES["ports"]))
+    for prec in (fdtd.F64, fdtd.F32):
+        with prep.plan(prec) as p:
+            z = np.zeros(2 * prep.case.plane, dtype=np.complex128)
+            p.set_source(z, z, prep.N, prep.case.time_period)
+            p.run(0, 3)
+            H, E = p.read_fields()
+        assert not H.any() and not E.any()
