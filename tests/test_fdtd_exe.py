@@ -1,0 +1,151 @@
+"""The drop-in FDTD executable (smatranem_amd/FDTD) end to end.
+
+GPU: on a file set written in the reference's formats, its stderr report equals, line
+for line (time banner aside), the report the reference's main (F:32-187) prints,
+composed from the oracle: setup, the time loop, and the S_calc projection (F:419-448)
+-- the |S|^2 values printed with %.20e must match to the last digit.
+CPU: the reference's fatal conditions exit with status 1 and its messages."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from smatranem_amd import casegen
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "smatranem_amd", "FDTD")
+
+
+def _case(seed=4, **kw):
+    args = dict(L0=6, L1=5, L2=24, outer=8, pml_xy=0, pml_z=3, omega=2 * np.pi / 12.0, core=(2, 2),
+                seed=seed, time_period=(2, 1, 5), iBc=(1, -1, 0), metal_cells=3, lossy_cells=2)
+    args.update(kw)
+    return casegen.waveguide_case(**args)
+
+
+def _run(inp, env=None):
+    e = dict(os.environ)
+    e.update(env or {})
+    p = subprocess.run([EXE, inp], capture_output=True, text=True, env=e, timeout=300)
+    return p.returncode, p.stderr.splitlines()[2:]  # drop the time banner and blank line
+
+
+def _expected(case, Xi, n_real):
+    """The reference's report for this file set, from the oracle."""
+    L = case.L
+    N, iR, iI, Dt = oracle.set_tau(case.omega, case.min_dt / case.dt_factor)
+    st = oracle.setup_from_media(L, case.mu, case.eps, iR=iR, iI=iI)
+    iBTI = (case.iB, case.iT, case.iI, case.inc_mode)
+    Hinc = np.zeros(2 * case.plane, np.complex128)
+    Einc = np.zeros(2 * case.plane, np.complex128)
+    P = case.plane
+    NA = 2 * P
+    colnum = case.inc_mode - 1 if case.inc_mode > 0 else -case.inc_mode - 1 + NA
+    col = Xi[:, colnum]
+    Hinc[0::2], Hinc[1::2] = col[:P], col[P:2 * P]
+    Einc[1::2], Einc[0::2] = -col[2 * P:3 * P], col[3 * P:4 * P]
+    H, E, mons, sums = oracle.run(L, st, case.iBc, iBTI, Hinc, Einc, N, case.time_period)
+    tp = case.time_period
+    lines = [f"wavelength = {1.3e-6:.5e}, k_0 = {2 * np.pi / (1.3e-6 * case.meta['omega0']):.5e}",
+             f"Ramp-period = {tp[0]}, cw-period = {tp[1]}, stop-period = {tp[2]}",
+             f"Dt_factor = {case.dt_factor:.5e}, split field impl. does not use stable_factor."]
+    sig = st["sigma"]
+    off = (0, 2 * L[0], 2 * L[0] + 2 * L[1])
+    for d in range(3):
+        for j in range(L[d]):
+            for h in range(2):
+                v = sig[off[d] + h + 2 * j]
+                if v != 0.0:
+                    lines.append(f"l{d} = {j}.{5 if h else 0}, sigma = {v:.5e}")
+    pn = casegen.plane_name
+    lines.append(f"Start = {case.iB} for {pn(L, case.iB)}, Incident point = {case.iI} for {pn(L, case.iI)}, "
+                 f"Incident mode = {case.inc_mode:+d}, Target_T = {case.iT} for {pn(L, case.iT)}")
+    lines.append(f"omega = {case.omega:.5e}, min_Dt = {case.min_dt / case.dt_factor:.5e}, N_2pi = {N}, "
+                 f"Dt = {Dt:.5e}, tau_R = {1 / iR:.5e}, tau_I = {1 / iI:.5e}")
+    lines.append(f"L3d = {case.n}, Lnotn = {st['Lnotn']}")
+    lines.append(f"sum00 = {sums[0]:.5e}, sum05 = {sums[1]:.5e}")
+    lines.append(f"Bc[0] = {case.iBc[0]}, Bc[1] = {case.iBc[1]}, Bc[2] = {case.iBc[2]}")
+    from tests.test_oracle_vs_ref import tilxi_columns
+    cols = list(range(n_real)) + [NA + m for m in range(n_real)]
+    tcols = np.ascontiguousarray(tilxi_columns(Xi, n_real, cols)).view(np.float64).reshape(-1)
+    for title, (h, e) in (("Bottom", (mons[0], mons[1])), ("Top", (mons[2], mons[3]))):
+        S = np.zeros(2 * len(cols))
+        oracle.load_oracle().orc_s_project(oracle.i32(L), len(cols), tcols, h, e, S)
+        S = S.view(np.complex128)
+        lines.append(f"{title} |S_(m,{case.inc_mode:+d})|^2 :")
+        for m in range(n_real):
+            a, b = S[m], S[n_real + m]
+            lines.append(f"{m + 1:+d}, {a.real * a.real + a.imag * a.imag:.20e}, {-(m + 1):+d}, "
+                         f"{b.real * b.real + b.imag * b.imag:.20e}")
+    return lines
+
+
+def _write(case, tmp_path, n_real=3):
+    case.meta["omega0"] = case.omega
+    inp, Xi = casegen.write_case_files(case, str(tmp_path / "Data"), n_real=n_real)
+    return inp, Xi
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("inc", [1, -2])
+def test_fdtd_report_matches_reference(tmp_path, inc):
+    case = _case(inc_mode=1)
+    case.inc_mode = inc
+    inp, Xi = _write(case, tmp_path)
+    if inc < 0:  # the E half of the incident column must be real (F:503)
+        assert not Xi[2 * case.plane * 1:, -inc - 1 + 2 * case.plane].imag.any()
+    rc, got = _run(inp)
+    assert rc == 0, "\n".join(got)
+    exp = _expected(case, Xi, 3)
+    assert got == exp
+
+
+@pytest.mark.gpu
+def test_fdtd_report_f32_close(tmp_path):
+    case = _case()
+    inp, Xi = _write(case, tmp_path)
+    rc, got = _run(inp, {"SMX_FDTD_PRECISION": "f32"})
+    assert rc == 0, "\n".join(got)
+    exp = _expected(case, Xi, 3)
+    s_got = [line for line in got if line.startswith(("+", "-"))]
+    s_exp = [line for line in exp if line.startswith(("+", "-"))]
+    assert len(s_got) == len(s_exp) > 0
+    for a, b in zip(s_got, s_exp):
+        va = [float(v) for v in a.split(",")[1::2]]
+        vb = [float(v) for v in b.split(",")[1::2]]
+        for x, y in zip(va, vb):
+            assert abs(np.sqrt(x) - np.sqrt(y)) <= 1e-3 * max(1.0, np.sqrt(max(vb)))
+
+
+def test_exit_paths(tmp_path):
+    """Fatal conditions that occur before any GPU work: status 1 and the reference's text."""
+    rc, out = _run("/nonexistent/input.txt")
+    assert rc == 1 and out[-1] == "open error!: open input-file0!"
+    case = _case()
+    inp, Xi = _write(case, tmp_path)
+    text = open(inp).read()
+    # monitor order: Target_B >= Target_T (F:83)
+    bad = text.replace(f"1st = {casegen.plane_name(case.L, case.iB)}", "1st = T5")
+    p2 = str(tmp_path / "bad1.txt")
+    open(p2, "w").write(bad)
+    rc, out = _run(p2)
+    assert rc == 1 and out[-1].startswith("Target_B = ")
+    # incident mode beyond the propagating count (F:499)
+    bad = text.replace("incident-mode = +1", "incident-mode = +7")
+    p3 = str(tmp_path / "bad2.txt")
+    open(p3, "w").write(bad)
+    rc, out = _run(p3)
+    assert rc == 1 and out[-1] == "inci_mode = 7 > 3"
+    # missing key line
+    p4 = str(tmp_path / "bad3.txt")
+    open(p4, "w").write("\n".join(line for line in text.splitlines() if "FDTD:" not in line))
+    rc, out = _run(p4)
+    assert rc == 1 and out[-1].startswith("4 control commands cannot be read")
+    # omega mismatch between the Med0 and bMode files (F:751)
+    mode = str(tmp_path / "Data" / "joint_bMode.dat")
+    s = open(mode).read().replace("# Omega              : k/k_0 = ", "# Omega              : k/k_0 = 1")
+    open(mode, "w").write(s)
+    rc, out = _run(inp)
+    assert rc == 1 and out[-1].startswith("Omega = ") and out[-1].endswith("in input_Lf!")
