@@ -135,6 +135,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 leg")
     ap.add_argument("--no-box", action="store_true", help="skip the all-faces-PML variant")
     ap.add_argument("--small", action="store_true", help="32x32x48 grid (quick check)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run 5 device copies of MIB MiB (scale for rocprofv3 byte counters)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -148,6 +150,16 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
+    if args.calibrate_copy:
+        n = args.calibrate_copy * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes_read": 4 * n, "calibrate_copy_bytes_written": 4 * n,
+                          "copies": 5}), flush=True)
+        return
     K, W = args.steps, args.warmup
 
     t_setup = time.perf_counter()

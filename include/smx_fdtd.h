@@ -146,6 +146,21 @@ int smx_plan_run_steps(smx_plan *plan, long long step0, long long nsteps, double
 /* Wait for all queued work of the plan. */
 int smx_plan_sync(smx_plan *plan);
 
+/*
+ * Extension: running DFT of the tangential fields on up to 8 z-planes at up to 32
+ * frequencies (the reference's monitors, F:318-328, sample one frequency on two
+ * planes).  w_per_step[f] = omega_f*Dt.  Run step k (k counted from 0 by
+ * smx_plan_run_steps, or by successive smx_plan_step calls) adds
+ * e^{i*w*(k+1)}*(Hx, Hy) and e^{i*w*(k+1.5)}*(Ex, Ey) of the main slots (the total
+ * field in PML cells); with w = 2*pi/N_2pi this is the reference's monitor.  Samples
+ * are buffered and folded into FP64 accumulators every `chunk` (1..32) steps.
+ * nplanes = 0 switches the DFT off.  No counterpart in the reference.
+ */
+int smx_plan_set_dft(smx_plan *plan, int nplanes, const int *planes, int nfreq, const double *w_per_step,
+                     int chunk);
+/* Accumulated DFT, complex[nplanes][4 (Hx, Hy, Ex, Ey)][nfreq][L0*L1]. */
+int smx_plan_read_dft(smx_plan *plan, double *out);
+
 /* The plan's HIP stream (a hipStream_t), for callers that order or time their own
  * work against the plan's kernels.  Valid until smx_plan_destroy. */
 void *smx_plan_stream(const smx_plan *plan);

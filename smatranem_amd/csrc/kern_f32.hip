@@ -20,6 +20,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <vector>
@@ -388,14 +389,472 @@ f32_args base_args(const smx_f32_state *st, const smx_geom &g)
     a.iI = g.iI;
     a.sgn = (float)g.sgn;
     for (int i = 0; i < 6; i++) {
-        a.T[i] = st->T[i];
-        a.S[i] = st->S[i];
+        a.T[i] = st->T[st->cur][i];
+        a.S[i] = st->S[st->cur][i];
         a.b[i] = st->b[i];
     }
     a.pml = st->pml;
     a.pmlflag = st->pmlflag;
     a.special = st->special;
     return a;
+}
+
+/* ---------------------------------------------------------------------------------- */
+/* Fused step: H and E of one step in a single pass over HBM                          */
+/* ---------------------------------------------------------------------------------- */
+/*
+ * A block owns an FX x FY tile of (j0, j1) and marches an FZ-plane chunk of j2.  At
+ * plane k it holds in LDS the old E of plane k+1 on the tile grown by one cell on
+ * every side (for the backward curl of H) and the new H of plane k on the tile grown
+ * by one cell on the +x/+y sides (for the forward curl of E); the z neighbours travel
+ * in registers.  Each iteration computes H(k+1) on the grown tile, then E(k) on the
+ * tile, so every field word is read and written once per step; halo H cells are
+ * recomputed, not exchanged.  Reads come from buffer cur, writes go to 1 - cur.
+ * Lossy/metal cells run their class branch in two small passes around this kernel:
+ * special_h_pass before it (this kernel then reads their new H) and special_e_pass
+ * after it (the metal E update takes the backward curl of the new H, F:310, which
+ * reaches outside the tile).
+ */
+constexpr int FX = 32, FY = 8, FZ = 16;
+constexpr int EXW = FX + 2, EYW = FY + 2;   /* E stage: x0-1 .. x0+FX, y0-1 .. y0+FY */
+constexpr int HXW = FX + 1, HYW = FY + 1;   /* H plane: x0 .. x0+FX, y0 .. y0+FY */
+constexpr int NHALO = HXW * HYW - FX * FY;  /* 41 grown-tile H cells outside the tile */
+constexpr int NESLOT = EXW * EYW;           /* 340 */
+static_assert(NHALO <= kThreads && NESLOT <= 2 * kThreads && FX * FY == kThreads, "tile shape");
+
+struct fused_args {
+    int L0, L1, L2;
+    int gx, gy, nblocks;
+    int64_t sz;
+    float w[3];
+    int iB, iT, iI;
+    float sgn;
+    const float *To[6];
+    float *Tn[6];
+    const float *So[6];
+    float *Sn[6];
+    const float *b[6];
+    const float *pml;
+    const unsigned char *pmlflag;
+    const smx_special_f32 *special;
+    const int64_t *special_cells;
+    int nspecial;
+    const double *srcE_plane, *srcH_plane, *Einc, *Hinc;
+    double srcE_f2, srcE_re, srcE_im, srcH_f2, srcH_re, srcH_im;
+    double *HB, *EB, *HT, *ET;
+    double monH_re, monH_im, monE_re, monE_im;
+};
+
+__device__ inline int wrap_index(int v, int L)
+{
+    const int r = v % L;
+    return r < 0 ? r + L : r;
+}
+
+__device__ inline double plane_source(const double *plane, const double *inc, double f2, double re, double im,
+                                      int64_t q, int v)
+{
+    if (plane) return plane[v + 2 * q];
+    const double *z = inc + 2 * (v + 2 * q);
+    return f2 * (re * z[0] + im * z[1]);
+}
+
+__device__ inline int axis_len(const fused_args &a, int d) { return d == 0 ? a.L0 : d == 1 ? a.L1 : a.L2; }
+
+__device__ inline const float *axis_tab(const fused_args &a, int d)
+{
+    return d == 0 ? a.pml : d == 1 ? a.pml + 4 * a.L0 : a.pml + 4 * (a.L0 + a.L1);
+}
+
+/* Neighbour offsets/weights of cell (x, y, z): backward (fwd = false) or forward. */
+__device__ inline nb3 nbrs_at(const fused_args &a, int x, int y, int z, bool fwd)
+{
+    const int64_t sy = a.L0;
+    nb3 n;
+    if (fwd) {
+        n.o[0] = x + 1 == a.L0 ? -(int64_t)(a.L0 - 1) : 1;
+        n.o[1] = y + 1 == a.L1 ? -(int64_t)(a.L1 - 1) * sy : sy;
+        n.o[2] = z + 1 == a.L2 ? -(int64_t)(a.L2 - 1) * a.sz : a.sz;
+        n.w[0] = x + 1 == a.L0 ? a.w[0] : 1.f;
+        n.w[1] = y + 1 == a.L1 ? a.w[1] : 1.f;
+        n.w[2] = z + 1 == a.L2 ? a.w[2] : 1.f;
+    } else {
+        n.o[0] = x == 0 ? (int64_t)(a.L0 - 1) : -1;
+        n.o[1] = y == 0 ? (int64_t)(a.L1 - 1) * sy : -sy;
+        n.o[2] = z == 0 ? (int64_t)(a.L2 - 1) * a.sz : -a.sz;
+        n.w[0] = x == 0 ? a.w[0] : 1.f;
+        n.w[1] = y == 0 ? a.w[1] : 1.f;
+        n.w[2] = z == 0 ? a.w[2] : 1.f;
+    }
+    return n;
+}
+
+/* H of a lossy/metal cell from the old buffers (the class branches of F:259-283). */
+__device__ void fused_special_h(const fused_args &a, int x, int y, int z, int64_t l, const smx_special_f32 &sp,
+                                float Hn[3], float Sn[3])
+{
+    float *const *E = const_cast<float *const *>(a.To + 3);
+    float *const *Es = const_cast<float *const *>(a.So + 3);
+    const int k = sp.cls;
+    for (int c = 0; c < 3; c++) {
+        Hn[c] = a.To[c][l];
+        Sn[c] = a.So[c][l];
+    }
+    if (k == 'e') return; /* frozen */
+    const nb3 n = nbrs_at(a, x, y, z, false);
+    float P[3], M[3];
+    if (k == 'n' || k == 'a') {
+        curl_halves<false>(E, nullptr, n, l, P, M);
+        if (k == 'n') {
+            float W[3] = {P[0] + M[0], P[1] + M[1], P[2] + M[2]};
+            if (z == a.iI) {
+                const int64_t q = x + (int64_t)a.L0 * y;
+                W[0] += a.sgn * (float)plane_source(a.srcE_plane, a.Einc, a.srcE_f2, a.srcE_re, a.srcE_im, q, 1);
+                W[1] -= a.sgn * (float)plane_source(a.srcE_plane, a.Einc, a.srcE_f2, a.srcE_re, a.srcE_im, q, 0);
+            }
+            for (int c = 0; c < 3; c++) Hn[c] = sp.hA[c] * Hn[c] - sp.hB[c] * W[c];
+        } else {
+            for (int c = 0; c < 3; c++) {
+                const float s = Sn[c], m = Hn[c] - s;
+                const float s2 = sp.hAs[c] * s - sp.hBs[c] * M[c];
+                Hn[c] = sp.hA[c] * m - sp.hB[c] * P[c] + s2;
+                Sn[c] = s2;
+            }
+        }
+        return;
+    }
+    curl_halves<false>(E, Es, n, l, P, M); /* 'h', 'b': curl of the E main slots */
+    for (int c = 0; c < 3; c++) {
+        const float s = Sn[c];
+        float m = Hn[c] - s + 0.5f * s;
+        const float s2 = sp.hAs[c] * s - sp.hBs[c] * (P[c] + M[c]);
+        m += 0.5f * s2;
+        Hn[c] = m + s2;
+        Sn[c] = s2;
+    }
+}
+
+/* Kinds of a cell in the fused step. */
+enum { K_NORMAL = 0, K_PML = 1, K_SPECIAL = 2 };
+
+/* New H at (x, y, z) from the old E at the cell (ec) and at its -x, -y, -z neighbours. */
+__device__ inline int fused_h(const fused_args &a, int x, int y, int z, int64_t l, const float ec[3],
+                              const float exm[3], const float eym[3], const float ezm[3], float Hn[3], float Sn[3])
+{
+    const float b0 = a.b[0][l];
+    if (__builtin_isnan(b0)) { /* computed by special_h_pass before this kernel */
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            Hn[c] = a.Tn[c][l];
+            Sn[c] = a.Sn[c][l];
+        }
+        return K_SPECIAL;
+    }
+    const float bc[3] = {b0, a.b[1][l], a.b[2][l]};
+    const float wx = x == 0 ? a.w[0] : 1.f, wy = y == 0 ? a.w[1] : 1.f, wz = z == 0 ? a.w[2] : 1.f;
+    float P[3], M[3];
+    P[0] = ec[2] - wy * eym[2];
+    M[0] = -ec[1] + wz * ezm[1];
+    P[1] = ec[0] - wz * ezm[0];
+    M[1] = -ec[2] + wx * exm[2];
+    P[2] = ec[1] - wx * exm[1];
+    M[2] = -ec[0] + wy * eym[0];
+    const bool is_pml = a.pmlflag[x] | a.pmlflag[a.L0 + y] | a.pmlflag[a.L0 + a.L1 + z];
+    if (!is_pml) {
+        if (z == a.iI) {
+            const int64_t q = x + (int64_t)a.L0 * y;
+            P[0] += a.sgn * (float)plane_source(a.srcE_plane, a.Einc, a.srcE_f2, a.srcE_re, a.srcE_im, q, 1);
+            P[1] -= a.sgn * (float)plane_source(a.srcE_plane, a.Einc, a.srcE_f2, a.srcE_re, a.srcE_im, q, 0);
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) Hn[c] = a.To[c][l] - bc[c] * (P[c] + M[c]);
+        return K_NORMAL;
+    }
+    const int jj[3] = {x, y, z};
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const int dm = c == 2 ? 0 : c + 1, ds = c == 0 ? 2 : c - 1;
+        const float *tm = axis_tab(a, dm), *ts = axis_tab(a, ds);
+        const int Lm = axis_len(a, dm), Ls = axis_len(a, ds);
+        const float aM = tm[jj[dm]], gM = tm[Lm + jj[dm]];
+        const float aS = ts[jj[ds]], gS = ts[Ls + jj[ds]];
+        const float s = a.So[c][l], m = a.To[c][l] - s;
+        const float s2 = aS * s - bc[c] * gS * M[c];
+        Hn[c] = aM * m - bc[c] * gM * P[c] + s2;
+        Sn[c] = s2;
+    }
+    return K_PML;
+}
+
+__device__ inline void store3(float *const *dst, int64_t l, const float v[3])
+{
+    dst[0][l] = v[0];
+    dst[1][l] = v[1];
+    dst[2][l] = v[2];
+}
+
+__global__ __launch_bounds__(kThreads) void fused_step_f32(fused_args a)
+{
+    __shared__ float sE[3][EYW][EXW];
+    __shared__ float sH[3][HYW][HXW];
+
+    const int id = xcd_logical(blockIdx.x, a.nblocks);
+    const int bxi = id % a.gx, rest = id / a.gx;
+    const int byi = rest % a.gy, bzi = rest / a.gy;
+    const int x0 = bxi * FX, y0 = byi * FY, z0 = bzi * FZ;
+    const int z1 = min(z0 + FZ, a.L2);
+    const int t = threadIdx.x;
+
+    /* tile cell of this thread (slot A) and, for t < NHALO, one grown-tile cell (slot B) */
+    const int ai = t % FX, aj = t / FX;
+    const bool owned = x0 + ai < a.L0 && y0 + aj < a.L1;
+    const int ax = wrap_index(x0 + ai, a.L0), ay = wrap_index(y0 + aj, a.L1);
+    const int64_t aq = ax + (int64_t)a.L0 * ay;
+    const bool hasB = t < NHALO;
+    const int bi = t < HXW ? t : FX, bj = t < HXW ? FY : t - HXW;
+    const int bx = wrap_index(x0 + bi, a.L0), by = wrap_index(y0 + bj, a.L1);
+    const int64_t bq = bx + (int64_t)a.L0 * by;
+
+    /* E stage slots loaded by this thread: t and t + 256 (< 340) */
+    int64_t eq[2];
+    int es[2];
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+        const int s = t + r * kThreads;
+        es[r] = s < NESLOT ? s : -1;
+        const int ex = s % EXW, ey = s / EXW;
+        eq[r] = wrap_index(x0 - 1 + ex, a.L0) + (int64_t)a.L0 * wrap_index(y0 - 1 + ey, a.L1);
+    }
+    auto stage_E = [&](int plane) {
+        const int64_t off = (int64_t)plane * a.sz;
+#pragma unroll
+        for (int r = 0; r < 2; r++) {
+            if (es[r] < 0) continue;
+            const int ex = es[r] % EXW, ey = es[r] / EXW;
+#pragma unroll
+            for (int c = 0; c < 3; c++) sE[c][ey][ex] = a.To[3 + c][off + eq[r]];
+        }
+    };
+    auto evalH = [&](int i, int j, int x, int y, int z, int64_t l, const float ezm[3], float Hn[3], float Sn[3],
+                     float ecenter[3]) {
+        float exm[3], eym[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            ecenter[c] = sE[c][j + 1][i + 1];
+            exm[c] = sE[c][j + 1][i];
+            eym[c] = sE[c][j][i + 1];
+        }
+        return fused_h(a, x, y, z, l, ecenter, exm, eym, ezm, Hn, Sn);
+    };
+
+    /* prologue: H(z0) on the grown tile */
+    float eA[3], eB[3]; /* old E of the previous plane at slots A, B (z neighbours) */
+    float hA[3], hB[3], sA[3], sB[3];
+    {
+        stage_E(z0);
+        const int zm = z0 == 0 ? a.L2 - 1 : z0 - 1;
+        const int64_t offm = (int64_t)zm * a.sz;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            eA[c] = a.To[3 + c][offm + aq];
+            eB[c] = hasB ? a.To[3 + c][offm + bq] : 0.f;
+        }
+        __syncthreads();
+        const int64_t off = (int64_t)z0 * a.sz;
+        float ec[3];
+        const int kind = evalH(ai, aj, ax, ay, z0, off + aq, eA, hA, sA, ec);
+        if (owned && kind != K_SPECIAL) {
+            store3(a.Tn, off + aq, hA);
+            if (kind == K_PML) store3(a.Sn, off + aq, sA);
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) eA[c] = ec[c];
+        if (hasB) {
+            evalH(bi, bj, bx, by, z0, off + bq, eB, hB, sB, ec);
+#pragma unroll
+            for (int c = 0; c < 3; c++) eB[c] = ec[c];
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            sH[c][aj][ai] = hA[c];
+            if (hasB) sH[c][bj][bi] = hB[c];
+        }
+        __syncthreads();
+    }
+
+    for (int k = z0; k < z1; k++) {
+        const int p = k + 1 == a.L2 ? 0 : k + 1;
+        const bool p_owned = k + 1 < z1;
+        const int64_t offk = (int64_t)k * a.sz, offp = (int64_t)p * a.sz;
+        stage_E(p);
+        __syncthreads();
+        /* H(k+1) on the grown tile; eA/eB hold old E(k), the -z neighbours */
+        float hnA[3], hnB[3], snA[3], snB[3], ecA[3], ecB[3];
+        const int kindA = evalH(ai, aj, ax, ay, p, offp + aq, eA, hnA, snA, ecA);
+        if (owned && p_owned && kindA != K_SPECIAL) {
+            store3(a.Tn, offp + aq, hnA);
+            if (kindA == K_PML) store3(a.Sn, offp + aq, snA);
+        }
+        if (hasB) evalH(bi, bj, bx, by, p, offp + bq, eB, hnB, snB, ecB);
+
+        /* E(k) on the tile cell: forward curl of the new H */
+        const int64_t l = offk + aq;
+        const float be0 = a.b[3][l];
+        if (owned && !__builtin_isnan(be0)) {
+            const float bc[3] = {be0, a.b[4][l], a.b[5][l]};
+            float hc[3], hxp[3], hyp[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                hc[c] = sH[c][aj][ai];
+                hxp[c] = sH[c][aj][ai + 1];
+                hyp[c] = sH[c][aj + 1][ai];
+            }
+            const float wx = ax + 1 == a.L0 ? a.w[0] : 1.f, wy = ay + 1 == a.L1 ? a.w[1] : 1.f;
+            const float wz = k + 1 == a.L2 ? a.w[2] : 1.f;
+            float P[3], M[3];
+            P[0] = wy * hyp[2] - hc[2];
+            M[0] = -wz * hnA[1] + hc[1];
+            P[1] = wz * hnA[0] - hc[0];
+            M[1] = -wx * hxp[2] + hc[2];
+            P[2] = wx * hxp[1] - hc[1];
+            M[2] = -wy * hyp[0] + hc[0];
+            float En[3];
+            const bool is_pml = a.pmlflag[ax] | a.pmlflag[a.L0 + ay] | a.pmlflag[a.L0 + a.L1 + k];
+            if (!is_pml) {
+                if (k == a.iI) {
+                    P[0] += a.sgn * (float)plane_source(a.srcH_plane, a.Hinc, a.srcH_f2, a.srcH_re, a.srcH_im, aq, 1);
+                    P[1] -= a.sgn * (float)plane_source(a.srcH_plane, a.Hinc, a.srcH_f2, a.srcH_re, a.srcH_im, aq, 0);
+                }
+#pragma unroll
+                for (int c = 0; c < 3; c++) En[c] = eA[c] + bc[c] * (P[c] + M[c]);
+                store3(a.Tn + 3, l, En);
+                for (int m = 0; m < 2; m++) { /* monitors: main = total here (S = 0) */
+                    if (k != (m == 0 ? a.iB : a.iT)) continue;
+                    double *HM = m == 0 ? a.HB : a.HT, *EM = m == 0 ? a.EB : a.ET;
+                    for (int v = 0; v < 2; v++) {
+                        const int64_t o = 2 * (v + 2 * aq);
+                        HM[o] += a.monH_re * (double)hc[v];
+                        HM[o + 1] += a.monH_im * (double)hc[v];
+                        EM[o] += a.monE_re * (double)En[v];
+                        EM[o + 1] += a.monE_im * (double)En[v];
+                    }
+                }
+            } else {
+                const int jj[3] = {ax, ay, k};
+                float Sn[3];
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const int dm = c == 2 ? 0 : c + 1, ds = c == 0 ? 2 : c - 1;
+                    const float *tm = axis_tab(a, dm), *ts = axis_tab(a, ds);
+                    const int Lm = axis_len(a, dm), Ls = axis_len(a, ds);
+                    const float aM = tm[2 * Lm + jj[dm]], gM = tm[3 * Lm + jj[dm]];
+                    const float aS = ts[2 * Ls + jj[ds]], gS = ts[3 * Ls + jj[ds]];
+                    const float s = a.So[3 + c][l], m = eA[c] - s;
+                    const float s2 = aS * s + bc[c] * gS * M[c];
+                    En[c] = aM * m + bc[c] * gM * P[c] + s2;
+                    Sn[c] = s2;
+                }
+                store3(a.Tn + 3, l, En);
+                store3(a.Sn + 3, l, Sn);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            eA[c] = ecA[c];
+            if (hasB) eB[c] = ecB[c];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            sH[c][aj][ai] = hnA[c];
+            if (hasB) sH[c][bj][bi] = hnB[c];
+        }
+    }
+}
+
+/* H of the lossy/metal cells, before the fused pass (which reads the result for its
+ * E update and leaves these cells' H alone). */
+__global__ __launch_bounds__(kThreads) void special_h_pass(fused_args a)
+{
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= a.nspecial) return;
+    const int64_t l = a.special_cells[i];
+    const int x = (int)(l % a.L0), y = (int)((l / a.L0) % a.L1), z = (int)(l / a.sz);
+    float Hn[3], Sn[3];
+    fused_special_h(a, x, y, z, l, a.special[i], Hn, Sn);
+    for (int c = 0; c < 3; c++) {
+        a.Tn[c][l] = Hn[c];
+        a.Sn[c][l] = Sn[c];
+    }
+}
+
+/* E of the lossy/metal cells after the fused pass (the class branches of F:291-314),
+ * from the new H in buffer 1 - cur; also their monitor samples. */
+__global__ __launch_bounds__(kThreads) void special_e_pass(fused_args a)
+{
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= a.nspecial) return;
+    const int64_t l = a.special_cells[i];
+    const smx_special_f32 &sp = a.special[i];
+    const int x = (int)(l % a.L0), y = (int)((l / a.L0) % a.L1), z = (int)(l / a.sz);
+    float *const *H = a.Tn;
+    float *const *Hs = a.Sn;
+    float En[3], Sn[3];
+    for (int c = 0; c < 3; c++) {
+        En[c] = a.To[3 + c][l];
+        Sn[c] = a.So[3 + c][l];
+    }
+    const int k = sp.cls;
+    float P[3], M[3];
+    if (k == 'n' || k == 'a') {
+        curl_halves<true>(H, nullptr, nbrs_at(a, x, y, z, true), l, P, M);
+        if (k == 'n') {
+            float W[3] = {P[0] + M[0], P[1] + M[1], P[2] + M[2]};
+            if (z == a.iI) {
+                const int64_t q = x + (int64_t)a.L0 * y;
+                W[0] += a.sgn * (float)plane_source(a.srcH_plane, a.Hinc, a.srcH_f2, a.srcH_re, a.srcH_im, q, 1);
+                W[1] -= a.sgn * (float)plane_source(a.srcH_plane, a.Hinc, a.srcH_f2, a.srcH_re, a.srcH_im, q, 0);
+            }
+            for (int c = 0; c < 3; c++) En[c] = sp.eA[c] * En[c] + sp.eB[c] * W[c];
+        } else {
+            for (int c = 0; c < 3; c++) {
+                const float s = Sn[c], m = En[c] - s;
+                const float s2 = sp.eAs[c] * s + sp.eBs[c] * M[c];
+                En[c] = sp.eA[c] * m + sp.eB[c] * P[c] + s2;
+                Sn[c] = s2;
+            }
+        }
+    } else if (k != 'h') { /* 'e', 'b': backward curl of the new H main slots (F:310) */
+        curl_halves<false>(H, Hs, nbrs_at(a, x, y, z, false), l, P, M);
+        for (int c = 0; c < 3; c++) {
+            const float s = Sn[c];
+            float m = En[c] - s + 0.5f * s;
+            const float s2 = sp.eAs[c] * s + sp.eBs[c] * (P[c] + M[c]);
+            m += 0.5f * s2;
+            En[c] = m + s2;
+            Sn[c] = s2;
+        }
+    }
+    for (int c = 0; c < 3; c++) {
+        a.Tn[3 + c][l] = En[c];
+        a.Sn[3 + c][l] = Sn[c];
+    }
+    if (z == a.iB || z == a.iT) {
+        const int64_t q = x + (int64_t)a.L0 * y;
+        for (int m = 0; m < 2; m++) {
+            if (z != (m == 0 ? a.iB : a.iT)) continue;
+            double *HM = m == 0 ? a.HB : a.HT, *EM = m == 0 ? a.EB : a.ET;
+            for (int v = 0; v < 2; v++) {
+                const double hv = (double)H[v][l] - (double)Hs[v][l];
+                const double ev = (double)En[v] - (double)Sn[v];
+                const int64_t o = 2 * (v + 2 * q);
+                HM[o] += a.monH_re * hv;
+                HM[o + 1] += a.monH_im * hv;
+                EM[o] += a.monE_re * ev;
+                EM[o + 1] += a.monE_im * ev;
+            }
+        }
+    }
 }
 
 } // namespace
@@ -405,17 +864,22 @@ const char *smx_f32_last_error(void) { return f32_err; }
 void smx_f32_free(smx_f32_state *st)
 {
     for (int i = 0; i < 6; i++) {
-        if (st->T[i]) (void)hipFree(st->T[i]);
-        if (st->S[i]) (void)hipFree(st->S[i]);
+        for (int k = 0; k < 2; k++) {
+            if (st->T[k][i]) (void)hipFree(st->T[k][i]);
+            if (st->S[k][i]) (void)hipFree(st->S[k][i]);
+            st->T[k][i] = st->S[k][i] = nullptr;
+        }
         if (st->b[i]) (void)hipFree(st->b[i]);
-        st->T[i] = st->S[i] = st->b[i] = nullptr;
+        st->b[i] = nullptr;
     }
     if (st->pml) (void)hipFree(st->pml);
     if (st->pmlflag) (void)hipFree(st->pmlflag);
     if (st->special) (void)hipFree(st->special);
+    if (st->special_cells) (void)hipFree(st->special_cells);
     st->pml = nullptr;
     st->pmlflag = nullptr;
     st->special = nullptr;
+    st->special_cells = nullptr;
     delete[] st->cont_host;
     delete[] st->addr_host;
     st->cont_host = nullptr;
@@ -460,6 +924,7 @@ int smx_f32_create(smx_f32_state *st, const smx_geom *g, double iR, double iI, c
     /* coefficient words; lossy and metal cells go to the special table */
     std::vector<float> bw(6 * (size_t)n);
     std::vector<smx_special_f32> special;
+    std::vector<int64_t> special_cells;
     for (int64_t l = 0; l < n; l++) {
         const int64_t x = l % L0, y = (l / L0) % L1, z = l / (L0 * L1);
         const bool is_a = flag[x] || flag[L0 + y] || flag[L0 + L1 + z];
@@ -497,6 +962,7 @@ int smx_f32_create(smx_f32_state *st, const smx_geom *g, double iR, double iI, c
         }
         const uint32_t idx = (uint32_t)special.size();
         special.push_back(sp);
+        special_cells.push_back(l);
         float fidx;
         std::memcpy(&fidx, &idx, sizeof fidx);
         const float nan = std::numeric_limits<float>::quiet_NaN();
@@ -509,12 +975,17 @@ int smx_f32_create(smx_f32_state *st, const smx_geom *g, double iR, double iI, c
     }
     st->nspecial = (int)special.size();
 
+    const char *mode = getenv("SMX_F32_KERNEL");
+    st->fused = !(mode && !strcmp(mode, "twopass"));
+    st->cur = 0;
     for (int i = 0; i < 6; i++) {
-        F32_HIP(hipMalloc(&st->T[i], sizeof(float) * n));
-        F32_HIP(hipMalloc(&st->S[i], sizeof(float) * n));
+        for (int k = 0; k < (st->fused ? 2 : 1); k++) {
+            F32_HIP(hipMalloc(&st->T[k][i], sizeof(float) * n));
+            F32_HIP(hipMalloc(&st->S[k][i], sizeof(float) * n));
+            F32_HIP(hipMemsetAsync(st->T[k][i], 0, sizeof(float) * n, stream));
+            F32_HIP(hipMemsetAsync(st->S[k][i], 0, sizeof(float) * n, stream));
+        }
         F32_HIP(hipMalloc(&st->b[i], sizeof(float) * n));
-        F32_HIP(hipMemsetAsync(st->T[i], 0, sizeof(float) * n, stream));
-        F32_HIP(hipMemsetAsync(st->S[i], 0, sizeof(float) * n, stream));
         F32_HIP(hipMemcpyAsync(st->b[i], bw.data() + (size_t)i * n, sizeof(float) * n, hipMemcpyHostToDevice,
                                stream));
     }
@@ -525,6 +996,9 @@ int smx_f32_create(smx_f32_state *st, const smx_geom *g, double iR, double iI, c
     if (!special.empty()) {
         F32_HIP(hipMalloc(&st->special, sizeof(smx_special_f32) * special.size()));
         F32_HIP(hipMemcpyAsync(st->special, special.data(), sizeof(smx_special_f32) * special.size(),
+                               hipMemcpyHostToDevice, stream));
+        F32_HIP(hipMalloc(&st->special_cells, sizeof(int64_t) * special_cells.size()));
+        F32_HIP(hipMemcpyAsync(st->special_cells, special_cells.data(), sizeof(int64_t) * special_cells.size(),
                                hipMemcpyHostToDevice, stream));
     }
     st->cont_host = new char[n];
@@ -537,6 +1011,70 @@ int smx_f32_create(smx_f32_state *st, const smx_geom *g, double iR, double iI, c
 
 hipError_t smx_f32_step(smx_f32_state *st, const smx_f32_launch_args *la, hipStream_t stream)
 {
+    if (st->fused) {
+        fused_args a;
+        std::memset(&a, 0, sizeof a);
+        const smx_geom &g = la->g;
+        a.L0 = g.L0;
+        a.L1 = g.L1;
+        a.L2 = g.L2;
+        a.gx = (g.L0 + FX - 1) / FX;
+        a.gy = (g.L1 + FY - 1) / FY;
+        a.nblocks = a.gx * a.gy * ((g.L2 + FZ - 1) / FZ);
+        a.sz = (int64_t)g.L0 * g.L1;
+        a.w[0] = (float)g.iBc0;
+        a.w[1] = (float)g.iBc1;
+        a.w[2] = (float)g.iBc2;
+        a.iB = g.iB;
+        a.iT = g.iT;
+        a.iI = g.iI;
+        a.sgn = (float)g.sgn;
+        for (int i = 0; i < 6; i++) {
+            a.To[i] = st->T[st->cur][i];
+            a.Tn[i] = st->T[1 - st->cur][i];
+            a.So[i] = st->S[st->cur][i];
+            a.Sn[i] = st->S[1 - st->cur][i];
+            a.b[i] = st->b[i];
+        }
+        a.pml = st->pml;
+        a.pmlflag = st->pmlflag;
+        a.special = st->special;
+        a.special_cells = st->special_cells;
+        a.nspecial = st->nspecial;
+        a.srcE_plane = la->srcE_plane;
+        a.srcH_plane = la->srcH_plane;
+        a.Einc = la->Einc;
+        a.Hinc = la->Hinc;
+        a.srcE_f2 = la->s.srcE_f2;
+        a.srcE_re = la->s.srcE_re;
+        a.srcE_im = la->s.srcE_im;
+        a.srcH_f2 = la->s.srcH_f2;
+        a.srcH_re = la->s.srcH_re;
+        a.srcH_im = la->s.srcH_im;
+        a.HB = la->HB;
+        a.EB = la->EB;
+        a.HT = la->HT;
+        a.ET = la->ET;
+        a.monH_re = la->s.monH_re;
+        a.monH_im = la->s.monH_im;
+        a.monE_re = la->s.monE_re;
+        a.monE_im = la->s.monE_im;
+        hipError_t e;
+        if (a.nspecial > 0) {
+            hipLaunchKernelGGL(special_h_pass, dim3((a.nspecial + kThreads - 1) / kThreads), dim3(kThreads), 0,
+                               stream, a);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+        }
+        hipLaunchKernelGGL(fused_step_f32, dim3(a.nblocks), dim3(kThreads), 0, stream, a);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if (a.nspecial > 0) {
+            hipLaunchKernelGGL(special_e_pass, dim3((a.nspecial + kThreads - 1) / kThreads), dim3(kThreads), 0,
+                               stream, a);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+        }
+        st->cur = 1 - st->cur;
+        return hipSuccess;
+    }
     f32_args a = base_args(st, la->g);
     a.HB = la->HB;
     a.EB = la->EB;
@@ -583,10 +1121,15 @@ int smx_f32_write_fields(smx_f32_state *st, const double *H_t, const double *E_t
             S[(size_t)(3 + c) * n + l] = (float)es;
         }
     }
-    for (int i = 0; i < 6; i++) {
-        F32_HIP(hipMemcpyAsync(st->T[i], T.data() + (size_t)i * n, sizeof(float) * n, hipMemcpyHostToDevice, stream));
-        F32_HIP(hipMemcpyAsync(st->S[i], S.data() + (size_t)i * n, sizeof(float) * n, hipMemcpyHostToDevice, stream));
-    }
+    /* both buffers: cells that never update (frozen metal fields) must read the same
+     * value from either */
+    for (int i = 0; i < 6; i++)
+        for (int k = 0; k < (st->fused ? 2 : 1); k++) {
+            F32_HIP(hipMemcpyAsync(st->T[k][i], T.data() + (size_t)i * n, sizeof(float) * n, hipMemcpyHostToDevice,
+                                   stream));
+            F32_HIP(hipMemcpyAsync(st->S[k][i], S.data() + (size_t)i * n, sizeof(float) * n, hipMemcpyHostToDevice,
+                                   stream));
+        }
     F32_HIP(hipStreamSynchronize(stream));
     return SMX_OK;
 }
@@ -596,8 +1139,10 @@ int smx_f32_read_fields(const smx_f32_state *st, double *H_t, double *E_t, hipSt
     const int64_t n = st->n;
     std::vector<float> T(6 * (size_t)n), S(6 * (size_t)n);
     for (int i = 0; i < 6; i++) {
-        F32_HIP(hipMemcpyAsync(T.data() + (size_t)i * n, st->T[i], sizeof(float) * n, hipMemcpyDeviceToHost, stream));
-        F32_HIP(hipMemcpyAsync(S.data() + (size_t)i * n, st->S[i], sizeof(float) * n, hipMemcpyDeviceToHost, stream));
+        F32_HIP(hipMemcpyAsync(T.data() + (size_t)i * n, st->T[st->cur][i], sizeof(float) * n, hipMemcpyDeviceToHost,
+                               stream));
+        F32_HIP(hipMemcpyAsync(S.data() + (size_t)i * n, st->S[st->cur][i], sizeof(float) * n, hipMemcpyDeviceToHost,
+                               stream));
     }
     F32_HIP(hipStreamSynchronize(stream));
     const size_t flen = 3 * (size_t)(n + st->Lnotn + 1);

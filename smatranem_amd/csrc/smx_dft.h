@@ -1,0 +1,36 @@
+/*
+ * smx_dft.h -- multi-frequency port-plane DFT (kern_dft.hip): kernel arguments.
+ */
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define SMX_DFT_MAX_PLANES 8
+#define SMX_DFT_MAX_FREQ 32
+#define SMX_DFT_MAX_CHUNK 32
+
+struct smx_dft_sample_args {
+    int L0, L1, L2;
+    int nplanes;
+    int planes[SMX_DFT_MAX_PLANES];
+    int64_t plane_cells;            /* L0*L1 */
+    const float *T32[6], *S32[6];   /* FP32 plan: total and split (current buffer) */
+    const unsigned char *pmlflag;   /* FP32 plan: per-axis PML flags */
+    const double *H64, *E64;        /* FP64 plan: reference layout (main slots) */
+    void *series;                   /* ring: [chunk][plane][4][L0*L1], float or double */
+};
+
+struct smx_dft_fold_args {
+    int nplanes, nfreq, count;
+    long long k0;                   /* run step index of ring slot 0 */
+    int64_t plane_cells;
+    double w[SMX_DFT_MAX_FREQ];     /* phase advance per step, omega_f*Dt */
+    const void *series;
+    double *acc;                    /* [plane][4][freq][L0*L1] complex */
+};
+
+extern "C" {
+hipError_t smx_dft_sample(const smx_dft_sample_args *a, int slot, hipStream_t stream);
+hipError_t smx_dft_fold(const smx_dft_fold_args *a, int f64, hipStream_t stream);
+}

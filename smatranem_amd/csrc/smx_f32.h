@@ -14,6 +14,8 @@
  *         samples) with A = (iR - iI*s)/(iR + iI*s), B = b*iR/(iR + iI*s), plus the
  *         per-axis 'a' flag.
  *   special  full per-component coefficients and class of lossy/metal cells.
+ * Fields are double-buffered: the fused step reads buffer cur and writes 1 - cur, so a
+ * block's halo reads never see a neighbour block's new values.
  */
 #pragma once
 
@@ -32,14 +34,18 @@ struct smx_special_f32 {
 struct smx_f32_state {
     int L[3];
     int64_t n;
-    float *T[6];
-    float *S[6];
+    float *T[2][6];         /* ping-pong buffers; [cur] holds the current state */
+    float *S[2][6];
     float *b[6];
     float *pml;             /* 4*(L0+L1+L2) floats: per axis aH[L], gH[L], aE[L], gE[L] */
     unsigned char *pmlflag; /* L0+L1+L2 */
     smx_special_f32 *special;
+    int64_t *special_cells; /* linear index of each special cell, table order */
     int nspecial;
-    int bx; /* x width of a block (32 or 64) */
+    int cur;
+    int fused; /* 1: one fused H+E launch per step (+ the special-cell E pass);
+                  0: two in-place sweeps */
+    int bx;    /* x width of a block of the in-place sweeps (32 or 64) */
     /* host copies for the reference-layout field import/export */
     char *cont_host;
     int *addr_host;

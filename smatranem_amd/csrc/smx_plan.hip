@@ -17,6 +17,7 @@
 
 #include "smx_fdtd.h"
 #include "smx_internal.h"
+#include "smx_dft.h"
 #include "smx_f32.h"
 
 namespace {
@@ -69,6 +70,15 @@ struct smx_plan {
     int N = 0;
     int tp[3] = {0, 0, 0};
     std::vector<double> expn; /* complex[2N] */
+    long long next_step = 0;  /* run step index of the next smx_plan_step */
+
+    /* multi-frequency port DFT (kern_dft.hip) */
+    int dft_nplanes = 0, dft_nfreq = 0, dft_chunk = 0, dft_fill = 0;
+    int dft_planes[SMX_DFT_MAX_PLANES] = {0};
+    double dft_w[SMX_DFT_MAX_FREQ] = {0};
+    long long dft_k0 = 0;
+    void *dft_series = nullptr;
+    double *dft_acc = nullptr;
 };
 
 namespace {
@@ -173,7 +183,8 @@ void free_plan(smx_plan *p)
 {
     if (!p) return;
     if (p->stream) (void)hipStreamSynchronize(p->stream);
-    void *bufs[] = {p->Ah, p->Bh, p->Ae, p->Be, p->H, p->E, p->Cont, p->addr, p->mon, p->inc, p->src};
+    void *bufs[] = {p->Ah, p->Bh, p->Ae, p->Be, p->H, p->E, p->Cont, p->addr, p->mon, p->inc, p->src,
+                    p->dft_series, p->dft_acc};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     smx_f32_free(&p->f32);
@@ -241,6 +252,56 @@ smx_f64_args f64_args(smx_plan *p)
     a.ET = p->mon + 12 * p->plane;
     std::memset(&a.s, 0, sizeof a.s);
     return a;
+}
+
+/* Fold the DFT samples held in the ring into the accumulators. */
+int dft_flush(smx_plan *p)
+{
+    if (!p->dft_nplanes || p->dft_fill == 0) return SMX_OK;
+    smx_dft_fold_args f;
+    std::memset(&f, 0, sizeof f);
+    f.nplanes = p->dft_nplanes;
+    f.nfreq = p->dft_nfreq;
+    f.count = p->dft_fill;
+    f.k0 = p->dft_k0;
+    f.plane_cells = p->plane;
+    std::memcpy(f.w, p->dft_w, sizeof f.w);
+    f.series = p->dft_series;
+    f.acc = p->dft_acc;
+    SMX_HIP(smx_dft_fold(&f, p->g.precision == SMX_F64, p->stream));
+    p->dft_fill = 0;
+    return SMX_OK;
+}
+
+/* Sample the DFT planes after run step k (steps must come in order within a chunk). */
+int dft_after_step(smx_plan *p, long long k)
+{
+    if (!p->dft_nplanes) return SMX_OK;
+    int rc;
+    if (p->dft_fill > 0 && k != p->dft_k0 + p->dft_fill && (rc = dft_flush(p))) return rc;
+    if (p->dft_fill == 0) p->dft_k0 = k;
+    smx_dft_sample_args a;
+    std::memset(&a, 0, sizeof a);
+    a.L0 = p->g.L[0];
+    a.L1 = p->g.L[1];
+    a.L2 = p->g.L[2];
+    a.nplanes = p->dft_nplanes;
+    std::memcpy(a.planes, p->dft_planes, sizeof a.planes);
+    a.plane_cells = p->plane;
+    if (p->g.precision == SMX_F64) {
+        a.H64 = p->H;
+        a.E64 = p->E;
+    } else {
+        for (int i = 0; i < 6; i++) {
+            a.T32[i] = p->f32.T[p->f32.cur][i];
+            a.S32[i] = p->f32.S[p->f32.cur][i];
+        }
+        a.pmlflag = p->f32.pmlflag;
+    }
+    a.series = p->dft_series;
+    SMX_HIP(smx_dft_sample(&a, p->dft_fill, p->stream));
+    if (++p->dft_fill == p->dft_chunk) return dft_flush(p);
+    return SMX_OK;
 }
 
 int launch_step(smx_plan *p, const smx_step_scalars &s, bool explicit_src)
@@ -420,7 +481,8 @@ int smx_plan_step(smx_plan *p, const double *Hi, const double *Ei, const double 
     s.monE_re = ExpN2[2];
     s.monE_im = ExpN2[3];
     int rc = launch_step(p, s, true);
-    if (rc) return rc;
+    if (rc || (rc = dft_after_step(p, p->next_step))) return rc;
+    p->next_step++;
     SMX_HIP(hipStreamSynchronize(p->stream));
     return SMX_OK;
 }
@@ -460,8 +522,9 @@ int smx_plan_run_steps(smx_plan *p, long long step0, long long nsteps, double su
             sums[0] += fH;
             sums[1] += fE;
         }
-        if ((rc = launch_step(p, s, false))) return rc;
+        if ((rc = launch_step(p, s, false)) || (rc = dft_after_step(p, k))) return rc;
     }
+    p->next_step = step0 + nsteps;
     return SMX_OK;
 }
 
@@ -477,6 +540,55 @@ int smx_plan_run(smx_plan *p, int P0, int P1, double sums[2])
 }
 
 void *smx_plan_stream(const smx_plan *p) { return p ? (void *)p->stream : nullptr; }
+
+int smx_plan_set_dft(smx_plan *p, int nplanes, const int *planes, int nfreq, const double *w_per_step,
+                     int chunk)
+{
+    if (!p) return arg_fail("null pointer");
+    if (nplanes < 0 || nplanes > SMX_DFT_MAX_PLANES) return arg_fail("DFT planes: 0..8");
+    if (nplanes > 0 && (!planes || !w_per_step)) return arg_fail("null pointer");
+    if (nplanes > 0 && (nfreq < 1 || nfreq > SMX_DFT_MAX_FREQ)) return arg_fail("DFT frequencies: 1..32");
+    if (nplanes > 0 && (chunk < 1 || chunk > SMX_DFT_MAX_CHUNK)) return arg_fail("DFT chunk: 1..32");
+    for (int m = 0; m < nplanes; m++)
+        if (planes[m] < 0 || planes[m] >= p->g.L[2]) return arg_fail("DFT plane outside the grid");
+    SMX_HIP(hipStreamSynchronize(p->stream));
+    if (p->dft_series) SMX_HIP(hipFree(p->dft_series));
+    if (p->dft_acc) SMX_HIP(hipFree(p->dft_acc));
+    p->dft_series = nullptr;
+    p->dft_acc = nullptr;
+    p->dft_nplanes = 0;
+    p->dft_fill = 0;
+    if (nplanes == 0) return SMX_OK;
+    const size_t word = p->g.precision == SMX_F64 ? sizeof(double) : sizeof(float);
+    const size_t cells = (size_t)nplanes * 4 * (size_t)p->plane;
+    SMX_HIP(hipMalloc(&p->dft_series, word * cells * (size_t)chunk));
+    SMX_HIP(hipMalloc(&p->dft_acc, sizeof(double) * 2 * cells * (size_t)nfreq));
+    SMX_HIP(hipMemsetAsync(p->dft_acc, 0, sizeof(double) * 2 * cells * (size_t)nfreq, p->stream));
+    p->dft_nplanes = nplanes;
+    p->dft_nfreq = nfreq;
+    p->dft_chunk = chunk;
+    std::memset(p->dft_planes, 0, sizeof p->dft_planes);
+    std::memset(p->dft_w, 0, sizeof p->dft_w);
+    std::memcpy(p->dft_planes, planes, sizeof(int) * nplanes);
+    std::memcpy(p->dft_w, w_per_step, sizeof(double) * nfreq);
+    SMX_HIP(hipStreamSynchronize(p->stream));
+    return SMX_OK;
+}
+
+int smx_plan_read_dft(smx_plan *p, double *out)
+{
+    if (!p || !out) return arg_fail("null pointer");
+    if (!p->dft_nplanes) {
+        snprintf(g_last_error, sizeof g_last_error, "no DFT planes set");
+        return SMX_ERR_STATE;
+    }
+    int rc = dft_flush(p);
+    if (rc) return rc;
+    const size_t len = 2 * (size_t)p->dft_nplanes * 4 * (size_t)p->plane * (size_t)p->dft_nfreq;
+    SMX_HIP(hipMemcpyAsync(out, p->dft_acc, sizeof(double) * len, hipMemcpyDeviceToHost, p->stream));
+    SMX_HIP(hipStreamSynchronize(p->stream));
+    return SMX_OK;
+}
 
 int smx_plan_sync(smx_plan *p)
 {

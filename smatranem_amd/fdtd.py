@@ -87,6 +87,10 @@ def lib():
     L.smx_plan_run_steps.restype = C.c_int
     L.smx_plan_stream.argtypes = [C.c_void_p]
     L.smx_plan_stream.restype = C.c_void_p
+    L.smx_plan_set_dft.argtypes = [C.c_void_p, C.c_int, _ip, C.c_int, _dp, C.c_int]
+    L.smx_plan_set_dft.restype = C.c_int
+    L.smx_plan_read_dft.argtypes = [C.c_void_p, _dp]
+    L.smx_plan_read_dft.restype = C.c_int
     L.smx_plan_sync.argtypes = [C.c_void_p]
     L.smx_plan_sync.restype = C.c_int
     L.smx_set_tau.argtypes = [C.c_double, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double),
@@ -313,6 +317,19 @@ class Plan:
         _check(lib().smx_plan_run_steps(self._h, int(step0), int(nsteps),
                                         acc.ctypes.data_as(C.c_void_p)))
         return acc
+
+    def set_dft(self, planes, w_per_step, chunk=32):
+        """Port DFT on z-planes `planes` at phase advances w_per_step = omega_f*Dt."""
+        self._dft = (len(planes), len(w_per_step))
+        _check(lib().smx_plan_set_dft(self._h, len(planes), _i32(planes), len(w_per_step),
+                                      _f64(w_per_step), int(chunk)))
+
+    def read_dft(self):
+        """complex128[nplanes, 4 (Hx, Hy, Ex, Ey), nfreq, L0*L1]."""
+        m, f = self._dft
+        out = np.zeros(2 * m * 4 * f * self.grid.plane)
+        _check(lib().smx_plan_read_dft(self._h, out))
+        return out.view(np.complex128).reshape(m, 4, f, self.grid.plane)
 
     @property
     def stream(self):

@@ -1,0 +1,97 @@
+"""Multi-frequency port DFT (smx_plan_set_dft), against the reference monitors and a
+direct DFT of the oracle's per-step plane samples."""
+import numpy as np
+import pytest
+
+import oracle
+from smatranem_amd import casegen, fdtd
+from tests.cases import rel_l2, small_waveguide
+
+pytestmark = pytest.mark.gpu
+
+CASE = dict(pml_xy=0, pml_z=3, iBc=(1, -1, 0), metal_cells=4, lossy_cells=3)
+
+
+def _prep():
+    return casegen.prepare(small_waveguide(seed=6, **CASE))
+
+
+@pytest.mark.parametrize("chunk", [1, 7, 32])
+def test_dft_at_omega_equals_reference_monitors(chunk):
+    prep = _prep()
+    c = prep.case
+    w = 2 * np.pi / prep.N
+    with prep.plan(fdtd.F64) as p:
+        p.set_dft([c.iB, c.iT], [w, 0.5 * w], chunk=chunk)
+        p.run(0, c.time_period[2])
+        HB, EB, HT, ET = p.read_monitors()
+        D = p.read_dft()
+    for m, (Hm, Em) in enumerate(((HB, EB), (HT, ET))):
+        ref = np.stack([Hm[0::2], Hm[1::2], Em[0::2], Em[1::2]])
+        assert np.linalg.norm(ref) > 0
+        assert rel_l2(D[m, :, 0, :], ref) < 1e-12
+
+
+def _oracle_samples(prep, nsteps):
+    """Per-step main-slot samples (Hx, Hy, Ex, Ey) of the monitor planes from the oracle."""
+    c = prep.case
+    N, iR, iI, _ = oracle.set_tau(c.omega, c.min_dt)
+    st = oracle.setup_from_media(c.L, c.mu, c.eps, iR=iR, iI=iI)
+    n, plane = c.n, c.plane
+    flen = 3 * (n + st["Lnotn"] + 1)
+    H, E = np.zeros(flen), np.zeros(flen)
+    mons = [np.zeros(4 * plane) for _ in range(4)]
+    ExpN = oracle.expn_table(N)
+    Hinc = np.ascontiguousarray(c.Hinc).view(np.float64)
+    Einc = np.ascontiguousarray(c.Einc).view(np.float64)
+    out = np.zeros((nsteps, 2, 4, plane))
+    for k in range(nsteps):
+        t, P = k % N, k // N
+        tn = (t + 1) % N
+        fE = 2.0 * oracle.set_env(N, c.time_period, 1, t, P)
+        fH = 2.0 * oracle.set_env(N, c.time_period, 0, t + 1, P)
+        Ei = fE * (ExpN[4 * t + 2] * Einc[0::2] + ExpN[4 * t + 3] * Einc[1::2])
+        Hi = fH * (ExpN[4 * tn] * Hinc[0::2] + ExpN[4 * tn + 1] * Hinc[1::2])
+        oracle.main_calc(oracle.load_oracle().orc_main_calc, c.L, st["Ah"], st["Bh"], st["Ae"], st["Be"], c.iBc,
+                         H, E, (c.iB, c.iT, c.iI, c.inc_mode), np.ascontiguousarray(Hi),
+                         np.ascontiguousarray(Ei), ExpN[4 * tn:4 * tn + 4].copy(), *mons, st["Cont"], st["addr"])
+        for m, z in enumerate((c.iB, c.iT)):
+            base = 3 * plane * z
+            out[k, m, 0] = H[base + 0:base + 3 * plane:3]
+            out[k, m, 1] = H[base + 1:base + 3 * plane:3]
+            out[k, m, 2] = E[base + 0:base + 3 * plane:3]
+            out[k, m, 3] = E[base + 1:base + 3 * plane:3]
+    return out
+
+
+@pytest.mark.parametrize("prec,tol", [(fdtd.F64, 1e-12), (fdtd.F32, 2e-4)])
+def test_dft_multifrequency_vs_direct(prec, tol, monkeypatch):
+    prep = _prep()
+    c = prep.case
+    nsteps = 3 * prep.N + 5
+    ws = np.array([0.1, 2 * np.pi / prep.N, 0.37, 1.1])
+    S = _oracle_samples(prep, nsteps)
+    k = np.arange(nsteps)
+    ref = np.zeros((2, 4, len(ws), c.plane), dtype=np.complex128)
+    for fi, w in enumerate(ws):
+        pH = np.exp(1j * w * (k + 1))
+        pE = np.exp(1j * w * (k + 1.5))
+        for v in range(4):
+            ph = pH if v < 2 else pE
+            ref[:, v, fi, :] = np.einsum("k,kmq->mq", ph, S[:, :, v, :])
+    with prep.plan(prec) as p:
+        p.set_dft([c.iB, c.iT], ws, chunk=16)
+        p.run_steps(0, nsteps)
+        D = p.read_dft()
+    assert rel_l2(D, ref) < tol
+
+
+def test_dft_argument_checks():
+    prep = _prep()
+    with prep.plan(fdtd.F32) as p:
+        for planes, ws, chunk in (([prep.case.L[2]], [0.1], 4), ([0], [0.1] * 40, 4), ([0], [0.1], 0)):
+            with pytest.raises(fdtd.SmxError):
+                p.set_dft(planes, ws, chunk)
+        p._dft = (1, 1)
+        with pytest.raises(fdtd.SmxError):
+            p.read_dft()

@@ -80,8 +80,15 @@ def test_plan_run_f64_bitwise(name):
     assert _eq(sums, so)
 
 
+@pytest.fixture(params=["fused", "twopass"])
+def f32_kernel(request, monkeypatch):
+    """The FP32 plan's step kernel: one fused launch, or the two in-place sweeps."""
+    monkeypatch.setenv("SMX_F32_KERNEL", request.param)
+    return request.param
+
+
 @pytest.mark.parametrize("name", list(CASES))
-def test_plan_run_f32_close(name):
+def test_plan_run_f32_close(name, f32_kernel):
     """FP32 fields vs the FP64 oracle: relative L2 <= 2e-4 on fields, monitors."""
     prep = casegen.prepare(small_waveguide(seed=3, **CASES[name]))
     with prep.plan(fdtd.F32) as p:
@@ -96,7 +103,26 @@ def test_plan_run_f32_close(name):
             assert rel_l2(a.view(np.float64), b) < 2e-4
 
 
-def test_f32_round_trip_fields():
+@pytest.mark.parametrize("dims", [(37, 19, 41), (33, 9, 17), (5, 3, 20)])
+def test_fused_partial_tiles(dims, f32_kernel):
+    """Grids that are not multiples of the fused tile (32 x 8 x 16): partial tiles,
+    wrapped halos and chunk edges, against the FP64 oracle."""
+    L0, L1, L2 = dims
+    prep = casegen.prepare(small_waveguide(seed=8, L0=L0, L1=L1, L2=L2, outer=max(L2 // 4, 4),
+                                           pml_xy=0, pml_z=2, iBc=(1, -1, 1), metal_cells=3,
+                                           lossy_cells=2, core=(2, 2)))
+    with prep.plan(fdtd.F32) as p:
+        p.run(0, prep.case.time_period[2])
+        H, E = p.read_fields()
+        mons = p.read_monitors()
+    Ho, Eo, mo, _ = _oracle_run(prep, 0, prep.case.time_period[2])
+    assert rel_l2(H, Ho) < 2e-4 and rel_l2(E, Eo) < 2e-4, (rel_l2(H, Ho), rel_l2(E, Eo))
+    for a, b in zip(mons, mo):
+        if np.linalg.norm(b) > 0:
+            assert rel_l2(a.view(np.float64), b) < 2e-4
+
+
+def test_f32_round_trip_fields(f32_kernel):
     """write_fields/read_fields of the FP32 plan: total/split packing is lossless up to
     FP32 rounding of the main and split slots."""
     prep = casegen.prepare(small_waveguide(seed=5, **CASES["box"]))
@@ -135,7 +161,7 @@ def test_run_errors():
         assert e.value.status == 101
 
 
-def test_zero_source_keeps_fields_zero():
+def test_zero_source_keeps_fields_zero(f32_kernel):
     prep = casegen.prepare(small_waveguide(seed=2, **CASES["ports"]))
     for prec in (fdtd.F64, fdtd.F32):
         with prep.plan(prec) as p:
