@@ -161,6 +161,23 @@ int smx_plan_set_dft(smx_plan *plan, int nplanes, const int *planes, int nfreq, 
 /* Accumulated DFT, complex[nplanes][4 (Hx, Hy, Ex, Ey)][nfreq][L0*L1]. */
 int smx_plan_read_dft(smx_plan *plan, double *out);
 
+/*
+ * Extension: adjoint gradient of a port objective (FP32 plans without lossy or metal
+ * cells).  No counterpart in the reference.
+ *   J = sum_{v = x, y} sum_q |D_v(q)|^2,  D_v(q) = sum_k e^{i*w*(k+1.5)} E_v(q, zT) after step k
+ * over steps k = 0 .. nsteps-1 of the plan's source loop, from zero fields.
+ * smx_adjoint_forward runs the forward pass, storing the full state every ckpt_every
+ * steps, and returns J.  smx_adjoint_backward runs the discrete adjoint in reverse
+ * time, recomputing each checkpoint segment forward to get E in the design box, and
+ * writes grad[3][nbox] = dJ/d eps_c (c = x, y, z) for the cells of the design box
+ * box = {x0, x1, y0, y1, z0, z1} (half-open, x fastest), which must hold only 'n'
+ * cells.  Both leave the plan's fields at the last recomputed state and its monitors
+ * accumulated over the recomputation.
+ */
+int smx_adjoint_forward(smx_plan *plan, long long nsteps, int ckpt_every, int zT, double w, const int box[6],
+                        double *J);
+int smx_adjoint_backward(smx_plan *plan, double *grad);
+
 /* The plan's HIP stream (a hipStream_t), for callers that order or time their own
  * work against the plan's kernels.  Valid until smx_plan_destroy. */
 void *smx_plan_stream(const smx_plan *plan);

@@ -60,6 +60,11 @@ def load_oracle():
                                 _ip, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _cp, _ip, _dp]
         lib.orc_run.restype = C.c_int
         lib.orc_s_project.argtypes = [_ip, C.c_int, _dp, _dp, _dp, _dp]
+        lib.orc_step_adjoint.argtypes = [_ip, _dp, _dp, _dp, _dp, _ip, _dp, _dp, _cp, _ip, C.c_int]
+        lib.orc_step_adjoint.restype = C.c_int
+        lib.orc_adjoint_run.argtypes = [_ip, _dp, _dp, _dp, _dp, _ip, _ip, _dp, _dp, C.c_int, _ip, _cp, _ip,
+                                        C.c_int, _dp, _dp, C.c_double, C.c_double, C.c_int, C.c_double, _dp]
+        lib.orc_adjoint_run.restype = C.c_double
         _oracle = lib
     return _oracle
 
@@ -193,3 +198,16 @@ def run(L, coeffs, iBc, iBTI, Hinc, Einc, N, tp, P0=0, P1=None, fields=None, mon
     if rc:
         raise RuntimeError(f"iBTIin[{rc - 1}] error!")
     return H, E, tuple(mons), sums
+
+
+def adjoint_run(L, st, iBc, iBTI, Hinc, Einc, N, tp, zT, w, iR, iI):
+    """orc_adjoint_run: (J, dJ/d eps_c [3n]) of the port-DFT objective on plane zT."""
+    lib = load_oracle()
+    n = int(L[0]) * int(L[1]) * int(L[2])
+    grad = np.zeros(3 * n)
+    J = lib.orc_adjoint_run(i32(L), st["Ah"], st["Bh"], st["Ae"], st["Be"], i32(iBc), i32(iBTI),
+                            np.ascontiguousarray(Hinc, dtype=np.complex128).view(np.float64),
+                            np.ascontiguousarray(Einc, dtype=np.complex128).view(np.float64), N, i32(tp),
+                            cont_bytes(st["Cont"]), i32(st["addr"]), st["Lnotn"], st["eps"], st["sigma"],
+                            iR, iI, int(zT), float(w), grad)
+    return J, grad

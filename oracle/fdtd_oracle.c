@@ -34,6 +34,9 @@
  *                        and the metal stable form of check_files/StableForm.pdf p.3.
  *   orc_run              composition of the above in the order of F:153-167.
  *   orc_s_project        bitwise vs reference S_calc (F:419-448) reading the same files.
+ *   orc_step_adjoint,    no reference counterpart (this build's adjoint): the transpose
+ *   orc_adjoint_run      is pinned by the dot-product test <M x, y> = <x, M^T y> and the
+ *                        gradient by central finite differences of orc_run.
  */
 #include <complex.h>
 #include <float.h>
@@ -642,4 +645,210 @@ void orc_s_project(const int L[4], int ncol, const double *tcols, const double *
         SB[2 * k] = creal(acc);
         SB[2 * k + 1] = cimag(acc);
     }
+}
+
+/* ---------------------------------------------------------------------------------- */
+/* Discrete adjoint of the step (no reference counterpart; checks the GPU adjoint)     */
+/* ---------------------------------------------------------------------------------- */
+
+/*
+ * Transposed curl pieces: for every entry of P that curl_*(half, P, map) reads with
+ * coefficient kappa into W[c], add kappa*u[c] to lP at that entry.
+ */
+static inline double *lslot(double *P, const int *map, int c, int64_t q)
+{
+    return map ? &P[c + 3 * (int64_t)map[q]] : &P[c + 3 * q];
+}
+
+static void curl_backward_T(int half, const orc_nb *nb, double *lP, const int *map, int64_t l, const double u[3])
+{
+    for (int c = 0; c < 3; c++) {
+        int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+        if (half != 1) {
+            *lslot(lP, map, c2, l) += u[c];
+            *lslot(lP, map, c2, l + nb->off[c1]) += -nb->w[c1] * u[c];
+        }
+        if (half != 0) {
+            *lslot(lP, map, c1, l) += -u[c];
+            *lslot(lP, map, c1, l + nb->off[c2]) += nb->w[c2] * u[c];
+        }
+    }
+}
+
+static void curl_forward_T(int half, const orc_nb *nb, double *lP, const int *map, int64_t l, const double u[3])
+{
+    for (int c = 0; c < 3; c++) {
+        int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+        if (half != 1) {
+            *lslot(lP, map, c2, l + nb->off[c1]) += nb->w[c1] * u[c];
+            *lslot(lP, map, c2, l) += -u[c];
+        }
+        if (half != 0) {
+            *lslot(lP, map, c1, l + nb->off[c2]) += -nb->w[c2] * u[c];
+            *lslot(lP, map, c1, l) += u[c];
+        }
+    }
+}
+
+/*
+ * Reverse of one orc_main_calc step for grids of 'n' and 'a' cells: on entry lH, lE
+ * hold dJ/d(state after the step), on exit dJ/d(state before it).  The step is
+ * affine in the fields (the source does not depend on them), so this is its
+ * transpose.  The zero slot of lH/lE is cleared.  Returns 1 if a metal cell is
+ * present (not supported), else 0.
+ */
+int orc_step_adjoint(const int L[4], const double *Ah, const double *Bh, const double *Ae, const double *Be,
+                     const int iBc[3], double *lH, double *lE, const char *Cont, const int *addr, int Lnotn)
+{
+    const int64_t L0 = L[0], L1 = L[1], L2 = L[2], n = L0 * L1 * L2;
+    double *lHs = lH + 3 * n, *lEs = lE + 3 * n;
+    int j[3];
+    for (int64_t l = 0; l < n; l++)
+        if (Cont[l] != 'n' && Cont[l] != 'a') return 1;
+    /* reverse of the E sweep: E_new = Ae*E_old + Be*(curl_f(H) + curl_f_map(Hs) [+ src]) */
+    for (j[2] = 0; j[2] < L2; j[2]++)
+        for (j[1] = 0; j[1] < L1; j[1]++)
+            for (j[0] = 0; j[0] < L0; j[0]++) {
+                const int64_t l = j[0] + L0 * j[1] + L0 * L1 * j[2];
+                orc_nb nb = nb_forward(L, iBc, j);
+                if (Cont[l] == 'n') {
+                    double u[3];
+                    for (int c = 0; c < 3; c++) u[c] = Be[c + 3 * l] * lE[c + 3 * l];
+                    curl_forward_T(2, &nb, lH, NULL, l, u);
+                    curl_forward_T(2, &nb, lHs, addr, l, u);
+                    for (int c = 0; c < 3; c++) lE[c + 3 * l] *= Ae[c + 3 * l];
+                } else {
+                    for (int half = 0; half < 2; half++) {
+                        const int64_t s = half == 0 ? 3 * l : 3 * n + 3 * (int64_t)addr[l];
+                        double u[3];
+                        for (int c = 0; c < 3; c++) u[c] = Be[s + c] * lE[s + c];
+                        curl_forward_T(half, &nb, lH, NULL, l, u);
+                        curl_forward_T(half, &nb, lHs, addr, l, u);
+                        for (int c = 0; c < 3; c++) lE[s + c] *= Ae[s + c];
+                    }
+                }
+            }
+    for (int c = 0; c < 3; c++) lHs[3 * (int64_t)Lnotn + c] = 0.;
+    /* reverse of the H sweep: H_new = Ah*H_old - Bh*(curl_b(E) + curl_b_map(Es) [+ src]) */
+    for (j[2] = 0; j[2] < L2; j[2]++)
+        for (j[1] = 0; j[1] < L1; j[1]++)
+            for (j[0] = 0; j[0] < L0; j[0]++) {
+                const int64_t l = j[0] + L0 * j[1] + L0 * L1 * j[2];
+                orc_nb nb = nb_backward(L, iBc, j);
+                if (Cont[l] == 'n') {
+                    double u[3];
+                    for (int c = 0; c < 3; c++) u[c] = -Bh[c + 3 * l] * lH[c + 3 * l];
+                    curl_backward_T(2, &nb, lE, NULL, l, u);
+                    curl_backward_T(2, &nb, lEs, addr, l, u);
+                    for (int c = 0; c < 3; c++) lH[c + 3 * l] *= Ah[c + 3 * l];
+                } else {
+                    for (int half = 0; half < 2; half++) {
+                        const int64_t s = half == 0 ? 3 * l : 3 * n + 3 * (int64_t)addr[l];
+                        double u[3];
+                        for (int c = 0; c < 3; c++) u[c] = -Bh[s + c] * lH[s + c];
+                        curl_backward_T(half, &nb, lE, NULL, l, u);
+                        curl_backward_T(half, &nb, lEs, addr, l, u);
+                        for (int c = 0; c < 3; c++) lH[s + c] *= Ah[s + c];
+                    }
+                }
+            }
+    for (int c = 0; c < 3; c++) lEs[3 * (int64_t)Lnotn + c] = 0.;
+    return 0;
+}
+
+/*
+ * Objective and gradient of a whole run (FP64, all states kept; small grids only).
+ * Forward: the loop of F:153-167 from zero fields for periods [0, tp[2]) -- K steps.
+ * Objective: J = sum_{v = x, y} sum_q |D_v(q)|^2, D_v(q) = sum_k e^{i*w*(k+1.5)}
+ * E_v(q, zT) after step k (the E monitor's phase convention, F:323-325); E_v is the
+ * main slot, and main + split (the total field) in PML cells.
+ * Gradient: gradc[3n] = dJ/d(eps_c of each cell), through the E update coefficients
+ * of 'n' cells (dBe/deps, dAe/deps from the coefficient formula, F:120-121); other
+ * cells get 0.  eps: complex[3n] as given to orc_build_coeffs; sigma likewise.
+ * Returns J, or NAN for unsupported input (metal cells).
+ */
+double orc_adjoint_run(const int L[4], const double *Ah, const double *Bh, const double *Ae, const double *Be,
+                       const int iBc[3], const int iBTIin[4], const double *Hinc, const double *Einc, int N,
+                       const int tp[3], const char *Cont, const int *addr, int Lnotn, const double *eps,
+                       const double *sigma, double iR, double iI, int zT, double w, double *gradc)
+{
+    const int64_t L0 = L[0], L1 = L[1], n = L0 * L1 * L[2], plane = L0 * L1;
+    const int64_t flen = 3 * (n + Lnotn + 1);
+    const long K = (long)tp[2] * N;
+    for (int64_t l = 0; l < n; l++)
+        if (Cont[l] != 'n' && Cont[l] != 'a') return NAN;
+    double *states = calloc((size_t)(K + 1) * 2 * flen, sizeof(double)); /* (H, E) after each step */
+    double *ExpN = malloc(sizeof(double) * 4 * N);
+    double *Hi = malloc(sizeof(double) * 2 * plane), *Ei = malloc(sizeof(double) * 2 * plane);
+    double *mons = calloc(16 * plane, sizeof(double));
+    orc_expn_table(N, ExpN);
+    for (long k = 0; k < K; k++) {
+        const int t = (int)(k % N), P = (int)(k / N);
+        double *H = states + (k + 1) * 2 * flen, *E = H + flen;
+        memcpy(H, states + k * 2 * flen, sizeof(double) * 2 * flen);
+        orc_source_planes(2 * plane, N, tp, t, P, ExpN, Hinc, Einc, Hi, Ei, NULL, NULL);
+        const int bti[4] = {-1, -1, iBTIin[2], iBTIin[3]};
+        orc_main_calc(L, Ah, Bh, Ae, Be, iBc, H, E, bti, Hi, Ei, &ExpN[4 * ((t + 1) % N)], mons, mons + 4 * plane,
+                      mons + 8 * plane, mons + 12 * plane, Cont, addr);
+    }
+    /* D and J */
+    double *D = calloc(4 * plane, sizeof(double)); /* [v][q] complex */
+    for (long k = 0; k < K; k++) {
+        const double *E = states + (k + 1) * 2 * flen + flen;
+        const double th = w * ((double)k + 1.5), cs = cos(th), sn = sin(th);
+        for (int v = 0; v < 2; v++)
+            for (int64_t q = 0; q < plane; q++) {
+                const int64_t l = q + plane * zT;
+                double x = E[v + 3 * l]; /* main slot; the total field in PML cells */
+                if (Cont[l] == 'a') x += E[3 * n + v + 3 * (int64_t)addr[l]];
+                D[2 * (v * plane + q)] += cs * x;
+                D[2 * (v * plane + q) + 1] += sn * x;
+            }
+    }
+    double J = 0.;
+    for (int64_t i = 0; i < 2 * plane; i++) J += D[2 * i] * D[2 * i] + D[2 * i + 1] * D[2 * i + 1];
+    /* reverse sweep */
+    double *lH = calloc(flen, sizeof(double)), *lE = calloc(flen, sizeof(double));
+    memset(gradc, 0, sizeof(double) * 3 * n);
+    const int64_t off[3] = {0, 2 * L0, 2 * L0 + 2 * L1};
+    for (long k = K - 1; k >= 0; k--) {
+        /* objective sample of the state after step k */
+        const double th = w * ((double)k + 1.5), cs = cos(th), sn = sin(th);
+        for (int v = 0; v < 2; v++)
+            for (int64_t q = 0; q < plane; q++) {
+                const int64_t l = q + plane * zT;
+                const double g = 2. * (D[2 * (v * plane + q)] * cs + D[2 * (v * plane + q) + 1] * sn);
+                lE[v + 3 * l] += g;
+                if (Cont[l] == 'a') lE[3 * n + v + 3 * (int64_t)addr[l]] += g;
+            }
+        /* gradient of step k: E_new = Ae*E_old + Be*W in 'n' cells, W = (E_new - Ae*E_old)/Be */
+        const double *Eo = states + k * 2 * flen + flen, *En = states + (k + 1) * 2 * flen + flen;
+        for (int64_t l = 0; l < n; l++) {
+            if (Cont[l] != 'n') continue;
+            const int64_t pos[3] = {l % L0, (l / L0) % L1, l / plane};
+            for (int c = 0; c < 3; c++) {
+                const int64_t i = c + 3 * l;
+                const int d = (c + 1) % 3;
+                const double s = sigma[off[d] + 1 + 2 * pos[d]];
+                const double er = eps[2 * i], ei = eps[2 * i + 1];
+                const double den = iR * er + iI * (er * s + ei);
+                const double B = Be[i], A = Ae[i];
+                const double dB = -(iR + iI * s) * B * B;
+                const double dA = (iR - iI * s) * B + (iR * er - iI * (er * s + ei)) * dB;
+                (void)den;
+                const double W = (En[i] - A * Eo[i]) / B;
+                gradc[i] += lE[i] * (dA * Eo[i] + dB * W);
+            }
+        }
+        orc_step_adjoint(L, Ah, Bh, Ae, Be, iBc, lH, lE, Cont, addr, Lnotn);
+    }
+    free(states);
+    free(ExpN);
+    free(Hi);
+    free(Ei);
+    free(mons);
+    free(D);
+    free(lH);
+    free(lE);
+    return J;
 }

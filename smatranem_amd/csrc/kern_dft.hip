@@ -56,10 +56,16 @@ __global__ __launch_bounds__(kThreads) void sample_planes_f64(smx_dft_sample_arg
     const int64_t q = i - m * per_plane;
     const int64_t l = q + per_plane * a.planes[m];
     double *dst = (double *)a.series + ((int64_t)slot * a.nplanes + m) * 4 * per_plane + q;
-    dst[0 * per_plane] = a.H64[0 + 3 * l];
-    dst[1 * per_plane] = a.H64[1 + 3 * l];
-    dst[2 * per_plane] = a.E64[0 + 3 * l];
-    dst[3 * per_plane] = a.E64[1 + 3 * l];
+    double v[4] = {a.H64[0 + 3 * l], a.H64[1 + 3 * l], a.E64[0 + 3 * l], a.E64[1 + 3 * l]};
+    if (a.cont[l] == 'a') { /* PML: main + split = total field */
+        const int64_t s = 3 * a.n + 3 * (int64_t)a.addr[l];
+        v[0] += a.H64[s + 0];
+        v[1] += a.H64[s + 1];
+        v[2] += a.E64[s + 0];
+        v[3] += a.E64[s + 1];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) dst[k * per_plane] = v[k];
 }
 
 /*

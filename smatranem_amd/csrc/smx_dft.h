@@ -17,7 +17,10 @@ struct smx_dft_sample_args {
     int64_t plane_cells;            /* L0*L1 */
     const float *T32[6], *S32[6];   /* FP32 plan: total and split (current buffer) */
     const unsigned char *pmlflag;   /* FP32 plan: per-axis PML flags */
-    const double *H64, *E64;        /* FP64 plan: reference layout (main slots) */
+    const double *H64, *E64;        /* FP64 plan: reference layout */
+    const char *cont;               /* FP64 plan: cell classes ('a' cells add the split slot) */
+    const int *addr;
+    int64_t n;
     void *series;                   /* ring: [chunk][plane][4][L0*L1], float or double */
 };
 

@@ -11,12 +11,14 @@
 
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <new>
 #include <vector>
 
 #include "smx_fdtd.h"
 #include "smx_internal.h"
+#include "smx_adjoint.h"
 #include "smx_dft.h"
 #include "smx_f32.h"
 
@@ -79,6 +81,14 @@ struct smx_plan {
     long long dft_k0 = 0;
     void *dft_series = nullptr;
     double *dft_acc = nullptr;
+
+    /* adjoint (kern_adjoint.hip): checkpoints of the forward run, adjoint fields */
+    long long adj_K = 0;
+    int adj_C = 0, adj_zT = 0, adj_box[6] = {0};
+    double adj_w = 0.;
+    bool adj_ready = false;
+    float *adj_ckpt = nullptr; /* [ceil(K/C)][12][n]: T[6], S[6] at steps 0, C, 2C, ... */
+    double *adj_D = nullptr;   /* [2][L0*L1] complex */
 };
 
 namespace {
@@ -184,7 +194,7 @@ void free_plan(smx_plan *p)
     if (!p) return;
     if (p->stream) (void)hipStreamSynchronize(p->stream);
     void *bufs[] = {p->Ah, p->Bh, p->Ae, p->Be, p->H, p->E, p->Cont, p->addr, p->mon, p->inc, p->src,
-                    p->dft_series, p->dft_acc};
+                    p->dft_series, p->dft_acc, p->adj_ckpt, p->adj_D};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     smx_f32_free(&p->f32);
@@ -291,6 +301,9 @@ int dft_after_step(smx_plan *p, long long k)
     if (p->g.precision == SMX_F64) {
         a.H64 = p->H;
         a.E64 = p->E;
+        a.cont = p->Cont;
+        a.addr = p->addr;
+        a.n = p->n;
     } else {
         for (int i = 0; i < 6; i++) {
             a.T32[i] = p->f32.T[p->f32.cur][i];
@@ -595,6 +608,192 @@ int smx_plan_sync(smx_plan *p)
     if (!p) return arg_fail("null pointer");
     SMX_HIP(hipStreamSynchronize(p->stream));
     return SMX_OK;
+}
+
+/* ---------------------------------------------------------------------------------- */
+/* Adjoint gradient (extension)                                                        */
+/* ---------------------------------------------------------------------------------- */
+
+static smx_adj_args adj_args(smx_plan *p)
+{
+    smx_adj_args a;
+    std::memset(&a, 0, sizeof a);
+    a.L0 = p->g.L[0];
+    a.L1 = p->g.L[1];
+    a.L2 = p->g.L[2];
+    a.n = p->n;
+    a.sz = p->plane;
+    for (int d = 0; d < 3; d++) a.w[d] = (float)p->g.iBc[d];
+    a.zT = p->adj_zT;
+    std::memcpy(a.box, p->adj_box, sizeof a.box);
+    a.bx = a.box[1] - a.box[0];
+    a.by = a.box[3] - a.box[2];
+    a.nbox = (int64_t)a.bx * a.by * (a.box[5] - a.box[4]);
+    for (int i = 0; i < 6; i++) a.b[i] = p->f32.b[i];
+    a.pml = p->f32.pml;
+    a.pmlflag = p->f32.pmlflag;
+    a.D = p->adj_D;
+    return a;
+}
+
+int smx_adjoint_forward(smx_plan *p, long long nsteps, int ckpt_every, int zT, double w, const int box[6],
+                        double *J)
+{
+    if (!p || !box || !J) return arg_fail("null pointer");
+    if (p->g.precision != SMX_F32) return arg_fail("the adjoint runs on FP32 plans");
+    if (p->f32.nspecial > 0) return arg_fail("the adjoint does not support lossy or metal cells");
+    if (!p->have_source) {
+        snprintf(g_last_error, sizeof g_last_error, "smx_adjoint_forward before smx_plan_set_source");
+        return SMX_ERR_STATE;
+    }
+    if (nsteps < 1 || ckpt_every < 1) return arg_fail("nsteps and ckpt_every must be >= 1");
+    if (zT < 0 || zT >= p->g.L[2]) return arg_fail("objective plane outside the grid");
+    for (int d = 0; d < 3; d++)
+        if (box[2 * d] < 0 || box[2 * d + 1] > p->g.L[d] || box[2 * d] >= box[2 * d + 1])
+            return arg_fail("design box outside the grid or empty");
+    for (int z = box[4]; z < box[5]; z++)
+        for (int y = box[2]; y < box[3]; y++)
+            for (int x = box[0]; x < box[1]; x++)
+                if (p->f32.cont_host[x + (int64_t)p->g.L[0] * (y + (int64_t)p->g.L[1] * z)] != 'n')
+                    return arg_fail("design box holds a PML or special cell");
+    const int64_t n = p->n;
+    const long long nck = (nsteps + ckpt_every - 1) / ckpt_every;
+    SMX_HIP(hipStreamSynchronize(p->stream));
+    if (p->adj_ckpt) SMX_HIP(hipFree(p->adj_ckpt));
+    if (p->adj_D) SMX_HIP(hipFree(p->adj_D));
+    p->adj_ckpt = nullptr;
+    p->adj_D = nullptr;
+    p->adj_ready = false;
+    SMX_HIP(hipMalloc(&p->adj_ckpt, sizeof(float) * 12 * (size_t)n * (size_t)nck));
+    SMX_HIP(hipMalloc(&p->adj_D, sizeof(double) * 4 * (size_t)p->plane));
+    SMX_HIP(hipMemsetAsync(p->adj_D, 0, sizeof(double) * 4 * (size_t)p->plane, p->stream));
+    p->adj_K = nsteps;
+    p->adj_C = ckpt_every;
+    p->adj_zT = zT;
+    p->adj_w = w;
+    std::memcpy(p->adj_box, box, sizeof p->adj_box);
+    /* the run starts from zero fields */
+    for (int k = 0; k < 2; k++)
+        for (int i = 0; i < 6; i++) {
+            if (p->f32.T[k][i]) SMX_HIP(hipMemsetAsync(p->f32.T[k][i], 0, sizeof(float) * n, p->stream));
+            if (p->f32.S[k][i]) SMX_HIP(hipMemsetAsync(p->f32.S[k][i], 0, sizeof(float) * n, p->stream));
+        }
+    smx_adj_args a = adj_args(p);
+    for (long long k = 0; k < nsteps; k++) {
+        if (k % ckpt_every == 0) {
+            float *dst = p->adj_ckpt + (size_t)(k / ckpt_every) * 12 * (size_t)n;
+            for (int i = 0; i < 6; i++) {
+                SMX_HIP(hipMemcpyAsync(dst + (size_t)i * n, p->f32.T[p->f32.cur][i], sizeof(float) * n,
+                                       hipMemcpyDeviceToDevice, p->stream));
+                SMX_HIP(hipMemcpyAsync(dst + (size_t)(6 + i) * n, p->f32.S[p->f32.cur][i], sizeof(float) * n,
+                                       hipMemcpyDeviceToDevice, p->stream));
+            }
+        }
+        smx_step_scalars s;
+        double fH, fE;
+        int rc = step_scalars(p, (int)(k / p->N), (int)(k % p->N), &s, &fH, &fE);
+        if (rc || (rc = launch_step(p, s, false))) return rc;
+        const double th = w * ((double)k + 1.5);
+        SMX_HIP(smx_adj_launch_sample(&a, p->f32.T[p->f32.cur][3], p->f32.T[p->f32.cur][4], std::cos(th),
+                                      std::sin(th), p->stream));
+    }
+    std::vector<double> D(4 * (size_t)p->plane);
+    SMX_HIP(hipMemcpyAsync(D.data(), p->adj_D, sizeof(double) * D.size(), hipMemcpyDeviceToHost, p->stream));
+    SMX_HIP(hipStreamSynchronize(p->stream));
+    double acc = 0.;
+    for (size_t i = 0; i < D.size(); i++) acc += D[i] * D[i];
+    *J = acc;
+    p->adj_ready = true;
+    return SMX_OK;
+}
+
+int smx_adjoint_backward(smx_plan *p, double *grad)
+{
+    if (!p || !grad) return arg_fail("null pointer");
+    if (!p->adj_ready) {
+        snprintf(g_last_error, sizeof g_last_error, "smx_adjoint_backward before smx_adjoint_forward");
+        return SMX_ERR_STATE;
+    }
+    const int64_t n = p->n;
+    smx_adj_args a = adj_args(p);
+    const long long K = p->adj_K, C = p->adj_C, nck = (K + C - 1) / C;
+    float *lam = nullptr, *seg = nullptr;
+    double *G = nullptr, *gdev = nullptr;
+    int rc = SMX_OK;
+    auto fail = [&](hipError_t e, const char *what) {
+        rc = hip_fail(e, what);
+        return rc;
+    };
+    hipError_t e;
+    if ((e = hipMalloc(&lam, sizeof(float) * 12 * (size_t)n)) != hipSuccess) return fail(e, "hipMalloc(lambda)");
+    if ((e = hipMalloc(&seg, sizeof(float) * 3 * (size_t)a.nbox * (size_t)(C + 1))) != hipSuccess ||
+        (e = hipMalloc(&G, sizeof(double) * 3 * (size_t)a.nbox)) != hipSuccess ||
+        (e = hipMalloc(&gdev, sizeof(double) * 3 * (size_t)a.nbox)) != hipSuccess) {
+        fail(e, "hipMalloc(adjoint)");
+    }
+    if (rc == SMX_OK) {
+        for (int i = 0; i < 6; i++) {
+            a.lT[i] = lam + (size_t)i * n;
+            a.lS[i] = lam + (size_t)(6 + i) * n;
+        }
+        a.G = G;
+        (void)hipMemsetAsync(lam, 0, sizeof(float) * 12 * (size_t)n, p->stream);
+        (void)hipMemsetAsync(G, 0, sizeof(double) * 3 * (size_t)a.nbox, p->stream);
+        const double th_last = p->adj_w * ((double)(K - 1) + 1.5);
+        if ((e = smx_adj_launch_inject(&a, std::cos(th_last), std::sin(th_last), p->stream)) != hipSuccess)
+            fail(e, "adjoint inject");
+    }
+    bool first = true;
+    for (long long s = nck - 1; s >= 0 && rc == SMX_OK; s--) {
+        const long long ks = s * C, ke = std::min(ks + C, K);
+        const float *src = p->adj_ckpt + (size_t)s * 12 * (size_t)n;
+        for (int i = 0; i < 6 && rc == SMX_OK; i++) {
+            if ((e = hipMemcpyAsync(p->f32.T[p->f32.cur][i], src + (size_t)i * n, sizeof(float) * n,
+                                    hipMemcpyDeviceToDevice, p->stream)) != hipSuccess ||
+                (e = hipMemcpyAsync(p->f32.S[p->f32.cur][i], src + (size_t)(6 + i) * n, sizeof(float) * n,
+                                    hipMemcpyDeviceToDevice, p->stream)) != hipSuccess)
+                fail(e, "restore checkpoint");
+        }
+        auto save = [&](long long slot) {
+            float **T = p->f32.T[p->f32.cur];
+            hipError_t ee = smx_adj_launch_save_box(&a, T[3], T[4], T[5], seg + (size_t)slot * 3 * a.nbox, p->stream);
+            if (ee != hipSuccess) fail(ee, "save design box");
+        };
+        if (rc == SMX_OK) save(0);
+        for (long long k = ks; k < ke && rc == SMX_OK; k++) {
+            smx_step_scalars sc;
+            double fH, fE;
+            if ((rc = step_scalars(p, (int)(k / p->N), (int)(k % p->N), &sc, &fH, &fE))) break;
+            if ((rc = launch_step(p, sc, false))) break;
+            save(k - ks + 1);
+        }
+        for (long long k = ke - 1; k >= ks && rc == SMX_OK; k--) {
+            const float *cur = seg + (size_t)(k - ks) * 3 * a.nbox, *next = cur + 3 * a.nbox;
+            if ((e = smx_adj_launch_a(&a, first ? 0 : 1, cur, next, p->stream)) != hipSuccess) {
+                fail(e, "adjoint kernel A");
+                break;
+            }
+            first = false;
+            const double th = p->adj_w * ((double)(k - 1) + 1.5);
+            if ((e = smx_adj_launch_b(&a, k >= 1, std::cos(th), std::sin(th), p->stream)) != hipSuccess) {
+                fail(e, "adjoint kernel B");
+                break;
+            }
+        }
+    }
+    if (rc == SMX_OK) {
+        if ((e = smx_adj_launch_finish(&a, p->g.invtau_R, gdev, p->stream)) != hipSuccess ||
+            (e = hipMemcpyAsync(grad, gdev, sizeof(double) * 3 * (size_t)a.nbox, hipMemcpyDeviceToHost,
+                                p->stream)) != hipSuccess ||
+            (e = hipStreamSynchronize(p->stream)) != hipSuccess)
+            fail(e, "adjoint finish");
+    }
+    (void)hipStreamSynchronize(p->stream);
+    if (lam) (void)hipFree(lam);
+    if (seg) (void)hipFree(seg);
+    if (G) (void)hipFree(G);
+    if (gdev) (void)hipFree(gdev);
+    return rc;
 }
 
 int smx_main_calc(const int L[4], const double *Ah, const double *Bh, const double *Ae,

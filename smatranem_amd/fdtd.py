@@ -91,6 +91,11 @@ def lib():
     L.smx_plan_set_dft.restype = C.c_int
     L.smx_plan_read_dft.argtypes = [C.c_void_p, _dp]
     L.smx_plan_read_dft.restype = C.c_int
+    L.smx_adjoint_forward.argtypes = [C.c_void_p, C.c_longlong, C.c_int, C.c_int, C.c_double, _ip,
+                                      C.POINTER(C.c_double)]
+    L.smx_adjoint_forward.restype = C.c_int
+    L.smx_adjoint_backward.argtypes = [C.c_void_p, _dp]
+    L.smx_adjoint_backward.restype = C.c_int
     L.smx_plan_sync.argtypes = [C.c_void_p]
     L.smx_plan_sync.restype = C.c_int
     L.smx_set_tau.argtypes = [C.c_double, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double),
@@ -330,6 +335,21 @@ class Plan:
         out = np.zeros(2 * m * 4 * f * self.grid.plane)
         _check(lib().smx_plan_read_dft(self._h, out))
         return out.view(np.complex128).reshape(m, 4, f, self.grid.plane)
+
+    def adjoint_forward(self, nsteps, ckpt_every, zT, w, box):
+        """Forward pass of the port-objective adjoint; returns J (smx_adjoint_forward)."""
+        J = C.c_double()
+        self._box = tuple(int(v) for v in box)
+        _check(lib().smx_adjoint_forward(self._h, int(nsteps), int(ckpt_every), int(zT), float(w),
+                                         _i32(box), C.byref(J)))
+        return J.value
+
+    def adjoint_backward(self):
+        """dJ/d eps_c on the design box: float64[3, nz, ny, nx] (smx_adjoint_backward)."""
+        x0, x1, y0, y1, z0, z1 = self._box
+        out = np.zeros(3 * (x1 - x0) * (y1 - y0) * (z1 - z0))
+        _check(lib().smx_adjoint_backward(self._h, out))
+        return out.reshape(3, z1 - z0, y1 - y0, x1 - x0)
 
     @property
     def stream(self):

@@ -91,7 +91,7 @@ def _write(case, tmp_path, n_real=3):
 @pytest.mark.gpu
 @pytest.mark.parametrize("inc", [1, -2])
 def test_fdtd_report_matches_reference(tmp_path, inc):
-    case = _case(inc_mode=1)
+    case = _case()
     case.inc_mode = inc
     inp, Xi = _write(case, tmp_path)
     if inc < 0:  # the E half of the incident column must be real (F:503)

@@ -1,0 +1,83 @@
+"""GPU adjoint (FP32) against the oracle's FP64 adjoint and against central finite
+differences of the FP64 GPU forward."""
+import numpy as np
+import pytest
+
+import oracle
+from smatranem_amd import casegen, fdtd
+from tests.cases import small_waveguide
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(**kw):
+    args = dict(seed=3, pml_xy=2, pml_z=3, iBc=(0, 0, 0), metal_cells=0, lossy_cells=0, time_period=(1, 1, 3))
+    args.update(kw)
+    return small_waveguide(**args)
+
+
+def _box(case):
+    L0, L1, L2 = case.L[:3]
+    return (3, L0 - 3, 3, L1 - 3, L2 // 2 - 3, L2 // 2 + 3)
+
+
+@pytest.mark.parametrize("kernel,ckpt", [("fused", 7), ("twopass", 5), ("fused", 1000)])
+def test_adjoint_matches_oracle(kernel, ckpt, monkeypatch):
+    monkeypatch.setenv("SMX_F32_KERNEL", kernel)
+    case = _case()
+    prep = casegen.prepare(case)
+    zT = case.L[2] - 5
+    w = 0.9 * 2 * np.pi / prep.N
+    K = case.time_period[2] * prep.N
+    box = _box(case)
+    with prep.plan(fdtd.F32) as p:
+        J = p.adjoint_forward(K, ckpt, zT, w, box)
+        g = p.adjoint_backward()
+    N, iR, iI, _ = oracle.set_tau(case.omega, case.min_dt)
+    st = oracle.setup_from_media(case.L, case.mu, case.eps, iR=iR, iI=iI)
+    Jo, go = oracle.adjoint_run(case.L, st, case.iBc, (-1, -1, case.iI, 1), case.Hinc, case.Einc, N,
+                                case.time_period, zT, w, iR, iI)
+    assert abs(J - Jo) <= 1e-4 * Jo
+    x0, x1, y0, y1, z0, z1 = box
+    go3 = go.reshape(case.L[2], case.L[1], case.L[0], 3)[z0:z1, y0:y1, x0:x1, :].transpose(3, 0, 1, 2)
+    err = np.linalg.norm(g - go3) / np.linalg.norm(go3)
+    assert err <= 1e-3, err
+
+
+def test_adjoint_vs_finite_differences():
+    """Central differences of J on the FP64 GPU forward (delta eps = 1e-3), on cells
+    where the gradient is large enough to resolve."""
+    case = _case(seed=5)
+    prep = casegen.prepare(case)
+    zT = case.L[2] - 5
+    w = 0.9 * 2 * np.pi / prep.N
+    K = case.time_period[2] * prep.N
+    box = _box(case)
+    with prep.plan(fdtd.F32) as p:
+        p.adjoint_forward(K, 10, zT, w, box)
+        g = p.adjoint_backward()
+
+    def J64(eps):
+        c2 = casegen.Case(**{**case.__dict__, "eps": eps})
+        pr = casegen.prepare(c2)
+        assert pr.N == prep.N
+        with pr.plan(fdtd.F64) as q:
+            q.set_dft([zT], [w], chunk=16)
+            q.run_steps(0, K)
+            D = q.read_dft()[0]
+        return float(np.sum(np.abs(D[2:4, 0, :]) ** 2))
+
+    x0, x1, y0, y1, z0, z1 = box
+    L0, L1 = case.L[0], case.L[1]
+    flat = np.abs(g).reshape(-1)
+    order = np.argsort(flat)[::-1][:4]
+    for idx in order:
+        c, zz, yy, xx = np.unravel_index(idx, g.shape)
+        cell = (x0 + xx) + L0 * ((y0 + yy) + L1 * (z0 + zz))
+        h = 1e-3
+        eps_p = case.eps.copy()
+        eps_m = case.eps.copy()
+        eps_p[3 * cell + c] += h
+        eps_m[3 * cell + c] -= h
+        fd = (J64(eps_p) - J64(eps_m)) / (2 * h)
+        assert abs(fd - g[c, zz, yy, xx]) <= 1e-3 * abs(fd) + 1e-3 * flat.max(), (fd, g[c, zz, yy, xx])
