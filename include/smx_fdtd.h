@@ -178,6 +178,15 @@ int smx_adjoint_forward(smx_plan *plan, long long nsteps, int ckpt_every, int zT
                         double *J);
 int smx_adjoint_backward(smx_plan *plan, double *grad);
 
+/*
+ * Autotuner of the FP32 fused step: times each tile shape (32x8, 64x4, 16x16 cells
+ * of (j0, j1)) with each z chunk (8, 16, 32, 64 planes) for trial_steps steps on this
+ * plan's grid, keeps the fastest, and restores fields and monitors.  choice (may be
+ * NULL) receives the tile x, y and z sizes in use; trial_steps = 0 only reports.
+ * Plans of other kinds report zeros.  No counterpart in the reference.
+ */
+int smx_plan_autotune(smx_plan *plan, int trial_steps, int choice[3]);
+
 /* The plan's HIP stream (a hipStream_t), for callers that order or time their own
  * work against the plan's kernels.  Valid until smx_plan_destroy. */
 void *smx_plan_stream(const smx_plan *plan);

@@ -403,7 +403,7 @@ f32_args base_args(const smx_f32_state *st, const smx_geom &g)
 /* Fused step: H and E of one step in a single pass over HBM                          */
 /* ---------------------------------------------------------------------------------- */
 /*
- * A block owns an FX x FY tile of (j0, j1) and marches an FZ-plane chunk of j2.  At
+ * A block owns an FX x FY tile of (j0, j1) and marches an fz-plane chunk of j2.  At
  * plane k it holds in LDS the old E of plane k+1 on the tile grown by one cell on
  * every side (for the backward curl of H) and the new H of plane k on the tile grown
  * by one cell on the +x/+y sides (for the forward curl of E); the z neighbours travel
@@ -415,16 +415,18 @@ f32_args base_args(const smx_f32_state *st, const smx_geom &g)
  * after it (the metal E update takes the backward curl of the new H, F:310, which
  * reaches outside the tile).
  */
-constexpr int FX = 32, FY = 8, FZ = 16;
-constexpr int EXW = FX + 2, EYW = FY + 2;   /* E stage: x0-1 .. x0+FX, y0-1 .. y0+FY */
-constexpr int HXW = FX + 1, HYW = FY + 1;   /* H plane: x0 .. x0+FX, y0 .. y0+FY */
-constexpr int NHALO = HXW * HYW - FX * FY;  /* 41 grown-tile H cells outside the tile */
-constexpr int NESLOT = EXW * EYW;           /* 340 */
-static_assert(NHALO <= kThreads && NESLOT <= 2 * kThreads && FX * FY == kThreads, "tile shape");
+/* Tile shapes (FX x FY = 256 threads) the autotuner chooses among; FZ is a runtime
+ * chunk length. */
+struct tile_shape {
+    int fx, fy;
+};
+constexpr tile_shape kTiles[3] = {{32, 8}, {64, 4}, {16, 16}};
+constexpr int kDefaultTile = 0, kDefaultFZ = 16;
 
 struct fused_args {
     int L0, L1, L2;
     int gx, gy, nblocks;
+    int fz;
     int64_t sz;
     float w[3];
     int iB, iT, iI;
@@ -593,16 +595,22 @@ __device__ inline void store3(float *const *dst, int64_t l, const float v[3])
     dst[2][l] = v[2];
 }
 
+template <int FX, int FY>
 __global__ __launch_bounds__(kThreads) void fused_step_f32(fused_args a)
 {
+    constexpr int EXW = FX + 2, EYW = FY + 2;  /* E stage: x0-1 .. x0+FX, y0-1 .. y0+FY */
+    constexpr int HXW = FX + 1, HYW = FY + 1;  /* H plane: x0 .. x0+FX, y0 .. y0+FY */
+    constexpr int NHALO = HXW * HYW - FX * FY; /* grown-tile H cells outside the tile */
+    constexpr int NESLOT = EXW * EYW;
+    static_assert(NHALO <= kThreads && NESLOT <= 2 * kThreads && FX * FY == kThreads, "tile shape");
     __shared__ float sE[3][EYW][EXW];
     __shared__ float sH[3][HYW][HXW];
 
     const int id = xcd_logical(blockIdx.x, a.nblocks);
     const int bxi = id % a.gx, rest = id / a.gx;
     const int byi = rest % a.gy, bzi = rest / a.gy;
-    const int x0 = bxi * FX, y0 = byi * FY, z0 = bzi * FZ;
-    const int z1 = min(z0 + FZ, a.L2);
+    const int x0 = bxi * FX, y0 = byi * FY, z0 = bzi * a.fz;
+    const int z1 = min(z0 + a.fz, a.L2);
     const int t = threadIdx.x;
 
     /* tile cell of this thread (slot A) and, for t < NHALO, one grown-tile cell (slot B) */
@@ -978,6 +986,15 @@ int smx_f32_create(smx_f32_state *st, const smx_geom *g, double iR, double iI, c
     const char *mode = getenv("SMX_F32_KERNEL");
     st->fused = !(mode && !strcmp(mode, "twopass"));
     st->cur = 0;
+    st->tile = kDefaultTile;
+    st->fz = kDefaultFZ;
+    if (const char *t = getenv("SMX_F32_TILE")) { /* "<shape 0..2>,<z chunk>", e.g. "1,32" */
+        int ti = kDefaultTile, zc = kDefaultFZ;
+        if (sscanf(t, "%d,%d", &ti, &zc) >= 1 && ti >= 0 && ti < 3 && zc >= 1) {
+            st->tile = ti;
+            st->fz = zc;
+        }
+    }
     for (int i = 0; i < 6; i++) {
         for (int k = 0; k < (st->fused ? 2 : 1); k++) {
             F32_HIP(hipMalloc(&st->T[k][i], sizeof(float) * n));
@@ -1018,9 +1035,11 @@ hipError_t smx_f32_step(smx_f32_state *st, const smx_f32_launch_args *la, hipStr
         a.L0 = g.L0;
         a.L1 = g.L1;
         a.L2 = g.L2;
-        a.gx = (g.L0 + FX - 1) / FX;
-        a.gy = (g.L1 + FY - 1) / FY;
-        a.nblocks = a.gx * a.gy * ((g.L2 + FZ - 1) / FZ);
+        const tile_shape ts = kTiles[st->tile];
+        a.fz = st->fz;
+        a.gx = (g.L0 + ts.fx - 1) / ts.fx;
+        a.gy = (g.L1 + ts.fy - 1) / ts.fy;
+        a.nblocks = a.gx * a.gy * ((g.L2 + a.fz - 1) / a.fz);
         a.sz = (int64_t)g.L0 * g.L1;
         a.w[0] = (float)g.iBc0;
         a.w[1] = (float)g.iBc1;
@@ -1065,7 +1084,11 @@ hipError_t smx_f32_step(smx_f32_state *st, const smx_f32_launch_args *la, hipStr
                                stream, a);
             if ((e = hipGetLastError()) != hipSuccess) return e;
         }
-        hipLaunchKernelGGL(fused_step_f32, dim3(a.nblocks), dim3(kThreads), 0, stream, a);
+        switch (st->tile) {
+        case 1: hipLaunchKernelGGL((fused_step_f32<64, 4>), dim3(a.nblocks), dim3(kThreads), 0, stream, a); break;
+        case 2: hipLaunchKernelGGL((fused_step_f32<16, 16>), dim3(a.nblocks), dim3(kThreads), 0, stream, a); break;
+        default: hipLaunchKernelGGL((fused_step_f32<32, 8>), dim3(a.nblocks), dim3(kThreads), 0, stream, a);
+        }
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if (a.nspecial > 0) {
             hipLaunchKernelGGL(special_e_pass, dim3((a.nspecial + kThreads - 1) / kThreads), dim3(kThreads), 0,

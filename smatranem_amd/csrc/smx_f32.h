@@ -46,6 +46,8 @@ struct smx_f32_state {
     int fused; /* 1: one fused H+E launch per step (+ the special-cell E pass);
                   0: two in-place sweeps */
     int bx;    /* x width of a block of the in-place sweeps (32 or 64) */
+    int tile;  /* fused tile shape: 0 = 32x8, 1 = 64x4, 2 = 16x16 (x by y) */
+    int fz;    /* fused z chunk */
     /* host copies for the reference-layout field import/export */
     char *cont_host;
     int *addr_host;

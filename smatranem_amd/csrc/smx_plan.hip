@@ -796,6 +796,76 @@ int smx_adjoint_backward(smx_plan *p, double *grad)
     return rc;
 }
 
+int smx_plan_autotune(smx_plan *p, int trial_steps, int choice[3])
+{
+    static const int shapes[3][2] = {{32, 8}, {64, 4}, {16, 16}}; /* kern_f32.hip kTiles */
+    static const int chunks[4] = {8, 16, 32, 64};
+    if (!p) return arg_fail("null pointer");
+    smx_f32_state &st = p->f32;
+    if (p->g.precision == SMX_F32 && st.fused && trial_steps > 0) {
+        const int64_t n = p->n;
+        const size_t field = sizeof(float) * (size_t)n;
+        float *save = nullptr;
+        double *mon = nullptr;
+        SMX_HIP(hipMalloc(&save, 12 * field));
+        SMX_HIP(hipMalloc(&mon, sizeof(double) * 16 * (size_t)p->plane));
+        for (int i = 0; i < 6; i++) {
+            SMX_HIP(hipMemcpyAsync((char *)save + i * field, st.T[st.cur][i], field, hipMemcpyDeviceToDevice, p->stream));
+            SMX_HIP(hipMemcpyAsync((char *)save + (6 + i) * field, st.S[st.cur][i], field, hipMemcpyDeviceToDevice,
+                                   p->stream));
+        }
+        SMX_HIP(hipMemcpyAsync(mon, p->mon, sizeof(double) * 16 * (size_t)p->plane, hipMemcpyDeviceToDevice, p->stream));
+        smx_step_scalars s;
+        std::memset(&s, 0, sizeof s);
+        hipEvent_t e0, e1;
+        SMX_HIP(hipEventCreate(&e0));
+        SMX_HIP(hipEventCreate(&e1));
+        float best = 1e30f;
+        int best_t = st.tile, best_z = st.fz;
+        for (int t = 0; t < 3; t++)
+            for (int zi = 0; zi < 4; zi++) {
+                st.tile = t;
+                st.fz = chunks[zi];
+                int rc;
+                for (int k = 0; k < 2; k++)
+                    if ((rc = launch_step(p, s, false))) return rc;
+                SMX_HIP(hipEventRecord(e0, p->stream));
+                for (int k = 0; k < trial_steps; k++)
+                    if ((rc = launch_step(p, s, false))) return rc;
+                SMX_HIP(hipEventRecord(e1, p->stream));
+                SMX_HIP(hipEventSynchronize(e1));
+                float ms = 0.f;
+                SMX_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (ms < best) {
+                    best = ms;
+                    best_t = t;
+                    best_z = chunks[zi];
+                }
+            }
+        st.tile = best_t;
+        st.fz = best_z;
+        for (int k = 0; k < 2; k++)
+            for (int i = 0; i < 6; i++) {
+                SMX_HIP(hipMemcpyAsync(st.T[k][i], (char *)save + i * field, field, hipMemcpyDeviceToDevice, p->stream));
+                SMX_HIP(hipMemcpyAsync(st.S[k][i], (char *)save + (6 + i) * field, field, hipMemcpyDeviceToDevice,
+                                       p->stream));
+            }
+        SMX_HIP(hipMemcpyAsync(p->mon, mon, sizeof(double) * 16 * (size_t)p->plane, hipMemcpyDeviceToDevice, p->stream));
+        SMX_HIP(hipStreamSynchronize(p->stream));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipFree(save);
+        (void)hipFree(mon);
+    }
+    if (choice) {
+        const bool f = p->g.precision == SMX_F32 && st.fused;
+        choice[0] = f ? shapes[st.tile][0] : 0;
+        choice[1] = f ? shapes[st.tile][1] : 0;
+        choice[2] = f ? st.fz : 0;
+    }
+    return SMX_OK;
+}
+
 int smx_main_calc(const int L[4], const double *Ah, const double *Bh, const double *Ae,
                   const double *Be, const int iBc[3], double *H_t, double *E_t,
                   const int iBTIin[4], const double *Hi, const double *Ei, const double *ExpN2,

@@ -96,6 +96,8 @@ def lib():
     L.smx_adjoint_forward.restype = C.c_int
     L.smx_adjoint_backward.argtypes = [C.c_void_p, _dp]
     L.smx_adjoint_backward.restype = C.c_int
+    L.smx_plan_autotune.argtypes = [C.c_void_p, C.c_int, _ip]
+    L.smx_plan_autotune.restype = C.c_int
     L.smx_plan_sync.argtypes = [C.c_void_p]
     L.smx_plan_sync.restype = C.c_int
     L.smx_set_tau.argtypes = [C.c_double, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double),
@@ -350,6 +352,12 @@ class Plan:
         out = np.zeros(3 * (x1 - x0) * (y1 - y0) * (z1 - z0))
         _check(lib().smx_adjoint_backward(self._h, out))
         return out.reshape(3, z1 - z0, y1 - y0, x1 - x0)
+
+    def autotune(self, trial_steps=20):
+        """Pick the fastest fused tile shape and z chunk for this grid; returns (fx, fy, fz)."""
+        out = np.zeros(3, dtype=np.int32)
+        _check(lib().smx_plan_autotune(self._h, int(trial_steps), out))
+        return tuple(int(v) for v in out)
 
     @property
     def stream(self):

@@ -170,3 +170,37 @@ def test_zero_source_keeps_fields_zero(f32_kernel):
             p.run(0, 3)
             H, E = p.read_fields()
         assert not H.any() and not E.any()
+
+
+@pytest.mark.parametrize("tile", ["0,16", "1,8", "2,32", "0,5"])
+def test_fused_tile_shapes(tile, monkeypatch):
+    """Every fused tile shape and z chunk (SMX_F32_TILE) against the FP64 oracle."""
+    monkeypatch.setenv("SMX_F32_KERNEL", "fused")
+    monkeypatch.setenv("SMX_F32_TILE", tile)
+    prep = casegen.prepare(small_waveguide(seed=9, L0=37, L1=19, L2=41, outer=8, pml_xy=2, pml_z=3,
+                                           iBc=(1, 0, -1), metal_cells=3, lossy_cells=2, core=(3, 3)))
+    with prep.plan(fdtd.F32) as p:
+        assert p.autotune(0)[:2] == {"0": (32, 8), "1": (64, 4), "2": (16, 16)}[tile[0]]
+        p.run(0, prep.case.time_period[2])
+        H, E = p.read_fields()
+    Ho, Eo, _, _ = _oracle_run(prep, 0, prep.case.time_period[2])
+    assert rel_l2(H, Ho) < 2e-4 and rel_l2(E, Eo) < 2e-4, (rel_l2(H, Ho), rel_l2(E, Eo))
+
+
+def test_autotune_keeps_state():
+    """The autotuner restores fields and monitors, and the run it tunes stays correct."""
+    prep = casegen.prepare(small_waveguide(seed=9, **CASES["ports"]))
+    with prep.plan(fdtd.F32) as p:
+        p.run(0, 1)
+        H0, E0 = p.read_fields()
+        m0 = p.read_monitors()
+        choice = p.autotune(3)
+        H1, E1 = p.read_fields()
+        m1 = p.read_monitors()
+        assert _eq(H0, H1) and _eq(E0, E1)
+        assert all(_eq(a, b) for a, b in zip(m0, m1))
+        assert choice[0] * choice[1] == 256 and choice[2] in (8, 16, 32, 64)
+        p.run(1, prep.case.time_period[2])
+        H, E = p.read_fields()
+    Ho, Eo, _, _ = _oracle_run(prep, 0, prep.case.time_period[2])
+    assert rel_l2(H, Ho) < 2e-4 and rel_l2(E, Eo) < 2e-4
