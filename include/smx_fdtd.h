@@ -179,11 +179,12 @@ int smx_adjoint_forward(smx_plan *plan, long long nsteps, int ckpt_every, int zT
 int smx_adjoint_backward(smx_plan *plan, double *grad);
 
 /*
- * Autotuner of the FP32 fused step: times each tile shape (32x8, 64x4, 16x16 cells
- * of (j0, j1)) with each z chunk (8, 16, 32, 64 planes) for trial_steps steps on this
- * plan's grid, keeps the fastest, and restores fields and monitors.  choice (may be
- * NULL) receives the tile x, y and z sizes in use; trial_steps = 0 only reports.
- * Plans of other kinds report zeros.  No counterpart in the reference.
+ * Autotuner of the FP32 step: times the two in-place sweeps and the fused step with
+ * each tile shape (32x8, 64x4, 16x16 cells of (j0, j1)) and z chunk (8, 16, 32, 64
+ * planes) for trial_steps steps on this plan's grid, keeps the fastest, and restores
+ * fields and monitors.  choice (may be NULL) receives the fused tile x, y and z sizes
+ * in use, or zeros when the in-place sweeps won (and for FP64 plans); trial_steps = 0
+ * only reports.  No counterpart in the reference.
  */
 int smx_plan_autotune(smx_plan *plan, int trial_steps, int choice[3]);
 

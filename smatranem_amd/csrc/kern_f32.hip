@@ -595,6 +595,62 @@ __device__ inline void store3(float *const *dst, int64_t l, const float v[3])
     dst[2][l] = v[2];
 }
 
+/* New H of one cell from preloaded old values: E at the cell (ec) and its -x, -y, -z
+ * neighbours, old H (h), coefficient words (b), old split (sp, PML only). */
+__device__ inline int fused_h_pre(const fused_args &a, int x, int y, int z, int64_t l, const float ec[3],
+                                  const float exm[3], const float eym[3], const float ezm[3], const float h[3],
+                                  const float b[3], const float sp[3], bool pml, float Hn[3], float Sn[3])
+{
+    if (__builtin_isnan(b[0])) { /* computed by special_h_pass before this kernel */
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            Hn[c] = a.Tn[c][l];
+            Sn[c] = a.Sn[c][l];
+        }
+        return K_SPECIAL;
+    }
+    const float wx = x == 0 ? a.w[0] : 1.f, wy = y == 0 ? a.w[1] : 1.f, wz = z == 0 ? a.w[2] : 1.f;
+    float P[3], M[3];
+    P[0] = ec[2] - wy * eym[2];
+    M[0] = -ec[1] + wz * ezm[1];
+    P[1] = ec[0] - wz * ezm[0];
+    M[1] = -ec[2] + wx * exm[2];
+    P[2] = ec[1] - wx * exm[1];
+    M[2] = -ec[0] + wy * eym[0];
+    if (!pml) {
+        if (z == a.iI) {
+            const int64_t q = x + (int64_t)a.L0 * y;
+            P[0] += a.sgn * (float)plane_source(a.srcE_plane, a.Einc, a.srcE_f2, a.srcE_re, a.srcE_im, q, 1);
+            P[1] -= a.sgn * (float)plane_source(a.srcE_plane, a.Einc, a.srcE_f2, a.srcE_re, a.srcE_im, q, 0);
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) Hn[c] = h[c] - b[c] * (P[c] + M[c]);
+        return K_NORMAL;
+    }
+    const int jj[3] = {x, y, z};
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const int dm = c == 2 ? 0 : c + 1, ds = c == 0 ? 2 : c - 1;
+        const float *tm = axis_tab(a, dm), *ts = axis_tab(a, ds);
+        const int Lm = axis_len(a, dm), Ls = axis_len(a, ds);
+        const float aM = tm[jj[dm]], gM = tm[Lm + jj[dm]];
+        const float aS = ts[jj[ds]], gS = ts[Ls + jj[ds]];
+        const float s2 = aS * sp[c] - b[c] * gS * M[c];
+        Hn[c] = aM * (h[c] - sp[c]) - b[c] * gM * P[c] + s2;
+        Sn[c] = s2;
+    }
+    return K_PML;
+}
+
+/* Everything plane p contributes from global memory, loaded one plane ahead. */
+struct plane_bundle {
+    float e[2][3];                   /* old E of the two staged cells of this thread */
+    float hA[3], bA[3], sA[3];       /* slot A: old H, bH, old H split (PML) */
+    float hB[3], bB[3], sB[3];       /* slot B likewise */
+    float be[3], se[3];              /* slot A: bE, old E split (PML), for E at plane p */
+    bool pA, pB;                     /* slot A / B in PML at plane p */
+};
+
 template <int FX, int FY>
 __global__ __launch_bounds__(kThreads) void fused_step_f32(fused_args a)
 {
@@ -622,8 +678,10 @@ __global__ __launch_bounds__(kThreads) void fused_step_f32(fused_args a)
     const int bi = t < HXW ? t : FX, bj = t < HXW ? FY : t - HXW;
     const int bx = wrap_index(x0 + bi, a.L0), by = wrap_index(y0 + bj, a.L1);
     const int64_t bq = bx + (int64_t)a.L0 * by;
+    const bool pxyA = a.pmlflag[ax] | a.pmlflag[a.L0 + ay];
+    const bool pxyB = a.pmlflag[bx] | a.pmlflag[a.L0 + by];
 
-    /* E stage slots loaded by this thread: t and t + 256 (< 340) */
+    /* E stage cells loaded by this thread: t and t + 256 (< NESLOT) */
     int64_t eq[2];
     int es[2];
 #pragma unroll
@@ -633,18 +691,39 @@ __global__ __launch_bounds__(kThreads) void fused_step_f32(fused_args a)
         const int ex = s % EXW, ey = s / EXW;
         eq[r] = wrap_index(x0 - 1 + ex, a.L0) + (int64_t)a.L0 * wrap_index(y0 - 1 + ey, a.L1);
     }
-    auto stage_E = [&](int plane) {
-        const int64_t off = (int64_t)plane * a.sz;
+    auto load_bundle = [&](int p, plane_bundle &B) {
+        const int64_t off = (int64_t)p * a.sz;
+        const bool pz = a.pmlflag[a.L0 + a.L1 + p];
+        B.pA = pxyA | pz;
+        B.pB = pxyB | pz;
+#pragma unroll
+        for (int r = 0; r < 2; r++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) B.e[r][c] = es[r] >= 0 ? a.To[3 + c][off + eq[r]] : 0.f;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            B.hA[c] = a.To[c][off + aq];
+            B.bA[c] = a.b[c][off + aq];
+            B.sA[c] = B.pA ? a.So[c][off + aq] : 0.f;
+            B.be[c] = a.b[3 + c][off + aq];
+            B.se[c] = B.pA ? a.So[3 + c][off + aq] : 0.f;
+            B.hB[c] = hasB ? a.To[c][off + bq] : 0.f;
+            B.bB[c] = hasB ? a.b[c][off + bq] : 0.f;
+            B.sB[c] = hasB && B.pB ? a.So[c][off + bq] : 0.f;
+        }
+    };
+    auto stage_to_lds = [&](const plane_bundle &B) {
 #pragma unroll
         for (int r = 0; r < 2; r++) {
             if (es[r] < 0) continue;
             const int ex = es[r] % EXW, ey = es[r] / EXW;
 #pragma unroll
-            for (int c = 0; c < 3; c++) sE[c][ey][ex] = a.To[3 + c][off + eq[r]];
+            for (int c = 0; c < 3; c++) sE[c][ey][ex] = B.e[r][c];
         }
     };
-    auto evalH = [&](int i, int j, int x, int y, int z, int64_t l, const float ezm[3], float Hn[3], float Sn[3],
-                     float ecenter[3]) {
+    /* H at a grown-tile cell (i, j) of plane p from the staged E */
+    auto evalH = [&](int i, int j, int x, int y, int p, int64_t l, const float ezm[3], const float h[3],
+                     const float b[3], const float sp[3], bool pml, float Hn[3], float Sn[3], float ecenter[3]) {
         float exm[3], eym[3];
 #pragma unroll
         for (int c = 0; c < 3; c++) {
@@ -652,14 +731,16 @@ __global__ __launch_bounds__(kThreads) void fused_step_f32(fused_args a)
             exm[c] = sE[c][j + 1][i];
             eym[c] = sE[c][j][i + 1];
         }
-        return fused_h(a, x, y, z, l, ecenter, exm, eym, ezm, Hn, Sn);
+        return fused_h_pre(a, x, y, p, l, ecenter, exm, eym, ezm, h, b, sp, pml, Hn, Sn);
     };
 
+    float eA[3], eB[3]; /* old E of the previous plane at slots A, B (the -z neighbours) */
+    float beK[3], seK[3];
+    bool pK;
+    plane_bundle cur, nxt;
+
     /* prologue: H(z0) on the grown tile */
-    float eA[3], eB[3]; /* old E of the previous plane at slots A, B (z neighbours) */
-    float hA[3], hB[3], sA[3], sB[3];
     {
-        stage_E(z0);
         const int zm = z0 == 0 ? a.L2 - 1 : z0 - 1;
         const int64_t offm = (int64_t)zm * a.sz;
 #pragma unroll
@@ -667,10 +748,13 @@ __global__ __launch_bounds__(kThreads) void fused_step_f32(fused_args a)
             eA[c] = a.To[3 + c][offm + aq];
             eB[c] = hasB ? a.To[3 + c][offm + bq] : 0.f;
         }
+        load_bundle(z0, cur);
+        load_bundle(z0 + 1 == a.L2 ? 0 : z0 + 1, nxt);
+        stage_to_lds(cur);
         __syncthreads();
         const int64_t off = (int64_t)z0 * a.sz;
-        float ec[3];
-        const int kind = evalH(ai, aj, ax, ay, z0, off + aq, eA, hA, sA, ec);
+        float hA[3], sA[3], hB[3], sB[3], ec[3];
+        const int kind = evalH(ai, aj, ax, ay, z0, off + aq, eA, cur.hA, cur.bA, cur.sA, cur.pA, hA, sA, ec);
         if (owned && kind != K_SPECIAL) {
             store3(a.Tn, off + aq, hA);
             if (kind == K_PML) store3(a.Sn, off + aq, sA);
@@ -678,10 +762,16 @@ __global__ __launch_bounds__(kThreads) void fused_step_f32(fused_args a)
 #pragma unroll
         for (int c = 0; c < 3; c++) eA[c] = ec[c];
         if (hasB) {
-            evalH(bi, bj, bx, by, z0, off + bq, eB, hB, sB, ec);
+            evalH(bi, bj, bx, by, z0, off + bq, eB, cur.hB, cur.bB, cur.sB, cur.pB, hB, sB, ec);
 #pragma unroll
             for (int c = 0; c < 3; c++) eB[c] = ec[c];
         }
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            beK[c] = cur.be[c];
+            seK[c] = cur.se[c];
+        }
+        pK = cur.pA;
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             sH[c][aj][ai] = hA[c];
@@ -694,22 +784,22 @@ __global__ __launch_bounds__(kThreads) void fused_step_f32(fused_args a)
         const int p = k + 1 == a.L2 ? 0 : k + 1;
         const bool p_owned = k + 1 < z1;
         const int64_t offk = (int64_t)k * a.sz, offp = (int64_t)p * a.sz;
-        stage_E(p);
+        cur = nxt;                                   /* bundle of plane p */
+        if (k + 1 < z1) load_bundle(p + 1 == a.L2 ? 0 : p + 1, nxt); /* prefetch plane p+1 */
+        stage_to_lds(cur);
         __syncthreads();
         /* H(k+1) on the grown tile; eA/eB hold old E(k), the -z neighbours */
         float hnA[3], hnB[3], snA[3], snB[3], ecA[3], ecB[3];
-        const int kindA = evalH(ai, aj, ax, ay, p, offp + aq, eA, hnA, snA, ecA);
+        const int kindA = evalH(ai, aj, ax, ay, p, offp + aq, eA, cur.hA, cur.bA, cur.sA, cur.pA, hnA, snA, ecA);
         if (owned && p_owned && kindA != K_SPECIAL) {
             store3(a.Tn, offp + aq, hnA);
             if (kindA == K_PML) store3(a.Sn, offp + aq, snA);
         }
-        if (hasB) evalH(bi, bj, bx, by, p, offp + bq, eB, hnB, snB, ecB);
+        if (hasB) evalH(bi, bj, bx, by, p, offp + bq, eB, cur.hB, cur.bB, cur.sB, cur.pB, hnB, snB, ecB);
 
         /* E(k) on the tile cell: forward curl of the new H */
         const int64_t l = offk + aq;
-        const float be0 = a.b[3][l];
-        if (owned && !__builtin_isnan(be0)) {
-            const float bc[3] = {be0, a.b[4][l], a.b[5][l]};
+        if (owned && !__builtin_isnan(beK[0])) {
             float hc[3], hxp[3], hyp[3];
 #pragma unroll
             for (int c = 0; c < 3; c++) {
@@ -727,14 +817,13 @@ __global__ __launch_bounds__(kThreads) void fused_step_f32(fused_args a)
             P[2] = wx * hxp[1] - hc[1];
             M[2] = -wy * hyp[0] + hc[0];
             float En[3];
-            const bool is_pml = a.pmlflag[ax] | a.pmlflag[a.L0 + ay] | a.pmlflag[a.L0 + a.L1 + k];
-            if (!is_pml) {
+            if (!pK) {
                 if (k == a.iI) {
                     P[0] += a.sgn * (float)plane_source(a.srcH_plane, a.Hinc, a.srcH_f2, a.srcH_re, a.srcH_im, aq, 1);
                     P[1] -= a.sgn * (float)plane_source(a.srcH_plane, a.Hinc, a.srcH_f2, a.srcH_re, a.srcH_im, aq, 0);
                 }
 #pragma unroll
-                for (int c = 0; c < 3; c++) En[c] = eA[c] + bc[c] * (P[c] + M[c]);
+                for (int c = 0; c < 3; c++) En[c] = eA[c] + beK[c] * (P[c] + M[c]);
                 store3(a.Tn + 3, l, En);
                 for (int m = 0; m < 2; m++) { /* monitors: main = total here (S = 0) */
                     if (k != (m == 0 ? a.iB : a.iT)) continue;
@@ -757,9 +846,8 @@ __global__ __launch_bounds__(kThreads) void fused_step_f32(fused_args a)
                     const int Lm = axis_len(a, dm), Ls = axis_len(a, ds);
                     const float aM = tm[2 * Lm + jj[dm]], gM = tm[3 * Lm + jj[dm]];
                     const float aS = ts[2 * Ls + jj[ds]], gS = ts[3 * Ls + jj[ds]];
-                    const float s = a.So[3 + c][l], m = eA[c] - s;
-                    const float s2 = aS * s + bc[c] * gS * M[c];
-                    En[c] = aM * m + bc[c] * gM * P[c] + s2;
+                    const float s2 = aS * seK[c] + beK[c] * gS * M[c];
+                    En[c] = aM * (eA[c] - seK[c]) + beK[c] * gM * P[c] + s2;
                     Sn[c] = s2;
                 }
                 store3(a.Tn + 3, l, En);
@@ -770,7 +858,10 @@ __global__ __launch_bounds__(kThreads) void fused_step_f32(fused_args a)
         for (int c = 0; c < 3; c++) {
             eA[c] = ecA[c];
             if (hasB) eB[c] = ecB[c];
+            beK[c] = cur.be[c];
+            seK[c] = cur.se[c];
         }
+        pK = cur.pA;
         __syncthreads();
 #pragma unroll
         for (int c = 0; c < 3; c++) {
@@ -983,6 +1074,8 @@ int smx_f32_create(smx_f32_state *st, const smx_geom *g, double iR, double iI, c
     }
     st->nspecial = (int)special.size();
 
+    /* both buffers always exist, so the autotuner may switch between the fused step
+     * (buffer cur -> 1 - cur) and the in-place sweeps (on buffer cur) at any time */
     const char *mode = getenv("SMX_F32_KERNEL");
     st->fused = !(mode && !strcmp(mode, "twopass"));
     st->cur = 0;
@@ -996,7 +1089,7 @@ int smx_f32_create(smx_f32_state *st, const smx_geom *g, double iR, double iI, c
         }
     }
     for (int i = 0; i < 6; i++) {
-        for (int k = 0; k < (st->fused ? 2 : 1); k++) {
+        for (int k = 0; k < 2; k++) {
             F32_HIP(hipMalloc(&st->T[k][i], sizeof(float) * n));
             F32_HIP(hipMalloc(&st->S[k][i], sizeof(float) * n));
             F32_HIP(hipMemsetAsync(st->T[k][i], 0, sizeof(float) * n, stream));
@@ -1147,7 +1240,7 @@ int smx_f32_write_fields(smx_f32_state *st, const double *H_t, const double *E_t
     /* both buffers: cells that never update (frozen metal fields) must read the same
      * value from either */
     for (int i = 0; i < 6; i++)
-        for (int k = 0; k < (st->fused ? 2 : 1); k++) {
+        for (int k = 0; k < 2; k++) {
             F32_HIP(hipMemcpyAsync(st->T[k][i], T.data() + (size_t)i * n, sizeof(float) * n, hipMemcpyHostToDevice,
                                    stream));
             F32_HIP(hipMemcpyAsync(st->S[k][i], S.data() + (size_t)i * n, sizeof(float) * n, hipMemcpyHostToDevice,

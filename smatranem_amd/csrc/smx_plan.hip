@@ -802,7 +802,7 @@ int smx_plan_autotune(smx_plan *p, int trial_steps, int choice[3])
     static const int chunks[4] = {8, 16, 32, 64};
     if (!p) return arg_fail("null pointer");
     smx_f32_state &st = p->f32;
-    if (p->g.precision == SMX_F32 && st.fused && trial_steps > 0) {
+    if (p->g.precision == SMX_F32 && trial_steps > 0) {
         const int64_t n = p->n;
         const size_t field = sizeof(float) * (size_t)n;
         float *save = nullptr;
@@ -821,27 +821,32 @@ int smx_plan_autotune(smx_plan *p, int trial_steps, int choice[3])
         SMX_HIP(hipEventCreate(&e0));
         SMX_HIP(hipEventCreate(&e1));
         float best = 1e30f;
-        int best_t = st.tile, best_z = st.fz;
-        for (int t = 0; t < 3; t++)
-            for (int zi = 0; zi < 4; zi++) {
-                st.tile = t;
-                st.fz = chunks[zi];
-                int rc;
-                for (int k = 0; k < 2; k++)
-                    if ((rc = launch_step(p, s, false))) return rc;
-                SMX_HIP(hipEventRecord(e0, p->stream));
-                for (int k = 0; k < trial_steps; k++)
-                    if ((rc = launch_step(p, s, false))) return rc;
-                SMX_HIP(hipEventRecord(e1, p->stream));
-                SMX_HIP(hipEventSynchronize(e1));
-                float ms = 0.f;
-                SMX_HIP(hipEventElapsedTime(&ms, e0, e1));
-                if (ms < best) {
-                    best = ms;
-                    best_t = t;
-                    best_z = chunks[zi];
-                }
+        int best_f = st.fused, best_t = st.tile, best_z = st.fz;
+        /* candidate 0: the in-place sweeps; then every fused shape and z chunk */
+        for (int cand = 0; cand < 1 + 3 * 4; cand++) {
+            st.fused = cand > 0;
+            if (cand > 0) {
+                st.tile = (cand - 1) / 4;
+                st.fz = chunks[(cand - 1) % 4];
             }
+            int rc;
+            for (int k = 0; k < 2; k++)
+                if ((rc = launch_step(p, s, false))) return rc;
+            SMX_HIP(hipEventRecord(e0, p->stream));
+            for (int k = 0; k < trial_steps; k++)
+                if ((rc = launch_step(p, s, false))) return rc;
+            SMX_HIP(hipEventRecord(e1, p->stream));
+            SMX_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            SMX_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best) {
+                best = ms;
+                best_f = st.fused;
+                best_t = st.tile;
+                best_z = st.fz;
+            }
+        }
+        st.fused = best_f;
         st.tile = best_t;
         st.fz = best_z;
         for (int k = 0; k < 2; k++)
