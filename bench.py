@@ -28,6 +28,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "fdtd_f32_step_160x160x320_pmc.txt")
 METRIC = "Gcell-updates/s + % HBM peak, FP32 vs FP64 GPU path at <=1e-3 |S| error"
 
 
@@ -36,6 +37,50 @@ def bytes_per_cell(phi, field_bytes, coef_bytes):
     once (12 words), split/aux slots in a fraction phi of cells (12*phi words), one
     coefficient word per component (6 words)."""
     return field_bytes * (12 + 12 * phi) + 6 * coef_bytes
+
+
+def measured_traffic(kernels):
+    """HBM bytes of one step of `kernels` from the committed rocprofv3 FETCH_SIZE and
+    WRITE_SIZE averages (KiB per dispatch) of the 160x160x320 grid, each scaled by what
+    the counter reads for a plain device copy of known size.  None when a kernel is not
+    listed there."""
+    try:
+        with open(PMC_SUMMARY) as f:
+            lines = f.read().splitlines()
+    except OSError:
+        return None
+    scale, table = None, {}
+    for ln in lines:
+        w = ln.split()
+        if not w or w[0].startswith("#"):
+            continue
+        if w[0] == "scale":
+            scale = (float(w[2]), float(w[4]))
+        else:
+            table[w[0]] = (float(w[1]), float(w[2]))
+    if scale is None or any(k not in table for k in kernels):
+        return None
+    return sum((table[k][0] * scale[0] + table[k][1] * scale[1]) * 1024 for k in kernels)
+
+
+def copy_bandwidth(torch, mib=1024, reps=5):
+    """Achievable HBM bandwidth: a plain device-to-device copy of `mib` MiB, timed with
+    HIP events; bytes read + written per second, GB/s."""
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    ev1.record()
+    torch.cuda.synchronize()
+    gbs = 2 * 4 * n * reps / (ev0.elapsed_time(ev1) * 1e-3) / 1e9
+    del src, dst
+    torch.cuda.empty_cache()
+    return gbs
 
 
 def overlap(prep, H, E):
@@ -178,6 +223,8 @@ def main():
     with prep.plan(fdtd.F32) as p32:
         choice = p32.autotune(20)
         kernel = ("fused_step_f32 %dx%d, z chunk %d" % choice) if choice[0] else "h_sweep_f32 + e_sweep_f32"
+        step_kernels = ["fused_step_f32<%d,%d>" % choice[:2]] if choice[0] else ["h_sweep_f32<32>",
+                                                                                "e_sweep_f32<32>"]
         print(f"[bench] autotuner: {kernel}", file=sys.stderr, flush=True)
         wall_ms, ev_ms = time_plan(p32, K, W, torch)
         H32, E32 = p32.read_fields()
@@ -233,6 +280,10 @@ def main():
                    roofline_frac=round(bb * bcase.n / (evb / kb * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
         print(f"[bench] box-PML fp32: {box['value']:.2f} Gcell/s", file=sys.stderr, flush=True)
 
+    copy_gbs = copy_bandwidth(torch)
+    print(f"[bench] device copy: {copy_gbs:.0f} GB/s", file=sys.stderr, flush=True)
+    traffic = None if args.small else measured_traffic(step_kernels)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(prep, args.cpu_seconds)
@@ -251,7 +302,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "curvilinear split-field FDTD 160x160x320 (configs[1]): Si core "
+        "config": {"workload": "curvilinear split-field FDTD 160x160x320: Si core "
                                "eps 12.1 in eps 2.1, scale factors 1+0.3*g (8 Fourier modes), "
                                "5% PEC edges, 10-cell z PML, PEC x/y walls, modal sheet source, "
                                "reference DFT monitors on 2 port planes",
@@ -259,8 +310,12 @@ def main():
                    "pml_fraction": round(phi, 4), "pec_edges": case.meta.get("pec_edges"),
                    "small": bool(args.small)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": kernel + " (one step)",
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None if traffic is None else round(traffic),
+                     "traffic_unit": "B per step (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-scaled)",
+                     "algorithmic_bytes_per_step": round(b32 * n),
+                     "kernel": kernel + " (one step)", "copy_GBps": round(copy_gbs, 1),
+                     "frac_of_copy": round(achieved / copy_gbs, 4),
                      "bytes_per_cell": round(b32, 2), "kernel_ms_per_step": round(kern_ms, 5)},
         "fp64": fp64,
         "err_vs_fp64": err,
