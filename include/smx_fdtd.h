@@ -179,14 +179,16 @@ int smx_adjoint_forward(smx_plan *plan, long long nsteps, int ckpt_every, int zT
 int smx_adjoint_backward(smx_plan *plan, double *grad);
 
 /*
- * Autotuner of the FP32 step: times the two in-place sweeps and the fused step with
- * each tile shape (32x8, 64x4, 16x16 cells of (j0, j1)) and z chunk (8, 16, 32, 64
- * planes) for trial_steps steps on this plan's grid, keeps the fastest, and restores
- * fields and monitors.  choice (may be NULL) receives the fused tile x, y and z sizes
- * in use, or zeros when the in-place sweeps won (and for FP64 plans); trial_steps = 0
- * only reports.  No counterpart in the reference.
+ * Autotuner of the FP32 step: times the two in-place sweeps and the fused step in each
+ * variant -- tiles of 32x8, 64x4 and 16x16 cells of (j0, j1) staged through registers,
+ * and tiles of 32x8 and 16x16 staged by asynchronous copies to LDS -- with each z chunk
+ * (8, 16, 32, 64 planes) for trial_steps steps on this plan's grid, keeps the fastest,
+ * and restores fields and monitors.  choice (may be NULL) receives {tile x, tile y,
+ * z chunk, variant}: variant 0 = in-place sweeps (sizes 0, also for FP64 plans),
+ * 1 = register-staged fused step, 2 = LDS-staged fused step.  trial_steps = 0 only
+ * reports.  No counterpart in the reference.
  */
-int smx_plan_autotune(smx_plan *plan, int trial_steps, int choice[3]);
+int smx_plan_autotune(smx_plan *plan, int trial_steps, int choice[4]);
 
 /* The plan's HIP stream (a hipStream_t), for callers that order or time their own
  * work against the plan's kernels.  Valid until smx_plan_destroy. */

@@ -420,7 +420,8 @@ f32_args base_args(const smx_f32_state *st, const smx_geom &g)
 struct tile_shape {
     int fx, fy;
 };
-constexpr tile_shape kTiles[3] = {{32, 8}, {64, 4}, {16, 16}};
+constexpr tile_shape kTiles[5] = {{32, 8}, {64, 4}, {16, 16}, {32, 8}, {16, 16}};
+constexpr int kNumTiles = 5, kFirstDmaTile = 3;
 constexpr int kDefaultTile = 0, kDefaultFZ = 16;
 
 struct fused_args {
@@ -436,6 +437,7 @@ struct fused_args {
     const float *So[6];
     float *Sn[6];
     const float *b[6];
+    int64_t ncomp; /* component stride of To, So, b (their arrays are one allocation each) */
     const float *pml;
     const unsigned char *pmlflag;
     const smx_special_f32 *special;
@@ -624,7 +626,10 @@ __device__ inline int fused_h_pre(const fused_args &a, int x, int y, int z, int6
             P[1] -= a.sgn * (float)plane_source(a.srcE_plane, a.Einc, a.srcE_f2, a.srcE_re, a.srcE_im, q, 0);
         }
 #pragma unroll
-        for (int c = 0; c < 3; c++) Hn[c] = h[c] - b[c] * (P[c] + M[c]);
+        for (int c = 0; c < 3; c++) {
+            Hn[c] = h[c] - b[c] * (P[c] + M[c]);
+            Sn[c] = 0.f; /* defined on every path, so the array stays in registers */
+        }
         return K_NORMAL;
     }
     const int jj[3] = {x, y, z};
@@ -871,6 +876,415 @@ __global__ __launch_bounds__(kThreads) void fused_step_f32(fused_args a)
     }
 }
 
+/* ---------------------------------------------------------------------------------- */
+/* Fused step with asynchronous LDS staging                                           */
+/* ---------------------------------------------------------------------------------- */
+/*
+ * The dataflow of fused_step_f32 (H(p) on the grown tile, then E(p-1) on the tile,
+ * marching p over a z chunk), with every per-plane input copied from HBM straight into
+ * LDS by buffer_load_dword ... lds two planes ahead of the plane being computed, into a
+ * ring of three stage slots:
+ *   E    old E on the E halo region (x0-1 .. x0+FX, y0-1 .. y0+FY)
+ *   Ho   old H, bH (and the H split in PML stages) on the grown H tile
+ *   bE   bE (and the E split in PML stages) on the tile
+ * The copies occupy no registers, so a block keeps two planes of loads in flight while it
+ * computes, where fused_step_f32 could hold one plane in registers.
+ *
+ * Ordering: every copy instruction writes 64 consecutive LDS words (lane i -> word i)
+ * and counts in the issuing wave's vmcnt.  Vector memory operations of one wave complete
+ * in issue order, and each wave issues at least kMinOps copies per stage (the list of a
+ * stage is dealt round-robin to the 4 waves), so after issuing stage p+2 a wave that
+ * waits for vmcnt <= 2*kMinOps holds its share of stage p in LDS; the s_barrier that
+ * follows makes the whole stage visible.  Waiting for fewer outstanding operations than
+ * are actually in flight (PML stages carry more copies, stores add more) only waits
+ * longer.  Global stores and the few direct loads (source, monitors, lossy/metal cells)
+ * count in the same vmcnt and the compiler's own waits account for the copies.
+ */
+template <int FX, int FY>
+struct dma_layout {
+    static constexpr int EXW = FX + 2, EYW = FY + 2, NE = EXW * EYW, NEC = (NE + 63) / 64;
+    static constexpr int HXW = FX + 1, HYW = FY + 1, NH = HXW * HYW, NHC = (NH + 63) / 64;
+    static constexpr int NTC = FX * FY / 64;
+    /* word offsets inside a stage slot; each region is [3 components][chunks * 64] */
+    static constexpr int E_OFF = 0;
+    static constexpr int HO_OFF = E_OFF + 3 * NEC * 64;
+    static constexpr int BH_OFF = HO_OFF + 3 * NHC * 64;
+    static constexpr int BE_OFF = BH_OFF + 3 * NHC * 64;
+    static constexpr int SH_OFF = BE_OFF + 3 * NTC * 64;
+    static constexpr int SE_OFF = SH_OFF + 3 * NHC * 64;
+    static constexpr int SLOT = SE_OFF + 3 * NTC * 64;
+    static constexpr int NBASE = 3 * NEC + 6 * NHC + 3 * NTC; /* copies of every stage */
+    static constexpr int NSPLIT = 3 * NHC + 3 * NTC;          /* extra copies of PML stages */
+    static constexpr int kMinOps = NBASE / 4;                 /* per wave, at least */
+    static constexpr int kMaxFZ = 64;
+};
+
+constexpr unsigned kBufWord3 = 0x00020000u; /* raw 32-bit buffer resource (gfx9 family) */
+
+/* Buffer resource over one group of 6 component arrays (6*ncomp floats). */
+__device__ inline __amdgpu_buffer_rsrc_t group_rsrc(const float *base, int64_t ncomp)
+{
+    return __builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, (int)(unsigned)(24 * ncomp), (int)kBufWord3);
+}
+
+#define SMX_LDS(p) ((__attribute__((address_space(3))) void *)(p))
+
+/* Copies of stage slot `slot` (plane byte offset soff) dealt to wave W.  Component c of
+ * a group starts c*cb bytes into the group's buffer. */
+template <int FX, int FY, int W>
+__device__ inline void dma_issue(float *slot, __amdgpu_buffer_rsrc_t rT, __amdgpu_buffer_rsrc_t rB,
+                                 __amdgpu_buffer_rsrc_t rS, unsigned cb, unsigned soff, bool split, const unsigned *vE,
+                                 const unsigned *vH, const unsigned *vT)
+{
+    using Ly = dma_layout<FX, FY>;
+#pragma unroll
+    for (int j = W; j < Ly::NBASE; j += 4) {
+        if (j < 3 * Ly::NEC) {
+            const int c = j / Ly::NEC, ch = j % Ly::NEC;
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rT, SMX_LDS(slot + Ly::E_OFF + c * Ly::NEC * 64 + ch * 64), 4,
+                                                     vE[ch] + (3 + c) * cb, soff, 0, 0);
+        } else if (j < 3 * Ly::NEC + 3 * Ly::NHC) {
+            const int r = j - 3 * Ly::NEC, c = r / Ly::NHC, ch = r % Ly::NHC;
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rT, SMX_LDS(slot + Ly::HO_OFF + c * Ly::NHC * 64 + ch * 64), 4,
+                                                     vH[ch] + c * cb, soff, 0, 0);
+        } else if (j < 3 * Ly::NEC + 6 * Ly::NHC) {
+            const int r = j - 3 * Ly::NEC - 3 * Ly::NHC, c = r / Ly::NHC, ch = r % Ly::NHC;
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rB, SMX_LDS(slot + Ly::BH_OFF + c * Ly::NHC * 64 + ch * 64), 4,
+                                                     vH[ch] + c * cb, soff, 0, 0);
+        } else {
+            const int r = j - 3 * Ly::NEC - 6 * Ly::NHC, c = r / Ly::NTC, ch = r % Ly::NTC;
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rB, SMX_LDS(slot + Ly::BE_OFF + c * Ly::NTC * 64 + ch * 64), 4,
+                                                     vT[ch] + (3 + c) * cb, soff, 0, 0);
+        }
+    }
+    if (split) {
+#pragma unroll
+        for (int j = W; j < Ly::NSPLIT; j += 4) {
+            if (j < 3 * Ly::NHC) {
+                const int c = j / Ly::NHC, ch = j % Ly::NHC;
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(rS, SMX_LDS(slot + Ly::SH_OFF + c * Ly::NHC * 64 + ch * 64),
+                                                         4, vH[ch] + c * cb, soff, 0, 0);
+            } else {
+                const int r = j - 3 * Ly::NHC, c = r / Ly::NTC, ch = r % Ly::NTC;
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(rS, SMX_LDS(slot + Ly::SE_OFF + c * Ly::NTC * 64 + ch * 64),
+                                                         4, vT[ch] + (3 + c) * cb, soff, 0, 0);
+            }
+        }
+    }
+}
+
+/* s_waitcnt vmcnt(VM) lgkmcnt(0), then s_barrier; the empty asm statements keep the
+ * compiler from moving LDS accesses across it. */
+template <int VM>
+__device__ inline void wait_vm_barrier()
+{
+    static_assert(VM >= 0 && VM < 64, "vmcnt field");
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt((VM & 0xF) | ((VM >> 4) << 14) | (0x7 << 4));
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
+/* Wait for every outstanding vector memory operation of this wave.  Used right after
+ * the rare direct loads inside the staged loop (source planes, monitors, lossy/metal
+ * cells): the compiler does not count the asynchronous copies in its own vmcnt
+ * bookkeeping, so a load whose result it still thinks pending at a join would make it
+ * wait for vmcnt(0) -- every copy in flight -- on the common path too. */
+__device__ inline void retire_loads()
+{
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt((0x7 << 4) | (0xF << 8)); /* vmcnt(0) */
+    asm volatile("" ::: "memory");
+}
+
+/* New H of one regular cell from its curl halves: lossless X -= b*(P + M), or the PML
+ * main/split pair with the axis factors pa (A) and pg (B/b) of the cell. */
+__device__ inline void h_regular(const float h[3], const float b[3], const float sp[3], const float P[3],
+                                 const float M[3], bool pml, const float pa[3], const float pg[3], float Hn[3],
+                                 float Sn[3])
+{
+    if (!pml) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            Hn[c] = h[c] - b[c] * (P[c] + M[c]);
+            Sn[c] = 0.f; /* defined on every path, so the array stays in registers */
+        }
+        return;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const int dm = c == 2 ? 0 : c + 1, ds = c == 0 ? 2 : c - 1;
+        const float s2 = pa[ds] * sp[c] - b[c] * pg[ds] * M[c];
+        Hn[c] = pa[dm] * (h[c] - sp[c]) - b[c] * pg[dm] * P[c] + s2;
+        Sn[c] = s2;
+    }
+}
+
+template <int FX, int FY>
+__global__ __launch_bounds__(kThreads) void fused_dma_f32(fused_args a)
+{
+    using Ly = dma_layout<FX, FY>;
+    constexpr int EXW = Ly::EXW, HXW = Ly::HXW, HYW = Ly::HYW;
+    constexpr int NHALO = HXW * HYW - FX * FY;
+    static_assert(FX * FY == kThreads && NHALO <= kThreads, "tile shape");
+    /* One LDS array holds the three stage slots, the new H of the previous plane on the
+     * grown tile (sH) and the chunk's z tables.  It must stay a single variable: with
+     * several LDS variables the compiler starts tracking the asynchronous copies itself
+     * and waits for all of them (vmcnt(0)) before every read of a slot. */
+    constexpr int SH_BASE = 3 * Ly::SLOT, NHG = HYW * HXW;
+    constexpr int ZT_BASE = SH_BASE + 3 * NHG;
+    constexpr int FLAG = ZT_BASE + 5 * (Ly::kMaxFZ + 1);
+    __shared__ float lds[FLAG + 1];
+    auto sH = [&](int c, int j, int i) -> float & { return lds[SH_BASE + c * NHG + j * HXW + i]; };
+    auto zt = [&](int p, int k) -> float & { return lds[ZT_BASE + 5 * p + k]; }; /* aH, gH, aE, gE, flag */
+
+    const int id = xcd_logical(blockIdx.x, a.nblocks);
+    const int bxi = id % a.gx, rest = id / a.gx;
+    const int byi = rest % a.gy, bzi = rest / a.gy;
+    const int x0 = bxi * FX, y0 = byi * FY, z0 = bzi * a.fz;
+    const int z1 = min(z0 + a.fz, a.L2);
+    const int t = threadIdx.x, lane = t & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
+
+    /* per-lane source offsets (bytes within a plane) of every chunk of the three regions;
+     * lanes past a region's end copy a valid cell into the slot's padding */
+    unsigned vE[Ly::NEC], vH[Ly::NHC], vT[Ly::NTC];
+#pragma unroll
+    for (int ch = 0; ch < Ly::NEC; ch++) {
+        const int e = min(ch * 64 + lane, Ly::NE - 1);
+        vE[ch] = 4u * (unsigned)(wrap_index(x0 - 1 + e % EXW, a.L0) + a.L0 * wrap_index(y0 - 1 + e / EXW, a.L1));
+    }
+#pragma unroll
+    for (int ch = 0; ch < Ly::NHC; ch++) {
+        const int e = min(ch * 64 + lane, Ly::NH - 1);
+        vH[ch] = 4u * (unsigned)(wrap_index(x0 + e % HXW, a.L0) + a.L0 * wrap_index(y0 + e / HXW, a.L1));
+    }
+#pragma unroll
+    for (int ch = 0; ch < Ly::NTC; ch++) {
+        const int e = ch * 64 + lane;
+        vT[ch] = 4u * (unsigned)(wrap_index(x0 + e % FX, a.L0) + a.L0 * wrap_index(y0 + e / FX, a.L1));
+    }
+
+    /* this thread's tile cell (slot A) and, for t < NHALO, one grown-tile cell (slot B) */
+    const int ai = t % FX, aj = t / FX;
+    const bool owned = x0 + ai < a.L0 && y0 + aj < a.L1;
+    const int ax = wrap_index(x0 + ai, a.L0), ay = wrap_index(y0 + aj, a.L1);
+    const int64_t aq = ax + (int64_t)a.L0 * ay;
+    const bool hasB = t < NHALO;
+    const int bi = t < HXW ? t : FX, bj = t < HXW ? FY : t - HXW;
+    const int bx = wrap_index(x0 + bi, a.L0), by = wrap_index(y0 + bj, a.L1);
+    const int64_t bq = bx + (int64_t)a.L0 * by;
+
+    /* x/y PML factors of the two cells (H: integer samples, E: half samples) */
+    const float *tx = a.pml, *ty = a.pml + 4 * a.L0;
+    const bool pxyA = a.pmlflag[ax] | a.pmlflag[a.L0 + ay];
+    const bool pxyB = hasB && (a.pmlflag[bx] | a.pmlflag[a.L0 + by]);
+    const float hAx = tx[ax], hGx = tx[a.L0 + ax], eAx = tx[2 * a.L0 + ax], eGx = tx[3 * a.L0 + ax];
+    const float hAy = ty[ay], hGy = ty[a.L1 + ay], eAy = ty[2 * a.L1 + ay], eGy = ty[3 * a.L1 + ay];
+    const float hBx = hasB ? tx[bx] : 1.f, hBgx = hasB ? tx[a.L0 + bx] : 1.f;
+    const float hBy = hasB ? ty[by] : 1.f, hBgy = hasB ? ty[a.L1 + by] : 1.f;
+    /* z tables of the chunk's planes z0 .. z1 (z1 wraps to 0 at the top) */
+    for (int i = t; i <= z1 - z0; i += kThreads) {
+        const int z = z0 + i == a.L2 ? 0 : z0 + i;
+        const float *tz = a.pml + 4 * (a.L0 + a.L1);
+        zt(i, 0) = tz[z];
+        zt(i, 1) = tz[a.L2 + z];
+        zt(i, 2) = tz[2 * a.L2 + z];
+        zt(i, 3) = tz[3 * a.L2 + z];
+        zt(i, 4) = a.pmlflag[a.L0 + a.L1 + z] ? 1.f : 0.f;
+    }
+    /* does any cell of the grown tile lie in x/y PML?  (a vote through the same LDS
+     * array: __syncthreads_or would bring an LDS variable of its own) */
+    if (t == 0) lds[FLAG] = 0.f;
+    __syncthreads();
+    if (pxyA | pxyB) lds[FLAG] = 1.f;
+    __syncthreads();
+    const bool tile_pml = lds[FLAG] != 0.f;
+
+    /* old E of plane z0-1 at the two cells: the -z neighbours of H(z0) */
+    float eA[3], eB[3];
+    {
+        const int zm = z0 == 0 ? a.L2 - 1 : z0 - 1;
+        const int64_t offm = (int64_t)zm * a.sz;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            eA[c] = a.To[3 + c][offm + aq];
+            eB[c] = hasB ? a.To[3 + c][offm + bq] : 0.f;
+        }
+        /* retire these loads before the copies start, so that no wait for them lands
+         * inside the loop */
+        retire_loads();
+    }
+
+    const __amdgpu_buffer_rsrc_t rT = group_rsrc(a.To[0], a.ncomp), rB = group_rsrc(a.b[0], a.ncomp),
+                                 rS = group_rsrc(a.So[0], a.ncomp);
+    const unsigned cb = 4u * (unsigned)a.ncomp;
+    auto issue = [&](int p) { /* stage of chunk plane index p (0 .. z1-z0) into slot p % 3 */
+        const int z = z0 + p == a.L2 ? 0 : z0 + p;
+        const unsigned soff = 4u * (unsigned)((int64_t)z * a.sz);
+        const bool split = tile_pml || zt(p, 4) != 0.f;
+        float *slot = lds + (p % 3) * Ly::SLOT;
+        switch (wv) {
+        case 0: dma_issue<FX, FY, 0>(slot, rT, rB, rS, cb, soff, split, vE, vH, vT); break;
+        case 1: dma_issue<FX, FY, 1>(slot, rT, rB, rS, cb, soff, split, vE, vH, vT); break;
+        case 2: dma_issue<FX, FY, 2>(slot, rT, rB, rS, cb, soff, split, vE, vH, vT); break;
+        default: dma_issue<FX, FY, 3>(slot, rT, rB, rS, cb, soff, split, vE, vH, vT); break;
+        }
+    };
+
+    const int np = z1 - z0; /* planes p = 0 .. np: H at z0+p, E at z0+p-1 */
+    issue(0);
+    if (np >= 1) issue(1);
+
+    float beK[3] = {0.f, 0.f, 0.f}, seK[3] = {0.f, 0.f, 0.f}; /* bE, E split of plane p-1 */
+    bool pK = false;
+    for (int p = 0; p <= np; p++) {
+        if (p + 2 <= np) {
+            issue(p + 2);
+            wait_vm_barrier<2 * Ly::kMinOps>();
+        } else if (p + 1 <= np) {
+            wait_vm_barrier<Ly::kMinOps>();
+        } else {
+            wait_vm_barrier<0>();
+        }
+        const float *S = lds + (p % 3) * Ly::SLOT;
+        const int z = z0 + p == a.L2 ? 0 : z0 + p;
+        const int64_t offp = (int64_t)z * a.sz;
+        const bool zp = zt(p, 4) != 0.f;
+        const float hz[2] = {zt(p, 0), zt(p, 1)};
+
+        /* H(z) on the grown tile */
+        auto evalH = [&](int i, int j, int x, int y, int64_t l, const float ezm[3], bool pml, const float pa[3],
+                         const float pg[3], float Hn[3], float Sn[3], float ecen[3]) {
+            float h[3], b[3], sp[3], exm[3], eym[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const float *Ec = S + Ly::E_OFF + c * Ly::NEC * 64;
+                ecen[c] = Ec[(j + 1) * EXW + i + 1];
+                exm[c] = Ec[(j + 1) * EXW + i];
+                eym[c] = Ec[j * EXW + i + 1];
+                h[c] = S[Ly::HO_OFF + c * Ly::NHC * 64 + j * HXW + i];
+                b[c] = S[Ly::BH_OFF + c * Ly::NHC * 64 + j * HXW + i];
+                sp[c] = pml ? S[Ly::SH_OFF + c * Ly::NHC * 64 + j * HXW + i] : 0.f;
+            }
+            if (__builtin_isnan(b[0])) { /* lossy/metal: computed by special_h_pass */
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    Hn[c] = a.Tn[c][l];
+                    Sn[c] = a.Sn[c][l];
+                }
+                retire_loads();
+                return K_SPECIAL;
+            }
+            const float wx = x == 0 ? a.w[0] : 1.f, wy = y == 0 ? a.w[1] : 1.f, wz = z == 0 ? a.w[2] : 1.f;
+            float P[3], M[3];
+            P[0] = ecen[2] - wy * eym[2];
+            M[0] = -ecen[1] + wz * ezm[1];
+            P[1] = ecen[0] - wz * ezm[0];
+            M[1] = -ecen[2] + wx * exm[2];
+            P[2] = ecen[1] - wx * exm[1];
+            M[2] = -ecen[0] + wy * eym[0];
+            if (!pml && z == a.iI) {
+                const int64_t q = x + (int64_t)a.L0 * y;
+                P[0] += a.sgn * (float)plane_source(a.srcE_plane, a.Einc, a.srcE_f2, a.srcE_re, a.srcE_im, q, 1);
+                P[1] -= a.sgn * (float)plane_source(a.srcE_plane, a.Einc, a.srcE_f2, a.srcE_re, a.srcE_im, q, 0);
+                retire_loads();
+            }
+            h_regular(h, b, sp, P, M, pml, pa, pg, Hn, Sn);
+            return pml ? K_PML : K_NORMAL;
+        };
+        const bool pmlA = pxyA | zp, pmlB = pxyB | zp;
+        float hnA[3], snA[3], hnB[3], snB[3], ecA[3], ecB[3];
+        {
+            const float pa[3] = {hAx, hAy, hz[0]}, pg[3] = {hGx, hGy, hz[1]};
+            const int kind = evalH(ai, aj, ax, ay, offp + aq, eA, pmlA, pa, pg, hnA, snA, ecA);
+            if (owned && p < np && kind != K_SPECIAL) {
+                store3(a.Tn, offp + aq, hnA);
+                if (kind == K_PML) store3(a.Sn, offp + aq, snA);
+            }
+        }
+        if (hasB) {
+            const float pa[3] = {hBx, hBy, hz[0]}, pg[3] = {hBgx, hBgy, hz[1]};
+            evalH(bi, bj, bx, by, offp + bq, eB, pmlB, pa, pg, hnB, snB, ecB);
+        }
+
+        /* E(z-1) on the tile cell: forward curl of the new H */
+        if (p > 0 && owned && !__builtin_isnan(beK[0])) {
+            const int k = z0 + p - 1;
+            const int64_t l = (int64_t)k * a.sz + aq;
+            float hc[3], hxp[3], hyp[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                hc[c] = sH(c, aj, ai);
+                hxp[c] = sH(c, aj, ai + 1);
+                hyp[c] = sH(c, aj + 1, ai);
+            }
+            const float wx = ax + 1 == a.L0 ? a.w[0] : 1.f, wy = ay + 1 == a.L1 ? a.w[1] : 1.f;
+            const float wz = k + 1 == a.L2 ? a.w[2] : 1.f;
+            float P[3], M[3];
+            P[0] = wy * hyp[2] - hc[2];
+            M[0] = -wz * hnA[1] + hc[1];
+            P[1] = wz * hnA[0] - hc[0];
+            M[1] = -wx * hxp[2] + hc[2];
+            P[2] = wx * hxp[1] - hc[1];
+            M[2] = -wy * hyp[0] + hc[0];
+            float En[3];
+            if (!pK) {
+                if (k == a.iI) {
+                    P[0] += a.sgn * (float)plane_source(a.srcH_plane, a.Hinc, a.srcH_f2, a.srcH_re, a.srcH_im, aq, 1);
+                    P[1] -= a.sgn * (float)plane_source(a.srcH_plane, a.Hinc, a.srcH_f2, a.srcH_re, a.srcH_im, aq, 0);
+                    retire_loads();
+                }
+#pragma unroll
+                for (int c = 0; c < 3; c++) En[c] = eA[c] + beK[c] * (P[c] + M[c]);
+                store3(a.Tn + 3, l, En);
+                if (k == a.iB || k == a.iT) { /* monitors: main = total here (S = 0) */
+                    for (int m = 0; m < 2; m++) {
+                        if (k != (m == 0 ? a.iB : a.iT)) continue;
+                        double *HM = m == 0 ? a.HB : a.HT, *EM = m == 0 ? a.EB : a.ET;
+                        for (int v = 0; v < 2; v++) {
+                            const int64_t o = 2 * (v + 2 * aq);
+                            HM[o] += a.monH_re * (double)hc[v];
+                            HM[o + 1] += a.monH_im * (double)hc[v];
+                            EM[o] += a.monE_re * (double)En[v];
+                            EM[o + 1] += a.monE_im * (double)En[v];
+                        }
+                    }
+                    retire_loads();
+                }
+            } else {
+                const float pa[3] = {eAx, eAy, zt(p - 1, 2)}, pg[3] = {eGx, eGy, zt(p - 1, 3)};
+                float Sn[3];
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const int dm = c == 2 ? 0 : c + 1, ds = c == 0 ? 2 : c - 1;
+                    const float s2 = pa[ds] * seK[c] + beK[c] * pg[ds] * M[c];
+                    En[c] = pa[dm] * (eA[c] - seK[c]) + beK[c] * pg[dm] * P[c] + s2;
+                    Sn[c] = s2;
+                }
+                store3(a.Tn + 3, l, En);
+                store3(a.Sn + 3, l, Sn);
+            }
+        }
+        /* carry plane z to the next iteration */
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            eA[c] = ecA[c];
+            if (hasB) eB[c] = ecB[c];
+            beK[c] = S[Ly::BE_OFF + c * Ly::NTC * 64 + t];
+            seK[c] = pmlA ? S[Ly::SE_OFF + c * Ly::NTC * 64 + t] : 0.f;
+        }
+        pK = pmlA;
+        /* every wave is done with stage p and with sH before either is overwritten */
+        wait_vm_barrier<63>();
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            sH(c, aj, ai) = hnA[c];
+            if (hasB) sH(c, bj, bi) = hnB[c];
+        }
+    }
+}
+
 /* H of the lossy/metal cells, before the fused pass (which reads the result for its
  * E update and leaves these cells' H alone). */
 __global__ __launch_bounds__(kThreads) void special_h_pass(fused_args a)
@@ -960,16 +1374,28 @@ __global__ __launch_bounds__(kThreads) void special_e_pass(fused_args a)
 
 const char *smx_f32_last_error(void) { return f32_err; }
 
+int smx_f32_num_tiles(void) { return kNumTiles; }
+
+int smx_f32_tile_ok(const smx_f32_state *st, int tile, int fz)
+{
+    if (tile < 0 || tile >= kNumTiles || fz < 1) return 0;
+    if (tile < kFirstDmaTile) return 1;
+    /* the staged kernel keeps the chunk's z tables in LDS and addresses each group of
+     * six component arrays with 32-bit byte offsets */
+    return fz <= dma_layout<32, 8>::kMaxFZ && 24 * st->n < ((int64_t)1 << 32);
+}
+
 void smx_f32_free(smx_f32_state *st)
 {
+    for (int k = 0; k < 2; k++) {
+        if (st->Tmem[k]) (void)hipFree(st->Tmem[k]);
+        if (st->Smem[k]) (void)hipFree(st->Smem[k]);
+        st->Tmem[k] = st->Smem[k] = nullptr;
+    }
+    if (st->bmem) (void)hipFree(st->bmem);
+    st->bmem = nullptr;
     for (int i = 0; i < 6; i++) {
-        for (int k = 0; k < 2; k++) {
-            if (st->T[k][i]) (void)hipFree(st->T[k][i]);
-            if (st->S[k][i]) (void)hipFree(st->S[k][i]);
-            st->T[k][i] = st->S[k][i] = nullptr;
-        }
-        if (st->b[i]) (void)hipFree(st->b[i]);
-        st->b[i] = nullptr;
+        st->T[0][i] = st->T[1][i] = st->S[0][i] = st->S[1][i] = st->b[i] = nullptr;
     }
     if (st->pml) (void)hipFree(st->pml);
     if (st->pmlflag) (void)hipFree(st->pmlflag);
@@ -1083,21 +1509,25 @@ int smx_f32_create(smx_f32_state *st, const smx_geom *g, double iR, double iI, c
     st->fz = kDefaultFZ;
     if (const char *t = getenv("SMX_F32_TILE")) { /* "<shape 0..2>,<z chunk>", e.g. "1,32" */
         int ti = kDefaultTile, zc = kDefaultFZ;
-        if (sscanf(t, "%d,%d", &ti, &zc) >= 1 && ti >= 0 && ti < 3 && zc >= 1) {
+        if (sscanf(t, "%d,%d", &ti, &zc) >= 1 && smx_f32_tile_ok(st, ti, zc)) {
             st->tile = ti;
             st->fz = zc;
         }
     }
+    for (int k = 0; k < 2; k++) {
+        F32_HIP(hipMalloc(&st->Tmem[k], sizeof(float) * 6 * n));
+        F32_HIP(hipMalloc(&st->Smem[k], sizeof(float) * 6 * n));
+        F32_HIP(hipMemsetAsync(st->Tmem[k], 0, sizeof(float) * 6 * n, stream));
+        F32_HIP(hipMemsetAsync(st->Smem[k], 0, sizeof(float) * 6 * n, stream));
+    }
+    F32_HIP(hipMalloc(&st->bmem, sizeof(float) * 6 * n));
+    F32_HIP(hipMemcpyAsync(st->bmem, bw.data(), sizeof(float) * 6 * n, hipMemcpyHostToDevice, stream));
     for (int i = 0; i < 6; i++) {
         for (int k = 0; k < 2; k++) {
-            F32_HIP(hipMalloc(&st->T[k][i], sizeof(float) * n));
-            F32_HIP(hipMalloc(&st->S[k][i], sizeof(float) * n));
-            F32_HIP(hipMemsetAsync(st->T[k][i], 0, sizeof(float) * n, stream));
-            F32_HIP(hipMemsetAsync(st->S[k][i], 0, sizeof(float) * n, stream));
+            st->T[k][i] = st->Tmem[k] + (size_t)i * n;
+            st->S[k][i] = st->Smem[k] + (size_t)i * n;
         }
-        F32_HIP(hipMalloc(&st->b[i], sizeof(float) * n));
-        F32_HIP(hipMemcpyAsync(st->b[i], bw.data() + (size_t)i * n, sizeof(float) * n, hipMemcpyHostToDevice,
-                               stream));
+        st->b[i] = st->bmem + (size_t)i * n;
     }
     F32_HIP(hipMalloc(&st->pml, sizeof(float) * pml.size()));
     F32_HIP(hipMemcpyAsync(st->pml, pml.data(), sizeof(float) * pml.size(), hipMemcpyHostToDevice, stream));
@@ -1128,6 +1558,7 @@ hipError_t smx_f32_step(smx_f32_state *st, const smx_f32_launch_args *la, hipStr
         a.L0 = g.L0;
         a.L1 = g.L1;
         a.L2 = g.L2;
+        if (!smx_f32_tile_ok(st, st->tile, st->fz)) return hipErrorInvalidValue;
         const tile_shape ts = kTiles[st->tile];
         a.fz = st->fz;
         a.gx = (g.L0 + ts.fx - 1) / ts.fx;
@@ -1148,6 +1579,7 @@ hipError_t smx_f32_step(smx_f32_state *st, const smx_f32_launch_args *la, hipStr
             a.Sn[i] = st->S[1 - st->cur][i];
             a.b[i] = st->b[i];
         }
+        a.ncomp = st->n;
         a.pml = st->pml;
         a.pmlflag = st->pmlflag;
         a.special = st->special;
@@ -1178,6 +1610,8 @@ hipError_t smx_f32_step(smx_f32_state *st, const smx_f32_launch_args *la, hipStr
             if ((e = hipGetLastError()) != hipSuccess) return e;
         }
         switch (st->tile) {
+        case 4: hipLaunchKernelGGL((fused_dma_f32<16, 16>), dim3(a.nblocks), dim3(kThreads), 0, stream, a); break;
+        case 3: hipLaunchKernelGGL((fused_dma_f32<32, 8>), dim3(a.nblocks), dim3(kThreads), 0, stream, a); break;
         case 1: hipLaunchKernelGGL((fused_step_f32<64, 4>), dim3(a.nblocks), dim3(kThreads), 0, stream, a); break;
         case 2: hipLaunchKernelGGL((fused_step_f32<16, 16>), dim3(a.nblocks), dim3(kThreads), 0, stream, a); break;
         default: hipLaunchKernelGGL((fused_step_f32<32, 8>), dim3(a.nblocks), dim3(kThreads), 0, stream, a);

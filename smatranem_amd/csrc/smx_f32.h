@@ -37,6 +37,7 @@ struct smx_f32_state {
     float *T[2][6];         /* ping-pong buffers; [cur] holds the current state */
     float *S[2][6];
     float *b[6];
+    float *Tmem[2], *Smem[2], *bmem; /* one allocation per group: X[i] = Xmem + i*n */
     float *pml;             /* 4*(L0+L1+L2) floats: per axis aH[L], gH[L], aE[L], gE[L] */
     unsigned char *pmlflag; /* L0+L1+L2 */
     smx_special_f32 *special;
@@ -46,7 +47,8 @@ struct smx_f32_state {
     int fused; /* 1: one fused H+E launch per step (+ the special-cell E pass);
                   0: two in-place sweeps */
     int bx;    /* x width of a block of the in-place sweeps (32 or 64) */
-    int tile;  /* fused tile shape: 0 = 32x8, 1 = 64x4, 2 = 16x16 (x by y) */
+    int tile;  /* fused tile shape: 0 = 32x8, 1 = 64x4, 2 = 16x16 (x by y) staged through
+                  registers; 3 = 32x8, 4 = 16x16 staged by asynchronous copies to LDS */
     int fz;    /* fused z chunk */
     /* host copies for the reference-layout field import/export */
     char *cont_host;
@@ -70,3 +72,6 @@ hipError_t smx_f32_step(smx_f32_state *st, const smx_f32_launch_args *a, hipStre
 int smx_f32_write_fields(smx_f32_state *st, const double *H_t, const double *E_t, hipStream_t stream);
 int smx_f32_read_fields(const smx_f32_state *st, double *H_t, double *E_t, hipStream_t stream);
 const char *smx_f32_last_error(void);
+/* Number of fused tile variants, and whether (tile, z chunk) can run on this grid. */
+int smx_f32_num_tiles(void);
+int smx_f32_tile_ok(const smx_f32_state *st, int tile, int fz);

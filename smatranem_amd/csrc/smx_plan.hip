@@ -796,9 +796,10 @@ int smx_adjoint_backward(smx_plan *p, double *grad)
     return rc;
 }
 
-int smx_plan_autotune(smx_plan *p, int trial_steps, int choice[3])
+int smx_plan_autotune(smx_plan *p, int trial_steps, int choice[4])
 {
-    static const int shapes[3][2] = {{32, 8}, {64, 4}, {16, 16}}; /* kern_f32.hip kTiles */
+    /* kern_f32.hip kTiles: 0-2 register-staged, 3-4 staged by asynchronous copies to LDS */
+    static const int shapes[5][2] = {{32, 8}, {64, 4}, {16, 16}, {32, 8}, {16, 16}};
     static const int chunks[4] = {8, 16, 32, 64};
     if (!p) return arg_fail("null pointer");
     smx_f32_state &st = p->f32;
@@ -823,11 +824,12 @@ int smx_plan_autotune(smx_plan *p, int trial_steps, int choice[3])
         float best = 1e30f;
         int best_f = st.fused, best_t = st.tile, best_z = st.fz;
         /* candidate 0: the in-place sweeps; then every fused shape and z chunk */
-        for (int cand = 0; cand < 1 + 3 * 4; cand++) {
+        for (int cand = 0; cand < 1 + smx_f32_num_tiles() * 4; cand++) {
             st.fused = cand > 0;
             if (cand > 0) {
                 st.tile = (cand - 1) / 4;
                 st.fz = chunks[(cand - 1) % 4];
+                if (!smx_f32_tile_ok(&st, st.tile, st.fz)) continue;
             }
             int rc;
             for (int k = 0; k < 2; k++)
@@ -867,6 +869,7 @@ int smx_plan_autotune(smx_plan *p, int trial_steps, int choice[3])
         choice[0] = f ? shapes[st.tile][0] : 0;
         choice[1] = f ? shapes[st.tile][1] : 0;
         choice[2] = f ? st.fz : 0;
+        choice[3] = f ? (st.tile >= 3 ? 2 : 1) : 0;
     }
     return SMX_OK;
 }

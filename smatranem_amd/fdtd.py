@@ -354,8 +354,10 @@ class Plan:
         return out.reshape(3, z1 - z0, y1 - y0, x1 - x0)
 
     def autotune(self, trial_steps=20):
-        """Pick the fastest fused tile shape and z chunk for this grid; returns (fx, fy, fz)."""
-        out = np.zeros(3, dtype=np.int32)
+        """Pick the fastest FP32 step kernel for this grid; returns (fx, fy, fz, variant):
+        variant 0 = the two in-place sweeps (sizes 0), 1 = fused step staged through
+        registers, 2 = fused step staged by asynchronous copies to LDS."""
+        out = np.zeros(4, dtype=np.int32)
         _check(lib().smx_plan_autotune(self._h, int(trial_steps), out))
         return tuple(int(v) for v in out)
 

@@ -21,9 +21,11 @@ def _box(case):
     return (3, L0 - 3, 3, L1 - 3, L2 // 2 - 3, L2 // 2 + 3)
 
 
-@pytest.mark.parametrize("kernel,ckpt", [("fused", 7), ("twopass", 5), ("fused", 1000)])
+@pytest.mark.parametrize("kernel,ckpt", [("fused", 7), ("twopass", 5), ("fused", 1000), ("staged", 7)])
 def test_adjoint_matches_oracle(kernel, ckpt, monkeypatch):
-    monkeypatch.setenv("SMX_F32_KERNEL", kernel)
+    monkeypatch.setenv("SMX_F32_KERNEL", "twopass" if kernel == "twopass" else "fused")
+    if kernel == "staged":
+        monkeypatch.setenv("SMX_F32_TILE", "3,16")
     case = _case()
     prep = casegen.prepare(case)
     zT = case.L[2] - 5

@@ -80,10 +80,13 @@ def test_plan_run_f64_bitwise(name):
     assert _eq(sums, so)
 
 
-@pytest.fixture(params=["fused", "twopass"])
+@pytest.fixture(params=["fused", "staged", "twopass"])
 def f32_kernel(request, monkeypatch):
-    """The FP32 plan's step kernel: one fused launch, or the two in-place sweeps."""
-    monkeypatch.setenv("SMX_F32_KERNEL", request.param)
+    """The FP32 plan's step kernel: one fused launch staged through registers or by
+    asynchronous copies to LDS, or the two in-place sweeps."""
+    monkeypatch.setenv("SMX_F32_KERNEL", "twopass" if request.param == "twopass" else "fused")
+    if request.param == "staged":
+        monkeypatch.setenv("SMX_F32_TILE", "3,16")
     return request.param
 
 
@@ -172,15 +175,17 @@ def test_zero_source_keeps_fields_zero(f32_kernel):
         assert not H.any() and not E.any()
 
 
-@pytest.mark.parametrize("tile", ["0,16", "1,8", "2,32", "0,5"])
+@pytest.mark.parametrize("tile", ["0,16", "1,8", "2,32", "0,5", "3,16", "4,8", "3,5", "4,64", "3,41"])
 def test_fused_tile_shapes(tile, monkeypatch):
-    """Every fused tile shape and z chunk (SMX_F32_TILE) against the FP64 oracle."""
+    """Every fused tile shape and z chunk (SMX_F32_TILE: shapes 0-2 staged through
+    registers, 3-4 by asynchronous copies to LDS) against the FP64 oracle."""
     monkeypatch.setenv("SMX_F32_KERNEL", "fused")
     monkeypatch.setenv("SMX_F32_TILE", tile)
     prep = casegen.prepare(small_waveguide(seed=9, L0=37, L1=19, L2=41, outer=8, pml_xy=2, pml_z=3,
                                            iBc=(1, 0, -1), metal_cells=3, lossy_cells=2, core=(3, 3)))
     with prep.plan(fdtd.F32) as p:
-        assert p.autotune(0)[:2] == {"0": (32, 8), "1": (64, 4), "2": (16, 16)}[tile[0]]
+        shape, fz = (int(v) for v in tile.split(","))
+        assert p.autotune(0) == ((32, 8), (64, 4), (16, 16), (32, 8), (16, 16))[shape] + (fz, 1 + (shape >= 3))
         p.run(0, prep.case.time_period[2])
         H, E = p.read_fields()
     Ho, Eo, _, _ = _oracle_run(prep, 0, prep.case.time_period[2])
@@ -199,7 +204,8 @@ def test_autotune_keeps_state():
         m1 = p.read_monitors()
         assert _eq(H0, H1) and _eq(E0, E1)
         assert all(_eq(a, b) for a, b in zip(m0, m1))
-        assert choice == (0, 0, 0) or (choice[0] * choice[1] == 256 and choice[2] in (8, 16, 32, 64))
+        assert choice == (0, 0, 0, 0) or (choice[0] * choice[1] == 256 and choice[2] in (8, 16, 32, 64)
+                                          and choice[3] in (1, 2))
         p.run(1, prep.case.time_period[2])
         H, E = p.read_fields()
     Ho, Eo, _, _ = _oracle_run(prep, 0, prep.case.time_period[2])
