@@ -126,6 +126,49 @@ def time_plan(plan, steps, warmup, torch):
     return (t1 - t0) * 1e3, ev0.elapsed_time(ev1)
 
 
+def batch_run(count, steps, torch, small=False):
+    """Independent runs of seeded sizes (L0, L1 in [64, 192], L2 in [64, 256], dielectric
+    boxes, 8-cell PML on every face), FP32, each plan autotuned, advanced together by
+    smx_plan_run_batch; then the same plans one after another, for comparison."""
+    from smatranem_amd import casegen, fdtd
+    cases = casegen.batch_cases(count, seed=2025)
+    if small:
+        cases = [casegen.waveguide_case(24 + 8 * i, 20, 40, outer=8, pml_xy=4, pml_z=4, seed=i, blocks=2,
+                                        time_period=(5, 100000, 100010)) for i in range(count)]
+    preps = [casegen.prepare(c) for c in cases]
+    plans = [pr.plan(fdtd.F32) for pr in preps]
+
+    def sync_all():
+        for p in plans:
+            p.sync()
+        torch.cuda.synchronize()
+    try:
+        for p in plans:
+            p.autotune(10)
+        fdtd.run_batch(plans, 0, 20)
+        sync_all()
+        t0 = time.perf_counter()
+        fdtd.run_batch(plans, 20, steps)
+        sync_all()
+        t_batch = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        for p in plans:
+            p.run_steps(20 + steps, steps)
+        sync_all()
+        t_seq = time.perf_counter() - t0
+    finally:
+        for p in plans:
+            p.close()
+    cells = sum(c.n for c in cases)
+    alg = sum(c.n * bytes_per_cell(pr.pml_fraction, 4, 4) for c, pr in zip(cases, preps))
+    return dict(workload="%d independent runs, L0, L1 in [64, 192], L2 in [64, 256], dielectric boxes, "
+                         "8-cell PML on all faces, FP32, smx_plan_run_batch" % count,
+                grids=[list(c.L[:3]) for c in cases], cells=cells, steps=steps,
+                value=round(cells * steps / t_batch / 1e9, 4), unit="Gcell-updates/s",
+                roofline_frac=round(alg * steps / t_batch / 1e9 / HBM_PEAK_GBS, 4),
+                one_after_another=round(cells * steps / t_seq / 1e9, 4))
+
+
 def cpu_baseline(prep, seconds):
     """The reference's main_calc (compiled from its source, oracle/_ref) or, where that
     is not built, the oracle's restatement, single-threaded on this host, over whole
@@ -179,6 +222,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 leg")
     ap.add_argument("--no-box", action="store_true", help="skip the all-faces-PML variant")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batch of independent runs")
+    ap.add_argument("--batch-count", type=int, default=8)
+    ap.add_argument("--batch-steps", type=int, default=400)
     ap.add_argument("--small", action="store_true", help="32x32x48 grid (quick check)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device copies of MIB MiB (scale for rocprofv3 byte counters)")
@@ -285,6 +331,12 @@ def main():
                    roofline_frac=round(bb * bcase.n / (evb / kb * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
         print(f"[bench] box-PML fp32: {box['value']:.2f} Gcell/s", file=sys.stderr, flush=True)
 
+    batch = None
+    if not args.no_batch:
+        batch = batch_run(args.batch_count, args.batch_steps, torch, args.small)
+        print(f"[bench] batch of {args.batch_count}: {batch['value']:.2f} Gcell/s together, "
+              f"{batch['one_after_another']:.2f} one after another", file=sys.stderr, flush=True)
+
     copy_gbs = copy_bandwidth(torch)
     print(f"[bench] device copy: {copy_gbs:.0f} GB/s", file=sys.stderr, flush=True)
     traffic = None if args.small else measured_traffic(step_kernels)
@@ -325,6 +377,7 @@ def main():
         "fp64": fp64,
         "err_vs_fp64": err,
         "pml_all_faces": box,
+        "batch": batch,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

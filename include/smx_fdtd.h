@@ -143,6 +143,18 @@ int smx_plan_run(smx_plan *plan, int P0, int P1, double sums[2]);
  */
 int smx_plan_run_steps(smx_plan *plan, long long step0, long long nsteps, double sums[2]);
 
+/*
+ * Batch: global steps [step0, step0 + nsteps) of every plan in plans[0 .. nplans-1]
+ * (independent runs, each with its own grid, source and monitors), with the same
+ * results as smx_plan_run_steps on each plan in turn.  The steps are issued in
+ * step-major order, each plan on its own stream, so kernels of different plans run
+ * concurrently and small grids share the device.  sums (may be NULL) is double[2*nplans],
+ * the envelope sums of each plan.  On an error return the plans may have advanced by
+ * different numbers of steps.  No counterpart in the reference (its runs are separate
+ * processes, run_all.sh).
+ */
+int smx_plan_run_batch(smx_plan *const *plans, int nplans, long long step0, long long nsteps, double *sums);
+
 /* Wait for all queued work of the plan. */
 int smx_plan_sync(smx_plan *plan);
 

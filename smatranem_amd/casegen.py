@@ -128,7 +128,7 @@ def sheet_source(L0, L1, profile, n_eff=1.0):
 def waveguide_case(L0, L1, L2, outer, pml_xy, pml_z, *, omega=2 * np.pi / 20.0, eps_core=12.1,
                    eps_clad=2.1, core=(8, 6), scale_amp=0.0, n_modes=8, pec_fraction=0.0,
                    seed=0, iBc=(0, 0, 0), dt_factor=1.0, time_period=(2, 1, 8), monitor_gap=2,
-                   metal_cells=0, lossy_cells=0):
+                   metal_cells=0, lossy_cells=0, blocks=0):
     """A straight dielectric core along z in cladding, on an L0 x L1 x L2 grid with
     ``outer`` port slices at each end (B0.., T..), PML layers of ``pml_xy`` cells on the
     x/y faces and ``pml_z`` on the z faces (PMLcalc.c:188-260), optional curvilinear scale
@@ -136,8 +136,9 @@ def waveguide_case(L0, L1, L2, outer, pml_xy, pml_z, *, omega=2 * np.pi / 20.0, 
     normalised to max|g| = 1; fx_c = h_{c+1} h_{c+2} / h_c at each component's
     position, MixedData/BindData.c:223-228), seeded PEC slabs covering about
     ``pec_fraction`` of the E edges of the scatterer, and optional metal ('e'/'h'/'b')
-    and lossy cells.  Monitor planes sit between the z PML and the scatterer, the
-    source plane just above the bottom monitor."""
+    and lossy cells, and ``blocks`` seeded dielectric boxes (eps uniform in [1, 12.1]) in
+    the scatterer.  Monitor planes sit between the z PML and the scatterer, the source
+    plane just above the bottom monitor."""
     rng = np.random.default_rng(seed)
     n = L0 * L1 * L2
     z = np.arange(L2)[:, None, None]
@@ -152,6 +153,17 @@ def waveguide_case(L0, L1, L2, outer, pml_xy, pml_z, *, omega=2 * np.pi / 20.0, 
         px, py, _ = EPS_POS[c]
         inside = (np.abs(x + px - cx - 0.5) <= core[0] / 2) & (np.abs(y + py - cy - 0.5) <= core[1] / 2)
         eps3[..., c] = np.where(np.broadcast_to(inside, (L2, L1, L0)), eps_core, eps_clad)
+
+    # seeded dielectric boxes in the scatterer slices, clear of the x/y PML
+    for _ in range(blocks):
+        bz = rng.integers(outer + 1, max(outer + 2, L2 - outer - 4))
+        by = rng.integers(pml_xy, max(pml_xy + 1, L1 - pml_xy - 4))
+        bx = rng.integers(pml_xy, max(pml_xy + 1, L0 - pml_xy - 4))
+        dz = rng.integers(2, max(3, (L2 - 2 * outer) // 3))
+        dy = rng.integers(2, max(3, (L1 - 2 * pml_xy) // 3))
+        dx = rng.integers(2, max(3, (L0 - 2 * pml_xy) // 3))
+        eps3[bz:min(bz + dz, L2 - outer), by:min(by + dy, L1 - pml_xy), bx:min(bx + dx, L0 - pml_xy), :] = \
+            rng.uniform(1.0, 12.1)
 
     # curvilinear scale factors
     if scale_amp:
@@ -239,6 +251,21 @@ def waveguide_case(L0, L1, L2, outer, pml_xy, pml_z, *, omega=2 * np.pi / 20.0, 
                 dt_factor=dt_factor, iBc=tuple(iBc), iB=iB, iT=iT, iI=iI, inc_mode=1,
                 time_period=tuple(time_period), Hinc=Hinc, Einc=Einc,
                 meta=dict(pec_edges=int(3 * pec_mask.sum()), seed=seed))
+
+
+def batch_cases(count, seed=0, pml=8, time_period=(5, 100000, 100010)):
+    """``count`` independent runs of seeded sizes: L0, L1 uniform in [64, 192], L2 in
+    [64, 256], a waveguide core with 6 seeded dielectric boxes (eps uniform in [1, 12.1]),
+    ``pml``-cell PML on every face, modal sheet source."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        L0, L1 = (int(v) for v in rng.integers(64, 193, size=2))
+        L2 = int(rng.integers(64, 257))
+        out.append(waveguide_case(L0, L1, L2, outer=max(L2 // 8, pml + 4), pml_xy=pml, pml_z=pml,
+                                  omega=2 * np.pi / 20.0, core=(max(L0 // 8, 2), max(L1 // 10, 2)),
+                                  seed=seed * 1000 + i, blocks=6, time_period=time_period))
+    return out
 
 
 @dataclass

@@ -420,8 +420,11 @@ f32_args base_args(const smx_f32_state *st, const smx_geom &g)
 struct tile_shape {
     int fx, fy;
 };
-constexpr tile_shape kTiles[5] = {{32, 8}, {64, 4}, {16, 16}, {32, 8}, {16, 16}};
-constexpr int kNumTiles = 5, kFirstDmaTile = 3;
+/* 0-2: staged through registers; 3-4: staged by copies to LDS two planes ahead, split
+ * slots included; 5-7: the same three or four planes ahead without split slots, for
+ * grids with no PML cell */
+constexpr tile_shape kTiles[8] = {{32, 8}, {64, 4}, {16, 16}, {32, 8}, {16, 16}, {32, 8}, {16, 16}, {32, 8}};
+constexpr int kNumTiles = 8, kFirstDmaTile = 3, kFirstNoSplitTile = 5;
 constexpr int kDefaultTile = 0, kDefaultFZ = 16;
 
 struct fused_args {
@@ -447,6 +450,13 @@ struct fused_args {
     double srcE_f2, srcE_re, srcE_im, srcH_f2, srcH_re, srcH_im;
     double *HB, *EB, *HT, *ET;
     double monH_re, monH_im, monE_re, monE_im;
+    /* box of the staged kernel's tiles: [x0, x1) x [y0, y1) x [z0, z1) (the whole grid
+     * unless the cells outside it are left to the region sweeps) */
+    int box[6];
+    /* boxes of the region sweeps and their first thread index */
+    int nreg;
+    int reg[6][6];
+    int64_t reg_start[7];
 };
 
 __device__ inline int wrap_index(int v, int L)
@@ -540,55 +550,6 @@ __device__ void fused_special_h(const fused_args &a, int x, int y, int z, int64_
 
 /* Kinds of a cell in the fused step. */
 enum { K_NORMAL = 0, K_PML = 1, K_SPECIAL = 2 };
-
-/* New H at (x, y, z) from the old E at the cell (ec) and at its -x, -y, -z neighbours. */
-__device__ inline int fused_h(const fused_args &a, int x, int y, int z, int64_t l, const float ec[3],
-                              const float exm[3], const float eym[3], const float ezm[3], float Hn[3], float Sn[3])
-{
-    const float b0 = a.b[0][l];
-    if (__builtin_isnan(b0)) { /* computed by special_h_pass before this kernel */
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            Hn[c] = a.Tn[c][l];
-            Sn[c] = a.Sn[c][l];
-        }
-        return K_SPECIAL;
-    }
-    const float bc[3] = {b0, a.b[1][l], a.b[2][l]};
-    const float wx = x == 0 ? a.w[0] : 1.f, wy = y == 0 ? a.w[1] : 1.f, wz = z == 0 ? a.w[2] : 1.f;
-    float P[3], M[3];
-    P[0] = ec[2] - wy * eym[2];
-    M[0] = -ec[1] + wz * ezm[1];
-    P[1] = ec[0] - wz * ezm[0];
-    M[1] = -ec[2] + wx * exm[2];
-    P[2] = ec[1] - wx * exm[1];
-    M[2] = -ec[0] + wy * eym[0];
-    const bool is_pml = a.pmlflag[x] | a.pmlflag[a.L0 + y] | a.pmlflag[a.L0 + a.L1 + z];
-    if (!is_pml) {
-        if (z == a.iI) {
-            const int64_t q = x + (int64_t)a.L0 * y;
-            P[0] += a.sgn * (float)plane_source(a.srcE_plane, a.Einc, a.srcE_f2, a.srcE_re, a.srcE_im, q, 1);
-            P[1] -= a.sgn * (float)plane_source(a.srcE_plane, a.Einc, a.srcE_f2, a.srcE_re, a.srcE_im, q, 0);
-        }
-#pragma unroll
-        for (int c = 0; c < 3; c++) Hn[c] = a.To[c][l] - bc[c] * (P[c] + M[c]);
-        return K_NORMAL;
-    }
-    const int jj[3] = {x, y, z};
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-        const int dm = c == 2 ? 0 : c + 1, ds = c == 0 ? 2 : c - 1;
-        const float *tm = axis_tab(a, dm), *ts = axis_tab(a, ds);
-        const int Lm = axis_len(a, dm), Ls = axis_len(a, ds);
-        const float aM = tm[jj[dm]], gM = tm[Lm + jj[dm]];
-        const float aS = ts[jj[ds]], gS = ts[Ls + jj[ds]];
-        const float s = a.So[c][l], m = a.To[c][l] - s;
-        const float s2 = aS * s - bc[c] * gS * M[c];
-        Hn[c] = aM * m - bc[c] * gM * P[c] + s2;
-        Sn[c] = s2;
-    }
-    return K_PML;
-}
 
 __device__ inline void store3(float *const *dst, int64_t l, const float v[3])
 {
@@ -900,7 +861,7 @@ __global__ __launch_bounds__(kThreads) void fused_step_f32(fused_args a)
  * longer.  Global stores and the few direct loads (source, monitors, lossy/metal cells)
  * count in the same vmcnt and the compiler's own waits account for the copies.
  */
-template <int FX, int FY>
+template <int FX, int FY, bool SPLIT = true>
 struct dma_layout {
     static constexpr int EXW = FX + 2, EYW = FY + 2, NE = EXW * EYW, NEC = (NE + 63) / 64;
     static constexpr int HXW = FX + 1, HYW = FY + 1, NH = HXW * HYW, NHC = (NH + 63) / 64;
@@ -912,7 +873,8 @@ struct dma_layout {
     static constexpr int BE_OFF = BH_OFF + 3 * NHC * 64;
     static constexpr int SH_OFF = BE_OFF + 3 * NTC * 64;
     static constexpr int SE_OFF = SH_OFF + 3 * NHC * 64;
-    static constexpr int SLOT = SE_OFF + 3 * NTC * 64;
+    /* a slot carries the split regions only when the launch region may hold PML cells */
+    static constexpr int SLOT = SPLIT ? SE_OFF + 3 * NTC * 64 : SH_OFF;
     static constexpr int NBASE = 3 * NEC + 6 * NHC + 3 * NTC; /* copies of every stage */
     static constexpr int NSPLIT = 3 * NHC + 3 * NTC;          /* extra copies of PML stages */
     static constexpr int kMinOps = NBASE / 4;                 /* per wave, at least */
@@ -931,12 +893,12 @@ __device__ inline __amdgpu_buffer_rsrc_t group_rsrc(const float *base, int64_t n
 
 /* Copies of stage slot `slot` (plane byte offset soff) dealt to wave W.  Component c of
  * a group starts c*cb bytes into the group's buffer. */
-template <int FX, int FY, int W>
+template <int FX, int FY, bool SPLIT, int W>
 __device__ inline void dma_issue(float *slot, __amdgpu_buffer_rsrc_t rT, __amdgpu_buffer_rsrc_t rB,
                                  __amdgpu_buffer_rsrc_t rS, unsigned cb, unsigned soff, bool split, const unsigned *vE,
                                  const unsigned *vH, const unsigned *vT)
 {
-    using Ly = dma_layout<FX, FY>;
+    using Ly = dma_layout<FX, FY, SPLIT>;
 #pragma unroll
     for (int j = W; j < Ly::NBASE; j += 4) {
         if (j < 3 * Ly::NEC) {
@@ -957,7 +919,7 @@ __device__ inline void dma_issue(float *slot, __amdgpu_buffer_rsrc_t rT, __amdgp
                                                      vT[ch] + (3 + c) * cb, soff, 0, 0);
         }
     }
-    if (split) {
+    if (SPLIT && split) {
 #pragma unroll
         for (int j = W; j < Ly::NSPLIT; j += 4) {
             if (j < 3 * Ly::NHC) {
@@ -1020,10 +982,12 @@ __device__ inline void h_regular(const float h[3], const float b[3], const float
     }
 }
 
-template <int FX, int FY>
+template <int FX, int FY, int D, bool SPLIT>
 __global__ __launch_bounds__(kThreads) void fused_dma_f32(fused_args a)
 {
-    using Ly = dma_layout<FX, FY>;
+    using Ly = dma_layout<FX, FY, SPLIT>;
+    constexpr int NSLOT = D + 1;
+    static_assert(D >= 1 && D * Ly::kMinOps < 64, "copies in flight must fit the vmcnt field");
     constexpr int EXW = Ly::EXW, HXW = Ly::HXW, HYW = Ly::HYW;
     constexpr int NHALO = HXW * HYW - FX * FY;
     static_assert(FX * FY == kThreads && NHALO <= kThreads, "tile shape");
@@ -1031,7 +995,7 @@ __global__ __launch_bounds__(kThreads) void fused_dma_f32(fused_args a)
      * grown tile (sH) and the chunk's z tables.  It must stay a single variable: with
      * several LDS variables the compiler starts tracking the asynchronous copies itself
      * and waits for all of them (vmcnt(0)) before every read of a slot. */
-    constexpr int SH_BASE = 3 * Ly::SLOT, NHG = HYW * HXW;
+    constexpr int SH_BASE = NSLOT * Ly::SLOT, NHG = HYW * HXW;
     constexpr int ZT_BASE = SH_BASE + 3 * NHG;
     constexpr int FLAG = ZT_BASE + 5 * (Ly::kMaxFZ + 1);
     __shared__ float lds[FLAG + 1];
@@ -1041,8 +1005,8 @@ __global__ __launch_bounds__(kThreads) void fused_dma_f32(fused_args a)
     const int id = xcd_logical(blockIdx.x, a.nblocks);
     const int bxi = id % a.gx, rest = id / a.gx;
     const int byi = rest % a.gy, bzi = rest / a.gy;
-    const int x0 = bxi * FX, y0 = byi * FY, z0 = bzi * a.fz;
-    const int z1 = min(z0 + a.fz, a.L2);
+    const int x0 = a.box[0] + bxi * FX, y0 = a.box[2] + byi * FY, z0 = a.box[4] + bzi * a.fz;
+    const int z1 = min(z0 + a.fz, a.box[5]);
     const int t = threadIdx.x, lane = t & 63;
     const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
 
@@ -1067,7 +1031,7 @@ __global__ __launch_bounds__(kThreads) void fused_dma_f32(fused_args a)
 
     /* this thread's tile cell (slot A) and, for t < NHALO, one grown-tile cell (slot B) */
     const int ai = t % FX, aj = t / FX;
-    const bool owned = x0 + ai < a.L0 && y0 + aj < a.L1;
+    const bool owned = x0 + ai < a.box[1] && y0 + aj < a.box[3];
     const int ax = wrap_index(x0 + ai, a.L0), ay = wrap_index(y0 + aj, a.L1);
     const int64_t aq = ax + (int64_t)a.L0 * ay;
     const bool hasB = t < NHALO;
@@ -1077,8 +1041,9 @@ __global__ __launch_bounds__(kThreads) void fused_dma_f32(fused_args a)
 
     /* x/y PML factors of the two cells (H: integer samples, E: half samples) */
     const float *tx = a.pml, *ty = a.pml + 4 * a.L0;
-    const bool pxyA = a.pmlflag[ax] | a.pmlflag[a.L0 + ay];
-    const bool pxyB = hasB && (a.pmlflag[bx] | a.pmlflag[a.L0 + by]);
+    /* without split slots the launch region holds no PML cell (checked on the host) */
+    const bool pxyA = SPLIT && (a.pmlflag[ax] | a.pmlflag[a.L0 + ay]);
+    const bool pxyB = SPLIT && hasB && (a.pmlflag[bx] | a.pmlflag[a.L0 + by]);
     const float hAx = tx[ax], hGx = tx[a.L0 + ax], eAx = tx[2 * a.L0 + ax], eGx = tx[3 * a.L0 + ax];
     const float hAy = ty[ay], hGy = ty[a.L1 + ay], eAy = ty[2 * a.L1 + ay], eGy = ty[3 * a.L1 + ay];
     const float hBx = hasB ? tx[bx] : 1.f, hBgx = hasB ? tx[a.L0 + bx] : 1.f;
@@ -1091,7 +1056,7 @@ __global__ __launch_bounds__(kThreads) void fused_dma_f32(fused_args a)
         zt(i, 1) = tz[a.L2 + z];
         zt(i, 2) = tz[2 * a.L2 + z];
         zt(i, 3) = tz[3 * a.L2 + z];
-        zt(i, 4) = a.pmlflag[a.L0 + a.L1 + z] ? 1.f : 0.f;
+        zt(i, 4) = SPLIT && a.pmlflag[a.L0 + a.L1 + z] ? 1.f : 0.f;
     }
     /* does any cell of the grown tile lie in x/y PML?  (a vote through the same LDS
      * array: __syncthreads_or would bring an LDS variable of its own) */
@@ -1119,35 +1084,39 @@ __global__ __launch_bounds__(kThreads) void fused_dma_f32(fused_args a)
     const __amdgpu_buffer_rsrc_t rT = group_rsrc(a.To[0], a.ncomp), rB = group_rsrc(a.b[0], a.ncomp),
                                  rS = group_rsrc(a.So[0], a.ncomp);
     const unsigned cb = 4u * (unsigned)a.ncomp;
-    auto issue = [&](int p) { /* stage of chunk plane index p (0 .. z1-z0) into slot p % 3 */
+    auto issue = [&](int p) { /* stage of chunk plane index p (0 .. z1-z0) into slot p % NSLOT */
         const int z = z0 + p == a.L2 ? 0 : z0 + p;
         const unsigned soff = 4u * (unsigned)((int64_t)z * a.sz);
         const bool split = tile_pml || zt(p, 4) != 0.f;
-        float *slot = lds + (p % 3) * Ly::SLOT;
+        float *slot = lds + (p % NSLOT) * Ly::SLOT;
         switch (wv) {
-        case 0: dma_issue<FX, FY, 0>(slot, rT, rB, rS, cb, soff, split, vE, vH, vT); break;
-        case 1: dma_issue<FX, FY, 1>(slot, rT, rB, rS, cb, soff, split, vE, vH, vT); break;
-        case 2: dma_issue<FX, FY, 2>(slot, rT, rB, rS, cb, soff, split, vE, vH, vT); break;
-        default: dma_issue<FX, FY, 3>(slot, rT, rB, rS, cb, soff, split, vE, vH, vT); break;
+        case 0: dma_issue<FX, FY, SPLIT, 0>(slot, rT, rB, rS, cb, soff, split, vE, vH, vT); break;
+        case 1: dma_issue<FX, FY, SPLIT, 1>(slot, rT, rB, rS, cb, soff, split, vE, vH, vT); break;
+        case 2: dma_issue<FX, FY, SPLIT, 2>(slot, rT, rB, rS, cb, soff, split, vE, vH, vT); break;
+        default: dma_issue<FX, FY, SPLIT, 3>(slot, rT, rB, rS, cb, soff, split, vE, vH, vT); break;
         }
     };
 
     const int np = z1 - z0; /* planes p = 0 .. np: H at z0+p, E at z0+p-1 */
-    issue(0);
-    if (np >= 1) issue(1);
+    for (int p = 0; p < D && p <= np; p++) issue(p);
 
     float beK[3] = {0.f, 0.f, 0.f}, seK[3] = {0.f, 0.f, 0.f}; /* bE, E split of plane p-1 */
     bool pK = false;
     for (int p = 0; p <= np; p++) {
-        if (p + 2 <= np) {
-            issue(p + 2);
-            wait_vm_barrier<2 * Ly::kMinOps>();
-        } else if (p + 1 <= np) {
-            wait_vm_barrier<Ly::kMinOps>();
+        /* stage p is complete once at most (stages issued after it) * kMinOps copies of
+         * this wave are outstanding */
+        if (p + D <= np) {
+            issue(p + D);
+            wait_vm_barrier<D * Ly::kMinOps>();
         } else {
-            wait_vm_barrier<0>();
+            switch (np - p) { /* stages already issued after p: np - p < D */
+            case 0: wait_vm_barrier<0>(); break;
+            case 1: wait_vm_barrier<Ly::kMinOps>(); break;
+            case 2: wait_vm_barrier<(D > 2 ? 2 : 0) * Ly::kMinOps>(); break;
+            default: wait_vm_barrier<(D > 3 ? 3 : 0) * Ly::kMinOps>(); break;
+            }
         }
-        const float *S = lds + (p % 3) * Ly::SLOT;
+        const float *S = lds + (p % NSLOT) * Ly::SLOT;
         const int z = z0 + p == a.L2 ? 0 : z0 + p;
         const int64_t offp = (int64_t)z * a.sz;
         const bool zp = zt(p, 4) != 0.f;
@@ -1165,7 +1134,7 @@ __global__ __launch_bounds__(kThreads) void fused_dma_f32(fused_args a)
                 eym[c] = Ec[j * EXW + i + 1];
                 h[c] = S[Ly::HO_OFF + c * Ly::NHC * 64 + j * HXW + i];
                 b[c] = S[Ly::BH_OFF + c * Ly::NHC * 64 + j * HXW + i];
-                sp[c] = pml ? S[Ly::SH_OFF + c * Ly::NHC * 64 + j * HXW + i] : 0.f;
+                sp[c] = SPLIT && pml ? S[Ly::SH_OFF + c * Ly::NHC * 64 + j * HXW + i] : 0.f;
             }
             if (__builtin_isnan(b[0])) { /* lossy/metal: computed by special_h_pass */
 #pragma unroll
@@ -1272,7 +1241,7 @@ __global__ __launch_bounds__(kThreads) void fused_dma_f32(fused_args a)
             eA[c] = ecA[c];
             if (hasB) eB[c] = ecB[c];
             beK[c] = S[Ly::BE_OFF + c * Ly::NTC * 64 + t];
-            seK[c] = pmlA ? S[Ly::SE_OFF + c * Ly::NTC * 64 + t] : 0.f;
+            seK[c] = SPLIT && pmlA ? S[Ly::SE_OFF + c * Ly::NTC * 64 + t] : 0.f;
         }
         pK = pmlA;
         /* every wave is done with stage p and with sH before either is overwritten */
@@ -1283,6 +1252,147 @@ __global__ __launch_bounds__(kThreads) void fused_dma_f32(fused_args a)
             if (hasB) sH(c, bj, bi) = hnB[c];
         }
     }
+}
+
+/* ---------------------------------------------------------------------------------- */
+/* Region sweeps: the cells outside the staged kernel's box                           */
+/* ---------------------------------------------------------------------------------- */
+/*
+ * On a grid with PML the staged kernel without split slots covers only the box of
+ * cells whose +x, +y, +z neighbours hold no PML cell either (its recomputed halo then
+ * needs no split value).  These two kernels update every other cell from the old
+ * buffer into the new one: region_h_f32 their H (independent of the staged kernel),
+ * then, after both, region_e_f32 their E from the new H of all cells.  Lossy/metal
+ * cells are left to the special passes.  The thread index runs x fastest through each
+ * of up to 6 boxes in turn.
+ */
+__device__ inline bool region_cell(const fused_args &a, int64_t g, int &x, int &y, int &z)
+{
+    if (g >= a.reg_start[a.nreg]) return false;
+    int r = 0;
+    while (r + 1 < a.nreg && g >= a.reg_start[r + 1]) r++;
+    const int *b = a.reg[r];
+    const int64_t o = g - a.reg_start[r];
+    const int wx = b[1] - b[0], wy = b[3] - b[2];
+    x = b[0] + (int)(o % wx);
+    y = b[2] + (int)((o / wx) % wy);
+    z = b[4] + (int)(o / ((int64_t)wx * wy));
+    return true;
+}
+
+/* PML factors of the cell for H (half = 0: integer samples) or E (half = 1). */
+__device__ inline void cell_pml(const fused_args &a, int x, int y, int z, int half, float pa[3], float pg[3])
+{
+    const int jj[3] = {x, y, z};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const float *t = axis_tab(a, d);
+        const int L = axis_len(a, d);
+        pa[d] = t[2 * half * L + jj[d]];
+        pg[d] = t[(2 * half + 1) * L + jj[d]];
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void region_h_f32(fused_args a)
+{
+    int x, y, z;
+    if (!region_cell(a, (int64_t)blockIdx.x * kThreads + threadIdx.x, x, y, z)) return;
+    const int64_t q = x + (int64_t)a.L0 * y, l = q + a.sz * z;
+    const float b0 = a.b[0][l];
+    if (__builtin_isnan(b0)) return; /* special_h_pass */
+    const float b[3] = {b0, a.b[1][l], a.b[2][l]};
+    const nb3 n = nbrs_at(a, x, y, z, false);
+    float ec[3], exm[3], eym[3], ezm[3], h[3], sp[3];
+    const bool pml = a.pmlflag[x] | a.pmlflag[a.L0 + y] | a.pmlflag[a.L0 + a.L1 + z];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        ec[c] = a.To[3 + c][l];
+        exm[c] = a.To[3 + c][l + n.o[0]];
+        eym[c] = a.To[3 + c][l + n.o[1]];
+        ezm[c] = a.To[3 + c][l + n.o[2]];
+        h[c] = a.To[c][l];
+        sp[c] = pml ? a.So[c][l] : 0.f;
+    }
+    const float wx = n.w[0], wy = n.w[1], wz = n.w[2];
+    float P[3], M[3];
+    P[0] = ec[2] - wy * eym[2];
+    M[0] = -ec[1] + wz * ezm[1];
+    P[1] = ec[0] - wz * ezm[0];
+    M[1] = -ec[2] + wx * exm[2];
+    P[2] = ec[1] - wx * exm[1];
+    M[2] = -ec[0] + wy * eym[0];
+    if (!pml && z == a.iI) {
+        P[0] += a.sgn * (float)plane_source(a.srcE_plane, a.Einc, a.srcE_f2, a.srcE_re, a.srcE_im, q, 1);
+        P[1] -= a.sgn * (float)plane_source(a.srcE_plane, a.Einc, a.srcE_f2, a.srcE_re, a.srcE_im, q, 0);
+    }
+    float pa[3], pg[3], Hn[3], Sn[3];
+    cell_pml(a, x, y, z, 0, pa, pg);
+    h_regular(h, b, sp, P, M, pml, pa, pg, Hn, Sn);
+    store3(a.Tn, l, Hn);
+    if (pml) store3(a.Sn, l, Sn);
+}
+
+__global__ __launch_bounds__(kThreads) void region_e_f32(fused_args a)
+{
+    int x, y, z;
+    if (!region_cell(a, (int64_t)blockIdx.x * kThreads + threadIdx.x, x, y, z)) return;
+    const int64_t q = x + (int64_t)a.L0 * y, l = q + a.sz * z;
+    const float b0 = a.b[3][l];
+    if (__builtin_isnan(b0)) return; /* special_e_pass */
+    const float b[3] = {b0, a.b[4][l], a.b[5][l]};
+    const nb3 n = nbrs_at(a, x, y, z, true);
+    float hc[3], hxp[3], hyp[3], hzp[3], e[3], se[3];
+    const bool pml = a.pmlflag[x] | a.pmlflag[a.L0 + y] | a.pmlflag[a.L0 + a.L1 + z];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        hc[c] = a.Tn[c][l];
+        hxp[c] = a.Tn[c][l + n.o[0]];
+        hyp[c] = a.Tn[c][l + n.o[1]];
+        hzp[c] = a.Tn[c][l + n.o[2]];
+        e[c] = a.To[3 + c][l];
+        se[c] = pml ? a.So[3 + c][l] : 0.f;
+    }
+    const float wx = n.w[0], wy = n.w[1], wz = n.w[2];
+    float P[3], M[3];
+    P[0] = wy * hyp[2] - hc[2];
+    M[0] = -wz * hzp[1] + hc[1];
+    P[1] = wz * hzp[0] - hc[0];
+    M[1] = -wx * hxp[2] + hc[2];
+    P[2] = wx * hxp[1] - hc[1];
+    M[2] = -wy * hyp[0] + hc[0];
+    float En[3];
+    if (!pml) {
+        if (z == a.iI) {
+            P[0] += a.sgn * (float)plane_source(a.srcH_plane, a.Hinc, a.srcH_f2, a.srcH_re, a.srcH_im, q, 1);
+            P[1] -= a.sgn * (float)plane_source(a.srcH_plane, a.Hinc, a.srcH_f2, a.srcH_re, a.srcH_im, q, 0);
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) En[c] = e[c] + b[c] * (P[c] + M[c]);
+        store3(a.Tn + 3, l, En);
+        for (int m = 0; m < 2; m++) { /* monitors: main = total here (S = 0) */
+            if (z != (m == 0 ? a.iB : a.iT)) continue;
+            double *HM = m == 0 ? a.HB : a.HT, *EM = m == 0 ? a.EB : a.ET;
+            for (int v = 0; v < 2; v++) {
+                const int64_t o = 2 * (v + 2 * q);
+                HM[o] += a.monH_re * (double)hc[v];
+                HM[o + 1] += a.monH_im * (double)hc[v];
+                EM[o] += a.monE_re * (double)En[v];
+                EM[o + 1] += a.monE_im * (double)En[v];
+            }
+        }
+        return;
+    }
+    float pa[3], pg[3], Sn[3];
+    cell_pml(a, x, y, z, 1, pa, pg);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const int dm = c == 2 ? 0 : c + 1, ds = c == 0 ? 2 : c - 1;
+        const float s2 = pa[ds] * se[c] + b[c] * pg[ds] * M[c];
+        En[c] = pa[dm] * (e[c] - se[c]) + b[c] * pg[dm] * P[c] + s2;
+        Sn[c] = s2;
+    }
+    store3(a.Tn + 3, l, En);
+    store3(a.Sn + 3, l, Sn);
 }
 
 /* H of the lossy/metal cells, before the fused pass (which reads the result for its
@@ -1372,14 +1482,73 @@ __global__ __launch_bounds__(kThreads) void special_e_pass(fused_args a)
 
 } // namespace
 
+/* The box of cells whose +x/+y/+z neighbours are PML-free too, and boxes covering the
+ * rest of the grid (see smx_f32_state). */
+static void plan_split(smx_f32_state *st, const unsigned char *flag)
+{
+    const int L[3] = {st->L[0], st->L[1], st->L[2]};
+    int lo[3], hi[3];
+    st->split_ok = 0;
+    st->nreg = 0;
+    const unsigned char *f = flag;
+    for (int d = 0; d < 3; d++) {
+        int first = -1, last = -1, count = 0;
+        for (int j = 0; j < L[d]; j++)
+            if (!f[j]) {
+                if (first < 0) first = j;
+                last = j;
+                count++;
+            }
+        f += L[d];
+        if (count == L[d]) { /* no PML on this axis: the box spans it, halos wrap */
+            lo[d] = 0;
+            hi[d] = L[d];
+            continue;
+        }
+        if (count == 0 || last - first + 1 != count) return; /* not one PML-free run */
+        lo[d] = first;
+        hi[d] = last; /* exclusive: cell `last` is the box's PML-free +1 neighbour */
+        if (hi[d] <= lo[d]) return;
+    }
+    for (int d = 0; d < 3; d++) {
+        st->fbox[2 * d] = lo[d];
+        st->fbox[2 * d + 1] = hi[d];
+    }
+    auto add = [&](int x0, int x1, int y0, int y1, int z0, int z1) {
+        if (x1 <= x0 || y1 <= y0 || z1 <= z0) return;
+        int *b = st->reg[st->nreg++];
+        b[0] = x0, b[1] = x1, b[2] = y0, b[3] = y1, b[4] = z0, b[5] = z1;
+    };
+    add(0, L[0], 0, L[1], 0, lo[2]);
+    add(0, L[0], 0, L[1], hi[2], L[2]);
+    add(0, L[0], 0, lo[1], lo[2], hi[2]);
+    add(0, L[0], hi[1], L[1], lo[2], hi[2]);
+    add(0, lo[0], lo[1], hi[1], lo[2], hi[2]);
+    add(hi[0], L[0], lo[1], hi[1], lo[2], hi[2]);
+    st->reg_start[0] = 0;
+    for (int r = 0; r < st->nreg; r++) {
+        const int *b = st->reg[r];
+        st->reg_start[r + 1] = st->reg_start[r] + (int64_t)(b[1] - b[0]) * (b[3] - b[2]) * (b[5] - b[4]);
+    }
+    st->split_ok = 1;
+}
+
 const char *smx_f32_last_error(void) { return f32_err; }
 
 int smx_f32_num_tiles(void) { return kNumTiles; }
+
+void smx_f32_tile_shape(int tile, int *fx, int *fy)
+{
+    const bool ok = tile >= 0 && tile < kNumTiles;
+    *fx = ok ? kTiles[tile].fx : 0;
+    *fy = ok ? kTiles[tile].fy : 0;
+}
 
 int smx_f32_tile_ok(const smx_f32_state *st, int tile, int fz)
 {
     if (tile < 0 || tile >= kNumTiles || fz < 1) return 0;
     if (tile < kFirstDmaTile) return 1;
+    if (tile >= kFirstNoSplitTile && st->has_pml && !st->split_ok) return 0;
     /* the staged kernel keeps the chunk's z tables in LDS and addresses each group of
      * six component arrays with 32-bit byte offsets */
     return fz <= dma_layout<32, 8>::kMaxFZ && 24 * st->n < ((int64_t)1 << 32);
@@ -1499,6 +1668,9 @@ int smx_f32_create(smx_f32_state *st, const smx_geom *g, double iR, double iI, c
         bw[5 * (size_t)n + l] = 0.f;
     }
     st->nspecial = (int)special.size();
+    st->has_pml = 0;
+    for (unsigned char f : flag) st->has_pml |= f;
+    plan_split(st, flag.data());
 
     /* both buffers always exist, so the autotuner may switch between the fused step
      * (buffer cur -> 1 - cur) and the in-place sweeps (on buffer cur) at any time */
@@ -1560,10 +1732,20 @@ hipError_t smx_f32_step(smx_f32_state *st, const smx_f32_launch_args *la, hipStr
         a.L2 = g.L2;
         if (!smx_f32_tile_ok(st, st->tile, st->fz)) return hipErrorInvalidValue;
         const tile_shape ts = kTiles[st->tile];
+        /* the staged kernel without split slots covers only the PML-free box on a grid
+         * with PML; the region sweeps take the rest */
+        const bool split = st->tile >= kFirstNoSplitTile && st->has_pml;
+        const int full[6] = {0, g.L0, 0, g.L1, 0, g.L2};
+        for (int i = 0; i < 6; i++) a.box[i] = split ? st->fbox[i] : full[i];
+        if (split) {
+            a.nreg = st->nreg;
+            std::memcpy(a.reg, st->reg, sizeof a.reg);
+            std::memcpy(a.reg_start, st->reg_start, sizeof a.reg_start);
+        }
         a.fz = st->fz;
-        a.gx = (g.L0 + ts.fx - 1) / ts.fx;
-        a.gy = (g.L1 + ts.fy - 1) / ts.fy;
-        a.nblocks = a.gx * a.gy * ((g.L2 + a.fz - 1) / a.fz);
+        a.gx = (a.box[1] - a.box[0] + ts.fx - 1) / ts.fx;
+        a.gy = (a.box[3] - a.box[2] + ts.fy - 1) / ts.fy;
+        a.nblocks = a.gx * a.gy * ((a.box[5] - a.box[4] + a.fz - 1) / a.fz);
         a.sz = (int64_t)g.L0 * g.L1;
         a.w[0] = (float)g.iBc0;
         a.w[1] = (float)g.iBc1;
@@ -1609,14 +1791,26 @@ hipError_t smx_f32_step(smx_f32_state *st, const smx_f32_launch_args *la, hipStr
                                stream, a);
             if ((e = hipGetLastError()) != hipSuccess) return e;
         }
+        const unsigned rblocks = (unsigned)((a.reg_start[a.nreg] + kThreads - 1) / kThreads);
+        if (a.nreg > 0) {
+            hipLaunchKernelGGL(region_h_f32, dim3(rblocks), dim3(kThreads), 0, stream, a);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+        }
         switch (st->tile) {
-        case 4: hipLaunchKernelGGL((fused_dma_f32<16, 16>), dim3(a.nblocks), dim3(kThreads), 0, stream, a); break;
-        case 3: hipLaunchKernelGGL((fused_dma_f32<32, 8>), dim3(a.nblocks), dim3(kThreads), 0, stream, a); break;
+        case 7: hipLaunchKernelGGL((fused_dma_f32<32, 8, 4, false>), dim3(a.nblocks), dim3(kThreads), 0, stream, a); break;
+        case 6: hipLaunchKernelGGL((fused_dma_f32<16, 16, 3, false>), dim3(a.nblocks), dim3(kThreads), 0, stream, a); break;
+        case 5: hipLaunchKernelGGL((fused_dma_f32<32, 8, 3, false>), dim3(a.nblocks), dim3(kThreads), 0, stream, a); break;
+        case 4: hipLaunchKernelGGL((fused_dma_f32<16, 16, 2, true>), dim3(a.nblocks), dim3(kThreads), 0, stream, a); break;
+        case 3: hipLaunchKernelGGL((fused_dma_f32<32, 8, 2, true>), dim3(a.nblocks), dim3(kThreads), 0, stream, a); break;
         case 1: hipLaunchKernelGGL((fused_step_f32<64, 4>), dim3(a.nblocks), dim3(kThreads), 0, stream, a); break;
         case 2: hipLaunchKernelGGL((fused_step_f32<16, 16>), dim3(a.nblocks), dim3(kThreads), 0, stream, a); break;
         default: hipLaunchKernelGGL((fused_step_f32<32, 8>), dim3(a.nblocks), dim3(kThreads), 0, stream, a);
         }
         if ((e = hipGetLastError()) != hipSuccess) return e;
+        if (a.nreg > 0) {
+            hipLaunchKernelGGL(region_e_f32, dim3(rblocks), dim3(kThreads), 0, stream, a);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+        }
         if (a.nspecial > 0) {
             hipLaunchKernelGGL(special_e_pass, dim3((a.nspecial + kThreads - 1) / kThreads), dim3(kThreads), 0,
                                stream, a);

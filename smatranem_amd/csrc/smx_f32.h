@@ -43,12 +43,23 @@ struct smx_f32_state {
     smx_special_f32 *special;
     int64_t *special_cells; /* linear index of each special cell, table order */
     int nspecial;
+    int has_pml; /* any PML cell on the grid */
+    /* split of a grid with PML for the staged kernel without split slots: fbox, the
+     * cells whose +x/+y/+z neighbours hold no PML cell either ([x0, x1) x [y0, y1) x
+     * [z0, z1)), and up to 6 boxes covering the rest (region sweeps); split_ok = 0
+     * when the PML-free cells do not form one box */
+    int split_ok;
+    int fbox[6];
+    int nreg;
+    int reg[6][6];
+    int64_t reg_start[7];
     int cur;
     int fused; /* 1: one fused H+E launch per step (+ the special-cell E pass);
                   0: two in-place sweeps */
     int bx;    /* x width of a block of the in-place sweeps (32 or 64) */
-    int tile;  /* fused tile shape: 0 = 32x8, 1 = 64x4, 2 = 16x16 (x by y) staged through
-                  registers; 3 = 32x8, 4 = 16x16 staged by asynchronous copies to LDS */
+    int tile;  /* fused tile variant (kern_f32.hip kTiles): 0 = 32x8, 1 = 64x4, 2 = 16x16
+                  (x by y) staged through registers; 3-7 staged by asynchronous copies
+                  to LDS (5-7 only on grids without PML cells) */
     int fz;    /* fused z chunk */
     /* host copies for the reference-layout field import/export */
     char *cont_host;
@@ -74,4 +85,5 @@ int smx_f32_read_fields(const smx_f32_state *st, double *H_t, double *E_t, hipSt
 const char *smx_f32_last_error(void);
 /* Number of fused tile variants, and whether (tile, z chunk) can run on this grid. */
 int smx_f32_num_tiles(void);
+void smx_f32_tile_shape(int tile, int *fx, int *fy);
 int smx_f32_tile_ok(const smx_f32_state *st, int tile, int fz);

@@ -541,6 +541,38 @@ int smx_plan_run_steps(smx_plan *p, long long step0, long long nsteps, double su
     return SMX_OK;
 }
 
+int smx_plan_run_batch(smx_plan *const *plans, int nplans, long long step0, long long nsteps, double *sums)
+{
+    if (!plans || nplans <= 0) return arg_fail("empty batch");
+    for (int i = 0; i < nplans; i++) {
+        if (!plans[i]) return arg_fail("null plan in batch");
+        for (int j = 0; j < i; j++)
+            if (plans[j] == plans[i]) return arg_fail("plan listed twice in batch");
+        if (!plans[i]->have_source) {
+            snprintf(g_last_error, sizeof g_last_error, "batch plan %d: run before smx_plan_set_source", i);
+            return SMX_ERR_STATE;
+        }
+    }
+    if (step0 < 0 || nsteps < 0) return arg_fail("negative step");
+    for (long long k = step0; k < step0 + nsteps; k++) {
+        for (int i = 0; i < nplans; i++) {
+            smx_plan *p = plans[i];
+            const int P = (int)(k / p->N), t = (int)(k % p->N);
+            smx_step_scalars s;
+            double fH, fE;
+            int rc = step_scalars(p, P, t, &s, &fH, &fE);
+            if (rc) return rc;
+            if (sums) {
+                sums[2 * i] += fH;
+                sums[2 * i + 1] += fE;
+            }
+            if ((rc = launch_step(p, s, false)) || (rc = dft_after_step(p, k))) return rc;
+            p->next_step = k + 1;
+        }
+    }
+    return SMX_OK;
+}
+
 int smx_plan_run(smx_plan *p, int P0, int P1, double sums[2])
 {
     if (!p) return arg_fail("null pointer");
@@ -798,8 +830,6 @@ int smx_adjoint_backward(smx_plan *p, double *grad)
 
 int smx_plan_autotune(smx_plan *p, int trial_steps, int choice[4])
 {
-    /* kern_f32.hip kTiles: 0-2 register-staged, 3-4 staged by asynchronous copies to LDS */
-    static const int shapes[5][2] = {{32, 8}, {64, 4}, {16, 16}, {32, 8}, {16, 16}};
     static const int chunks[4] = {8, 16, 32, 64};
     if (!p) return arg_fail("null pointer");
     smx_f32_state &st = p->f32;
@@ -866,8 +896,10 @@ int smx_plan_autotune(smx_plan *p, int trial_steps, int choice[4])
     }
     if (choice) {
         const bool f = p->g.precision == SMX_F32 && st.fused;
-        choice[0] = f ? shapes[st.tile][0] : 0;
-        choice[1] = f ? shapes[st.tile][1] : 0;
+        int fx = 0, fy = 0;
+        smx_f32_tile_shape(st.tile, &fx, &fy);
+        choice[0] = f ? fx : 0;
+        choice[1] = f ? fy : 0;
         choice[2] = f ? st.fz : 0;
         choice[3] = f ? (st.tile >= 3 ? 2 : 1) : 0;
     }

@@ -96,6 +96,9 @@ def lib():
     L.smx_adjoint_forward.restype = C.c_int
     L.smx_adjoint_backward.argtypes = [C.c_void_p, _dp]
     L.smx_adjoint_backward.restype = C.c_int
+    L.smx_plan_run_batch.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_longlong, C.c_longlong,
+                                     C.POINTER(C.c_double)]
+    L.smx_plan_run_batch.restype = C.c_int
     L.smx_plan_autotune.argtypes = [C.c_void_p, C.c_int, _ip]
     L.smx_plan_autotune.restype = C.c_int
     L.smx_plan_sync.argtypes = [C.c_void_p]
@@ -218,6 +221,16 @@ def main_calc(L, Ah, Bh, Ae, Be, iBc, H_t, E_t, iBTIin, Hi, Ei, ExpN2, HB, EB, H
                              _i32(iBTIin), _f64(Hi), _f64(Ei), _cplx(ExpN2), HB, EB, HT, ET,
                              _cont(Cont), _i32(address_notn))
     _check(rc)
+
+
+def run_batch(plans, step0, nsteps):
+    """smx_plan_run_batch: steps [step0, step0 + nsteps) of independent plans, each on its
+    own stream, issued step-major.  Returns the envelope sums, float64[len(plans), 2]."""
+    handles = (C.c_void_p * len(plans))(*[p._h.value for p in plans])
+    sums = np.zeros((len(plans), 2))
+    _check(lib().smx_plan_run_batch(handles, len(plans), int(step0), int(nsteps),
+                                    sums.ctypes.data_as(C.POINTER(C.c_double))))
+    return sums
 
 
 @dataclass

@@ -1,0 +1,98 @@
+"""Batched runs (smx_plan_run_batch): independent plans of different grids and
+precisions advanced together give exactly what each plan gives when run alone, and the
+FP64 members stay bit-identical to the oracle."""
+import numpy as np
+import pytest
+
+import oracle
+from smatranem_amd import casegen, fdtd
+from tests.cases import small_waveguide
+
+pytestmark = pytest.mark.gpu
+
+MEMBERS = [
+    (fdtd.F32, dict(seed=21, L0=12, L1=10, L2=28)),
+    (fdtd.F32, dict(seed=22, L0=37, L1=19, L2=41, outer=8, pml_xy=2, pml_z=3, core=(3, 3))),
+    (fdtd.F64, dict(seed=23, L0=9, L1=14, L2=30, outer=7, pml_xy=0, pml_z=3)),
+    (fdtd.F32, dict(seed=24, L0=20, L1=16, L2=33, outer=8, pml_xy=0, pml_z=0, iBc=(1, -1, 1))),
+]
+
+
+def _preps():
+    return [(prec, casegen.prepare(small_waveguide(**kw))) for prec, kw in MEMBERS]
+
+
+def _state(p):
+    H, E = p.read_fields()
+    return H, E, tuple(p.read_monitors())
+
+
+def test_batch_equals_separate_runs():
+    preps = _preps()
+    steps = min(pr.case.time_period[2] * pr.N for _, pr in preps)
+    alone = []
+    for prec, pr in preps:
+        with pr.plan(prec) as p:
+            sums = p.run_steps(0, steps)
+            alone.append(_state(p) + (sums,))
+    plans = [pr.plan(prec) for prec, pr in preps]
+    try:
+        sums = fdtd.run_batch(plans, 0, steps)
+        together = [_state(p) for p in plans]
+    finally:
+        for p in plans:
+            p.close()
+    for i, (a, b) in enumerate(zip(alone, together)):
+        assert np.abs(a[0]).max() > 0, i
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), i
+        for ma, mb in zip(a[2], b[2]):
+            assert np.array_equal(ma, mb), i
+        assert np.array_equal(a[3], sums[i]), i
+
+
+def test_batch_fp64_member_matches_oracle():
+    """Whole periods of the FP64 member, batched in two calls, against the oracle."""
+    preps = _preps()
+    N2 = preps[2][1].N
+    total = min(pr.case.time_period[2] * pr.N for _, pr in preps)
+    periods = total // N2
+    assert periods >= 2
+    plans = [pr.plan(prec) for prec, pr in preps]
+    try:
+        fdtd.run_batch(plans, 0, N2)
+        fdtd.run_batch(plans, N2, (periods - 1) * N2)
+        H, E = plans[2].read_fields()
+        mons = plans[2].read_monitors()
+    finally:
+        for p in plans:
+            p.close()
+    c = preps[2][1].case
+    N, iR, iI, _ = oracle.set_tau(c.omega, c.min_dt / c.dt_factor)
+    st = oracle.setup_from_media(c.L, c.mu, c.eps, iR=iR, iI=iI)
+    Ho, Eo, mo, _ = oracle.run(c.L, st, c.iBc, (c.iB, c.iT, c.iI, c.inc_mode), c.Hinc, c.Einc, N,
+                               c.time_period, 0, periods)
+    assert N == N2
+    assert np.array_equal(H, Ho) and np.array_equal(E, Eo)
+    for a, b in zip(mons, mo):
+        assert np.array_equal(a.view(np.float64), b)
+
+
+def test_batch_errors():
+    preps = _preps()[:2]
+    plans = [pr.plan(prec) for prec, pr in preps]
+    try:
+        with pytest.raises(fdtd.SmxError) as e:
+            fdtd.run_batch([plans[0], plans[0]], 0, 1)
+        assert e.value.status == 1
+        with pytest.raises(fdtd.SmxError) as e:
+            fdtd.run_batch(plans, 0, -1)
+        assert e.value.status == 1
+    finally:
+        for p in plans:
+            p.close()
+    prec, pr = preps[0]
+    g = pr.grid(prec)
+    with fdtd.Plan(g, mu=pr.mu, eps=pr.eps, sigma=pr.sigma, Cont=pr.Cont, address_notn=pr.addr) as bare:
+        with pytest.raises(fdtd.SmxError) as e:
+            fdtd.run_batch([bare], 0, 1)
+        assert e.value.status == 5

@@ -80,13 +80,17 @@ def test_plan_run_f64_bitwise(name):
     assert _eq(sums, so)
 
 
-@pytest.fixture(params=["fused", "staged", "twopass"])
+@pytest.fixture(params=["fused", "staged", "staged-deep", "twopass"])
 def f32_kernel(request, monkeypatch):
     """The FP32 plan's step kernel: one fused launch staged through registers or by
-    asynchronous copies to LDS, or the two in-place sweeps."""
+    asynchronous copies to LDS (two planes ahead with split slots, or three planes
+    ahead without them plus the region sweeps around the PML-free box), or the two
+    in-place sweeps."""
     monkeypatch.setenv("SMX_F32_KERNEL", "twopass" if request.param == "twopass" else "fused")
     if request.param == "staged":
         monkeypatch.setenv("SMX_F32_TILE", "3,16")
+    if request.param == "staged-deep":
+        monkeypatch.setenv("SMX_F32_TILE", "5,16")
     return request.param
 
 
@@ -175,21 +179,64 @@ def test_zero_source_keeps_fields_zero(f32_kernel):
         assert not H.any() and not E.any()
 
 
-@pytest.mark.parametrize("tile", ["0,16", "1,8", "2,32", "0,5", "3,16", "4,8", "3,5", "4,64", "3,41"])
+SHAPES = ((32, 8), (64, 4), (16, 16), (32, 8), (16, 16), (32, 8), (16, 16), (32, 8))
+
+
+@pytest.mark.parametrize("tile", ["0,16", "1,8", "2,32", "0,5", "3,16", "4,8", "3,5", "4,64", "3,41",
+                                  "5,16", "6,8", "7,64", "5,5", "7,41"])
 def test_fused_tile_shapes(tile, monkeypatch):
     """Every fused tile shape and z chunk (SMX_F32_TILE: shapes 0-2 staged through
-    registers, 3-4 by asynchronous copies to LDS) against the FP64 oracle."""
+    registers, 3-4 by asynchronous copies to LDS with split slots, 5-7 without them
+    plus the region sweeps) against the FP64 oracle."""
     monkeypatch.setenv("SMX_F32_KERNEL", "fused")
     monkeypatch.setenv("SMX_F32_TILE", tile)
     prep = casegen.prepare(small_waveguide(seed=9, L0=37, L1=19, L2=41, outer=8, pml_xy=2, pml_z=3,
                                            iBc=(1, 0, -1), metal_cells=3, lossy_cells=2, core=(3, 3)))
     with prep.plan(fdtd.F32) as p:
         shape, fz = (int(v) for v in tile.split(","))
-        assert p.autotune(0) == ((32, 8), (64, 4), (16, 16), (32, 8), (16, 16))[shape] + (fz, 1 + (shape >= 3))
+        assert p.autotune(0) == SHAPES[shape] + (fz, 1 + (shape >= 3))
         p.run(0, prep.case.time_period[2])
         H, E = p.read_fields()
     Ho, Eo, _, _ = _oracle_run(prep, 0, prep.case.time_period[2])
     assert rel_l2(H, Ho) < 2e-4 and rel_l2(E, Eo) < 2e-4, (rel_l2(H, Ho), rel_l2(E, Eo))
+
+
+@pytest.mark.parametrize("tile", ["5,16", "6,8", "7,64", "5,5", "7,41", "3,16"])
+def test_staged_without_pml(tile, monkeypatch):
+    """The staged variants without split slots (tiles 5-7, for grids without PML cells)
+    against the FP64 oracle, on periodic / antiperiodic walls with metal and lossy cells."""
+    monkeypatch.setenv("SMX_F32_KERNEL", "fused")
+    monkeypatch.setenv("SMX_F32_TILE", tile)
+    prep = casegen.prepare(small_waveguide(seed=10, L0=37, L1=19, L2=41, outer=8, pml_xy=0, pml_z=0,
+                                           iBc=(1, -1, 1), metal_cells=3, lossy_cells=2, core=(3, 3)))
+    assert not any(c == ord("a") for c in prep.Cont)
+    with prep.plan(fdtd.F32) as p:
+        shape, fz = (int(v) for v in tile.split(","))
+        assert p.autotune(0) == SHAPES[shape] + (fz, 2)
+        p.run(0, prep.case.time_period[2])
+        H, E = p.read_fields()
+    Ho, Eo, _, _ = _oracle_run(prep, 0, prep.case.time_period[2])
+    assert rel_l2(H, Ho) < 2e-4 and rel_l2(E, Eo) < 2e-4, (rel_l2(H, Ho), rel_l2(E, Eo))
+
+
+@pytest.mark.parametrize("tile", ["5,16", "7,8"])
+def test_region_sweeps_carry_monitor_plane(tile, monkeypatch):
+    """With the monitor planes right next to the z PML, the top one lies outside the
+    staged kernel's PML-free box: the region sweeps accumulate it."""
+    monkeypatch.setenv("SMX_F32_KERNEL", "fused")
+    monkeypatch.setenv("SMX_F32_TILE", tile)
+    prep = casegen.prepare(small_waveguide(seed=12, L0=21, L1=17, L2=36, outer=7, pml_xy=0, pml_z=3,
+                                           monitor_gap=0, iBc=(1, -1, 0), metal_cells=3, lossy_cells=2))
+    assert prep.case.iT == prep.case.L[2] - 4
+    with prep.plan(fdtd.F32) as p:
+        p.run(0, prep.case.time_period[2])
+        H, E = p.read_fields()
+        mons = p.read_monitors()
+    Ho, Eo, mo, _ = _oracle_run(prep, 0, prep.case.time_period[2])
+    assert rel_l2(H, Ho) < 2e-4 and rel_l2(E, Eo) < 2e-4
+    for a, b in zip(mons, mo):
+        assert np.linalg.norm(b) > 0
+        assert rel_l2(a.view(np.float64), b) < 2e-4
 
 
 def test_autotune_keeps_state():
