@@ -269,13 +269,8 @@ def main():
     with prep.plan(fdtd.F32) as p32:
         choice = p32.autotune(20)
         fx, fy, fz, variant = choice
-        fused_name = {1: "fused_step_f32", 2: "fused_dma_f32"}.get(variant)
-        if fused_name:
-            kernel = "%s %dx%d, z chunk %d" % (fused_name, fx, fy, fz)
-            step_kernels = ["%s<%d,%d>" % (fused_name, fx, fy)]
-        else:
-            kernel = "h_sweep_f32 + e_sweep_f32"
-            step_kernels = ["h_sweep_f32<32>", "e_sweep_f32<32>"]
+        step_kernels = p32.kernels.split(" + ")
+        kernel = p32.kernels + (", z chunk %d" % fz if variant else "")
         print(f"[bench] autotuner: {kernel}", file=sys.stderr, flush=True)
         wall_ms, ev_ms = time_plan(p32, K, W, torch)
         H32, E32 = p32.read_fields()

@@ -202,6 +202,11 @@ int smx_adjoint_backward(smx_plan *plan, double *grad);
  */
 int smx_plan_autotune(smx_plan *plan, int trial_steps, int choice[4]);
 
+/* Names of the kernels one step of the plan launches with its current kernel choice,
+ * " + "-separated, as profilers list them (lossy/metal passes not included); a static
+ * string.  No counterpart in the reference. */
+const char *smx_plan_kernels(const smx_plan *plan);
+
 /* The plan's HIP stream (a hipStream_t), for callers that order or time their own
  * work against the plan's kernels.  Valid until smx_plan_destroy. */
 void *smx_plan_stream(const smx_plan *plan);

@@ -421,10 +421,12 @@ struct tile_shape {
     int fx, fy;
 };
 /* 0-2: staged through registers; 3-4: staged by copies to LDS two planes ahead, split
- * slots included; 5-7: the same three or four planes ahead without split slots, for
- * grids with no PML cell */
-constexpr tile_shape kTiles[8] = {{32, 8}, {64, 4}, {16, 16}, {32, 8}, {16, 16}, {32, 8}, {16, 16}, {32, 8}};
-constexpr int kNumTiles = 8, kFirstDmaTile = 3, kFirstNoSplitTile = 5;
+ * slots included; 5-9: without split slots (on a grid with PML: the PML-free box, the
+ * region sweeps doing the rest), 5-6 three planes ahead, 7 four, 8 two and 9 one (the
+ * shallower rings leave room for three or four blocks per CU) */
+constexpr tile_shape kTiles[10] = {{32, 8}, {64, 4}, {16, 16}, {32, 8}, {16, 16},
+                                   {32, 8}, {16, 16}, {32, 8}, {32, 8}, {32, 8}};
+constexpr int kNumTiles = 10, kFirstDmaTile = 3, kFirstNoSplitTile = 5;
 constexpr int kDefaultTile = 0, kDefaultFZ = 16;
 
 struct fused_args {
@@ -1272,11 +1274,13 @@ __device__ inline bool region_cell(const fused_args &a, int64_t g, int &x, int &
     int r = 0;
     while (r + 1 < a.nreg && g >= a.reg_start[r + 1]) r++;
     const int *b = a.reg[r];
-    const int64_t o = g - a.reg_start[r];
-    const int wx = b[1] - b[0], wy = b[3] - b[2];
-    x = b[0] + (int)(o % wx);
-    y = b[2] + (int)((o / wx) % wy);
-    z = b[4] + (int)(o / ((int64_t)wx * wy));
+    /* 32-bit arithmetic: a box holds far fewer than 2^32 cells */
+    const unsigned o = (unsigned)(g - a.reg_start[r]);
+    const unsigned wx = (unsigned)(b[1] - b[0]), wy = (unsigned)(b[3] - b[2]);
+    const unsigned row = o / wx;
+    x = b[0] + (int)(o - row * wx);
+    y = b[2] + (int)(row % wy);
+    z = b[4] + (int)(row / wy);
     return true;
 }
 
@@ -1528,7 +1532,9 @@ static void plan_split(smx_f32_state *st, const unsigned char *flag)
     st->reg_start[0] = 0;
     for (int r = 0; r < st->nreg; r++) {
         const int *b = st->reg[r];
-        st->reg_start[r + 1] = st->reg_start[r] + (int64_t)(b[1] - b[0]) * (b[3] - b[2]) * (b[5] - b[4]);
+        const int64_t cells = (int64_t)(b[1] - b[0]) * (b[3] - b[2]) * (b[5] - b[4]);
+        if (cells >= ((int64_t)1 << 32)) return; /* region_cell decodes with 32-bit offsets */
+        st->reg_start[r + 1] = st->reg_start[r] + cells;
     }
     st->split_ok = 1;
 }
@@ -1536,6 +1542,25 @@ static void plan_split(smx_f32_state *st, const unsigned char *flag)
 const char *smx_f32_last_error(void) { return f32_err; }
 
 int smx_f32_num_tiles(void) { return kNumTiles; }
+
+const char *smx_f32_kernel_name(const smx_f32_state *st)
+{
+    static const char *const fused[kNumTiles] = {
+        "fused_step_f32<32,8>", "fused_step_f32<64,4>", "fused_step_f32<16,16>",
+        "fused_dma_f32<32,8,2,true>", "fused_dma_f32<16,16,2,true>", "fused_dma_f32<32,8,3,false>",
+        "fused_dma_f32<16,16,3,false>", "fused_dma_f32<32,8,4,false>", "fused_dma_f32<32,8,2,false>",
+        "fused_dma_f32<32,8,1,false>"};
+    static const char *const split[kNumTiles] = {
+        nullptr, nullptr, nullptr, nullptr, nullptr,
+        "region_h_f32 + fused_dma_f32<32,8,3,false> + region_e_f32",
+        "region_h_f32 + fused_dma_f32<16,16,3,false> + region_e_f32",
+        "region_h_f32 + fused_dma_f32<32,8,4,false> + region_e_f32",
+        "region_h_f32 + fused_dma_f32<32,8,2,false> + region_e_f32",
+        "region_h_f32 + fused_dma_f32<32,8,1,false> + region_e_f32"};
+    if (!st->fused) return st->bx == 64 ? "h_sweep_f32<64> + e_sweep_f32<64>" : "h_sweep_f32<32> + e_sweep_f32<32>";
+    if (st->tile < 0 || st->tile >= kNumTiles) return "";
+    return st->tile >= kFirstNoSplitTile && st->has_pml ? split[st->tile] : fused[st->tile];
+}
 
 void smx_f32_tile_shape(int tile, int *fx, int *fy)
 {
@@ -1797,6 +1822,8 @@ hipError_t smx_f32_step(smx_f32_state *st, const smx_f32_launch_args *la, hipStr
             if ((e = hipGetLastError()) != hipSuccess) return e;
         }
         switch (st->tile) {
+        case 9: hipLaunchKernelGGL((fused_dma_f32<32, 8, 1, false>), dim3(a.nblocks), dim3(kThreads), 0, stream, a); break;
+        case 8: hipLaunchKernelGGL((fused_dma_f32<32, 8, 2, false>), dim3(a.nblocks), dim3(kThreads), 0, stream, a); break;
         case 7: hipLaunchKernelGGL((fused_dma_f32<32, 8, 4, false>), dim3(a.nblocks), dim3(kThreads), 0, stream, a); break;
         case 6: hipLaunchKernelGGL((fused_dma_f32<16, 16, 3, false>), dim3(a.nblocks), dim3(kThreads), 0, stream, a); break;
         case 5: hipLaunchKernelGGL((fused_dma_f32<32, 8, 3, false>), dim3(a.nblocks), dim3(kThreads), 0, stream, a); break;

@@ -86,4 +86,7 @@ const char *smx_f32_last_error(void);
 /* Number of fused tile variants, and whether (tile, z chunk) can run on this grid. */
 int smx_f32_num_tiles(void);
 void smx_f32_tile_shape(int tile, int *fx, int *fy);
+/* Kernels of one step in the current choice, " + "-separated (the lossy/metal passes
+ * of the fused variants not included). */
+const char *smx_f32_kernel_name(const smx_f32_state *st);
 int smx_f32_tile_ok(const smx_f32_state *st, int tile, int fz);

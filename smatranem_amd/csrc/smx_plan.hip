@@ -586,6 +586,13 @@ int smx_plan_run(smx_plan *p, int P0, int P1, double sums[2])
 
 void *smx_plan_stream(const smx_plan *p) { return p ? (void *)p->stream : nullptr; }
 
+const char *smx_plan_kernels(const smx_plan *p)
+{
+    if (!p) return "";
+    if (p->g.precision == SMX_F64) return "h_sweep_f64 + e_sweep_f64";
+    return smx_f32_kernel_name(&p->f32);
+}
+
 int smx_plan_set_dft(smx_plan *p, int nplanes, const int *planes, int nfreq, const double *w_per_step,
                      int chunk)
 {

@@ -96,6 +96,8 @@ def lib():
     L.smx_adjoint_forward.restype = C.c_int
     L.smx_adjoint_backward.argtypes = [C.c_void_p, _dp]
     L.smx_adjoint_backward.restype = C.c_int
+    L.smx_plan_kernels.argtypes = [C.c_void_p]
+    L.smx_plan_kernels.restype = C.c_char_p
     L.smx_plan_run_batch.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_longlong, C.c_longlong,
                                      C.POINTER(C.c_double)]
     L.smx_plan_run_batch.restype = C.c_int
@@ -373,6 +375,11 @@ class Plan:
         out = np.zeros(4, dtype=np.int32)
         _check(lib().smx_plan_autotune(self._h, int(trial_steps), out))
         return tuple(int(v) for v in out)
+
+    @property
+    def kernels(self):
+        """Names of the kernels one step launches with the current kernel choice."""
+        return lib().smx_plan_kernels(self._h).decode()
 
     @property
     def stream(self):

@@ -179,11 +179,11 @@ def test_zero_source_keeps_fields_zero(f32_kernel):
         assert not H.any() and not E.any()
 
 
-SHAPES = ((32, 8), (64, 4), (16, 16), (32, 8), (16, 16), (32, 8), (16, 16), (32, 8))
+SHAPES = ((32, 8), (64, 4), (16, 16), (32, 8), (16, 16), (32, 8), (16, 16), (32, 8), (32, 8), (32, 8))
 
 
 @pytest.mark.parametrize("tile", ["0,16", "1,8", "2,32", "0,5", "3,16", "4,8", "3,5", "4,64", "3,41",
-                                  "5,16", "6,8", "7,64", "5,5", "7,41"])
+                                  "5,16", "6,8", "7,64", "5,5", "7,41", "8,16", "9,32", "9,3"])
 def test_fused_tile_shapes(tile, monkeypatch):
     """Every fused tile shape and z chunk (SMX_F32_TILE: shapes 0-2 staged through
     registers, 3-4 by asynchronous copies to LDS with split slots, 5-7 without them
@@ -201,7 +201,7 @@ def test_fused_tile_shapes(tile, monkeypatch):
     assert rel_l2(H, Ho) < 2e-4 and rel_l2(E, Eo) < 2e-4, (rel_l2(H, Ho), rel_l2(E, Eo))
 
 
-@pytest.mark.parametrize("tile", ["5,16", "6,8", "7,64", "5,5", "7,41", "3,16"])
+@pytest.mark.parametrize("tile", ["5,16", "6,8", "7,64", "5,5", "7,41", "3,16", "8,8", "9,16"])
 def test_staged_without_pml(tile, monkeypatch):
     """The staged variants without split slots (tiles 5-7, for grids without PML cells)
     against the FP64 oracle, on periodic / antiperiodic walls with metal and lossy cells."""
