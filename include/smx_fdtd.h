@@ -197,8 +197,10 @@ int smx_adjoint_backward(smx_plan *plan, double *grad);
  * (8, 16, 32, 64 planes) for trial_steps steps on this plan's grid, keeps the fastest,
  * and restores fields and monitors.  choice (may be NULL) receives {tile x, tile y,
  * z chunk, variant}: variant 0 = in-place sweeps (sizes 0, also for FP64 plans),
- * 1 = register-staged fused step, 2 = LDS-staged fused step.  trial_steps = 0 only
- * reports.  No counterpart in the reference.
+ * 1 = register-staged fused step, 2 = LDS-staged fused step, 3 = two steps per pass
+ * (grids without PML and lossy/metal cells; used for pairs of steps of the source loop
+ * while the DFT is off, single steps take the best one-step kernel).  trial_steps = 0
+ * only reports.  No counterpart in the reference.
  */
 int smx_plan_autotune(smx_plan *plan, int trial_steps, int choice[4]);
 

@@ -452,6 +452,9 @@ struct fused_args {
     double srcE_f2, srcE_re, srcE_im, srcH_f2, srcH_re, srcH_im;
     double *HB, *EB, *HT, *ET;
     double monH_re, monH_im, monE_re, monE_im;
+    /* the second step's scalars (two-step kernel) */
+    double srcE2_f2, srcE2_re, srcE2_im, srcH2_f2, srcH2_re, srcH2_im;
+    double monH2_re, monH2_im, monE2_re, monE2_im;
     /* box of the staged kernel's tiles: [x0, x1) x [y0, y1) x [z0, z1) (the whole grid
      * unless the cells outside it are left to the region sweeps) */
     int box[6];
@@ -1257,6 +1260,322 @@ __global__ __launch_bounds__(kThreads) void fused_dma_f32(fused_args a)
 }
 
 /* ---------------------------------------------------------------------------------- */
+/* Two steps per pass: temporal blocking of the staged z-march                        */
+/* ---------------------------------------------------------------------------------- */
+/*
+ * The step is bound by HBM (one step reads 12 words per cell and writes 6), so the way
+ * past the streaming ceiling is to read and write the fields once per two steps.  A
+ * block owns an FX x FY tile and a z chunk [z0, z1), and marches planes p = z0-1 ..
+ * z1+1 carrying two time levels, skewed by one plane:
+ *   a. H1(p)   on G3 from E0(p), E0(p-1)            (first step, H)
+ *   b. E1(p-1) on G2 from H1(p-1), H1(p)            (first step, E)
+ *   c. H2(p-1) on G1 from E1(p-1), E1(p-2)          (second step, H)
+ *   d. E2(p-2) on T  from H2(p-2), H2(p-1)          (second step, E)
+ * with T the tile, G1 = T grown by one cell on +x/+y, G2 = G1 grown by one on -x/-y,
+ * G3 = G2 grown by one on +x/+y, G4 = G3 grown by one on -x/-y (the stage region).
+ * Only H2 and E2 of the tile reach HBM; the halo levels are recomputed by every tile
+ * that needs them, and the chunk ends recompute two planes each side.  Per plane the
+ * old E, H, bH, bE of G4 come in by asynchronous copies to LDS (as in fused_dma_f32);
+ * H1, E1, H2 of the neighbours go through LDS, the -z / +z neighbours of a thread's own
+ * cells stay in its registers.
+ *
+ * This variant covers grids without PML and without lossy/metal cells (the host checks):
+ * every cell takes the lossless update X -/+= b*curl.  Source terms (both steps) and the
+ * reference monitors (both steps) are applied as in the single-step kernels.
+ */
+template <int FX, int FY>
+struct two_layout {
+    static constexpr int W4 = FX + 4, H4 = FY + 4, N4 = W4 * H4, NC = (N4 + 63) / 64;
+    static constexpr int ARR = 3 * NC * 64;   /* one array (3 components) of a stage */
+    static constexpr int SLOT = 4 * ARR;      /* E0, H0, bH, bE */
+    static constexpr int NBASE = 4 * 3 * NC;  /* copies per stage */
+    static constexpr int kMinOps = NBASE / 4; /* per wave, at least */
+    static constexpr int CELLS = (N4 + kThreads - 1) / kThreads; /* cells per thread */
+};
+
+/* Copies of one stage dealt to wave W: array order E0, H0, bH, bE, each 3 components x
+ * NC chunks of the G4 region (lane i -> word i of the chunk). */
+template <int FX, int FY, int W>
+__device__ inline void two_issue(float *slot, __amdgpu_buffer_rsrc_t rT, __amdgpu_buffer_rsrc_t rB, unsigned cb,
+                                 unsigned soff, const unsigned *vq)
+{
+    using Ly = two_layout<FX, FY>;
+#pragma unroll
+    for (int j = W; j < Ly::NBASE; j += 4) {
+        const int arr = j / (3 * Ly::NC), r = j % (3 * Ly::NC), c = r / Ly::NC, ch = r % Ly::NC;
+        /* E0 = T comp 3+c, H0 = T comp c, bH = b comp c, bE = b comp 3+c */
+        const unsigned comp = (arr == 0 || arr == 3) ? 3 + c : c;
+        float *dst = slot + arr * Ly::ARR + c * Ly::NC * 64 + ch * 64;
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(arr < 2 ? rT : rB, SMX_LDS(dst), 4, vq[ch] + comp * cb, soff, 0, 0);
+    }
+}
+
+template <int FX, int FY, int D>
+__global__ __launch_bounds__(kThreads) void two_step_f32(fused_args a)
+{
+    using Ly = two_layout<FX, FY>;
+    constexpr int W4 = Ly::W4, H4 = Ly::H4, N4 = Ly::N4, NSLOT = D + 1, CELLS = Ly::CELLS;
+    static_assert(FX * FY == kThreads && CELLS == 2, "tile shape");
+    static_assert(D >= 1 && D * Ly::kMinOps < 64, "copies in flight must fit the vmcnt field");
+    /* one LDS variable (see fused_dma_f32): stage slots, then H1, E1, H2 of the previous
+     * plane in G4 layout */
+    constexpr int L_H1 = NSLOT * Ly::SLOT, L_E1 = L_H1 + 3 * N4, L_H2 = L_E1 + 3 * N4;
+    __shared__ float lds[L_H2 + 3 * N4];
+
+    const int id = xcd_logical(blockIdx.x, a.nblocks);
+    const int bxi = id % a.gx, rest = id / a.gx;
+    const int byi = rest % a.gy, bzi = rest / a.gy;
+    const int x0 = a.box[0] + bxi * FX, y0 = a.box[2] + byi * FY, z0 = a.box[4] + bzi * a.fz;
+    const int z1 = min(z0 + a.fz, a.box[5]);
+    const int t = threadIdx.x, lane = t & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
+
+    unsigned vq[Ly::NC];
+#pragma unroll
+    for (int ch = 0; ch < Ly::NC; ch++) {
+        const int e = min(ch * 64 + lane, N4 - 1);
+        vq[ch] = 4u * (unsigned)(wrap_index(x0 - 2 + e % W4, a.L0) + a.L0 * wrap_index(y0 - 2 + e / W4, a.L1));
+    }
+
+    /* this thread's cells of G4: e = t and t + 256 */
+    int ce[CELLS], cx[CELLS], cy[CELLS];
+    bool inG3[CELLS], inG2[CELLS], inG1[CELLS], inT[CELLS];
+    int64_t cq[CELLS];
+#pragma unroll
+    for (int r = 0; r < CELLS; r++) {
+        const int e = t + r * kThreads;
+        const bool valid = e < N4;
+        const int gx = e % W4, gy = e / W4;
+        ce[r] = valid ? e : 0;
+        cx[r] = wrap_index(x0 - 2 + gx, a.L0);
+        cy[r] = wrap_index(y0 - 2 + gy, a.L1);
+        cq[r] = cx[r] + (int64_t)a.L0 * cy[r];
+        inG3[r] = valid && gx >= 1 && gy >= 1;
+        inG2[r] = inG3[r] && gx < W4 - 1 && gy < H4 - 1;
+        inG1[r] = inG2[r] && gx >= 2 && gy >= 2;
+        inT[r] = inG1[r] && gx < W4 - 2 && gy < H4 - 2 && x0 - 2 + gx < a.box[1] && y0 - 2 + gy < a.box[3];
+    }
+
+    const __amdgpu_buffer_rsrc_t rT = group_rsrc(a.To[0], a.ncomp), rB = group_rsrc(a.b[0], a.ncomp);
+    const unsigned cb = 4u * (unsigned)a.ncomp;
+    auto plane_of = [&](int s) { return wrap_index(z0 - 2 + s, a.L2); }; /* stage s: plane z0-2+s */
+    auto issue = [&](int s) {
+        const unsigned soff = 4u * (unsigned)((int64_t)plane_of(s) * a.sz);
+        float *slot = lds + (s % NSLOT) * Ly::SLOT;
+        switch (wv) {
+        case 0: two_issue<FX, FY, 0>(slot, rT, rB, cb, soff, vq); break;
+        case 1: two_issue<FX, FY, 1>(slot, rT, rB, cb, soff, vq); break;
+        case 2: two_issue<FX, FY, 2>(slot, rT, rB, cb, soff, vq); break;
+        default: two_issue<FX, FY, 3>(slot, rT, rB, cb, soff, vq); break;
+        }
+    };
+    /* stage arrays: 0 = E0, 1 = H0, 2 = bH, 3 = bE */
+    auto st = [&](const float *S, int arr, int c, int e) { return S[arr * Ly::ARR + c * Ly::NC * 64 + e]; };
+
+    const int ns = z1 - z0 + 4; /* stages: planes z0-2 .. z1+1 */
+    for (int s = 0; s < D && s < ns; s++) issue(s);
+
+    /* carried per cell: E0 and bE of the previous plane, bH and H1 of the previous plane,
+     * E1 and bE two planes back */
+    float e0p[CELLS][3], bep[CELLS][3], bhp[CELLS][3], h1p[CELLS][3], e1pp[CELLS][3], bepp[CELLS][3];
+#pragma unroll
+    for (int r = 0; r < CELLS; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) e0p[r][c] = bep[r][c] = bhp[r][c] = h1p[r][c] = e1pp[r][c] = bepp[r][c] = 0.f;
+
+    for (int s = 0; s < ns; s++) {
+        if (s + D < ns) {
+            issue(s + D);
+            wait_vm_barrier<D * Ly::kMinOps>();
+        } else {
+            switch (ns - 1 - s) { /* stages already issued after s */
+            case 0: wait_vm_barrier<0>(); break;
+            case 1: wait_vm_barrier<Ly::kMinOps>(); break;
+            default: wait_vm_barrier<(D > 2 ? 2 : 0) * Ly::kMinOps>(); break;
+            }
+        }
+        const float *S = lds + (s % NSLOT) * Ly::SLOT;
+        const int p = z0 - 2 + s;     /* unwrapped plane of this stage */
+        const int zp = plane_of(s);   /* its wrapped index */
+        const int zp1 = plane_of(s - 1), zp2 = plane_of(s - 2);
+        const bool doA = s >= 1, doB = p >= z0, doC = p >= z0 + 1, doD = p >= z0 + 2;
+
+        float h1n[CELLS][3], e1n[CELLS][3], h2n[CELLS][3];
+#pragma unroll
+        for (int r = 0; r < CELLS; r++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) h1n[r][c] = e1n[r][c] = h2n[r][c] = 0.f;
+        /* a. H1(p) on G3 */
+#pragma unroll
+        for (int r = 0; r < CELLS; r++) {
+            if (!(doA && inG3[r])) continue;
+            const int e = ce[r];
+            float P[3], M[3], ec[3], exm[3], eym[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                ec[c] = st(S, 0, c, e);
+                exm[c] = st(S, 0, c, e - 1);
+                eym[c] = st(S, 0, c, e - W4);
+            }
+            const float wx = cx[r] == 0 ? a.w[0] : 1.f, wy = cy[r] == 0 ? a.w[1] : 1.f, wz = zp == 0 ? a.w[2] : 1.f;
+            P[0] = ec[2] - wy * eym[2];
+            M[0] = -ec[1] + wz * e0p[r][1];
+            P[1] = ec[0] - wz * e0p[r][0];
+            M[1] = -ec[2] + wx * exm[2];
+            P[2] = ec[1] - wx * exm[1];
+            M[2] = -ec[0] + wy * eym[0];
+            if (zp == a.iI) {
+                P[0] += a.sgn * (float)plane_source(nullptr, a.Einc, a.srcE_f2, a.srcE_re, a.srcE_im, cq[r], 1);
+                P[1] -= a.sgn * (float)plane_source(nullptr, a.Einc, a.srcE_f2, a.srcE_re, a.srcE_im, cq[r], 0);
+                retire_loads();
+            }
+#pragma unroll
+            for (int c = 0; c < 3; c++) h1n[r][c] = st(S, 1, c, e) - st(S, 2, c, e) * (P[c] + M[c]);
+        }
+        /* b. E1(p-1) on G2: forward curl of H1 (previous plane from LDS, +z from a.) */
+#pragma unroll
+        for (int r = 0; r < CELLS; r++) {
+            if (!(doB && inG2[r])) continue;
+            const int e = ce[r];
+            float P[3], M[3], hc[3], hxp[3], hyp[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                hc[c] = lds[L_H1 + c * N4 + e];
+                hxp[c] = lds[L_H1 + c * N4 + e + 1];
+                hyp[c] = lds[L_H1 + c * N4 + e + W4];
+            }
+            const float wx = cx[r] + 1 == a.L0 ? a.w[0] : 1.f, wy = cy[r] + 1 == a.L1 ? a.w[1] : 1.f;
+            const float wz = zp1 + 1 == a.L2 ? a.w[2] : 1.f;
+            P[0] = wy * hyp[2] - hc[2];
+            M[0] = -wz * h1n[r][1] + hc[1];
+            P[1] = wz * h1n[r][0] - hc[0];
+            M[1] = -wx * hxp[2] + hc[2];
+            P[2] = wx * hxp[1] - hc[1];
+            M[2] = -wy * hyp[0] + hc[0];
+            if (zp1 == a.iI) {
+                P[0] += a.sgn * (float)plane_source(nullptr, a.Hinc, a.srcH_f2, a.srcH_re, a.srcH_im, cq[r], 1);
+                P[1] -= a.sgn * (float)plane_source(nullptr, a.Hinc, a.srcH_f2, a.srcH_re, a.srcH_im, cq[r], 0);
+                retire_loads();
+            }
+#pragma unroll
+            for (int c = 0; c < 3; c++) e1n[r][c] = e0p[r][c] + bep[r][c] * (P[c] + M[c]);
+            if (inT[r] && p - 1 >= z0 && p - 1 < z1 && (zp1 == a.iB || zp1 == a.iT)) { /* first step's monitors */
+                for (int m = 0; m < 2; m++) {
+                    if (zp1 != (m == 0 ? a.iB : a.iT)) continue;
+                    double *HM = m == 0 ? a.HB : a.HT, *EM = m == 0 ? a.EB : a.ET;
+                    for (int v = 0; v < 2; v++) {
+                        const int64_t o = 2 * (v + 2 * cq[r]);
+                        HM[o] += a.monH_re * (double)hc[v];
+                        HM[o + 1] += a.monH_im * (double)hc[v];
+                        EM[o] += a.monE_re * (double)e1n[r][v];
+                        EM[o + 1] += a.monE_im * (double)e1n[r][v];
+                    }
+                }
+                retire_loads();
+            }
+            lds[L_E1 + 0 * N4 + e] = e1n[r][0];
+            lds[L_E1 + 1 * N4 + e] = e1n[r][1];
+            lds[L_E1 + 2 * N4 + e] = e1n[r][2];
+        }
+        wait_vm_barrier<63>(); /* E1(p-1) of the neighbours in LDS */
+        /* c. H2(p-1) on G1 */
+#pragma unroll
+        for (int r = 0; r < CELLS; r++) {
+            if (!(doC && inG1[r])) continue;
+            const int e = ce[r];
+            float P[3], M[3], ec[3], exm[3], eym[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                ec[c] = lds[L_E1 + c * N4 + e];
+                exm[c] = lds[L_E1 + c * N4 + e - 1];
+                eym[c] = lds[L_E1 + c * N4 + e - W4];
+            }
+            const float wx = cx[r] == 0 ? a.w[0] : 1.f, wy = cy[r] == 0 ? a.w[1] : 1.f, wz = zp1 == 0 ? a.w[2] : 1.f;
+            P[0] = ec[2] - wy * eym[2];
+            M[0] = -ec[1] + wz * e1pp[r][1];
+            P[1] = ec[0] - wz * e1pp[r][0];
+            M[1] = -ec[2] + wx * exm[2];
+            P[2] = ec[1] - wx * exm[1];
+            M[2] = -ec[0] + wy * eym[0];
+            if (zp1 == a.iI) {
+                P[0] += a.sgn * (float)plane_source(nullptr, a.Einc, a.srcE2_f2, a.srcE2_re, a.srcE2_im, cq[r], 1);
+                P[1] -= a.sgn * (float)plane_source(nullptr, a.Einc, a.srcE2_f2, a.srcE2_re, a.srcE2_im, cq[r], 0);
+                retire_loads();
+            }
+#pragma unroll
+            for (int c = 0; c < 3; c++) h2n[r][c] = h1p[r][c] - bhp[r][c] * (P[c] + M[c]);
+            if (inT[r] && p - 1 < z1) store3(a.Tn, cq[r] + a.sz * zp1, h2n[r]);
+        }
+        /* d. E2(p-2) on T: forward curl of H2 (previous plane from LDS, +z from c.) */
+#pragma unroll
+        for (int r = 0; r < CELLS; r++) {
+            if (!(doD && inT[r])) continue;
+            const int e = ce[r];
+            float P[3], M[3], hc[3], hxp[3], hyp[3], en[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                hc[c] = lds[L_H2 + c * N4 + e];
+                hxp[c] = lds[L_H2 + c * N4 + e + 1];
+                hyp[c] = lds[L_H2 + c * N4 + e + W4];
+            }
+            const float wx = cx[r] + 1 == a.L0 ? a.w[0] : 1.f, wy = cy[r] + 1 == a.L1 ? a.w[1] : 1.f;
+            const float wz = zp2 + 1 == a.L2 ? a.w[2] : 1.f;
+            P[0] = wy * hyp[2] - hc[2];
+            M[0] = -wz * h2n[r][1] + hc[1];
+            P[1] = wz * h2n[r][0] - hc[0];
+            M[1] = -wx * hxp[2] + hc[2];
+            P[2] = wx * hxp[1] - hc[1];
+            M[2] = -wy * hyp[0] + hc[0];
+            if (zp2 == a.iI) {
+                P[0] += a.sgn * (float)plane_source(nullptr, a.Hinc, a.srcH2_f2, a.srcH2_re, a.srcH2_im, cq[r], 1);
+                P[1] -= a.sgn * (float)plane_source(nullptr, a.Hinc, a.srcH2_f2, a.srcH2_re, a.srcH2_im, cq[r], 0);
+                retire_loads();
+            }
+#pragma unroll
+            for (int c = 0; c < 3; c++) en[c] = e1pp[r][c] + bepp[r][c] * (P[c] + M[c]);
+            store3(a.Tn + 3, cq[r] + a.sz * zp2, en);
+            if (zp2 == a.iB || zp2 == a.iT) { /* second step's monitors */
+                for (int m = 0; m < 2; m++) {
+                    if (zp2 != (m == 0 ? a.iB : a.iT)) continue;
+                    double *HM = m == 0 ? a.HB : a.HT, *EM = m == 0 ? a.EB : a.ET;
+                    for (int v = 0; v < 2; v++) {
+                        const int64_t o = 2 * (v + 2 * cq[r]);
+                        HM[o] += a.monH2_re * (double)hc[v];
+                        HM[o + 1] += a.monH2_im * (double)hc[v];
+                        EM[o] += a.monE2_re * (double)en[v];
+                        EM[o + 1] += a.monE2_im * (double)en[v];
+                    }
+                }
+                retire_loads();
+            }
+        }
+        /* carry to the next plane */
+#pragma unroll
+        for (int r = 0; r < CELLS; r++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                e1pp[r][c] = e1n[r][c];
+                bepp[r][c] = bep[r][c];
+                h1p[r][c] = h1n[r][c];
+                bhp[r][c] = st(S, 2, c, ce[r]);
+                bep[r][c] = st(S, 3, c, ce[r]);
+                e0p[r][c] = st(S, 0, c, ce[r]);
+            }
+        /* every wave is done with this stage and with H1, H2 of the previous plane */
+        wait_vm_barrier<63>();
+#pragma unroll
+        for (int r = 0; r < CELLS; r++) {
+            if (doA && inG3[r])
+#pragma unroll
+                for (int c = 0; c < 3; c++) lds[L_H1 + c * N4 + ce[r]] = h1n[r][c];
+            if (doC && inG1[r])
+#pragma unroll
+                for (int c = 0; c < 3; c++) lds[L_H2 + c * N4 + ce[r]] = h2n[r][c];
+        }
+    }
+}
+
+/* ---------------------------------------------------------------------------------- */
 /* Region sweeps: the cells outside the staged kernel's box                           */
 /* ---------------------------------------------------------------------------------- */
 /*
@@ -1543,6 +1862,18 @@ const char *smx_f32_last_error(void) { return f32_err; }
 
 int smx_f32_num_tiles(void) { return kNumTiles; }
 
+/* two-step variants: 32x8 tiles, one or two planes of copies in flight */
+constexpr int kNumTwo = 2;
+
+int smx_f32_num_two(void) { return kNumTwo; }
+
+int smx_f32_two_ok(const smx_f32_state *st, int variant, int fz)
+{
+    /* lossless cells only: no PML, no lossy/metal cell; 32-bit offsets as the staged kernel */
+    return variant >= 0 && variant < kNumTwo && fz >= 1 && !st->has_pml && st->nspecial == 0 &&
+           24 * st->n < ((int64_t)1 << 32);
+}
+
 const char *smx_f32_kernel_name(const smx_f32_state *st)
 {
     static const char *const fused[kNumTiles] = {
@@ -1557,6 +1888,8 @@ const char *smx_f32_kernel_name(const smx_f32_state *st)
         "region_h_f32 + fused_dma_f32<32,8,4,false> + region_e_f32",
         "region_h_f32 + fused_dma_f32<32,8,2,false> + region_e_f32",
         "region_h_f32 + fused_dma_f32<32,8,1,false> + region_e_f32"};
+    static const char *const two[kNumTwo] = {"two_step_f32<32,8,1>", "two_step_f32<32,8,2>"};
+    if (st->two >= 0 && st->two < kNumTwo) return two[st->two];
     if (!st->fused) return st->bx == 64 ? "h_sweep_f32<64> + e_sweep_f32<64>" : "h_sweep_f32<32> + e_sweep_f32<32>";
     if (st->tile < 0 || st->tile >= kNumTiles) return "";
     return st->tile >= kFirstNoSplitTile && st->has_pml ? split[st->tile] : fused[st->tile];
@@ -1704,11 +2037,20 @@ int smx_f32_create(smx_f32_state *st, const smx_geom *g, double iR, double iI, c
     st->cur = 0;
     st->tile = kDefaultTile;
     st->fz = kDefaultFZ;
-    if (const char *t = getenv("SMX_F32_TILE")) { /* "<shape 0..2>,<z chunk>", e.g. "1,32" */
+    st->two = -1;
+    st->two_fz = kDefaultFZ;
+    if (const char *t = getenv("SMX_F32_TILE")) { /* "<variant>,<z chunk>", e.g. "1,32" */
         int ti = kDefaultTile, zc = kDefaultFZ;
         if (sscanf(t, "%d,%d", &ti, &zc) >= 1 && smx_f32_tile_ok(st, ti, zc)) {
             st->tile = ti;
             st->fz = zc;
+        }
+    }
+    if (const char *t = getenv("SMX_F32_TWO")) { /* two-step kernel "<variant>,<z chunk>" */
+        int v = -1, zc = kDefaultFZ;
+        if (sscanf(t, "%d,%d", &v, &zc) >= 1 && smx_f32_two_ok(st, v, zc)) {
+            st->two = v;
+            st->two_fz = zc;
         }
     }
     for (int k = 0; k < 2; k++) {
@@ -1875,6 +2217,73 @@ hipError_t smx_f32_step(smx_f32_state *st, const smx_f32_launch_args *la, hipStr
     if (st->bx == 64) hipLaunchKernelGGL(e_sweep_f32<64>, grid, block, 0, stream, a);
     else hipLaunchKernelGGL(e_sweep_f32<32>, grid, block, 0, stream, a);
     return hipGetLastError();
+}
+
+hipError_t smx_f32_step2(smx_f32_state *st, const smx_f32_launch_args *la, hipStream_t stream)
+{
+    if (!smx_f32_two_ok(st, st->two, st->two_fz)) return hipErrorInvalidValue;
+    fused_args a;
+    std::memset(&a, 0, sizeof a);
+    const smx_geom &g = la->g;
+    a.L0 = g.L0;
+    a.L1 = g.L1;
+    a.L2 = g.L2;
+    const int box[6] = {0, g.L0, 0, g.L1, 0, g.L2};
+    for (int i = 0; i < 6; i++) a.box[i] = box[i];
+    a.fz = st->two_fz;
+    a.gx = (g.L0 + 31) / 32;
+    a.gy = (g.L1 + 7) / 8;
+    a.nblocks = a.gx * a.gy * ((g.L2 + a.fz - 1) / a.fz);
+    a.sz = (int64_t)g.L0 * g.L1;
+    a.w[0] = (float)g.iBc0;
+    a.w[1] = (float)g.iBc1;
+    a.w[2] = (float)g.iBc2;
+    a.iB = g.iB;
+    a.iT = g.iT;
+    a.iI = g.iI;
+    a.sgn = (float)g.sgn;
+    for (int i = 0; i < 6; i++) {
+        a.To[i] = st->T[st->cur][i];
+        a.Tn[i] = st->T[1 - st->cur][i];
+        a.So[i] = st->S[st->cur][i];
+        a.Sn[i] = st->S[1 - st->cur][i];
+        a.b[i] = st->b[i];
+    }
+    a.ncomp = st->n;
+    a.pml = st->pml;
+    a.pmlflag = st->pmlflag;
+    a.Einc = la->Einc;
+    a.Hinc = la->Hinc;
+    a.srcE_f2 = la->s.srcE_f2;
+    a.srcE_re = la->s.srcE_re;
+    a.srcE_im = la->s.srcE_im;
+    a.srcH_f2 = la->s.srcH_f2;
+    a.srcH_re = la->s.srcH_re;
+    a.srcH_im = la->s.srcH_im;
+    a.monH_re = la->s.monH_re;
+    a.monH_im = la->s.monH_im;
+    a.monE_re = la->s.monE_re;
+    a.monE_im = la->s.monE_im;
+    a.srcE2_f2 = la->s2.srcE_f2;
+    a.srcE2_re = la->s2.srcE_re;
+    a.srcE2_im = la->s2.srcE_im;
+    a.srcH2_f2 = la->s2.srcH_f2;
+    a.srcH2_re = la->s2.srcH_re;
+    a.srcH2_im = la->s2.srcH_im;
+    a.monH2_re = la->s2.monH_re;
+    a.monH2_im = la->s2.monH_im;
+    a.monE2_re = la->s2.monE_re;
+    a.monE2_im = la->s2.monE_im;
+    a.HB = la->HB;
+    a.EB = la->EB;
+    a.HT = la->HT;
+    a.ET = la->ET;
+    if (st->two == 1) hipLaunchKernelGGL((two_step_f32<32, 8, 2>), dim3(a.nblocks), dim3(kThreads), 0, stream, a);
+    else hipLaunchKernelGGL((two_step_f32<32, 8, 1>), dim3(a.nblocks), dim3(kThreads), 0, stream, a);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    st->cur = 1 - st->cur;
+    return hipSuccess;
 }
 
 int smx_f32_write_fields(smx_f32_state *st, const double *H_t, const double *E_t, hipStream_t stream)

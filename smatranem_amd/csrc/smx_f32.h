@@ -61,6 +61,9 @@ struct smx_f32_state {
                   (x by y) staged through registers; 3-7 staged by asynchronous copies
                   to LDS (5-7 only on grids without PML cells) */
     int fz;    /* fused z chunk */
+    int two;   /* two-step kernel in use for pairs of steps: -1 off, else its variant
+                  (kern_f32.hip kTwo); single steps still take the choice above */
+    int two_fz;
     /* host copies for the reference-layout field import/export */
     char *cont_host;
     int *addr_host;
@@ -70,6 +73,7 @@ struct smx_f32_state {
 struct smx_f32_launch_args {
     smx_geom g;
     smx_step_scalars s;
+    smx_step_scalars s2; /* the second step's scalars (smx_f32_step2) */
     const double *srcH_plane, *srcE_plane;
     const double *Hinc, *Einc;
     double *HB, *EB, *HT, *ET;
@@ -80,6 +84,10 @@ int smx_f32_create(smx_f32_state *st, const smx_geom *g, double iR, double iI, c
                    hipStream_t stream);
 void smx_f32_free(smx_f32_state *st);
 hipError_t smx_f32_step(smx_f32_state *st, const smx_f32_launch_args *a, hipStream_t stream);
+/* Two steps in one pass (scalars a->s, then a->s2), when smx_f32_two_ok(st, variant, fz). */
+hipError_t smx_f32_step2(smx_f32_state *st, const smx_f32_launch_args *a, hipStream_t stream);
+int smx_f32_num_two(void);
+int smx_f32_two_ok(const smx_f32_state *st, int variant, int fz);
 int smx_f32_write_fields(smx_f32_state *st, const double *H_t, const double *E_t, hipStream_t stream);
 int smx_f32_read_fields(const smx_f32_state *st, double *H_t, double *E_t, hipStream_t stream);
 const char *smx_f32_last_error(void);

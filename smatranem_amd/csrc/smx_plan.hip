@@ -517,6 +517,56 @@ int smx_plan_set_source(smx_plan *p, const double *Hinc, const double *Einc, int
     return SMX_OK;
 }
 
+static int launch_step2(smx_plan *p, const smx_step_scalars &s, const smx_step_scalars &s2)
+{
+    smx_f32_launch_args a;
+    a.g = make_geom(p);
+    a.s = s;
+    a.s2 = s2;
+    a.srcH_plane = nullptr;
+    a.srcE_plane = nullptr;
+    a.Hinc = p->inc;
+    a.Einc = p->inc + 4 * p->plane;
+    a.HB = p->mon;
+    a.EB = p->mon + 4 * p->plane;
+    a.HT = p->mon + 8 * p->plane;
+    a.ET = p->mon + 12 * p->plane;
+    SMX_HIP(smx_f32_step2(&p->f32, &a, p->stream));
+    return SMX_OK;
+}
+
+/* Step k of the source loop, or steps k and k+1 in one pass when the plan's two-step
+ * kernel is in use, the DFT is off and both steps are to run; *done = steps taken.
+ * sums get the envelope values in the same order as step-by-step. */
+static int advance(smx_plan *p, long long k, long long nleft, double *sums, int *done)
+{
+    smx_step_scalars s;
+    double fH, fE;
+    int rc = step_scalars(p, (int)(k / p->N), (int)(k % p->N), &s, &fH, &fE);
+    if (rc) return rc;
+    if (p->g.precision == SMX_F32 && p->f32.two >= 0 && p->dft_nplanes == 0 && nleft >= 2) {
+        smx_step_scalars s2;
+        double fH2, fE2;
+        if (step_scalars(p, (int)((k + 1) / p->N), (int)((k + 1) % p->N), &s2, &fH2, &fE2) == SMX_OK) {
+            if (sums) {
+                sums[0] += fH;
+                sums[1] += fE;
+                sums[0] += fH2;
+                sums[1] += fE2;
+            }
+            *done = 2;
+            return launch_step2(p, s, s2);
+        }
+    }
+    if (sums) {
+        sums[0] += fH;
+        sums[1] += fE;
+    }
+    *done = 1;
+    if ((rc = launch_step(p, s, false))) return rc;
+    return dft_after_step(p, k);
+}
+
 int smx_plan_run_steps(smx_plan *p, long long step0, long long nsteps, double sums[2])
 {
     if (!p) return arg_fail("null pointer");
@@ -525,17 +575,12 @@ int smx_plan_run_steps(smx_plan *p, long long step0, long long nsteps, double su
         return SMX_ERR_STATE;
     }
     if (step0 < 0 || nsteps < 0) return arg_fail("negative step");
-    for (long long k = step0; k < step0 + nsteps; k++) {
-        const int P = (int)(k / p->N), t = (int)(k % p->N);
-        smx_step_scalars s;
-        double fH, fE;
-        int rc = step_scalars(p, P, t, &s, &fH, &fE);
+    for (long long k = step0; k < step0 + nsteps;) {
+        int done = 0;
+        const int rc = advance(p, k, step0 + nsteps - k, sums, &done);
         if (rc) return rc;
-        if (sums) {
-            sums[0] += fH;
-            sums[1] += fE;
-        }
-        if ((rc = launch_step(p, s, false)) || (rc = dft_after_step(p, k))) return rc;
+        k += done;
+        p->next_step = k;
     }
     p->next_step = step0 + nsteps;
     return SMX_OK;
@@ -554,20 +599,16 @@ int smx_plan_run_batch(smx_plan *const *plans, int nplans, long long step0, long
         }
     }
     if (step0 < 0 || nsteps < 0) return arg_fail("negative step");
+    std::vector<long long> next((size_t)nplans, step0);
     for (long long k = step0; k < step0 + nsteps; k++) {
         for (int i = 0; i < nplans; i++) {
+            if (next[(size_t)i] != k) continue; /* a two-step pass already took step k */
             smx_plan *p = plans[i];
-            const int P = (int)(k / p->N), t = (int)(k % p->N);
-            smx_step_scalars s;
-            double fH, fE;
-            int rc = step_scalars(p, P, t, &s, &fH, &fE);
+            int done = 0;
+            const int rc = advance(p, k, step0 + nsteps - k, sums ? sums + 2 * i : nullptr, &done);
             if (rc) return rc;
-            if (sums) {
-                sums[2 * i] += fH;
-                sums[2 * i + 1] += fE;
-            }
-            if ((rc = launch_step(p, s, false)) || (rc = dft_after_step(p, k))) return rc;
-            p->next_step = k + 1;
+            next[(size_t)i] = k + done;
+            p->next_step = k + done;
         }
     }
     return SMX_OK;
@@ -860,6 +901,7 @@ int smx_plan_autotune(smx_plan *p, int trial_steps, int choice[4])
         SMX_HIP(hipEventCreate(&e1));
         float best = 1e30f;
         int best_f = st.fused, best_t = st.tile, best_z = st.fz;
+        st.two = -1; /* single-step candidates first */
         /* candidate 0: the in-place sweeps; then every fused shape and z chunk */
         for (int cand = 0; cand < 1 + smx_f32_num_tiles() * 4; cand++) {
             st.fused = cand > 0;
@@ -888,6 +930,32 @@ int smx_plan_autotune(smx_plan *p, int trial_steps, int choice[4])
         st.fused = best_f;
         st.tile = best_t;
         st.fz = best_z;
+        /* two steps per pass: timed per step over the same number of steps (even) */
+        float best_step = best / (float)trial_steps;
+        int best_two = -1, best_two_fz = st.two_fz;
+        const int pairs = (trial_steps + 1) / 2;
+        for (int v = 0; v < smx_f32_num_two(); v++)
+            for (int ci = 0; ci < 4; ci++) {
+                if (!smx_f32_two_ok(&st, v, chunks[ci])) continue;
+                st.two = v;
+                st.two_fz = chunks[ci];
+                int rc;
+                if ((rc = launch_step2(p, s, s))) return rc;
+                SMX_HIP(hipEventRecord(e0, p->stream));
+                for (int k = 0; k < pairs; k++)
+                    if ((rc = launch_step2(p, s, s))) return rc;
+                SMX_HIP(hipEventRecord(e1, p->stream));
+                SMX_HIP(hipEventSynchronize(e1));
+                float ms = 0.f;
+                SMX_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (ms / (2.f * pairs) < best_step) {
+                    best_step = ms / (2.f * pairs);
+                    best_two = v;
+                    best_two_fz = chunks[ci];
+                }
+            }
+        st.two = best_two;
+        st.two_fz = best_two_fz;
         for (int k = 0; k < 2; k++)
             for (int i = 0; i < 6; i++) {
                 SMX_HIP(hipMemcpyAsync(st.T[k][i], (char *)save + i * field, field, hipMemcpyDeviceToDevice, p->stream));
@@ -909,6 +977,12 @@ int smx_plan_autotune(smx_plan *p, int trial_steps, int choice[4])
         choice[1] = f ? fy : 0;
         choice[2] = f ? st.fz : 0;
         choice[3] = f ? (st.tile >= 3 ? 2 : 1) : 0;
+        if (p->g.precision == SMX_F32 && st.two >= 0) {
+            choice[0] = 32;
+            choice[1] = 8;
+            choice[2] = st.two_fz;
+            choice[3] = 3;
+        }
     }
     return SMX_OK;
 }

@@ -14,7 +14,8 @@ MEMBERS = [
     (fdtd.F32, dict(seed=21, L0=12, L1=10, L2=28)),
     (fdtd.F32, dict(seed=22, L0=37, L1=19, L2=41, outer=8, pml_xy=2, pml_z=3, core=(3, 3))),
     (fdtd.F64, dict(seed=23, L0=9, L1=14, L2=30, outer=7, pml_xy=0, pml_z=3)),
-    (fdtd.F32, dict(seed=24, L0=20, L1=16, L2=33, outer=8, pml_xy=0, pml_z=0, iBc=(1, -1, 1))),
+    (fdtd.F32, dict(seed=24, L0=20, L1=16, L2=33, outer=8, pml_xy=0, pml_z=0, iBc=(1, -1, 1),
+                    metal_cells=0, lossy_cells=0)),
 ]
 
 
@@ -27,7 +28,12 @@ def _state(p):
     return H, E, tuple(p.read_monitors())
 
 
-def test_batch_equals_separate_runs():
+@pytest.mark.parametrize("two", [None, "0,16"])
+def test_batch_equals_separate_runs(two, monkeypatch):
+    """With two = "v,fz" the PML-free member runs two steps per pass (and the batch
+    loop has to keep the members' step counts in step)."""
+    if two:
+        monkeypatch.setenv("SMX_F32_TWO", two)
     preps = _preps()
     steps = min(pr.case.time_period[2] * pr.N for _, pr in preps)
     alone = []

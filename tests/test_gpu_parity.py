@@ -239,6 +239,64 @@ def test_region_sweeps_carry_monitor_plane(tile, monkeypatch):
         assert rel_l2(a.view(np.float64), b) < 2e-4
 
 
+@pytest.mark.parametrize("two,iBc", [("0,16", (1, -1, 1)), ("1,8", (0, 0, 0)), ("0,5", (1, 1, -1)),
+                                     ("1,64", (-1, 1, 1)), ("0,41", (1, -1, 0))])
+def test_two_step_pass(two, iBc, monkeypatch):
+    """Two steps per pass (grids without PML and lossy/metal cells) against the FP64
+    oracle, fields and both monitor planes, over a run with an odd number of steps (the
+    last one takes the one-step kernel)."""
+    monkeypatch.setenv("SMX_F32_KERNEL", "fused")
+    monkeypatch.setenv("SMX_F32_TWO", two)
+    prep = casegen.prepare(small_waveguide(seed=13, L0=37, L1=19, L2=41, outer=8, pml_xy=0, pml_z=0,
+                                           iBc=iBc, metal_cells=0, lossy_cells=0, core=(3, 3)))
+    assert all(c == ord("n") for c in prep.Cont)
+    steps = prep.case.time_period[2] * prep.N
+    with prep.plan(fdtd.F32) as p:
+        v, fz = (int(x) for x in two.split(","))
+        assert p.autotune(0) == (32, 8, fz, 3)
+        assert p.kernels == "two_step_f32<32,8,%d>" % (v + 1)
+        sums = p.run_steps(0, steps - 1)
+        p.run_steps(steps - 1, 1, sums)
+        H, E = p.read_fields()
+        mons = p.read_monitors()
+    Ho, Eo, mo, so = _oracle_run(prep, 0, prep.case.time_period[2])
+    assert np.abs(Ho).max() > 0
+    assert rel_l2(H, Ho) < 2e-4 and rel_l2(E, Eo) < 2e-4, (rel_l2(H, Ho), rel_l2(E, Eo))
+    for a, b in zip(mons, mo):
+        assert np.linalg.norm(b) > 0
+        assert rel_l2(a.view(np.float64), b) < 2e-4
+    assert np.allclose(sums, so, rtol=1e-14, atol=0)
+
+
+def test_two_step_equals_one_step_kernels(monkeypatch):
+    """The two-step pass against the one-step staged kernel on the same grid: both are
+    FP32 restatements of the same map, so they agree far below the FP32-vs-FP64 gap."""
+    prep = casegen.prepare(small_waveguide(seed=14, L0=40, L1=24, L2=36, outer=8, pml_xy=0, pml_z=0,
+                                           iBc=(1, -1, 1), metal_cells=0, lossy_cells=0))
+    out = []
+    for two in ("0,16", None):
+        monkeypatch.setenv("SMX_F32_KERNEL", "fused")
+        monkeypatch.setenv("SMX_F32_TILE", "3,16")
+        if two:
+            monkeypatch.setenv("SMX_F32_TWO", two)
+        else:
+            monkeypatch.delenv("SMX_F32_TWO", raising=False)
+        with prep.plan(fdtd.F32) as p:
+            p.run(0, prep.case.time_period[2])
+            out.append(p.read_fields())
+    assert rel_l2(out[0][0], out[1][0]) < 1e-5 and rel_l2(out[0][1], out[1][1]) < 1e-5
+
+
+def test_two_step_refused_with_pml_or_special(monkeypatch):
+    monkeypatch.setenv("SMX_F32_KERNEL", "fused")
+    monkeypatch.setenv("SMX_F32_TWO", "0,16")
+    for kw in (dict(pml_xy=0, pml_z=3, metal_cells=0, lossy_cells=0),
+               dict(pml_xy=0, pml_z=0, metal_cells=2, lossy_cells=0)):
+        prep = casegen.prepare(small_waveguide(seed=15, **kw))
+        with prep.plan(fdtd.F32) as p:
+            assert p.autotune(0)[3] != 3
+
+
 def test_autotune_keeps_state():
     """The autotuner restores fields and monitors, and the run it tunes stays correct."""
     prep = casegen.prepare(small_waveguide(seed=9, **CASES["ports"]))
@@ -252,7 +310,7 @@ def test_autotune_keeps_state():
         assert _eq(H0, H1) and _eq(E0, E1)
         assert all(_eq(a, b) for a, b in zip(m0, m1))
         assert choice == (0, 0, 0, 0) or (choice[0] * choice[1] == 256 and choice[2] in (8, 16, 32, 64)
-                                          and choice[3] in (1, 2))
+                                          and choice[3] in (1, 2, 3))
         p.run(1, prep.case.time_period[2])
         H, E = p.read_fields()
     Ho, Eo, _, _ = _oracle_run(prep, 0, prep.case.time_period[2])

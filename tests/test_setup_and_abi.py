@@ -26,6 +26,27 @@ def test_library_exports_header_symbols():
         assert hasattr(lib, name), name
 
 
+def test_library_exports_nothing_unprefixed():
+    """Every C symbol the library defines carries the smx_ prefix (C++ ones are mangled):
+    an unprefixed C name such as `advance` can be bound, by the dynamic linker, to a
+    same-named function of another library in the process."""
+    import shutil
+    import subprocess
+    if shutil.which("nm") is None:
+        pytest.skip("nm not available")
+    out = subprocess.run(["nm", "-D", "--defined-only", fdtd.library_path], capture_output=True, text=True,
+                         check=True).stdout
+    bad = []
+    for line in out.splitlines():
+        parts = line.split()
+        if len(parts) == 3 and parts[1] in "TWD":
+            name = parts[2]
+            if not (name.startswith("smx_") or name.startswith("_Z") or name.startswith("__hip")
+                    or name.startswith("_fini") or name.startswith("_init")):
+                bad.append(name)
+    assert not bad, bad
+
+
 def test_status_strings():
     lib = fdtd.lib()
     assert lib.smx_status_string(0) == b"ok"
