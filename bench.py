@@ -58,8 +58,8 @@ def measured_traffic(kernels):
             scale = (float(w[2]), float(w[4]))
         else:
             table[w[0]] = (float(w[1]), float(w[2]))
-    if scale is None or any(k not in table for k in kernels):
-        return None
+    if scale is None or any(k not in table for k in kernels) or len(set(kernels)) != len(kernels):
+        return None  # a kernel launched twice per step with different extents has no single average
     return sum((table[k][0] * scale[0] + table[k][1] * scale[1]) * 1024 for k in kernels)
 
 

@@ -462,6 +462,8 @@ struct fused_args {
     int nreg;
     int reg[6][6];
     int64_t reg_start[7];
+    /* region_e_f32 accumulates no monitor for cells inside this box */
+    int monskip[6];
 };
 
 __device__ inline int wrap_index(int v, int L)
@@ -1692,7 +1694,9 @@ __global__ __launch_bounds__(kThreads) void region_e_f32(fused_args a)
 #pragma unroll
         for (int c = 0; c < 3; c++) En[c] = e[c] + b[c] * (P[c] + M[c]);
         store3(a.Tn + 3, l, En);
-        for (int m = 0; m < 2; m++) { /* monitors: main = total here (S = 0) */
+        const bool skip = x >= a.monskip[0] && x < a.monskip[1] && y >= a.monskip[2] && y < a.monskip[3] &&
+                          z >= a.monskip[4] && z < a.monskip[5];
+        for (int m = 0; m < 2 && !skip; m++) { /* monitors: main = total here (S = 0) */
             if (z != (m == 0 ? a.iB : a.iT)) continue;
             double *HM = m == 0 ? a.HB : a.HT, *EM = m == 0 ? a.EB : a.ET;
             for (int v = 0; v < 2; v++) {
@@ -1807,55 +1811,104 @@ __global__ __launch_bounds__(kThreads) void special_e_pass(fused_args a)
 
 /* The box of cells whose +x/+y/+z neighbours are PML-free too, and boxes covering the
  * rest of the grid (see smx_f32_state). */
-static void plan_split(smx_f32_state *st, const unsigned char *flag)
+/* Boxes covering the grid outside `inner`; false when a box holds 2^32 cells or more
+ * (region_cell decodes with 32-bit offsets). */
+static bool complement(const int L[3], const int in[6], smx_region_set *rs)
 {
-    const int L[3] = {st->L[0], st->L[1], st->L[2]};
-    int lo[3], hi[3];
-    st->split_ok = 0;
-    st->nreg = 0;
+    rs->n = 0;
+    auto add = [&](int x0, int x1, int y0, int y1, int z0, int z1) {
+        if (x1 <= x0 || y1 <= y0 || z1 <= z0) return;
+        int *b = rs->box[rs->n++];
+        b[0] = x0, b[1] = x1, b[2] = y0, b[3] = y1, b[4] = z0, b[5] = z1;
+    };
+    add(0, L[0], 0, L[1], 0, in[4]);
+    add(0, L[0], 0, L[1], in[5], L[2]);
+    add(0, L[0], 0, in[2], in[4], in[5]);
+    add(0, L[0], in[3], L[1], in[4], in[5]);
+    add(0, in[0], in[2], in[3], in[4], in[5]);
+    add(in[1], L[0], in[2], in[3], in[4], in[5]);
+    rs->start[0] = 0;
+    for (int r = 0; r < rs->n; r++) {
+        const int *b = rs->box[r];
+        const int64_t cells = (int64_t)(b[1] - b[0]) * (b[3] - b[2]) * (b[5] - b[4]);
+        if (cells >= ((int64_t)1 << 32)) return false;
+        rs->start[r + 1] = rs->start[r] + cells;
+    }
+    return true;
+}
+
+/* The PML-free run [lo, hi) of every axis (lo = 0, hi = L and wrapping halos where an
+ * axis has no PML); false when the PML-free cells of an axis are not one run. */
+static bool pml_free_runs(const smx_f32_state *st, const unsigned char *flag, int lo[3], int hi[3], bool has[3])
+{
     const unsigned char *f = flag;
     for (int d = 0; d < 3; d++) {
+        const int L = st->L[d];
         int first = -1, last = -1, count = 0;
-        for (int j = 0; j < L[d]; j++)
+        for (int j = 0; j < L; j++)
             if (!f[j]) {
                 if (first < 0) first = j;
                 last = j;
                 count++;
             }
-        f += L[d];
-        if (count == L[d]) { /* no PML on this axis: the box spans it, halos wrap */
+        f += L;
+        has[d] = count != L;
+        if (!has[d]) {
             lo[d] = 0;
-            hi[d] = L[d];
+            hi[d] = L;
             continue;
         }
-        if (count == 0 || last - first + 1 != count) return; /* not one PML-free run */
+        if (count == 0 || last - first + 1 != count) return false;
         lo[d] = first;
-        hi[d] = last; /* exclusive: cell `last` is the box's PML-free +1 neighbour */
-        if (hi[d] <= lo[d]) return;
+        hi[d] = last + 1;
     }
+    return true;
+}
+
+/* The boxes of the staged kernel without split slots (one step) and of the two-step
+ * kernel on a grid with PML (see smx_f32_state). */
+static void plan_split(smx_f32_state *st, const unsigned char *flag)
+{
+    const int L[3] = {st->L[0], st->L[1], st->L[2]};
+    int lo[3], hi[3];
+    bool has[3];
+    st->split_ok = st->split2_ok = 0;
+    st->nreg = 0;
+    if (!pml_free_runs(st, flag, lo, hi, has)) return;
+    /* one step: cells whose +1 neighbours are PML-free, i.e. [lo, hi-1) on a PML axis */
+    int in1[6];
     for (int d = 0; d < 3; d++) {
-        st->fbox[2 * d] = lo[d];
-        st->fbox[2 * d + 1] = hi[d];
+        in1[2 * d] = lo[d];
+        in1[2 * d + 1] = has[d] ? hi[d] - 1 : hi[d];
     }
-    auto add = [&](int x0, int x1, int y0, int y1, int z0, int z1) {
-        if (x1 <= x0 || y1 <= y0 || z1 <= z0) return;
-        int *b = st->reg[st->nreg++];
-        b[0] = x0, b[1] = x1, b[2] = y0, b[3] = y1, b[4] = z0, b[5] = z1;
-    };
-    add(0, L[0], 0, L[1], 0, lo[2]);
-    add(0, L[0], 0, L[1], hi[2], L[2]);
-    add(0, L[0], 0, lo[1], lo[2], hi[2]);
-    add(0, L[0], hi[1], L[1], lo[2], hi[2]);
-    add(0, lo[0], lo[1], hi[1], lo[2], hi[2]);
-    add(hi[0], L[0], lo[1], hi[1], lo[2], hi[2]);
-    st->reg_start[0] = 0;
-    for (int r = 0; r < st->nreg; r++) {
-        const int *b = st->reg[r];
-        const int64_t cells = (int64_t)(b[1] - b[0]) * (b[3] - b[2]) * (b[5] - b[4]);
-        if (cells >= ((int64_t)1 << 32)) return; /* region_cell decodes with 32-bit offsets */
-        st->reg_start[r + 1] = st->reg_start[r] + cells;
+    bool ok = true;
+    for (int d = 0; d < 3; d++) ok &= in1[2 * d + 1] > in1[2 * d];
+    smx_region_set rs;
+    if (ok && complement(L, in1, &rs)) {
+        for (int i = 0; i < 6; i++) st->fbox[i] = in1[i];
+        st->nreg = rs.n;
+        std::memcpy(st->reg, rs.box, sizeof st->reg);
+        std::memcpy(st->reg_start, rs.start, sizeof st->reg_start);
+        st->split_ok = 1;
     }
-    st->split_ok = 1;
+    /* two steps: outputs on [a, b) need H1 on [a-1, b+1] PML-free: a = lo+1, b = hi-2 */
+    int f2[6], h1[6], e1[6];
+    ok = true;
+    for (int d = 0; d < 3; d++) {
+        const int a = has[d] ? lo[d] + 1 : 0, b = has[d] ? hi[d] - 2 : L[d];
+        f2[2 * d] = a;
+        f2[2 * d + 1] = b;
+        /* first step outside the box shrunk by one on both sides (H1), on + sides (E1) */
+        h1[2 * d] = has[d] ? a + 1 : 0;
+        h1[2 * d + 1] = has[d] ? b - 1 : L[d];
+        e1[2 * d] = has[d] ? a : 0;
+        e1[2 * d + 1] = has[d] ? b - 1 : L[d];
+        ok &= b - a >= 3;
+    }
+    if (ok && complement(L, f2, &st->regP2) && complement(L, h1, &st->regH1) && complement(L, e1, &st->regE1)) {
+        for (int i = 0; i < 6; i++) st->fbox2[i] = f2[i];
+        st->split2_ok = 1;
+    }
 }
 
 const char *smx_f32_last_error(void) { return f32_err; }
@@ -1869,8 +1922,9 @@ int smx_f32_num_two(void) { return kNumTwo; }
 
 int smx_f32_two_ok(const smx_f32_state *st, int variant, int fz)
 {
-    /* lossless cells only: no PML, no lossy/metal cell; 32-bit offsets as the staged kernel */
-    return variant >= 0 && variant < kNumTwo && fz >= 1 && !st->has_pml && st->nspecial == 0 &&
+    /* no lossy/metal cell; PML only outside the two-step box (region sweeps do the
+     * rest); 32-bit offsets as the staged kernel */
+    return variant >= 0 && variant < kNumTwo && fz >= 1 && st->nspecial == 0 && (!st->has_pml || st->split2_ok) &&
            24 * st->n < ((int64_t)1 << 32);
 }
 
@@ -1889,7 +1943,10 @@ const char *smx_f32_kernel_name(const smx_f32_state *st)
         "region_h_f32 + fused_dma_f32<32,8,2,false> + region_e_f32",
         "region_h_f32 + fused_dma_f32<32,8,1,false> + region_e_f32"};
     static const char *const two[kNumTwo] = {"two_step_f32<32,8,1>", "two_step_f32<32,8,2>"};
-    if (st->two >= 0 && st->two < kNumTwo) return two[st->two];
+    static const char *const two_split[kNumTwo] = {
+        "region_h_f32 + region_e_f32 + two_step_f32<32,8,1> + region_h_f32 + region_e_f32",
+        "region_h_f32 + region_e_f32 + two_step_f32<32,8,2> + region_h_f32 + region_e_f32"};
+    if (st->two >= 0 && st->two < kNumTwo) return st->has_pml ? two_split[st->two] : two[st->two];
     if (!st->fused) return st->bx == 64 ? "h_sweep_f32<64> + e_sweep_f32<64>" : "h_sweep_f32<32> + e_sweep_f32<32>";
     if (st->tile < 0 || st->tile >= kNumTiles) return "";
     return st->tile >= kFirstNoSplitTile && st->has_pml ? split[st->tile] : fused[st->tile];
@@ -1921,6 +1978,9 @@ void smx_f32_free(smx_f32_state *st)
     }
     if (st->bmem) (void)hipFree(st->bmem);
     st->bmem = nullptr;
+    if (st->tmpT) (void)hipFree(st->tmpT);
+    if (st->tmpS) (void)hipFree(st->tmpS);
+    st->tmpT = st->tmpS = nullptr;
     for (int i = 0; i < 6; i++) {
         st->T[0][i] = st->T[1][i] = st->S[0][i] = st->S[1][i] = st->b[i] = nullptr;
     }
@@ -2278,10 +2338,85 @@ hipError_t smx_f32_step2(smx_f32_state *st, const smx_f32_launch_args *la, hipSt
     a.EB = la->EB;
     a.HT = la->HT;
     a.ET = la->ET;
+    if (!st->has_pml) {
+        if (st->two == 1) hipLaunchKernelGGL((two_step_f32<32, 8, 2>), dim3(a.nblocks), dim3(kThreads), 0, stream, a);
+        else hipLaunchKernelGGL((two_step_f32<32, 8, 1>), dim3(a.nblocks), dim3(kThreads), 0, stream, a);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        st->cur = 1 - st->cur;
+        return hipSuccess;
+    }
+    /*
+     * With PML: the two-step kernel on its box F, the region sweeps on the rest P.
+     *   1. H1 on P and both one-cell shells of F        old -> tmp
+     *   2. E1 on P and the +x/+y/+z shell of F          old, tmp H1 -> tmp
+     *   3. two steps on F                               old -> new
+     *   4. H2 on P                                      tmp -> new
+     *   5. E2 on P                                      tmp E1, new H2 -> new
+     * Each region step reads only cells the steps before it wrote (the shells supply
+     * the first step next to F, where P's second step needs it; F's own outputs supply
+     * the second step's +x/+y/+z neighbours of P).
+     */
+    if (!st->tmpT) {
+        hipError_t e;
+        if ((e = hipMalloc(&st->tmpT, sizeof(float) * 6 * st->n)) != hipSuccess) return e;
+        if ((e = hipMalloc(&st->tmpS, sizeof(float) * 6 * st->n)) != hipSuccess) return e;
+        if ((e = hipMemsetAsync(st->tmpS, 0, sizeof(float) * 6 * st->n, stream)) != hipSuccess) return e;
+    }
+    float *tT[6], *tS[6];
+    for (int i = 0; i < 6; i++) {
+        tT[i] = st->tmpT + (size_t)i * st->n;
+        tS[i] = st->tmpS + (size_t)i * st->n;
+    }
+    float *const oT[6] = {st->T[st->cur][0], st->T[st->cur][1], st->T[st->cur][2],
+                          st->T[st->cur][3], st->T[st->cur][4], st->T[st->cur][5]};
+    float *const oS[6] = {st->S[st->cur][0], st->S[st->cur][1], st->S[st->cur][2],
+                          st->S[st->cur][3], st->S[st->cur][4], st->S[st->cur][5]};
+    float *const nT[6] = {st->T[1 - st->cur][0], st->T[1 - st->cur][1], st->T[1 - st->cur][2],
+                          st->T[1 - st->cur][3], st->T[1 - st->cur][4], st->T[1 - st->cur][5]};
+    float *const nS[6] = {st->S[1 - st->cur][0], st->S[1 - st->cur][1], st->S[1 - st->cur][2],
+                          st->S[1 - st->cur][3], st->S[1 - st->cur][4], st->S[1 - st->cur][5]};
+    auto region = [&](const smx_region_set &rs, bool e_sweep, const float *const *inT, const float *const *inS,
+                      float *const *outT, float *const *outS, int step) -> hipError_t {
+        fused_args r = a;
+        r.nreg = rs.n;
+        std::memcpy(r.reg, rs.box, sizeof r.reg);
+        std::memcpy(r.reg_start, rs.start, sizeof r.reg_start);
+        for (int i = 0; i < 6; i++) {
+            r.To[i] = inT[i];
+            r.So[i] = inS[i];
+            r.Tn[i] = outT[i];
+            r.Sn[i] = outS[i];
+            r.monskip[i] = st->fbox2[i];
+        }
+        if (step == 2) {
+            r.srcE_f2 = a.srcE2_f2, r.srcE_re = a.srcE2_re, r.srcE_im = a.srcE2_im;
+            r.srcH_f2 = a.srcH2_f2, r.srcH_re = a.srcH2_re, r.srcH_im = a.srcH2_im;
+            r.monH_re = a.monH2_re, r.monH_im = a.monH2_im, r.monE_re = a.monE2_re, r.monE_im = a.monE2_im;
+        }
+        if (rs.n == 0) return hipSuccess;
+        const unsigned blocks = (unsigned)((rs.start[rs.n] + kThreads - 1) / kThreads);
+        if (e_sweep) hipLaunchKernelGGL(region_e_f32, dim3(blocks), dim3(kThreads), 0, stream, r);
+        else hipLaunchKernelGGL(region_h_f32, dim3(blocks), dim3(kThreads), 0, stream, r);
+        return hipGetLastError();
+    };
+    hipError_t e;
+    /* 1: H1 reads old H, E, S; writes tmp */
+    if ((e = region(st->regH1, false, oT, oS, tT, tS, 1)) != hipSuccess) return e;
+    /* 2: E1 reads old E, S and the new H in its output set: tmp */
+    if ((e = region(st->regE1, true, oT, oS, tT, tS, 1)) != hipSuccess) return e;
+    /* 3: the box */
+    for (int i = 0; i < 6; i++) a.box[i] = st->fbox2[i];
+    a.gx = (a.box[1] - a.box[0] + 31) / 32;
+    a.gy = (a.box[3] - a.box[2] + 7) / 8;
+    a.nblocks = a.gx * a.gy * ((a.box[5] - a.box[4] + a.fz - 1) / a.fz);
     if (st->two == 1) hipLaunchKernelGGL((two_step_f32<32, 8, 2>), dim3(a.nblocks), dim3(kThreads), 0, stream, a);
     else hipLaunchKernelGGL((two_step_f32<32, 8, 1>), dim3(a.nblocks), dim3(kThreads), 0, stream, a);
-    const hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    /* 4: H2 from tmp (H1, E1, H1 split) into new */
+    if ((e = region(st->regP2, false, tT, tS, nT, nS, 2)) != hipSuccess) return e;
+    /* 5: E2 from tmp (E1, E1 split) and the new H into new */
+    if ((e = region(st->regP2, true, tT, tS, nT, nS, 2)) != hipSuccess) return e;
     st->cur = 1 - st->cur;
     return hipSuccess;
 }

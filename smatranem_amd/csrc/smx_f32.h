@@ -31,6 +31,14 @@ struct smx_special_f32 {
     float eA[3], eB[3], eAs[3], eBs[3]; /* E likewise */
 };
 
+/* Up to 6 boxes [x0, x1) x [y0, y1) x [z0, z1) covering the grid outside one inner
+ * box, with the running cell count before each (the region sweeps' thread index). */
+struct smx_region_set {
+    int n;
+    int box[6][6];
+    int64_t start[7];
+};
+
 struct smx_f32_state {
     int L[3];
     int64_t n;
@@ -53,6 +61,14 @@ struct smx_f32_state {
     int nreg;
     int reg[6][6];
     int64_t reg_start[7];
+    /* the same for the two-step kernel on a grid with PML: its box (cells whose
+     * two-step cone stays PML-free), the rest (P) and the two first-step regions
+     * around it (P and both one-cell shells of the box; P and its +x/+y/+z shell);
+     * tmpT/tmpS hold the first step there (allocated on first use) */
+    int split2_ok;
+    int fbox2[6];
+    smx_region_set regP2, regH1, regE1;
+    float *tmpT, *tmpS;
     int cur;
     int fused; /* 1: one fused H+E launch per step (+ the special-cell E pass);
                   0: two in-place sweeps */

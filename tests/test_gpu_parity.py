@@ -287,14 +287,49 @@ def test_two_step_equals_one_step_kernels(monkeypatch):
     assert rel_l2(out[0][0], out[1][0]) < 1e-5 and rel_l2(out[0][1], out[1][1]) < 1e-5
 
 
-def test_two_step_refused_with_pml_or_special(monkeypatch):
+@pytest.mark.parametrize("two,kw", [
+    ("0,16", dict(pml_xy=0, pml_z=3, iBc=(1, -1, 0))),                 # z PML, both monitors in the box
+    ("1,8", dict(pml_xy=0, pml_z=3, monitor_gap=0, iBc=(0, 0, 0))),    # monitor planes in the region sweeps
+    ("0,5", dict(pml_xy=2, pml_z=3, iBc=(0, 0, 0))),                   # PML on every face
+    ("1,64", dict(pml_xy=3, pml_z=4, iBc=(1, 1, 0), scale_amp=0.3)),
+])
+def test_two_step_pass_with_pml(two, kw, monkeypatch):
+    """Two steps per pass on grids with PML: the kernel on the box whose two-step cone is
+    PML-free, the region sweeps (first step into a scratch set, second into the new
+    buffer) on the rest, against the FP64 oracle."""
+    monkeypatch.setenv("SMX_F32_KERNEL", "fused")
+    monkeypatch.setenv("SMX_F32_TWO", two)
+    prep = casegen.prepare(small_waveguide(seed=16, L0=37, L1=22, L2=44, outer=9, metal_cells=0,
+                                           lossy_cells=0, core=(3, 3), **kw))
+    assert any(c == ord("a") for c in prep.Cont)
+    steps = prep.case.time_period[2] * prep.N
+    with prep.plan(fdtd.F32) as p:
+        v, fz = (int(x) for x in two.split(","))
+        assert p.autotune(0) == (32, 8, fz, 3)
+        assert "two_step_f32<32,8,%d>" % (v + 1) in p.kernels and "region_h_f32" in p.kernels
+        p.run_steps(0, steps - 3)
+        p.run_steps(steps - 3, 3)
+        H, E = p.read_fields()
+        mons = p.read_monitors()
+    Ho, Eo, mo, _ = _oracle_run(prep, 0, prep.case.time_period[2])
+    assert np.abs(Ho).max() > 0
+    assert rel_l2(H, Ho) < 2e-4 and rel_l2(E, Eo) < 2e-4, (rel_l2(H, Ho), rel_l2(E, Eo))
+    for a, b in zip(mons, mo):
+        if np.linalg.norm(b) > 0:
+            assert rel_l2(a.view(np.float64), b) < 2e-4
+
+
+def test_two_step_refused(monkeypatch):
+    """Lossy/metal cells, or a PML-free core too thin for the two-step box, keep the
+    one-step kernels."""
     monkeypatch.setenv("SMX_F32_KERNEL", "fused")
     monkeypatch.setenv("SMX_F32_TWO", "0,16")
-    for kw in (dict(pml_xy=0, pml_z=3, metal_cells=0, lossy_cells=0),
-               dict(pml_xy=0, pml_z=0, metal_cells=2, lossy_cells=0)):
+    for kw in (dict(pml_xy=0, pml_z=0, metal_cells=2, lossy_cells=0),
+               dict(L0=12, L1=12, pml_xy=4, pml_z=3, metal_cells=0, lossy_cells=0)):
         prep = casegen.prepare(small_waveguide(seed=15, **kw))
         with prep.plan(fdtd.F32) as p:
             assert p.autotune(0)[3] != 3
+            assert "two_step" not in p.kernels
 
 
 def test_autotune_keeps_state():
