@@ -1281,27 +1281,32 @@ __global__ __launch_bounds__(kThreads) void fused_dma_f32(fused_args a)
  * H1, E1, H2 of the neighbours go through LDS, the -z / +z neighbours of a thread's own
  * cells stay in its registers.
  *
- * This variant covers grids without PML and without lossy/metal cells (the host checks):
- * every cell takes the lossless update X -/+= b*curl.  Source terms (both steps) and the
- * reference monitors (both steps) are applied as in the single-step kernels.
+ * Grids with lossy/metal cells keep the one-step kernels (the host checks).  With SPLIT
+ * the stages of PML planes and of tiles touching x/y PML also carry the old splits, and
+ * PML cells take the main/split update of the one-step kernels, their splits carried
+ * like the fields; without it the kernel runs only where no PML cell is reached (the
+ * whole grid, or the box the region sweeps surround).  Source terms and the reference
+ * monitors of both steps are applied as in the one-step kernels.
  */
-template <int FX, int FY>
+template <int FX, int FY, bool SPLIT>
 struct two_layout {
     static constexpr int W4 = FX + 4, H4 = FY + 4, N4 = W4 * H4, NC = (N4 + 63) / 64;
-    static constexpr int ARR = 3 * NC * 64;   /* one array (3 components) of a stage */
-    static constexpr int SLOT = 4 * ARR;      /* E0, H0, bH, bE */
-    static constexpr int NBASE = 4 * 3 * NC;  /* copies per stage */
-    static constexpr int kMinOps = NBASE / 4; /* per wave, at least */
+    static constexpr int ARR = 3 * NC * 64;                /* one array (3 components) of a stage */
+    static constexpr int SLOT = (SPLIT ? 6 : 4) * ARR;     /* E0, H0, bH, bE [, SH0, SE0] */
+    static constexpr int NBASE = 4 * 3 * NC;               /* copies of every stage */
+    static constexpr int NSPLIT = 2 * 3 * NC;              /* extra copies of PML stages */
+    static constexpr int kMinOps = NBASE / 4;              /* per wave, at least */
     static constexpr int CELLS = (N4 + kThreads - 1) / kThreads; /* cells per thread */
+    static constexpr int kMaxFZ = 64;                      /* z tables of a chunk in LDS */
 };
 
-/* Copies of one stage dealt to wave W: array order E0, H0, bH, bE, each 3 components x
- * NC chunks of the G4 region (lane i -> word i of the chunk). */
-template <int FX, int FY, int W>
-__device__ inline void two_issue(float *slot, __amdgpu_buffer_rsrc_t rT, __amdgpu_buffer_rsrc_t rB, unsigned cb,
-                                 unsigned soff, const unsigned *vq)
+/* Copies of one stage dealt to wave W: arrays E0, H0, bH, bE (and SH0, SE0 when `split`),
+ * each 3 components x NC chunks of the G4 region (lane i -> word i of the chunk). */
+template <int FX, int FY, bool SPLIT, int W>
+__device__ inline void two_issue(float *slot, __amdgpu_buffer_rsrc_t rT, __amdgpu_buffer_rsrc_t rB,
+                                 __amdgpu_buffer_rsrc_t rS, unsigned cb, unsigned soff, bool split, const unsigned *vq)
 {
-    using Ly = two_layout<FX, FY>;
+    using Ly = two_layout<FX, FY, SPLIT>;
 #pragma unroll
     for (int j = W; j < Ly::NBASE; j += 4) {
         const int arr = j / (3 * Ly::NC), r = j % (3 * Ly::NC), c = r / Ly::NC, ch = r % Ly::NC;
@@ -1310,19 +1315,50 @@ __device__ inline void two_issue(float *slot, __amdgpu_buffer_rsrc_t rT, __amdgp
         float *dst = slot + arr * Ly::ARR + c * Ly::NC * 64 + ch * 64;
         __builtin_amdgcn_raw_ptr_buffer_load_lds(arr < 2 ? rT : rB, SMX_LDS(dst), 4, vq[ch] + comp * cb, soff, 0, 0);
     }
+    if (SPLIT && split) {
+#pragma unroll
+        for (int j = W; j < Ly::NSPLIT; j += 4) {
+            const int arr = j / (3 * Ly::NC), r = j % (3 * Ly::NC), c = r / Ly::NC, ch = r % Ly::NC;
+            float *dst = slot + (4 + arr) * Ly::ARR + c * Ly::NC * 64 + ch * 64; /* SH0 = S comp c, SE0 = 3+c */
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rS, SMX_LDS(dst), 4, vq[ch] + (arr * 3 + c) * cb, soff, 0, 0);
+        }
+    }
 }
 
-template <int FX, int FY, int D>
+/* New E of one regular cell: lossless X += b*(P + M), or the PML main/split pair. */
+__device__ inline void e_regular(const float e[3], const float b[3], const float se[3], const float P[3],
+                                 const float M[3], bool pml, const float pa[3], const float pg[3], float En[3],
+                                 float Sn[3])
+{
+    if (!pml) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            En[c] = e[c] + b[c] * (P[c] + M[c]);
+            Sn[c] = 0.f;
+        }
+        return;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const int dm = c == 2 ? 0 : c + 1, ds = c == 0 ? 2 : c - 1;
+        const float s2 = pa[ds] * se[c] + b[c] * pg[ds] * M[c];
+        En[c] = pa[dm] * (e[c] - se[c]) + b[c] * pg[dm] * P[c] + s2;
+        Sn[c] = s2;
+    }
+}
+
+template <int FX, int FY, int D, bool SPLIT>
 __global__ __launch_bounds__(kThreads) void two_step_f32(fused_args a)
 {
-    using Ly = two_layout<FX, FY>;
+    using Ly = two_layout<FX, FY, SPLIT>;
     constexpr int W4 = Ly::W4, H4 = Ly::H4, N4 = Ly::N4, NSLOT = D + 1, CELLS = Ly::CELLS;
     static_assert(FX * FY == kThreads && CELLS == 2, "tile shape");
     static_assert(D >= 1 && D * Ly::kMinOps < 64, "copies in flight must fit the vmcnt field");
-    /* one LDS variable (see fused_dma_f32): stage slots, then H1, E1, H2 of the previous
-     * plane in G4 layout */
+    /* one LDS variable (see fused_dma_f32): stage slots, H1, E1, H2 of the previous plane
+     * in G4 layout, the chunk's z tables, a flag word */
     constexpr int L_H1 = NSLOT * Ly::SLOT, L_E1 = L_H1 + 3 * N4, L_H2 = L_E1 + 3 * N4;
-    __shared__ float lds[L_H2 + 3 * N4];
+    constexpr int L_ZT = L_H2 + 3 * N4, L_FLAG = L_ZT + (SPLIT ? 5 * (Ly::kMaxFZ + 4) : 0);
+    __shared__ float lds[L_FLAG + 1];
 
     const int id = xcd_logical(blockIdx.x, a.nblocks);
     const int bxi = id % a.gx, rest = id / a.gx;
@@ -1331,6 +1367,8 @@ __global__ __launch_bounds__(kThreads) void two_step_f32(fused_args a)
     const int z1 = min(z0 + a.fz, a.box[5]);
     const int t = threadIdx.x, lane = t & 63;
     const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int ns = z1 - z0 + 4; /* stages: planes z0-2 .. z1+1 */
+    auto plane_of = [&](int s) { return wrap_index(z0 - 2 + s, a.L2); };
 
     unsigned vq[Ly::NC];
 #pragma unroll
@@ -1341,7 +1379,8 @@ __global__ __launch_bounds__(kThreads) void two_step_f32(fused_args a)
 
     /* this thread's cells of G4: e = t and t + 256 */
     int ce[CELLS], cx[CELLS], cy[CELLS];
-    bool inG3[CELLS], inG2[CELLS], inG1[CELLS], inT[CELLS];
+    bool inG3[CELLS], inG2[CELLS], inG1[CELLS], inT[CELLS], pxy[CELLS];
+    float tH[CELLS][4], tE[CELLS][4]; /* x/y PML factors: aH_x, gH_x, aH_y, gH_y (E likewise) */
     int64_t cq[CELLS];
 #pragma unroll
     for (int r = 0; r < CELLS; r++) {
@@ -1356,34 +1395,74 @@ __global__ __launch_bounds__(kThreads) void two_step_f32(fused_args a)
         inG2[r] = inG3[r] && gx < W4 - 1 && gy < H4 - 1;
         inG1[r] = inG2[r] && gx >= 2 && gy >= 2;
         inT[r] = inG1[r] && gx < W4 - 2 && gy < H4 - 2 && x0 - 2 + gx < a.box[1] && y0 - 2 + gy < a.box[3];
+        pxy[r] = false;
+#pragma unroll
+        for (int k = 0; k < 4; k++) tH[r][k] = tE[r][k] = 1.f;
+        if constexpr (SPLIT) {
+            const float *tx = a.pml, *ty = a.pml + 4 * a.L0;
+            pxy[r] = valid && (a.pmlflag[cx[r]] | a.pmlflag[a.L0 + cy[r]]);
+            tH[r][0] = tx[cx[r]], tH[r][1] = tx[a.L0 + cx[r]], tH[r][2] = ty[cy[r]], tH[r][3] = ty[a.L1 + cy[r]];
+            tE[r][0] = tx[2 * a.L0 + cx[r]], tE[r][1] = tx[3 * a.L0 + cx[r]];
+            tE[r][2] = ty[2 * a.L1 + cy[r]], tE[r][3] = ty[3 * a.L1 + cy[r]];
+        }
     }
+    bool tile_pml = false;
+    if constexpr (SPLIT) {
+        for (int i = t; i < ns; i += kThreads) {
+            const int z = plane_of(i);
+            const float *tz = a.pml + 4 * (a.L0 + a.L1);
+            lds[L_ZT + 5 * i + 0] = tz[z];
+            lds[L_ZT + 5 * i + 1] = tz[a.L2 + z];
+            lds[L_ZT + 5 * i + 2] = tz[2 * a.L2 + z];
+            lds[L_ZT + 5 * i + 3] = tz[3 * a.L2 + z];
+            lds[L_ZT + 5 * i + 4] = a.pmlflag[a.L0 + a.L1 + z] ? 1.f : 0.f;
+        }
+        /* does any cell of the G4 region lie in x/y PML? (vote through the LDS array) */
+        if (t == 0) lds[L_FLAG] = 0.f;
+        __syncthreads();
+        if (pxy[0] | pxy[1]) lds[L_FLAG] = 1.f;
+        __syncthreads();
+        tile_pml = lds[L_FLAG] != 0.f;
+        retire_loads(); /* the table loads, before the copies start */
+    }
+    auto zpml = [&](int s) { return SPLIT && lds[L_ZT + 5 * s + 4] != 0.f; };
 
-    const __amdgpu_buffer_rsrc_t rT = group_rsrc(a.To[0], a.ncomp), rB = group_rsrc(a.b[0], a.ncomp);
+    const __amdgpu_buffer_rsrc_t rT = group_rsrc(a.To[0], a.ncomp), rB = group_rsrc(a.b[0], a.ncomp),
+                                 rS = group_rsrc(a.So[0], a.ncomp);
     const unsigned cb = 4u * (unsigned)a.ncomp;
-    auto plane_of = [&](int s) { return wrap_index(z0 - 2 + s, a.L2); }; /* stage s: plane z0-2+s */
     auto issue = [&](int s) {
         const unsigned soff = 4u * (unsigned)((int64_t)plane_of(s) * a.sz);
+        const bool split = SPLIT && (tile_pml || zpml(s));
         float *slot = lds + (s % NSLOT) * Ly::SLOT;
         switch (wv) {
-        case 0: two_issue<FX, FY, 0>(slot, rT, rB, cb, soff, vq); break;
-        case 1: two_issue<FX, FY, 1>(slot, rT, rB, cb, soff, vq); break;
-        case 2: two_issue<FX, FY, 2>(slot, rT, rB, cb, soff, vq); break;
-        default: two_issue<FX, FY, 3>(slot, rT, rB, cb, soff, vq); break;
+        case 0: two_issue<FX, FY, SPLIT, 0>(slot, rT, rB, rS, cb, soff, split, vq); break;
+        case 1: two_issue<FX, FY, SPLIT, 1>(slot, rT, rB, rS, cb, soff, split, vq); break;
+        case 2: two_issue<FX, FY, SPLIT, 2>(slot, rT, rB, rS, cb, soff, split, vq); break;
+        default: two_issue<FX, FY, SPLIT, 3>(slot, rT, rB, rS, cb, soff, split, vq); break;
         }
     };
-    /* stage arrays: 0 = E0, 1 = H0, 2 = bH, 3 = bE */
+    /* stage arrays: 0 = E0, 1 = H0, 2 = bH, 3 = bE, 4 = SH0, 5 = SE0 */
     auto st = [&](const float *S, int arr, int c, int e) { return S[arr * Ly::ARR + c * Ly::NC * 64 + e]; };
+    /* PML factors of cell r on the plane of stage s: H (integer samples) or E (half) */
+    auto factors = [&](int r, int s, bool e_field, float pa[3], float pg[3]) {
+        const float *tt = e_field ? tE[r] : tH[r];
+        pa[0] = tt[0], pg[0] = tt[1], pa[1] = tt[2], pg[1] = tt[3];
+        pa[2] = SPLIT ? lds[L_ZT + 5 * s + (e_field ? 2 : 0)] : 1.f;
+        pg[2] = SPLIT ? lds[L_ZT + 5 * s + (e_field ? 3 : 1)] : 1.f;
+    };
 
-    const int ns = z1 - z0 + 4; /* stages: planes z0-2 .. z1+1 */
     for (int s = 0; s < D && s < ns; s++) issue(s);
 
-    /* carried per cell: E0 and bE of the previous plane, bH and H1 of the previous plane,
-     * E1 and bE two planes back */
-    float e0p[CELLS][3], bep[CELLS][3], bhp[CELLS][3], h1p[CELLS][3], e1pp[CELLS][3], bepp[CELLS][3];
+    /* carried per cell: E0, SE0, bE of the previous plane; bH, H1, SH1 of the previous
+     * plane; E1, SE1, bE two planes back */
+    float e0p[CELLS][3], se0p[CELLS][3], bep[CELLS][3], bhp[CELLS][3], h1p[CELLS][3], sh1p[CELLS][3];
+    float e1pp[CELLS][3], se1pp[CELLS][3], bepp[CELLS][3];
 #pragma unroll
     for (int r = 0; r < CELLS; r++)
 #pragma unroll
-        for (int c = 0; c < 3; c++) e0p[r][c] = bep[r][c] = bhp[r][c] = h1p[r][c] = e1pp[r][c] = bepp[r][c] = 0.f;
+        for (int c = 0; c < 3; c++)
+            e0p[r][c] = se0p[r][c] = bep[r][c] = bhp[r][c] = h1p[r][c] = sh1p[r][c] = e1pp[r][c] = se1pp[r][c] =
+                bepp[r][c] = 0.f;
 
     for (int s = 0; s < ns; s++) {
         if (s + D < ns) {
@@ -1401,23 +1480,28 @@ __global__ __launch_bounds__(kThreads) void two_step_f32(fused_args a)
         const int zp = plane_of(s);   /* its wrapped index */
         const int zp1 = plane_of(s - 1), zp2 = plane_of(s - 2);
         const bool doA = s >= 1, doB = p >= z0, doC = p >= z0 + 1, doD = p >= z0 + 2;
+        const bool zs = zpml(s), zs1 = s >= 1 && zpml(s - 1), zs2 = s >= 2 && zpml(s - 2);
 
-        float h1n[CELLS][3], e1n[CELLS][3], h2n[CELLS][3];
+        float h1n[CELLS][3], sh1n[CELLS][3], e1n[CELLS][3], se1n[CELLS][3], h2n[CELLS][3];
 #pragma unroll
         for (int r = 0; r < CELLS; r++)
 #pragma unroll
-            for (int c = 0; c < 3; c++) h1n[r][c] = e1n[r][c] = h2n[r][c] = 0.f;
+            for (int c = 0; c < 3; c++) h1n[r][c] = sh1n[r][c] = e1n[r][c] = se1n[r][c] = h2n[r][c] = 0.f;
         /* a. H1(p) on G3 */
 #pragma unroll
         for (int r = 0; r < CELLS; r++) {
             if (!(doA && inG3[r])) continue;
             const int e = ce[r];
-            float P[3], M[3], ec[3], exm[3], eym[3];
+            const bool pml = pxy[r] || zs;
+            float P[3], M[3], ec[3], exm[3], eym[3], h[3], b[3], sp[3];
 #pragma unroll
             for (int c = 0; c < 3; c++) {
                 ec[c] = st(S, 0, c, e);
                 exm[c] = st(S, 0, c, e - 1);
                 eym[c] = st(S, 0, c, e - W4);
+                h[c] = st(S, 1, c, e);
+                b[c] = st(S, 2, c, e);
+                sp[c] = SPLIT && pml ? st(S, 4, c, e) : 0.f;
             }
             const float wx = cx[r] == 0 ? a.w[0] : 1.f, wy = cy[r] == 0 ? a.w[1] : 1.f, wz = zp == 0 ? a.w[2] : 1.f;
             P[0] = ec[2] - wy * eym[2];
@@ -1426,19 +1510,21 @@ __global__ __launch_bounds__(kThreads) void two_step_f32(fused_args a)
             M[1] = -ec[2] + wx * exm[2];
             P[2] = ec[1] - wx * exm[1];
             M[2] = -ec[0] + wy * eym[0];
-            if (zp == a.iI) {
+            if (!pml && zp == a.iI) {
                 P[0] += a.sgn * (float)plane_source(nullptr, a.Einc, a.srcE_f2, a.srcE_re, a.srcE_im, cq[r], 1);
                 P[1] -= a.sgn * (float)plane_source(nullptr, a.Einc, a.srcE_f2, a.srcE_re, a.srcE_im, cq[r], 0);
                 retire_loads();
             }
-#pragma unroll
-            for (int c = 0; c < 3; c++) h1n[r][c] = st(S, 1, c, e) - st(S, 2, c, e) * (P[c] + M[c]);
+            float pa[3], pg[3];
+            factors(r, s, false, pa, pg);
+            h_regular(h, b, sp, P, M, pml, pa, pg, h1n[r], sh1n[r]);
         }
         /* b. E1(p-1) on G2: forward curl of H1 (previous plane from LDS, +z from a.) */
 #pragma unroll
         for (int r = 0; r < CELLS; r++) {
             if (!(doB && inG2[r])) continue;
             const int e = ce[r];
+            const bool pml = pxy[r] || zs1;
             float P[3], M[3], hc[3], hxp[3], hyp[3];
 #pragma unroll
             for (int c = 0; c < 3; c++) {
@@ -1454,15 +1540,16 @@ __global__ __launch_bounds__(kThreads) void two_step_f32(fused_args a)
             M[1] = -wx * hxp[2] + hc[2];
             P[2] = wx * hxp[1] - hc[1];
             M[2] = -wy * hyp[0] + hc[0];
-            if (zp1 == a.iI) {
+            if (!pml && zp1 == a.iI) {
                 P[0] += a.sgn * (float)plane_source(nullptr, a.Hinc, a.srcH_f2, a.srcH_re, a.srcH_im, cq[r], 1);
                 P[1] -= a.sgn * (float)plane_source(nullptr, a.Hinc, a.srcH_f2, a.srcH_re, a.srcH_im, cq[r], 0);
                 retire_loads();
             }
-#pragma unroll
-            for (int c = 0; c < 3; c++) e1n[r][c] = e0p[r][c] + bep[r][c] * (P[c] + M[c]);
-            if (inT[r] && p - 1 >= z0 && p - 1 < z1 && (zp1 == a.iB || zp1 == a.iT)) { /* first step's monitors */
-                for (int m = 0; m < 2; m++) {
+            float pa[3], pg[3];
+            factors(r, s - 1, true, pa, pg);
+            e_regular(e0p[r], bep[r], se0p[r], P, M, pml, pa, pg, e1n[r], se1n[r]);
+            if (!pml && inT[r] && p - 1 >= z0 && p - 1 < z1 && (zp1 == a.iB || zp1 == a.iT)) {
+                for (int m = 0; m < 2; m++) { /* first step's monitors: main = total (S = 0) */
                     if (zp1 != (m == 0 ? a.iB : a.iT)) continue;
                     double *HM = m == 0 ? a.HB : a.HT, *EM = m == 0 ? a.EB : a.ET;
                     for (int v = 0; v < 2; v++) {
@@ -1485,7 +1572,8 @@ __global__ __launch_bounds__(kThreads) void two_step_f32(fused_args a)
         for (int r = 0; r < CELLS; r++) {
             if (!(doC && inG1[r])) continue;
             const int e = ce[r];
-            float P[3], M[3], ec[3], exm[3], eym[3];
+            const bool pml = pxy[r] || zs1;
+            float P[3], M[3], ec[3], exm[3], eym[3], sh2[3];
 #pragma unroll
             for (int c = 0; c < 3; c++) {
                 ec[c] = lds[L_E1 + c * N4 + e];
@@ -1499,21 +1587,26 @@ __global__ __launch_bounds__(kThreads) void two_step_f32(fused_args a)
             M[1] = -ec[2] + wx * exm[2];
             P[2] = ec[1] - wx * exm[1];
             M[2] = -ec[0] + wy * eym[0];
-            if (zp1 == a.iI) {
+            if (!pml && zp1 == a.iI) {
                 P[0] += a.sgn * (float)plane_source(nullptr, a.Einc, a.srcE2_f2, a.srcE2_re, a.srcE2_im, cq[r], 1);
                 P[1] -= a.sgn * (float)plane_source(nullptr, a.Einc, a.srcE2_f2, a.srcE2_re, a.srcE2_im, cq[r], 0);
                 retire_loads();
             }
-#pragma unroll
-            for (int c = 0; c < 3; c++) h2n[r][c] = h1p[r][c] - bhp[r][c] * (P[c] + M[c]);
-            if (inT[r] && p - 1 < z1) store3(a.Tn, cq[r] + a.sz * zp1, h2n[r]);
+            float pa[3], pg[3];
+            factors(r, s - 1, false, pa, pg);
+            h_regular(h1p[r], bhp[r], sh1p[r], P, M, pml, pa, pg, h2n[r], sh2);
+            if (inT[r] && p - 1 < z1) {
+                store3(a.Tn, cq[r] + a.sz * zp1, h2n[r]);
+                if (SPLIT && pml) store3(a.Sn, cq[r] + a.sz * zp1, sh2);
+            }
         }
         /* d. E2(p-2) on T: forward curl of H2 (previous plane from LDS, +z from c.) */
 #pragma unroll
         for (int r = 0; r < CELLS; r++) {
             if (!(doD && inT[r])) continue;
             const int e = ce[r];
-            float P[3], M[3], hc[3], hxp[3], hyp[3], en[3];
+            const bool pml = pxy[r] || zs2;
+            float P[3], M[3], hc[3], hxp[3], hyp[3], en[3], sn[3];
 #pragma unroll
             for (int c = 0; c < 3; c++) {
                 hc[c] = lds[L_H2 + c * N4 + e];
@@ -1528,15 +1621,17 @@ __global__ __launch_bounds__(kThreads) void two_step_f32(fused_args a)
             M[1] = -wx * hxp[2] + hc[2];
             P[2] = wx * hxp[1] - hc[1];
             M[2] = -wy * hyp[0] + hc[0];
-            if (zp2 == a.iI) {
+            if (!pml && zp2 == a.iI) {
                 P[0] += a.sgn * (float)plane_source(nullptr, a.Hinc, a.srcH2_f2, a.srcH2_re, a.srcH2_im, cq[r], 1);
                 P[1] -= a.sgn * (float)plane_source(nullptr, a.Hinc, a.srcH2_f2, a.srcH2_re, a.srcH2_im, cq[r], 0);
                 retire_loads();
             }
-#pragma unroll
-            for (int c = 0; c < 3; c++) en[c] = e1pp[r][c] + bepp[r][c] * (P[c] + M[c]);
+            float pa[3], pg[3];
+            factors(r, s - 2, true, pa, pg);
+            e_regular(e1pp[r], bepp[r], se1pp[r], P, M, pml, pa, pg, en, sn);
             store3(a.Tn + 3, cq[r] + a.sz * zp2, en);
-            if (zp2 == a.iB || zp2 == a.iT) { /* second step's monitors */
+            if (SPLIT && pml) store3(a.Sn + 3, cq[r] + a.sz * zp2, sn);
+            if (!pml && (zp2 == a.iB || zp2 == a.iT)) { /* second step's monitors */
                 for (int m = 0; m < 2; m++) {
                     if (zp2 != (m == 0 ? a.iB : a.iT)) continue;
                     double *HM = m == 0 ? a.HB : a.HT, *EM = m == 0 ? a.EB : a.ET;
@@ -1552,16 +1647,20 @@ __global__ __launch_bounds__(kThreads) void two_step_f32(fused_args a)
             }
         }
         /* carry to the next plane */
+        const bool split_s = SPLIT && (tile_pml || zs);
 #pragma unroll
         for (int r = 0; r < CELLS; r++)
 #pragma unroll
             for (int c = 0; c < 3; c++) {
                 e1pp[r][c] = e1n[r][c];
+                se1pp[r][c] = se1n[r][c];
                 bepp[r][c] = bep[r][c];
                 h1p[r][c] = h1n[r][c];
+                sh1p[r][c] = sh1n[r][c];
                 bhp[r][c] = st(S, 2, c, ce[r]);
                 bep[r][c] = st(S, 3, c, ce[r]);
                 e0p[r][c] = st(S, 0, c, ce[r]);
+                se0p[r][c] = split_s ? st(S, 5, c, ce[r]) : 0.f;
             }
         /* every wave is done with this stage and with H1, H2 of the previous plane */
         wait_vm_barrier<63>();
@@ -1915,17 +2014,21 @@ const char *smx_f32_last_error(void) { return f32_err; }
 
 int smx_f32_num_tiles(void) { return kNumTiles; }
 
-/* two-step variants: 32x8 tiles, one or two planes of copies in flight */
-constexpr int kNumTwo = 2;
+/* two-step variants, 32x8 tiles: 0, 1 = one or two planes of copies in flight, no
+ * split slots (PML only outside the kernel's box, region sweeps for the rest); 2 = one
+ * plane in flight with split slots, PML cells handled in the kernel */
+constexpr int kNumTwo = 3;
 
 int smx_f32_num_two(void) { return kNumTwo; }
 
 int smx_f32_two_ok(const smx_f32_state *st, int variant, int fz)
 {
-    /* no lossy/metal cell; PML only outside the two-step box (region sweeps do the
-     * rest); 32-bit offsets as the staged kernel */
-    return variant >= 0 && variant < kNumTwo && fz >= 1 && st->nspecial == 0 && (!st->has_pml || st->split2_ok) &&
-           24 * st->n < ((int64_t)1 << 32);
+    /* no lossy/metal cell; 32-bit offsets as the staged kernel; variants 0-1 need the
+     * PML outside the kernel's box, variant 2 keeps the chunk's z tables in LDS */
+    if (variant < 0 || variant >= kNumTwo || fz < 1 || st->nspecial != 0 || 24 * st->n >= ((int64_t)1 << 32))
+        return 0;
+    if (variant == 2) return fz <= two_layout<32, 8, true>::kMaxFZ;
+    return !st->has_pml || st->split2_ok;
 }
 
 const char *smx_f32_kernel_name(const smx_f32_state *st)
@@ -1942,10 +2045,12 @@ const char *smx_f32_kernel_name(const smx_f32_state *st)
         "region_h_f32 + fused_dma_f32<32,8,4,false> + region_e_f32",
         "region_h_f32 + fused_dma_f32<32,8,2,false> + region_e_f32",
         "region_h_f32 + fused_dma_f32<32,8,1,false> + region_e_f32"};
-    static const char *const two[kNumTwo] = {"two_step_f32<32,8,1>", "two_step_f32<32,8,2>"};
+    static const char *const two[kNumTwo] = {"two_step_f32<32,8,1,false>", "two_step_f32<32,8,2,false>",
+                                             "two_step_f32<32,8,1,true>"};
     static const char *const two_split[kNumTwo] = {
-        "region_h_f32 + region_e_f32 + two_step_f32<32,8,1> + region_h_f32 + region_e_f32",
-        "region_h_f32 + region_e_f32 + two_step_f32<32,8,2> + region_h_f32 + region_e_f32"};
+        "region_h_f32 + region_e_f32 + two_step_f32<32,8,1,false> + region_h_f32 + region_e_f32",
+        "region_h_f32 + region_e_f32 + two_step_f32<32,8,2,false> + region_h_f32 + region_e_f32",
+        "two_step_f32<32,8,1,true>"};
     if (st->two >= 0 && st->two < kNumTwo) return st->has_pml ? two_split[st->two] : two[st->two];
     if (!st->fused) return st->bx == 64 ? "h_sweep_f32<64> + e_sweep_f32<64>" : "h_sweep_f32<32> + e_sweep_f32<32>";
     if (st->tile < 0 || st->tile >= kNumTiles) return "";
@@ -2338,9 +2443,12 @@ hipError_t smx_f32_step2(smx_f32_state *st, const smx_f32_launch_args *la, hipSt
     a.EB = la->EB;
     a.HT = la->HT;
     a.ET = la->ET;
-    if (!st->has_pml) {
-        if (st->two == 1) hipLaunchKernelGGL((two_step_f32<32, 8, 2>), dim3(a.nblocks), dim3(kThreads), 0, stream, a);
-        else hipLaunchKernelGGL((two_step_f32<32, 8, 1>), dim3(a.nblocks), dim3(kThreads), 0, stream, a);
+    if (!st->has_pml || st->two == 2) {
+        if (st->two == 2)
+            hipLaunchKernelGGL((two_step_f32<32, 8, 1, true>), dim3(a.nblocks), dim3(kThreads), 0, stream, a);
+        else if (st->two == 1)
+            hipLaunchKernelGGL((two_step_f32<32, 8, 2, false>), dim3(a.nblocks), dim3(kThreads), 0, stream, a);
+        else hipLaunchKernelGGL((two_step_f32<32, 8, 1, false>), dim3(a.nblocks), dim3(kThreads), 0, stream, a);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         st->cur = 1 - st->cur;
@@ -2410,8 +2518,8 @@ hipError_t smx_f32_step2(smx_f32_state *st, const smx_f32_launch_args *la, hipSt
     a.gx = (a.box[1] - a.box[0] + 31) / 32;
     a.gy = (a.box[3] - a.box[2] + 7) / 8;
     a.nblocks = a.gx * a.gy * ((a.box[5] - a.box[4] + a.fz - 1) / a.fz);
-    if (st->two == 1) hipLaunchKernelGGL((two_step_f32<32, 8, 2>), dim3(a.nblocks), dim3(kThreads), 0, stream, a);
-    else hipLaunchKernelGGL((two_step_f32<32, 8, 1>), dim3(a.nblocks), dim3(kThreads), 0, stream, a);
+    if (st->two == 1) hipLaunchKernelGGL((two_step_f32<32, 8, 2, false>), dim3(a.nblocks), dim3(kThreads), 0, stream, a);
+    else hipLaunchKernelGGL((two_step_f32<32, 8, 1, false>), dim3(a.nblocks), dim3(kThreads), 0, stream, a);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     /* 4: H2 from tmp (H1, E1, H1 split) into new */
     if ((e = region(st->regP2, false, tT, tS, nT, nS, 2)) != hipSuccess) return e;

@@ -239,8 +239,11 @@ def test_region_sweeps_carry_monitor_plane(tile, monkeypatch):
         assert rel_l2(a.view(np.float64), b) < 2e-4
 
 
+TWO_NAMES = ("two_step_f32<32,8,1,false>", "two_step_f32<32,8,2,false>", "two_step_f32<32,8,1,true>")
+
+
 @pytest.mark.parametrize("two,iBc", [("0,16", (1, -1, 1)), ("1,8", (0, 0, 0)), ("0,5", (1, 1, -1)),
-                                     ("1,64", (-1, 1, 1)), ("0,41", (1, -1, 0))])
+                                     ("1,64", (-1, 1, 1)), ("0,41", (1, -1, 0)), ("2,16", (1, -1, 1))])
 def test_two_step_pass(two, iBc, monkeypatch):
     """Two steps per pass (grids without PML and lossy/metal cells) against the FP64
     oracle, fields and both monitor planes, over a run with an odd number of steps (the
@@ -254,7 +257,7 @@ def test_two_step_pass(two, iBc, monkeypatch):
     with prep.plan(fdtd.F32) as p:
         v, fz = (int(x) for x in two.split(","))
         assert p.autotune(0) == (32, 8, fz, 3)
-        assert p.kernels == "two_step_f32<32,8,%d>" % (v + 1)
+        assert p.kernels == TWO_NAMES[v]
         sums = p.run_steps(0, steps - 1)
         p.run_steps(steps - 1, 1, sums)
         H, E = p.read_fields()
@@ -292,11 +295,16 @@ def test_two_step_equals_one_step_kernels(monkeypatch):
     ("1,8", dict(pml_xy=0, pml_z=3, monitor_gap=0, iBc=(0, 0, 0))),    # monitor planes in the region sweeps
     ("0,5", dict(pml_xy=2, pml_z=3, iBc=(0, 0, 0))),                   # PML on every face
     ("1,64", dict(pml_xy=3, pml_z=4, iBc=(1, 1, 0), scale_amp=0.3)),
+    # split slots: PML cells inside the two-step kernel
+    ("2,16", dict(pml_xy=0, pml_z=3, iBc=(1, -1, 0))),
+    ("2,8", dict(pml_xy=0, pml_z=3, monitor_gap=0, iBc=(0, 0, 0))),
+    ("2,5", dict(pml_xy=2, pml_z=3, iBc=(0, 0, 0))),
+    ("2,64", dict(pml_xy=3, pml_z=4, iBc=(1, 1, 0), scale_amp=0.3)),
 ])
 def test_two_step_pass_with_pml(two, kw, monkeypatch):
-    """Two steps per pass on grids with PML: the kernel on the box whose two-step cone is
-    PML-free, the region sweeps (first step into a scratch set, second into the new
-    buffer) on the rest, against the FP64 oracle."""
+    """Two steps per pass on grids with PML, against the FP64 oracle: variants 0-1 on the
+    box whose two-step cone is PML-free with the region sweeps (first step into a scratch
+    set, second into the new buffer) on the rest; variant 2 with the PML cells inside."""
     monkeypatch.setenv("SMX_F32_KERNEL", "fused")
     monkeypatch.setenv("SMX_F32_TWO", two)
     prep = casegen.prepare(small_waveguide(seed=16, L0=37, L1=22, L2=44, outer=9, metal_cells=0,
@@ -306,7 +314,7 @@ def test_two_step_pass_with_pml(two, kw, monkeypatch):
     with prep.plan(fdtd.F32) as p:
         v, fz = (int(x) for x in two.split(","))
         assert p.autotune(0) == (32, 8, fz, 3)
-        assert "two_step_f32<32,8,%d>" % (v + 1) in p.kernels and "region_h_f32" in p.kernels
+        assert TWO_NAMES[v] in p.kernels and ("region_h_f32" in p.kernels) == (v < 2)
         p.run_steps(0, steps - 3)
         p.run_steps(steps - 3, 3)
         H, E = p.read_fields()
