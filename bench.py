@@ -39,10 +39,12 @@ def bytes_per_cell(phi, field_bytes, coef_bytes):
     return field_bytes * (12 + 12 * phi) + 6 * coef_bytes
 
 
-def measured_traffic(kernels):
-    """HBM bytes of one step of `kernels` from the committed rocprofv3 FETCH_SIZE and
-    WRITE_SIZE averages (KiB per dispatch) of the 160x160x320 grid, each scaled by what
-    the counter reads for a plain device copy of known size.  None when a kernel is not
+def measured_traffic(kernels, steps_per_pass=1):
+    """HBM bytes of one step from the committed rocprofv3 FETCH_SIZE and WRITE_SIZE
+    averages (KiB per dispatch) of the 160x160x320 grid, each scaled by what the counter
+    reads for a plain device copy of known size.  `kernels` lists the launches of one
+    pass (a kernel launched twice per pass appears twice; its average is over all its
+    dispatches), and a pass advances `steps_per_pass` steps.  None when a kernel is not
     listed there."""
     try:
         with open(PMC_SUMMARY) as f:
@@ -58,9 +60,9 @@ def measured_traffic(kernels):
             scale = (float(w[2]), float(w[4]))
         else:
             table[w[0]] = (float(w[1]), float(w[2]))
-    if scale is None or any(k not in table for k in kernels) or len(set(kernels)) != len(kernels):
-        return None  # a kernel launched twice per step with different extents has no single average
-    return sum((table[k][0] * scale[0] + table[k][1] * scale[1]) * 1024 for k in kernels)
+    if scale is None or any(k not in table for k in kernels):
+        return None
+    return sum((table[k][0] * scale[0] + table[k][1] * scale[1]) * 1024 for k in kernels) / steps_per_pass
 
 
 def copy_bandwidth(torch, mib=1024, reps=5):
@@ -334,7 +336,7 @@ def main():
 
     copy_gbs = copy_bandwidth(torch)
     print(f"[bench] device copy: {copy_gbs:.0f} GB/s", file=sys.stderr, flush=True)
-    traffic = None if args.small else measured_traffic(step_kernels)
+    traffic = None if args.small else measured_traffic(step_kernels, 2 if variant == 3 else 1)
 
     cpu = None
     if not args.no_cpu:
