@@ -1353,6 +1353,7 @@ __global__ __launch_bounds__(kThreads) void two_step_f32(fused_args a)
     using Ly = two_layout<FX, FY, SPLIT>;
     constexpr int W4 = Ly::W4, H4 = Ly::H4, N4 = Ly::N4, NSLOT = D + 1, CELLS = Ly::CELLS;
     static_assert(FX * FY == kThreads && CELLS == 2, "tile shape");
+    static_assert((FY + 1 + FX) + (H4 - 2 + W4 - 3) + (H4 - 1 + W4 - 2) <= kThreads, "rings fit one cell per thread");
     static_assert(D >= 1 && D * Ly::kMinOps < 64, "copies in flight must fit the vmcnt field");
     /* one LDS variable (see fused_dma_f32): stage slots, H1, E1, H2 of the previous plane
      * in G4 layout, the chunk's z tables, a flag word */
@@ -1377,24 +1378,53 @@ __global__ __launch_bounds__(kThreads) void two_step_f32(fused_args a)
         vq[ch] = 4u * (unsigned)(wrap_index(x0 - 2 + e % W4, a.L0) + a.L0 * wrap_index(y0 - 2 + e / W4, a.L1));
     }
 
-    /* this thread's cells of G4: e = t and t + 256 */
+    /* this thread's cells: r = 0 the tile cell t (every step computes it); r = 1 one cell
+     * of the rings the first three updates add around the tile -- G1\T (used by c, b,
+     * a), G2\G1 (b, a), G3\G2 (a), 129 cells for 32x8 -- ring cell h on thread
+     * (h % 4)*64 + h/4, so the four waves share the extra work evenly; G4\G3 is only
+     * read.  Local coordinates (gx, gy) count from the G4 corner (x0-2, y0-2). */
+    constexpr int RC = FY + 1 + FX, RB = RC + H4 - 2 + W4 - 3, RA = RB + H4 - 1 + W4 - 2; /* ring ends */
     int ce[CELLS], cx[CELLS], cy[CELLS];
     bool inG3[CELLS], inG2[CELLS], inG1[CELLS], inT[CELLS], pxy[CELLS];
     float tH[CELLS][4], tE[CELLS][4]; /* x/y PML factors: aH_x, gH_x, aH_y, gH_y (E likewise) */
     int64_t cq[CELLS];
 #pragma unroll
     for (int r = 0; r < CELLS; r++) {
-        const int e = t + r * kThreads;
-        const bool valid = e < N4;
-        const int gx = e % W4, gy = e / W4;
+        int gx, gy, ring = 0; /* ring: 0 tile, 1 G1\T, 2 G2\G1, 3 G3\G2, 4 none */
+        if (r == 0) {
+            gx = 2 + t % FX;
+            gy = 2 + t / FX;
+        } else {
+            const int h = (t & 63) * 4 + (t >> 6);
+            if (h < RC) { /* x = x0+FX column, then y = y0+FY row */
+                ring = 1;
+                gx = h < FY + 1 ? W4 - 2 : 2 + (h - (FY + 1));
+                gy = h < FY + 1 ? 2 + h : H4 - 2;
+            } else if (h < RB) { /* x = x0-1 column, then y = y0-1 row */
+                const int k = h - RC;
+                ring = 2;
+                gx = k < H4 - 2 ? 1 : 2 + (k - (H4 - 2));
+                gy = k < H4 - 2 ? 1 + k : 1;
+            } else if (h < RA) { /* x = x0+FX+1 column, then y = y0+FY+1 row */
+                const int k = h - RB;
+                ring = 3;
+                gx = k < H4 - 1 ? W4 - 1 : 1 + (k - (H4 - 1));
+                gy = k < H4 - 1 ? 1 + k : H4 - 1;
+            } else {
+                ring = 4;
+                gx = gy = 0;
+            }
+        }
+        const bool valid = ring < 4;
+        const int e = gy * W4 + gx;
         ce[r] = valid ? e : 0;
         cx[r] = wrap_index(x0 - 2 + gx, a.L0);
         cy[r] = wrap_index(y0 - 2 + gy, a.L1);
         cq[r] = cx[r] + (int64_t)a.L0 * cy[r];
-        inG3[r] = valid && gx >= 1 && gy >= 1;
-        inG2[r] = inG3[r] && gx < W4 - 1 && gy < H4 - 1;
-        inG1[r] = inG2[r] && gx >= 2 && gy >= 2;
-        inT[r] = inG1[r] && gx < W4 - 2 && gy < H4 - 2 && x0 - 2 + gx < a.box[1] && y0 - 2 + gy < a.box[3];
+        inG3[r] = ring <= 3;
+        inG2[r] = ring <= 2;
+        inG1[r] = ring <= 1;
+        inT[r] = ring == 0 && x0 - 2 + gx < a.box[1] && y0 - 2 + gy < a.box[3];
         pxy[r] = false;
 #pragma unroll
         for (int k = 0; k < 4; k++) tH[r][k] = tE[r][k] = 1.f;
