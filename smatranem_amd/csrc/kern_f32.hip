@@ -458,6 +458,8 @@ struct fused_args {
     /* box of the staged kernel's tiles: [x0, x1) x [y0, y1) x [z0, z1) (the whole grid
      * unless the cells outside it are left to the region sweeps) */
     int box[6];
+    /* two-step kernel: x of its first tile column, and the (unwrapped) x range it stores */
+    int xorg, xlo, xhi;
     /* boxes of the region sweeps and their first thread index */
     int nreg;
     int reg[6][6];
@@ -896,7 +898,10 @@ __device__ inline __amdgpu_buffer_rsrc_t group_rsrc(const float *base, int64_t n
     return __builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, (int)(unsigned)(24 * ncomp), (int)kBufWord3);
 }
 
-#define SMX_LDS(p) ((__attribute__((address_space(3))) void *)(p))
+/* LDS address of a generic pointer into the kernel's LDS array: its low 32 bits (a
+ * plain cast would add a null test, which the code generator cannot emit when the
+ * pointer folds to the array's base) */
+#define SMX_LDS(p) ((__attribute__((address_space(3))) void *)(size_t)(unsigned)(size_t)(p))
 
 /* Copies of stage slot `slot` (plane byte offset soff) dealt to wave W.  Component c of
  * a group starts c*cb bytes into the group's buffer. */
@@ -1288,39 +1293,51 @@ __global__ __launch_bounds__(kThreads) void fused_dma_f32(fused_args a)
  * whole grid, or the box the region sweeps surround).  Source terms and the reference
  * monitors of both steps are applied as in the one-step kernels.
  */
-template <int FX, int FY, bool SPLIT>
+template <int FX, int FY, bool SPLIT, bool WIDE>
 struct two_layout {
-    static constexpr int W4 = FX + 4, H4 = FY + 4, N4 = W4 * H4, NC = (N4 + 63) / 64;
-    static constexpr int ARR = 3 * NC * 64;                /* one array (3 components) of a stage */
+    static constexpr int NT = FX * FY, NW = NT / 64;        /* one thread per tile cell */
+    /* G4 starts OX columns left of the tile.  WIDE: every copy moves 16 bytes per lane (4
+     * consecutive x, lane i -> words 4i..4i+3 of a 256-word run), the region starts at a
+     * multiple of 4 and is 4 columns wider, a quarter of the copy instructions */
+    static constexpr int OX = WIDE ? 4 : 2;
+    static constexpr int W4 = FX + 2 * OX, H4 = FY + 4, N4 = W4 * H4;
+    static constexpr int CH = WIDE ? 256 : 64;             /* words per copy instruction */
+    static constexpr int NC = (N4 + CH - 1) / CH;          /* copies per component */
+    static constexpr int ARR = 3 * NC * CH;                /* one array (3 components) of a stage */
     static constexpr int SLOT = (SPLIT ? 6 : 4) * ARR;     /* E0, H0, bH, bE [, SH0, SE0] */
     static constexpr int NBASE = 4 * 3 * NC;               /* copies of every stage */
     static constexpr int NSPLIT = 2 * 3 * NC;              /* extra copies of PML stages */
-    static constexpr int kMinOps = NBASE / 4;              /* per wave, at least */
-    static constexpr int CELLS = (N4 + kThreads - 1) / kThreads; /* cells per thread */
+    static constexpr int kMinOps = NBASE / NW;             /* per wave, at least */
+    static constexpr int CELLS = 2;                        /* a tile cell and a ring cell */
     static constexpr int kMaxFZ = 64;                      /* z tables of a chunk in LDS */
 };
 
 /* Copies of one stage dealt to wave W: arrays E0, H0, bH, bE (and SH0, SE0 when `split`),
- * each 3 components x NC chunks of the G4 region (lane i -> word i of the chunk). */
-template <int FX, int FY, bool SPLIT, int W>
+ * each 3 components x NC runs of CH words of the G4 region. */
+template <int FX, int FY, bool SPLIT, bool WIDE, int W>
 __device__ inline void two_issue(float *slot, __amdgpu_buffer_rsrc_t rT, __amdgpu_buffer_rsrc_t rB,
                                  __amdgpu_buffer_rsrc_t rS, unsigned cb, unsigned soff, bool split, const unsigned *vq)
 {
-    using Ly = two_layout<FX, FY, SPLIT>;
+    using Ly = two_layout<FX, FY, SPLIT, WIDE>;
+    /* the copy's size must be a literal */
+    auto copy = [](__amdgpu_buffer_rsrc_t r, float *dst, unsigned voff, unsigned so) {
+        if constexpr (WIDE) __builtin_amdgcn_raw_ptr_buffer_load_lds(r, SMX_LDS(dst), 16, voff, so, 0, 0);
+        else __builtin_amdgcn_raw_ptr_buffer_load_lds(r, SMX_LDS(dst), 4, voff, so, 0, 0);
+    };
 #pragma unroll
-    for (int j = W; j < Ly::NBASE; j += 4) {
+    for (int j = W; j < Ly::NBASE; j += Ly::NW) {
         const int arr = j / (3 * Ly::NC), r = j % (3 * Ly::NC), c = r / Ly::NC, ch = r % Ly::NC;
         /* E0 = T comp 3+c, H0 = T comp c, bH = b comp c, bE = b comp 3+c */
         const unsigned comp = (arr == 0 || arr == 3) ? 3 + c : c;
-        float *dst = slot + arr * Ly::ARR + c * Ly::NC * 64 + ch * 64;
-        __builtin_amdgcn_raw_ptr_buffer_load_lds(arr < 2 ? rT : rB, SMX_LDS(dst), 4, vq[ch] + comp * cb, soff, 0, 0);
+        float *dst = slot + arr * Ly::ARR + (c * Ly::NC + ch) * Ly::CH;
+        copy(arr < 2 ? rT : rB, dst, vq[ch] + comp * cb, soff);
     }
     if (SPLIT && split) {
 #pragma unroll
-        for (int j = W; j < Ly::NSPLIT; j += 4) {
+        for (int j = W; j < Ly::NSPLIT; j += Ly::NW) {
             const int arr = j / (3 * Ly::NC), r = j % (3 * Ly::NC), c = r / Ly::NC, ch = r % Ly::NC;
-            float *dst = slot + (4 + arr) * Ly::ARR + c * Ly::NC * 64 + ch * 64; /* SH0 = S comp c, SE0 = 3+c */
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(rS, SMX_LDS(dst), 4, vq[ch] + (arr * 3 + c) * cb, soff, 0, 0);
+            float *dst = slot + (4 + arr) * Ly::ARR + (c * Ly::NC + ch) * Ly::CH; /* SH0 = S comp c, SE0 = 3+c */
+            copy(rS, dst, vq[ch] + (arr * 3 + c) * cb, soff);
         }
     }
 }
@@ -1347,13 +1364,16 @@ __device__ inline void e_regular(const float e[3], const float b[3], const float
     }
 }
 
-template <int FX, int FY, int D, bool SPLIT>
-__global__ __launch_bounds__(kThreads) void two_step_f32(fused_args a)
+template <int FX, int FY, int D, bool SPLIT, bool WIDE>
+__global__ __launch_bounds__(FX * FY) void two_step_f32(fused_args a)
 {
-    using Ly = two_layout<FX, FY, SPLIT>;
-    constexpr int W4 = Ly::W4, H4 = Ly::H4, N4 = Ly::N4, NSLOT = D + 1, CELLS = Ly::CELLS;
-    static_assert(FX * FY == kThreads && CELLS == 2, "tile shape");
-    static_assert((FY + 1 + FX) + (H4 - 2 + W4 - 3) + (H4 - 1 + W4 - 2) <= kThreads, "rings fit one cell per thread");
+    using Ly = two_layout<FX, FY, SPLIT, WIDE>;
+    constexpr int W4 = Ly::W4, N4 = Ly::N4, NSLOT = D + 1, CELLS = Ly::CELLS;
+    constexpr int NT = Ly::NT, OX = Ly::OX;
+    static_assert((NT == 256 || NT == 512) && FX % 32 == 0, "tile shape");
+    /* ring cells: G1\T, G2\G1, G3\G2 */
+    constexpr int RC = FY + 1 + FX, RB = RC + FY + 2 + FX + 1, RA = RB + FY + 3 + FX + 2;
+    static_assert(RA <= NT, "rings fit one cell per thread");
     static_assert(D >= 1 && D * Ly::kMinOps < 64, "copies in flight must fit the vmcnt field");
     /* one LDS variable (see fused_dma_f32): stage slots, H1, E1, H2 of the previous plane
      * in G4 layout, the chunk's z tables, a flag word */
@@ -1364,26 +1384,27 @@ __global__ __launch_bounds__(kThreads) void two_step_f32(fused_args a)
     const int id = xcd_logical(blockIdx.x, a.nblocks);
     const int bxi = id % a.gx, rest = id / a.gx;
     const int byi = rest % a.gy, bzi = rest / a.gy;
-    const int x0 = a.box[0] + bxi * FX, y0 = a.box[2] + byi * FY, z0 = a.box[4] + bzi * a.fz;
+    const int x0 = a.xorg + bxi * FX, y0 = a.box[2] + byi * FY, z0 = a.box[4] + bzi * a.fz;
     const int z1 = min(z0 + a.fz, a.box[5]);
     const int t = threadIdx.x, lane = t & 63;
     const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
     const int ns = z1 - z0 + 4; /* stages: planes z0-2 .. z1+1 */
     auto plane_of = [&](int s) { return wrap_index(z0 - 2 + s, a.L2); };
 
+    /* byte offset in a plane of the first word lane `lane` copies in run ch (WIDE: a group
+     * of 4 x; the groups past the region repeat the last one, into the run's padding) */
     unsigned vq[Ly::NC];
 #pragma unroll
     for (int ch = 0; ch < Ly::NC; ch++) {
-        const int e = min(ch * 64 + lane, N4 - 1);
-        vq[ch] = 4u * (unsigned)(wrap_index(x0 - 2 + e % W4, a.L0) + a.L0 * wrap_index(y0 - 2 + e / W4, a.L1));
+        const int e = WIDE ? 4 * min(ch * 64 + lane, N4 / 4 - 1) : min(ch * 64 + lane, N4 - 1);
+        vq[ch] = 4u * (unsigned)(wrap_index(x0 - OX + e % W4, a.L0) + a.L0 * wrap_index(y0 - 2 + e / W4, a.L1));
     }
 
     /* this thread's cells: r = 0 the tile cell t (every step computes it); r = 1 one cell
      * of the rings the first three updates add around the tile -- G1\T (used by c, b,
      * a), G2\G1 (b, a), G3\G2 (a), 129 cells for 32x8 -- ring cell h on thread
      * (h % 4)*64 + h/4, so the four waves share the extra work evenly; G4\G3 is only
-     * read.  Local coordinates (gx, gy) count from the G4 corner (x0-2, y0-2). */
-    constexpr int RC = FY + 1 + FX, RB = RC + H4 - 2 + W4 - 3, RA = RB + H4 - 1 + W4 - 2; /* ring ends */
+     * read.  Local coordinates (gx, gy) count from the G4 corner (x0-OX, y0-2). */
     int ce[CELLS], cx[CELLS], cy[CELLS];
     bool inG3[CELLS], inG2[CELLS], inG1[CELLS], inT[CELLS], pxy[CELLS];
     float tH[CELLS][4], tE[CELLS][4]; /* x/y PML factors: aH_x, gH_x, aH_y, gH_y (E likewise) */
@@ -1392,24 +1413,27 @@ __global__ __launch_bounds__(kThreads) void two_step_f32(fused_args a)
     for (int r = 0; r < CELLS; r++) {
         int gx, gy, ring = 0; /* ring: 0 tile, 1 G1\T, 2 G2\G1, 3 G3\G2, 4 none */
         if (r == 0) {
-            gx = 2 + t % FX;
+            gx = OX + t % FX;
             gy = 2 + t / FX;
         } else {
-            const int h = (t & 63) * 4 + (t >> 6);
+            /* 256 threads (two blocks per CU): ring cells spread over the 4 waves; 512
+             * threads (one block per CU): on the first waves, so the SIMDs that run
+             * them run a ring-free wave beside each */
+            const int h = NT == 256 ? (t & 63) * 4 + (t >> 6) : t;
             if (h < RC) { /* x = x0+FX column, then y = y0+FY row */
                 ring = 1;
-                gx = h < FY + 1 ? W4 - 2 : 2 + (h - (FY + 1));
-                gy = h < FY + 1 ? 2 + h : H4 - 2;
+                gx = h < FY + 1 ? OX + FX : OX + (h - (FY + 1));
+                gy = h < FY + 1 ? 2 + h : FY + 2;
             } else if (h < RB) { /* x = x0-1 column, then y = y0-1 row */
                 const int k = h - RC;
                 ring = 2;
-                gx = k < H4 - 2 ? 1 : 2 + (k - (H4 - 2));
-                gy = k < H4 - 2 ? 1 + k : 1;
+                gx = k < FY + 2 ? OX - 1 : OX + (k - (FY + 2));
+                gy = k < FY + 2 ? 1 + k : 1;
             } else if (h < RA) { /* x = x0+FX+1 column, then y = y0+FY+1 row */
                 const int k = h - RB;
                 ring = 3;
-                gx = k < H4 - 1 ? W4 - 1 : 1 + (k - (H4 - 1));
-                gy = k < H4 - 1 ? 1 + k : H4 - 1;
+                gx = k < FY + 3 ? OX + FX + 1 : OX - 1 + (k - (FY + 3));
+                gy = k < FY + 3 ? 1 + k : FY + 3;
             } else {
                 ring = 4;
                 gx = gy = 0;
@@ -1417,14 +1441,14 @@ __global__ __launch_bounds__(kThreads) void two_step_f32(fused_args a)
         }
         const bool valid = ring < 4;
         const int e = gy * W4 + gx;
-        ce[r] = valid ? e : 0;
-        cx[r] = wrap_index(x0 - 2 + gx, a.L0);
+        ce[r] = valid ? e : W4 + 1; /* idle lanes compute on a cell whose neighbours exist */
+        cx[r] = wrap_index(x0 - OX + gx, a.L0);
         cy[r] = wrap_index(y0 - 2 + gy, a.L1);
         cq[r] = cx[r] + (int64_t)a.L0 * cy[r];
         inG3[r] = ring <= 3;
         inG2[r] = ring <= 2;
         inG1[r] = ring <= 1;
-        inT[r] = ring == 0 && x0 - 2 + gx < a.box[1] && y0 - 2 + gy < a.box[3];
+        inT[r] = r == 0 && x0 - OX + gx >= a.xlo && x0 - OX + gx < a.xhi && y0 - 2 + gy < a.box[3];
         pxy[r] = false;
 #pragma unroll
         for (int k = 0; k < 4; k++) tH[r][k] = tE[r][k] = 1.f;
@@ -1438,7 +1462,7 @@ __global__ __launch_bounds__(kThreads) void two_step_f32(fused_args a)
     }
     bool tile_pml = false;
     if constexpr (SPLIT) {
-        for (int i = t; i < ns; i += kThreads) {
+        for (int i = t; i < ns; i += NT) {
             const int z = plane_of(i);
             const float *tz = a.pml + 4 * (a.L0 + a.L1);
             lds[L_ZT + 5 * i + 0] = tz[z];
@@ -1460,19 +1484,32 @@ __global__ __launch_bounds__(kThreads) void two_step_f32(fused_args a)
     const __amdgpu_buffer_rsrc_t rT = group_rsrc(a.To[0], a.ncomp), rB = group_rsrc(a.b[0], a.ncomp),
                                  rS = group_rsrc(a.So[0], a.ncomp);
     const unsigned cb = 4u * (unsigned)a.ncomp;
-    auto issue = [&](int s) {
-        const unsigned soff = 4u * (unsigned)((int64_t)plane_of(s) * a.sz);
+    auto issue = [&](int s, int z) { /* stage s, plane z = plane_of(s) */
+        const unsigned soff = 4u * (unsigned)((int64_t)z * a.sz);
         const bool split = SPLIT && (tile_pml || zpml(s));
         float *slot = lds + (s % NSLOT) * Ly::SLOT;
-        switch (wv) {
-        case 0: two_issue<FX, FY, SPLIT, 0>(slot, rT, rB, rS, cb, soff, split, vq); break;
-        case 1: two_issue<FX, FY, SPLIT, 1>(slot, rT, rB, rS, cb, soff, split, vq); break;
-        case 2: two_issue<FX, FY, SPLIT, 2>(slot, rT, rB, rS, cb, soff, split, vq); break;
-        default: two_issue<FX, FY, SPLIT, 3>(slot, rT, rB, rS, cb, soff, split, vq); break;
+        if constexpr (Ly::NW == 4) {
+            switch (wv) {
+            case 0: two_issue<FX, FY, SPLIT, WIDE, 0>(slot, rT, rB, rS, cb, soff, split, vq); break;
+            case 1: two_issue<FX, FY, SPLIT, WIDE, 1>(slot, rT, rB, rS, cb, soff, split, vq); break;
+            case 2: two_issue<FX, FY, SPLIT, WIDE, 2>(slot, rT, rB, rS, cb, soff, split, vq); break;
+            default: two_issue<FX, FY, SPLIT, WIDE, 3>(slot, rT, rB, rS, cb, soff, split, vq); break;
+            }
+        } else {
+            switch (wv) {
+            case 0: two_issue<FX, FY, SPLIT, WIDE, 0>(slot, rT, rB, rS, cb, soff, split, vq); break;
+            case 1: two_issue<FX, FY, SPLIT, WIDE, 1>(slot, rT, rB, rS, cb, soff, split, vq); break;
+            case 2: two_issue<FX, FY, SPLIT, WIDE, 2>(slot, rT, rB, rS, cb, soff, split, vq); break;
+            case 3: two_issue<FX, FY, SPLIT, WIDE, 3>(slot, rT, rB, rS, cb, soff, split, vq); break;
+            case 4: two_issue<FX, FY, SPLIT, WIDE, 4>(slot, rT, rB, rS, cb, soff, split, vq); break;
+            case 5: two_issue<FX, FY, SPLIT, WIDE, 5>(slot, rT, rB, rS, cb, soff, split, vq); break;
+            case 6: two_issue<FX, FY, SPLIT, WIDE, 6>(slot, rT, rB, rS, cb, soff, split, vq); break;
+            default: two_issue<FX, FY, SPLIT, WIDE, 7>(slot, rT, rB, rS, cb, soff, split, vq); break;
+            }
         }
     };
     /* stage arrays: 0 = E0, 1 = H0, 2 = bH, 3 = bE, 4 = SH0, 5 = SE0 */
-    auto st = [&](const float *S, int arr, int c, int e) { return S[arr * Ly::ARR + c * Ly::NC * 64 + e]; };
+    auto st = [&](const float *S, int arr, int c, int e) { return S[arr * Ly::ARR + c * Ly::NC * Ly::CH + e]; };
     /* PML factors of cell r on the plane of stage s: H (integer samples) or E (half) */
     auto factors = [&](int r, int s, bool e_field, float pa[3], float pg[3]) {
         const float *tt = e_field ? tE[r] : tH[r];
@@ -1481,8 +1518,8 @@ __global__ __launch_bounds__(kThreads) void two_step_f32(fused_args a)
         pg[2] = SPLIT ? lds[L_ZT + 5 * s + (e_field ? 3 : 1)] : 1.f;
     };
 
-    for (int s = 0; s < D && s < ns; s++) issue(s);
-
+    for (int s = 0; s < D && s < ns; s++) issue(s, plane_of(s));
+    int zi = plane_of(D); /* plane of the next stage to issue */
     /* carried per cell: E0, SE0, bE of the previous plane; bH, H1, SH1 of the previous
      * plane; E1, SE1, bE two planes back */
     float e0p[CELLS][3], se0p[CELLS][3], bep[CELLS][3], bhp[CELLS][3], h1p[CELLS][3], sh1p[CELLS][3];
@@ -1494,9 +1531,11 @@ __global__ __launch_bounds__(kThreads) void two_step_f32(fused_args a)
             e0p[r][c] = se0p[r][c] = bep[r][c] = bhp[r][c] = h1p[r][c] = sh1p[r][c] = e1pp[r][c] = se1pp[r][c] =
                 bepp[r][c] = 0.f;
 
+    int zq0 = plane_of(0), zq1 = plane_of(-1), zq2 = plane_of(-2); /* planes of stages s, s-1, s-2 */
     for (int s = 0; s < ns; s++) {
         if (s + D < ns) {
-            issue(s + D);
+            issue(s + D, zi);
+            zi = zi + 1 == a.L2 ? 0 : zi + 1;
             wait_vm_barrier<D * Ly::kMinOps>();
         } else {
             switch (ns - 1 - s) { /* stages already issued after s */
@@ -1507,8 +1546,7 @@ __global__ __launch_bounds__(kThreads) void two_step_f32(fused_args a)
         }
         const float *S = lds + (s % NSLOT) * Ly::SLOT;
         const int p = z0 - 2 + s;     /* unwrapped plane of this stage */
-        const int zp = plane_of(s);   /* its wrapped index */
-        const int zp1 = plane_of(s - 1), zp2 = plane_of(s - 2);
+        const int zp = zq0, zp1 = zq1, zp2 = zq2; /* wrapped planes of stages s, s-1, s-2 */
         const bool doA = s >= 1, doB = p >= z0, doC = p >= z0 + 1, doD = p >= z0 + 2;
         const bool zs = zpml(s), zs1 = s >= 1 && zpml(s - 1), zs2 = s >= 2 && zpml(s - 2);
 
@@ -1520,7 +1558,9 @@ __global__ __launch_bounds__(kThreads) void two_step_f32(fused_args a)
         /* a. H1(p) on G3 */
 #pragma unroll
         for (int r = 0; r < CELLS; r++) {
-            if (!(doA && inG3[r])) continue;
+            /* the ring cell runs on every lane (no divergent branch, so the two cells'
+             * updates interleave); a lane outside the update's region writes nothing */
+            if (!doA) continue;
             const int e = ce[r];
             const bool pml = pxy[r] || zs;
             float P[3], M[3], ec[3], exm[3], eym[3], h[3], b[3], sp[3];
@@ -1552,7 +1592,7 @@ __global__ __launch_bounds__(kThreads) void two_step_f32(fused_args a)
         /* b. E1(p-1) on G2: forward curl of H1 (previous plane from LDS, +z from a.) */
 #pragma unroll
         for (int r = 0; r < CELLS; r++) {
-            if (!(doB && inG2[r])) continue;
+            if (!doB) continue;
             const int e = ce[r];
             const bool pml = pxy[r] || zs1;
             float P[3], M[3], hc[3], hxp[3], hyp[3];
@@ -1592,15 +1632,17 @@ __global__ __launch_bounds__(kThreads) void two_step_f32(fused_args a)
                 }
                 retire_loads();
             }
-            lds[L_E1 + 0 * N4 + e] = e1n[r][0];
-            lds[L_E1 + 1 * N4 + e] = e1n[r][1];
-            lds[L_E1 + 2 * N4 + e] = e1n[r][2];
+            if (inG2[r]) {
+                lds[L_E1 + 0 * N4 + e] = e1n[r][0];
+                lds[L_E1 + 1 * N4 + e] = e1n[r][1];
+                lds[L_E1 + 2 * N4 + e] = e1n[r][2];
+            }
         }
         wait_vm_barrier<63>(); /* E1(p-1) of the neighbours in LDS */
         /* c. H2(p-1) on G1 */
 #pragma unroll
         for (int r = 0; r < CELLS; r++) {
-            if (!(doC && inG1[r])) continue;
+            if (!doC) continue;
             const int e = ce[r];
             const bool pml = pxy[r] || zs1;
             float P[3], M[3], ec[3], exm[3], eym[3], sh2[3];
@@ -1692,6 +1734,9 @@ __global__ __launch_bounds__(kThreads) void two_step_f32(fused_args a)
                 e0p[r][c] = st(S, 0, c, ce[r]);
                 se0p[r][c] = split_s ? st(S, 5, c, ce[r]) : 0.f;
             }
+        zq2 = zq1;
+        zq1 = zq0;
+        zq0 = zq0 + 1 == a.L2 ? 0 : zq0 + 1;
         /* every wave is done with this stage and with H1, H2 of the previous plane */
         wait_vm_barrier<63>();
 #pragma unroll
@@ -2020,11 +2065,14 @@ static void plan_split(smx_f32_state *st, const unsigned char *flag)
         std::memcpy(st->reg_start, rs.start, sizeof st->reg_start);
         st->split_ok = 1;
     }
-    /* two steps: outputs on [a, b) need H1 on [a-1, b+1] PML-free: a = lo+1, b = hi-2 */
+    /* two steps: outputs on [a, b) need H1 on [a-1, b+1] PML-free: a = lo+1, b = hi-2;
+     * the x start rounded up to a multiple of 4 (the 16-byte copies read 4-aligned x) */
     int f2[6], h1[6], e1[6];
     ok = true;
     for (int d = 0; d < 3; d++) {
-        const int a = has[d] ? lo[d] + 1 : 0, b = has[d] ? hi[d] - 2 : L[d];
+        int a = has[d] ? lo[d] + 1 : 0;
+        const int b = has[d] ? hi[d] - 2 : L[d];
+        if (d == 0) a = (a + 3) / 4 * 4;
         f2[2 * d] = a;
         f2[2 * d + 1] = b;
         /* first step outside the box shrunk by one on both sides (H1), on + sides (E1) */
@@ -2044,20 +2092,50 @@ const char *smx_f32_last_error(void) { return f32_err; }
 
 int smx_f32_num_tiles(void) { return kNumTiles; }
 
-/* two-step variants, 32x8 tiles: 0, 1 = one or two planes of copies in flight, no
- * split slots (PML only outside the kernel's box, region sweeps for the rest); 2 = one
- * plane in flight with split slots, PML cells handled in the kernel */
-constexpr int kNumTwo = 3;
+/* two-step variants: 32x8 tiles (one thread per cell), d planes of copies in flight,
+ * split slots (PML cells inside the kernel) or not (PML only outside the kernel's box,
+ * region sweeps for the rest), 4- or 16-byte copies (wide: x extent and box start
+ * multiples of 4) */
+struct two_variant {
+    int fx, fy, d;
+    bool split, wide;
+    int ox; /* columns of the stage region left of a tile */
+    const char *name, *with_regions;
+};
+#define SMX_TWO(FX, FY, D, SP, WI)                                                                         \
+    {FX, FY, D, SP, WI, two_layout<FX, FY, SP, WI>::OX, "two_step_f32<" #FX "," #FY "," #D "," #SP "," #WI ">", \
+     "region_h_f32 + region_e_f32 + two_step_f32<" #FX "," #FY "," #D "," #SP "," #WI                      \
+     "> + region_h_f32 + region_e_f32"}
+constexpr two_variant kTwo[] = {SMX_TWO(32, 8, 1, false, false), SMX_TWO(32, 8, 2, false, false),
+                                SMX_TWO(32, 8, 1, true, false), SMX_TWO(32, 8, 1, false, true)};
+#undef SMX_TWO
+constexpr int kNumTwo = sizeof kTwo / sizeof kTwo[0];
 
 int smx_f32_num_two(void) { return kNumTwo; }
 
+/* the kernels of kTwo, in its order */
+using two_kernel = void (*)(fused_args);
+constexpr two_kernel kTwoKernel[] = {two_step_f32<32, 8, 1, false, false>, two_step_f32<32, 8, 2, false, false>,
+                                     two_step_f32<32, 8, 1, true, false>, two_step_f32<32, 8, 1, false, true>};
+static_assert(sizeof kTwoKernel / sizeof kTwoKernel[0] == kNumTwo, "one kernel per variant");
+
+void smx_f32_two_shape(int variant, int *fx, int *fy)
+{
+    const bool ok = variant >= 0 && variant < kNumTwo;
+    *fx = ok ? kTwo[variant].fx : 0;
+    *fy = ok ? kTwo[variant].fy : 0;
+}
+
 int smx_f32_two_ok(const smx_f32_state *st, int variant, int fz)
 {
-    /* no lossy/metal cell; 32-bit offsets as the staged kernel; variants 0-1 need the
-     * PML outside the kernel's box, variant 2 keeps the chunk's z tables in LDS */
+    /* no lossy/metal cell; 32-bit offsets as the staged kernel; the variants without
+     * split slots need the PML outside the kernel's box, those with them keep the
+     * chunk's z tables in LDS; 16-byte copies need x groups of 4 that never straddle
+     * the x wrap (the two-step box starts at a multiple of 4) */
     if (variant < 0 || variant >= kNumTwo || fz < 1 || st->nspecial != 0 || 24 * st->n >= ((int64_t)1 << 32))
         return 0;
-    if (variant == 2) return fz <= two_layout<32, 8, true>::kMaxFZ;
+    if (kTwo[variant].wide && st->L[0] % 4 != 0) return 0;
+    if (kTwo[variant].split) return fz <= two_layout<32, 8, true, false>::kMaxFZ;
     return !st->has_pml || st->split2_ok;
 }
 
@@ -2075,13 +2153,8 @@ const char *smx_f32_kernel_name(const smx_f32_state *st)
         "region_h_f32 + fused_dma_f32<32,8,4,false> + region_e_f32",
         "region_h_f32 + fused_dma_f32<32,8,2,false> + region_e_f32",
         "region_h_f32 + fused_dma_f32<32,8,1,false> + region_e_f32"};
-    static const char *const two[kNumTwo] = {"two_step_f32<32,8,1,false>", "two_step_f32<32,8,2,false>",
-                                             "two_step_f32<32,8,1,true>"};
-    static const char *const two_split[kNumTwo] = {
-        "region_h_f32 + region_e_f32 + two_step_f32<32,8,1,false> + region_h_f32 + region_e_f32",
-        "region_h_f32 + region_e_f32 + two_step_f32<32,8,2,false> + region_h_f32 + region_e_f32",
-        "two_step_f32<32,8,1,true>"};
-    if (st->two >= 0 && st->two < kNumTwo) return st->has_pml ? two_split[st->two] : two[st->two];
+    if (st->two >= 0 && st->two < kNumTwo)
+        return st->has_pml && !kTwo[st->two].split ? kTwo[st->two].with_regions : kTwo[st->two].name;
     if (!st->fused) return st->bx == 64 ? "h_sweep_f32<64> + e_sweep_f32<64>" : "h_sweep_f32<32> + e_sweep_f32<32>";
     if (st->tile < 0 || st->tile >= kNumTiles) return "";
     return st->tile >= kFirstNoSplitTile && st->has_pml ? split[st->tile] : fused[st->tile];
@@ -2414,6 +2487,25 @@ hipError_t smx_f32_step(smx_f32_state *st, const smx_f32_launch_args *la, hipStr
     return hipGetLastError();
 }
 
+/* x tiles of the two-step kernel over a.box, placed so that every stage row (ox columns
+ * left of its tile) starts on a 128-byte line: a row of FX + 2*ox words then spans 2
+ * lines where it would span 3, and the copies ask the L2 for a third fewer lines.  A box
+ * over the whole x axis shifts its tiles by ox (the last tile wraps around); a box
+ * inside it starts its tiles at the aligned column at or below its start, and the
+ * kernel stores only inside it. */
+static void two_tiles_x(fused_args &a, int ox, int fx, int L0)
+{
+    if (a.box[0] == 0 && a.box[1] == L0) {
+        a.xorg = a.xlo = ox;
+        a.xhi = L0 + ox;
+    } else {
+        a.xorg = a.box[0] - ((a.box[0] - ox) % 32 + 32) % 32;
+        a.xlo = a.box[0];
+        a.xhi = a.box[1];
+    }
+    a.gx = (a.xhi - a.xorg + fx - 1) / fx;
+}
+
 hipError_t smx_f32_step2(smx_f32_state *st, const smx_f32_launch_args *la, hipStream_t stream)
 {
     if (!smx_f32_two_ok(st, st->two, st->two_fz)) return hipErrorInvalidValue;
@@ -2426,9 +2518,15 @@ hipError_t smx_f32_step2(smx_f32_state *st, const smx_f32_launch_args *la, hipSt
     const int box[6] = {0, g.L0, 0, g.L1, 0, g.L2};
     for (int i = 0; i < 6; i++) a.box[i] = box[i];
     a.fz = st->two_fz;
-    a.gx = (g.L0 + 31) / 32;
-    a.gy = (g.L1 + 7) / 8;
+    const two_variant tv = kTwo[st->two];
+    two_tiles_x(a, tv.ox, tv.fx, g.L0);
+    a.gy = (g.L1 + tv.fy - 1) / tv.fy;
     a.nblocks = a.gx * a.gy * ((g.L2 + a.fz - 1) / a.fz);
+    auto launch_two = [&](const fused_args &k) {
+        const dim3 grid(k.nblocks), block(tv.fx * tv.fy);
+        hipLaunchKernelGGL(kTwoKernel[st->two], grid, block, 0, stream, k);
+        return hipGetLastError();
+    };
     a.sz = (int64_t)g.L0 * g.L1;
     a.w[0] = (float)g.iBc0;
     a.w[1] = (float)g.iBc1;
@@ -2473,13 +2571,8 @@ hipError_t smx_f32_step2(smx_f32_state *st, const smx_f32_launch_args *la, hipSt
     a.EB = la->EB;
     a.HT = la->HT;
     a.ET = la->ET;
-    if (!st->has_pml || st->two == 2) {
-        if (st->two == 2)
-            hipLaunchKernelGGL((two_step_f32<32, 8, 1, true>), dim3(a.nblocks), dim3(kThreads), 0, stream, a);
-        else if (st->two == 1)
-            hipLaunchKernelGGL((two_step_f32<32, 8, 2, false>), dim3(a.nblocks), dim3(kThreads), 0, stream, a);
-        else hipLaunchKernelGGL((two_step_f32<32, 8, 1, false>), dim3(a.nblocks), dim3(kThreads), 0, stream, a);
-        const hipError_t e = hipGetLastError();
+    if (!st->has_pml || tv.split) {
+        const hipError_t e = launch_two(a);
         if (e != hipSuccess) return e;
         st->cur = 1 - st->cur;
         return hipSuccess;
@@ -2545,12 +2638,10 @@ hipError_t smx_f32_step2(smx_f32_state *st, const smx_f32_launch_args *la, hipSt
     if ((e = region(st->regE1, true, oT, oS, tT, tS, 1)) != hipSuccess) return e;
     /* 3: the box */
     for (int i = 0; i < 6; i++) a.box[i] = st->fbox2[i];
-    a.gx = (a.box[1] - a.box[0] + 31) / 32;
-    a.gy = (a.box[3] - a.box[2] + 7) / 8;
+    two_tiles_x(a, tv.ox, tv.fx, g.L0);
+    a.gy = (a.box[3] - a.box[2] + tv.fy - 1) / tv.fy;
     a.nblocks = a.gx * a.gy * ((a.box[5] - a.box[4] + a.fz - 1) / a.fz);
-    if (st->two == 1) hipLaunchKernelGGL((two_step_f32<32, 8, 2, false>), dim3(a.nblocks), dim3(kThreads), 0, stream, a);
-    else hipLaunchKernelGGL((two_step_f32<32, 8, 1, false>), dim3(a.nblocks), dim3(kThreads), 0, stream, a);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = launch_two(a)) != hipSuccess) return e;
     /* 4: H2 from tmp (H1, E1, H1 split) into new */
     if ((e = region(st->regP2, false, tT, tS, nT, nS, 2)) != hipSuccess) return e;
     /* 5: E2 from tmp (E1, E1 split) and the new H into new */

@@ -104,6 +104,8 @@ hipError_t smx_f32_step(smx_f32_state *st, const smx_f32_launch_args *a, hipStre
 hipError_t smx_f32_step2(smx_f32_state *st, const smx_f32_launch_args *a, hipStream_t stream);
 int smx_f32_num_two(void);
 int smx_f32_two_ok(const smx_f32_state *st, int variant, int fz);
+/* tile of two-step variant `variant`: fx x fy cells, one thread each */
+void smx_f32_two_shape(int variant, int *fx, int *fy);
 int smx_f32_write_fields(smx_f32_state *st, const double *H_t, const double *E_t, hipStream_t stream);
 int smx_f32_read_fields(const smx_f32_state *st, double *H_t, double *E_t, hipStream_t stream);
 const char *smx_f32_last_error(void);

@@ -978,8 +978,7 @@ int smx_plan_autotune(smx_plan *p, int trial_steps, int choice[4])
         choice[2] = f ? st.fz : 0;
         choice[3] = f ? (st.tile >= 3 ? 2 : 1) : 0;
         if (p->g.precision == SMX_F32 && st.two >= 0) {
-            choice[0] = 32;
-            choice[1] = 8;
+            smx_f32_two_shape(st.two, &choice[0], &choice[1]);
             choice[2] = st.two_fz;
             choice[3] = 3;
         }

@@ -28,7 +28,7 @@ def _state(p):
     return H, E, tuple(p.read_monitors())
 
 
-@pytest.mark.parametrize("two", [None, "0,16"])
+@pytest.mark.parametrize("two", [None, "0,16", "3,16"])
 def test_batch_equals_separate_runs(two, monkeypatch):
     """With two = "v,fz" the PML-free member runs two steps per pass (and the batch
     loop has to keep the members' step counts in step)."""

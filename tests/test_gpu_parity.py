@@ -239,23 +239,29 @@ def test_region_sweeps_carry_monitor_plane(tile, monkeypatch):
         assert rel_l2(a.view(np.float64), b) < 2e-4
 
 
-TWO_NAMES = ("two_step_f32<32,8,1,false>", "two_step_f32<32,8,2,false>", "two_step_f32<32,8,1,true>")
+# two-step variants: (copies in flight, split slots, 16-byte copies); tiles are 32 x 8
+TWO_VARIANTS = ((1, False, False), (2, False, False), (1, True, False), (1, False, True))
+TWO_NAMES = tuple(f"two_step_f32<32,8,{d},{str(sp).lower()},{str(wi).lower()}>" for d, sp, wi in TWO_VARIANTS)
+TWO_SPLIT = tuple(v[1] for v in TWO_VARIANTS)
 
 
 @pytest.mark.parametrize("two,iBc", [("0,16", (1, -1, 1)), ("1,8", (0, 0, 0)), ("0,5", (1, 1, -1)),
-                                     ("1,64", (-1, 1, 1)), ("0,41", (1, -1, 0)), ("2,16", (1, -1, 1))])
+                                     ("1,64", (-1, 1, 1)), ("0,41", (1, -1, 0)), ("2,16", (1, -1, 1)),
+                                     ("3,16", (1, -1, 1)), ("3,5", (0, 0, 0)), ("3,64", (1, 1, -1)),
+                                     ("3,41", (1, -1, 0)), ("3,8", (-1, 1, 1))])
 def test_two_step_pass(two, iBc, monkeypatch):
     """Two steps per pass (grids without PML and lossy/metal cells) against the FP64
     oracle, fields and both monitor planes, over a run with an odd number of steps (the
     last one takes the one-step kernel)."""
     monkeypatch.setenv("SMX_F32_KERNEL", "fused")
     monkeypatch.setenv("SMX_F32_TWO", two)
-    prep = casegen.prepare(small_waveguide(seed=13, L0=37, L1=19, L2=41, outer=8, pml_xy=0, pml_z=0,
+    v, fz = (int(x) for x in two.split(","))
+    L0 = 68 if TWO_VARIANTS[v][2] else 37  # 16-byte copies need L0 % 4 == 0
+    prep = casegen.prepare(small_waveguide(seed=13, L0=L0, L1=19, L2=41, outer=8, pml_xy=0, pml_z=0,
                                            iBc=iBc, metal_cells=0, lossy_cells=0, core=(3, 3)))
     assert all(c == ord("n") for c in prep.Cont)
     steps = prep.case.time_period[2] * prep.N
     with prep.plan(fdtd.F32) as p:
-        v, fz = (int(x) for x in two.split(","))
         assert p.autotune(0) == (32, 8, fz, 3)
         assert p.kernels == TWO_NAMES[v]
         sums = p.run_steps(0, steps - 1)
@@ -271,13 +277,14 @@ def test_two_step_pass(two, iBc, monkeypatch):
     assert np.allclose(sums, so, rtol=1e-14, atol=0)
 
 
-def test_two_step_equals_one_step_kernels(monkeypatch):
+@pytest.mark.parametrize("variant", ["0,16", "3,16"])
+def test_two_step_equals_one_step_kernels(variant, monkeypatch):
     """The two-step pass against the one-step staged kernel on the same grid: both are
     FP32 restatements of the same map, so they agree far below the FP32-vs-FP64 gap."""
     prep = casegen.prepare(small_waveguide(seed=14, L0=40, L1=24, L2=36, outer=8, pml_xy=0, pml_z=0,
                                            iBc=(1, -1, 1), metal_cells=0, lossy_cells=0))
     out = []
-    for two in ("0,16", None):
+    for two in (variant, None):
         monkeypatch.setenv("SMX_F32_KERNEL", "fused")
         monkeypatch.setenv("SMX_F32_TILE", "3,16")
         if two:
@@ -300,21 +307,28 @@ def test_two_step_equals_one_step_kernels(monkeypatch):
     ("2,8", dict(pml_xy=0, pml_z=3, monitor_gap=0, iBc=(0, 0, 0))),
     ("2,5", dict(pml_xy=2, pml_z=3, iBc=(0, 0, 0))),
     ("2,64", dict(pml_xy=3, pml_z=4, iBc=(1, 1, 0), scale_amp=0.3)),
+    # 16-byte copies
+    ("3,16", dict(pml_xy=0, pml_z=3, iBc=(1, -1, 0))),
+    ("3,8", dict(pml_xy=0, pml_z=3, monitor_gap=0, iBc=(0, 0, 0))),
+    ("3,5", dict(pml_xy=2, pml_z=3, iBc=(0, 0, 0))),
+    ("3,64", dict(pml_xy=3, pml_z=4, iBc=(1, 1, 0), scale_amp=0.3)),
 ])
 def test_two_step_pass_with_pml(two, kw, monkeypatch):
     """Two steps per pass on grids with PML, against the FP64 oracle: variants 0-1 on the
     box whose two-step cone is PML-free with the region sweeps (first step into a scratch
-    set, second into the new buffer) on the rest; variant 2 with the PML cells inside."""
+    set, second into the new buffer) on the rest; the split-slot variants with the PML
+    cells inside."""
     monkeypatch.setenv("SMX_F32_KERNEL", "fused")
     monkeypatch.setenv("SMX_F32_TWO", two)
-    prep = casegen.prepare(small_waveguide(seed=16, L0=37, L1=22, L2=44, outer=9, metal_cells=0,
+    v, fz = (int(x) for x in two.split(","))
+    L0 = 44 if TWO_VARIANTS[v][2] else 37
+    prep = casegen.prepare(small_waveguide(seed=16, L0=L0, L1=22, L2=44, outer=9, metal_cells=0,
                                            lossy_cells=0, core=(3, 3), **kw))
     assert any(c == ord("a") for c in prep.Cont)
     steps = prep.case.time_period[2] * prep.N
     with prep.plan(fdtd.F32) as p:
-        v, fz = (int(x) for x in two.split(","))
         assert p.autotune(0) == (32, 8, fz, 3)
-        assert TWO_NAMES[v] in p.kernels and ("region_h_f32" in p.kernels) == (v < 2)
+        assert TWO_NAMES[v] in p.kernels and ("region_h_f32" in p.kernels) == (not TWO_SPLIT[v])
         p.run_steps(0, steps - 3)
         p.run_steps(steps - 3, 3)
         H, E = p.read_fields()
@@ -329,11 +343,13 @@ def test_two_step_pass_with_pml(two, kw, monkeypatch):
 
 def test_two_step_refused(monkeypatch):
     """Lossy/metal cells, or a PML-free core too thin for the two-step box, keep the
-    one-step kernels."""
+    one-step kernels; so does an x extent that is not a multiple of 4 for the 16-byte
+    copies."""
     monkeypatch.setenv("SMX_F32_KERNEL", "fused")
-    monkeypatch.setenv("SMX_F32_TWO", "0,16")
-    for kw in (dict(pml_xy=0, pml_z=0, metal_cells=2, lossy_cells=0),
-               dict(L0=12, L1=12, pml_xy=4, pml_z=3, metal_cells=0, lossy_cells=0)):
+    for two, kw in (("0,16", dict(pml_xy=0, pml_z=0, metal_cells=2, lossy_cells=0)),
+                    ("0,16", dict(L0=12, L1=12, pml_xy=4, pml_z=3, metal_cells=0, lossy_cells=0)),
+                    ("3,16", dict(L0=37, pml_xy=0, pml_z=0, metal_cells=0, lossy_cells=0))):
+        monkeypatch.setenv("SMX_F32_TWO", two)
         prep = casegen.prepare(small_waveguide(seed=15, **kw))
         with prep.plan(fdtd.F32) as p:
             assert p.autotune(0)[3] != 3
@@ -352,7 +368,7 @@ def test_autotune_keeps_state():
         m1 = p.read_monitors()
         assert _eq(H0, H1) and _eq(E0, E1)
         assert all(_eq(a, b) for a, b in zip(m0, m1))
-        assert choice == (0, 0, 0, 0) or (choice[0] * choice[1] == 256 and choice[2] in (8, 16, 32, 64)
+        assert choice == (0, 0, 0, 0) or (choice[0] * choice[1] in (256, 512) and choice[2] in (8, 16, 32, 64)
                                           and choice[3] in (1, 2, 3))
         p.run(1, prep.case.time_period[2])
         H, E = p.read_fields()
