@@ -273,6 +273,8 @@ def main():
         fx, fy, fz, variant = choice
         step_kernels = p32.kernels.split(" + ")
         kernel = p32.kernels + (", z chunk %d" % fz if variant else "")
+        if variant == 3 and p32.z_balance:
+            kernel += " (chunks balanced, z-PML stage counted %g)" % (1 + p32.z_balance / 2)
         print(f"[bench] autotuner: {kernel}", file=sys.stderr, flush=True)
         wall_ms, ev_ms = time_plan(p32, K, W, torch)
         H32, E32 = p32.read_fields()

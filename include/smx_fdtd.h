@@ -196,13 +196,15 @@ int smx_adjoint_backward(smx_plan *plan, double *grad);
  * and tiles of 32x8 and 16x16 staged by asynchronous copies to LDS -- with each z chunk
  * (8, 16, 32, 64 planes) for trial_steps steps on this plan's grid, keeps the fastest,
  * and restores fields and monitors.  choice (may be NULL) receives {tile x, tile y,
- * z chunk, variant}: variant 0 = in-place sweeps (sizes 0, also for FP64 plans),
- * 1 = register-staged fused step, 2 = LDS-staged fused step, 3 = two steps per pass
- * (grids without PML and lossy/metal cells; used for pairs of steps of the source loop
- * while the DFT is off, single steps take the best one-step kernel).  trial_steps = 0
- * only reports.  No counterpart in the reference.
+ * z chunk, variant, z balance}: variant 0 = in-place sweeps (sizes 0, also for FP64
+ * plans), 1 = register-staged fused step, 2 = LDS-staged fused step, 3 = two steps per
+ * pass (grids without lossy/metal cells; used for pairs of steps of the source loop
+ * while the DFT is off, single steps take the best one-step kernel); z balance (variant
+ * 3 only) 0 = z chunks of equal size, k > 0 = chunks balanced with a z-PML stage
+ * counted 1 + k/2 times.  trial_steps = 0 only reports.  No counterpart in the
+ * reference.
  */
-int smx_plan_autotune(smx_plan *plan, int trial_steps, int choice[4]);
+int smx_plan_autotune(smx_plan *plan, int trial_steps, int choice[5]);
 
 /* Names of the kernels one step of the plan launches with its current kernel choice,
  * " + "-separated, as profilers list them (lossy/metal passes not included); a static

@@ -429,6 +429,9 @@ constexpr tile_shape kTiles[10] = {{32, 8}, {64, 4}, {16, 16}, {32, 8}, {16, 16}
 constexpr int kNumTiles = 10, kFirstDmaTile = 3, kFirstNoSplitTile = 5;
 constexpr int kDefaultTile = 0, kDefaultFZ = 16;
 
+/* most z chunks of one two-step launch */
+constexpr int kMaxZChunks = 256;
+
 struct fused_args {
     int L0, L1, L2;
     int gx, gy, nblocks;
@@ -458,8 +461,11 @@ struct fused_args {
     /* box of the staged kernel's tiles: [x0, x1) x [y0, y1) x [z0, z1) (the whole grid
      * unless the cells outside it are left to the region sweeps) */
     int box[6];
-    /* two-step kernel: x of its first tile column, and the (unwrapped) x range it stores */
+    /* two-step kernel: x of its first tile column, and the (unwrapped) x range it stores;
+     * its z chunks [zc[i], zc[i+1]) for i < nzc */
     int xorg, xlo, xhi;
+    int nzc;
+    int zc[kMaxZChunks + 1];
     /* boxes of the region sweeps and their first thread index */
     int nreg;
     int reg[6][6];
@@ -1309,7 +1315,7 @@ struct two_layout {
     static constexpr int NSPLIT = 2 * 3 * NC;              /* extra copies of PML stages */
     static constexpr int kMinOps = NBASE / NW;             /* per wave, at least */
     static constexpr int CELLS = 2;                        /* a tile cell and a ring cell */
-    static constexpr int kMaxFZ = 64;                      /* z tables of a chunk in LDS */
+    static constexpr int kMaxFZ = 80;                      /* z tables of a chunk in LDS */
 };
 
 /* Copies of one stage dealt to wave W: arrays E0, H0, bH, bE (and SH0, SE0 when `split`),
@@ -1384,8 +1390,7 @@ __global__ __launch_bounds__(FX * FY) void two_step_f32(fused_args a)
     const int id = xcd_logical(blockIdx.x, a.nblocks);
     const int bxi = id % a.gx, rest = id / a.gx;
     const int byi = rest % a.gy, bzi = rest / a.gy;
-    const int x0 = a.xorg + bxi * FX, y0 = a.box[2] + byi * FY, z0 = a.box[4] + bzi * a.fz;
-    const int z1 = min(z0 + a.fz, a.box[5]);
+    const int x0 = a.xorg + bxi * FX, y0 = a.box[2] + byi * FY, z0 = a.zc[bzi], z1 = a.zc[bzi + 1];
     const int t = threadIdx.x, lane = t & 63;
     const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
     const int ns = z1 - z0 + 4; /* stages: planes z0-2 .. z1+1 */
@@ -1533,6 +1538,9 @@ __global__ __launch_bounds__(FX * FY) void two_step_f32(fused_args a)
 
     int zq0 = plane_of(0), zq1 = plane_of(-1), zq2 = plane_of(-2); /* planes of stages s, s-1, s-2 */
     for (int s = 0; s < ns; s++) {
+        const int p = z0 - 2 + s;     /* unwrapped plane of this stage */
+        const int zp = zq0, zp1 = zq1, zp2 = zq2; /* wrapped planes of stages s, s-1, s-2 */
+        const bool doA = s >= 1, doB = p >= z0, doC = p >= z0 + 1, doD = p >= z0 + 2;
         if (s + D < ns) {
             issue(s + D, zi);
             zi = zi + 1 == a.L2 ? 0 : zi + 1;
@@ -1545,9 +1553,6 @@ __global__ __launch_bounds__(FX * FY) void two_step_f32(fused_args a)
             }
         }
         const float *S = lds + (s % NSLOT) * Ly::SLOT;
-        const int p = z0 - 2 + s;     /* unwrapped plane of this stage */
-        const int zp = zq0, zp1 = zq1, zp2 = zq2; /* wrapped planes of stages s, s-1, s-2 */
-        const bool doA = s >= 1, doB = p >= z0, doC = p >= z0 + 1, doD = p >= z0 + 2;
         const bool zs = zpml(s), zs1 = s >= 1 && zpml(s - 1), zs2 = s >= 2 && zpml(s - 2);
 
         float h1n[CELLS][3], sh1n[CELLS][3], e1n[CELLS][3], se1n[CELLS][3], h2n[CELLS][3];
@@ -1619,18 +1624,20 @@ __global__ __launch_bounds__(FX * FY) void two_step_f32(fused_args a)
             factors(r, s - 1, true, pa, pg);
             e_regular(e0p[r], bep[r], se0p[r], P, M, pml, pa, pg, e1n[r], se1n[r]);
             if (!pml && inT[r] && p - 1 >= z0 && p - 1 < z1 && (zp1 == a.iB || zp1 == a.iT)) {
-                for (int m = 0; m < 2; m++) { /* first step's monitors: main = total (S = 0) */
+                /* first step's monitors (main = total, S = 0): atomic adds that return
+                 * nothing, so no load has to be retired behind the copies in flight; one
+                 * lane per cell, in step order */
+                for (int m = 0; m < 2; m++) {
                     if (zp1 != (m == 0 ? a.iB : a.iT)) continue;
                     double *HM = m == 0 ? a.HB : a.HT, *EM = m == 0 ? a.EB : a.ET;
                     for (int v = 0; v < 2; v++) {
                         const int64_t o = 2 * (v + 2 * cq[r]);
-                        HM[o] += a.monH_re * (double)hc[v];
-                        HM[o + 1] += a.monH_im * (double)hc[v];
-                        EM[o] += a.monE_re * (double)e1n[r][v];
-                        EM[o + 1] += a.monE_im * (double)e1n[r][v];
+                        unsafeAtomicAdd(HM + o, a.monH_re * (double)hc[v]);
+                        unsafeAtomicAdd(HM + o + 1, a.monH_im * (double)hc[v]);
+                        unsafeAtomicAdd(EM + o, a.monE_re * (double)e1n[r][v]);
+                        unsafeAtomicAdd(EM + o + 1, a.monE_im * (double)e1n[r][v]);
                     }
                 }
-                retire_loads();
             }
             if (inG2[r]) {
                 lds[L_E1 + 0 * N4 + e] = e1n[r][0];
@@ -1709,13 +1716,12 @@ __global__ __launch_bounds__(FX * FY) void two_step_f32(fused_args a)
                     double *HM = m == 0 ? a.HB : a.HT, *EM = m == 0 ? a.EB : a.ET;
                     for (int v = 0; v < 2; v++) {
                         const int64_t o = 2 * (v + 2 * cq[r]);
-                        HM[o] += a.monH2_re * (double)hc[v];
-                        HM[o + 1] += a.monH2_im * (double)hc[v];
-                        EM[o] += a.monE2_re * (double)en[v];
-                        EM[o + 1] += a.monE2_im * (double)en[v];
+                        unsafeAtomicAdd(HM + o, a.monH2_re * (double)hc[v]);
+                        unsafeAtomicAdd(HM + o + 1, a.monH2_im * (double)hc[v]);
+                        unsafeAtomicAdd(EM + o, a.monE2_re * (double)en[v]);
+                        unsafeAtomicAdd(EM + o + 1, a.monE2_im * (double)en[v]);
                     }
                 }
-                retire_loads();
             }
         }
         /* carry to the next plane */
@@ -2135,6 +2141,7 @@ int smx_f32_two_ok(const smx_f32_state *st, int variant, int fz)
     if (variant < 0 || variant >= kNumTwo || fz < 1 || st->nspecial != 0 || 24 * st->n >= ((int64_t)1 << 32))
         return 0;
     if (kTwo[variant].wide && st->L[0] % 4 != 0) return 0;
+    if ((st->L[2] + fz - 1) / fz > kMaxZChunks) return 0;
     if (kTwo[variant].split) return fz <= two_layout<32, 8, true, false>::kMaxFZ;
     return !st->has_pml || st->split2_ok;
 }
@@ -2194,6 +2201,8 @@ void smx_f32_free(smx_f32_state *st)
     }
     if (st->pml) (void)hipFree(st->pml);
     if (st->pmlflag) (void)hipFree(st->pmlflag);
+    delete[] st->zpml_host;
+    st->zpml_host = nullptr;
     if (st->special) (void)hipFree(st->special);
     if (st->special_cells) (void)hipFree(st->special_cells);
     st->pml = nullptr;
@@ -2307,6 +2316,7 @@ int smx_f32_create(smx_f32_state *st, const smx_geom *g, double iR, double iI, c
     st->fz = kDefaultFZ;
     st->two = -1;
     st->two_fz = kDefaultFZ;
+    st->two_bal = 0;
     if (const char *t = getenv("SMX_F32_TILE")) { /* "<variant>,<z chunk>", e.g. "1,32" */
         int ti = kDefaultTile, zc = kDefaultFZ;
         if (sscanf(t, "%d,%d", &ti, &zc) >= 1 && smx_f32_tile_ok(st, ti, zc)) {
@@ -2314,11 +2324,12 @@ int smx_f32_create(smx_f32_state *st, const smx_geom *g, double iR, double iI, c
             st->fz = zc;
         }
     }
-    if (const char *t = getenv("SMX_F32_TWO")) { /* two-step kernel "<variant>,<z chunk>" */
-        int v = -1, zc = kDefaultFZ;
-        if (sscanf(t, "%d,%d", &v, &zc) >= 1 && smx_f32_two_ok(st, v, zc)) {
+    if (const char *t = getenv("SMX_F32_TWO")) { /* two-step kernel "<variant>,<z chunk>[,<balance>]" */
+        int v = -1, zc = kDefaultFZ, bal = 0;
+        if (sscanf(t, "%d,%d,%d", &v, &zc, &bal) >= 1 && smx_f32_two_ok(st, v, zc) && bal >= 0 && bal <= 4) {
             st->two = v;
             st->two_fz = zc;
+            st->two_bal = bal;
         }
     }
     for (int k = 0; k < 2; k++) {
@@ -2340,6 +2351,8 @@ int smx_f32_create(smx_f32_state *st, const smx_geom *g, double iR, double iI, c
     F32_HIP(hipMemcpyAsync(st->pml, pml.data(), sizeof(float) * pml.size(), hipMemcpyHostToDevice, stream));
     F32_HIP(hipMalloc(&st->pmlflag, flag.size()));
     F32_HIP(hipMemcpyAsync(st->pmlflag, flag.data(), flag.size(), hipMemcpyHostToDevice, stream));
+    st->zpml_host = new unsigned char[g->L2];
+    for (int z = 0; z < g->L2; z++) st->zpml_host[z] = flag[g->L0 + g->L1 + z];
     if (!special.empty()) {
         F32_HIP(hipMalloc(&st->special, sizeof(smx_special_f32) * special.size()));
         F32_HIP(hipMemcpyAsync(st->special, special.data(), sizeof(smx_special_f32) * special.size(),
@@ -2487,6 +2500,52 @@ hipError_t smx_f32_step(smx_f32_state *st, const smx_f32_launch_args *la, hipStr
     return hipGetLastError();
 }
 
+/* z chunks of the two-step kernel over [a.box[4], a.box[5]): as many as chunks of fz
+ * planes would make, each of at most `maxlen` planes.  bal = 0 cuts them uniformly; bal > 0
+ * balances their cost, a stage (a chunk's planes and the two halo planes on each side)
+ * counting 2, a stage of a z-PML plane 2 + bal (split slots, the PML update) and one of a
+ * monitor or source plane 2 + 2*bal (the accumulators, the profile loads), so the chunks
+ * holding them get fewer planes: the launch lasts as long as its slowest block.  False
+ * when more than kMaxZChunks chunks would be needed. */
+static bool two_chunks_z(fused_args &a, const smx_f32_state *st, int fz, int bal, int maxlen)
+{
+    const int zb = a.box[4], ze = a.box[5], len = ze - zb;
+    const int n = (len + fz - 1) / fz;
+    if (n > kMaxZChunks || (n > 0 && (len + n - 1) / n > maxlen)) return false;
+    a.nzc = n;
+    if (bal == 0 || n <= 1) {
+        for (int i = 0; i <= n; i++) a.zc[i] = min(zb + i * fz, ze);
+        return true;
+    }
+    auto cost = [&](int p) { /* stage of plane p (unwrapped) */
+        const int z = ((p % st->L[2]) + st->L[2]) % st->L[2];
+        return 2 + (st->zpml_host[z] ? bal : 0) + (z == a.iB || z == a.iT || z == a.iI ? 2 * bal : 0);
+    };
+    /* greedy cut under a cost bound T: the number of chunks it needs */
+    auto cuts = [&](int64_t T, int *zc) {
+        int k = 0, start = zb;
+        if (zc) zc[0] = zb;
+        while (start < ze) {
+            int64_t c = cost(start - 2) + cost(start - 1) + cost(start) + cost(start + 1) + cost(start + 2);
+            int end = start + 1;
+            while (end < ze && end - start < maxlen && c + cost(end + 2) <= T) c += cost(end + 2), end++;
+            start = end;
+            k++;
+            if (zc && k <= kMaxZChunks) zc[k] = end;
+        }
+        return k;
+    };
+    int64_t lo = 1, hi = 0;
+    for (int p = zb - 2; p < ze + 2; p++) hi += cost(p);
+    while (lo < hi) { /* least bound that needs at most n chunks */
+        const int64_t mid = (lo + hi) / 2;
+        if (cuts(mid, nullptr) <= n) hi = mid;
+        else lo = mid + 1;
+    }
+    a.nzc = cuts(lo, a.zc);
+    return a.nzc <= kMaxZChunks;
+}
+
 /* x tiles of the two-step kernel over a.box, placed so that every stage row (ox columns
  * left of its tile) starts on a 128-byte line: a row of FX + 2*ox words then spans 2
  * lines where it would span 3, and the copies ask the L2 for a third fewer lines.  A box
@@ -2518,10 +2577,18 @@ hipError_t smx_f32_step2(smx_f32_state *st, const smx_f32_launch_args *la, hipSt
     const int box[6] = {0, g.L0, 0, g.L1, 0, g.L2};
     for (int i = 0; i < 6; i++) a.box[i] = box[i];
     a.fz = st->two_fz;
+    a.iB = g.iB; /* also read by the z chunking */
+    a.iT = g.iT;
+    a.iI = g.iI;
     const two_variant tv = kTwo[st->two];
     two_tiles_x(a, tv.ox, tv.fx, g.L0);
     a.gy = (g.L1 + tv.fy - 1) / tv.fy;
-    a.nblocks = a.gx * a.gy * ((g.L2 + a.fz - 1) / a.fz);
+    const int maxlen = tv.split ? two_layout<32, 8, true, false>::kMaxFZ : g.L2;
+    if (!two_chunks_z(a, st, a.fz, st->two_bal, maxlen)) {
+        snprintf(f32_err, sizeof f32_err, "two-step kernel: more than %d z chunks", kMaxZChunks);
+        return hipErrorInvalidValue;
+    }
+    a.nblocks = a.gx * a.gy * a.nzc;
     auto launch_two = [&](const fused_args &k) {
         const dim3 grid(k.nblocks), block(tv.fx * tv.fy);
         hipLaunchKernelGGL(kTwoKernel[st->two], grid, block, 0, stream, k);
@@ -2640,7 +2707,11 @@ hipError_t smx_f32_step2(smx_f32_state *st, const smx_f32_launch_args *la, hipSt
     for (int i = 0; i < 6; i++) a.box[i] = st->fbox2[i];
     two_tiles_x(a, tv.ox, tv.fx, g.L0);
     a.gy = (a.box[3] - a.box[2] + tv.fy - 1) / tv.fy;
-    a.nblocks = a.gx * a.gy * ((a.box[5] - a.box[4] + a.fz - 1) / a.fz);
+    if (!two_chunks_z(a, st, a.fz, st->two_bal, maxlen)) {
+        snprintf(f32_err, sizeof f32_err, "two-step kernel: more than %d z chunks", kMaxZChunks);
+        return hipErrorInvalidValue;
+    }
+    a.nblocks = a.gx * a.gy * a.nzc;
     if ((e = launch_two(a)) != hipSuccess) return e;
     /* 4: H2 from tmp (H1, E1, H1 split) into new */
     if ((e = region(st->regP2, false, tT, tS, nT, nS, 2)) != hipSuccess) return e;

@@ -80,6 +80,9 @@ struct smx_f32_state {
     int two;   /* two-step kernel in use for pairs of steps: -1 off, else its variant
                   (kern_f32.hip kTwo); single steps still take the choice above */
     int two_fz;
+    int two_bal; /* z chunks of the two-step kernel: 0 uniform (two_fz planes), k > 0
+                    balanced with a stage of a z-PML plane counted as 1 + k/2 */
+    unsigned char *zpml_host; /* z-PML flag of every plane (host copy) */
     /* host copies for the reference-layout field import/export */
     char *cont_host;
     int *addr_host;

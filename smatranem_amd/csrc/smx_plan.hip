@@ -876,7 +876,7 @@ int smx_adjoint_backward(smx_plan *p, double *grad)
     return rc;
 }
 
-int smx_plan_autotune(smx_plan *p, int trial_steps, int choice[4])
+int smx_plan_autotune(smx_plan *p, int trial_steps, int choice[5])
 {
     static const int chunks[4] = {8, 16, 32, 64};
     if (!p) return arg_fail("null pointer");
@@ -932,13 +932,17 @@ int smx_plan_autotune(smx_plan *p, int trial_steps, int choice[4])
         st.fz = best_z;
         /* two steps per pass: timed per step over the same number of steps (even) */
         float best_step = best / (float)trial_steps;
-        int best_two = -1, best_two_fz = st.two_fz;
+        int best_two = -1, best_two_fz = st.two_fz, best_two_bal = 0;
         const int pairs = (trial_steps + 1) / 2;
+        bool zpml = false; /* balanced z chunks differ from uniform ones only with z PML */
+        for (int z = 0; z < st.L[2]; z++) zpml |= st.zpml_host[z] != 0;
         for (int v = 0; v < smx_f32_num_two(); v++)
-            for (int ci = 0; ci < 4; ci++) {
-                if (!smx_f32_two_ok(&st, v, chunks[ci])) continue;
+            for (int ci = 0; ci < 4 * 3; ci++) {
+                const int fz = chunks[ci % 4], bal = ci / 4;
+                if (!smx_f32_two_ok(&st, v, fz) || (bal > 0 && !zpml)) continue;
                 st.two = v;
-                st.two_fz = chunks[ci];
+                st.two_fz = fz;
+                st.two_bal = bal;
                 int rc;
                 if ((rc = launch_step2(p, s, s))) return rc;
                 SMX_HIP(hipEventRecord(e0, p->stream));
@@ -951,11 +955,13 @@ int smx_plan_autotune(smx_plan *p, int trial_steps, int choice[4])
                 if (ms / (2.f * pairs) < best_step) {
                     best_step = ms / (2.f * pairs);
                     best_two = v;
-                    best_two_fz = chunks[ci];
+                    best_two_fz = fz;
+                    best_two_bal = bal;
                 }
             }
         st.two = best_two;
         st.two_fz = best_two_fz;
+        st.two_bal = best_two_bal;
         for (int k = 0; k < 2; k++)
             for (int i = 0; i < 6; i++) {
                 SMX_HIP(hipMemcpyAsync(st.T[k][i], (char *)save + i * field, field, hipMemcpyDeviceToDevice, p->stream));
@@ -977,10 +983,12 @@ int smx_plan_autotune(smx_plan *p, int trial_steps, int choice[4])
         choice[1] = f ? fy : 0;
         choice[2] = f ? st.fz : 0;
         choice[3] = f ? (st.tile >= 3 ? 2 : 1) : 0;
+        choice[4] = 0;
         if (p->g.precision == SMX_F32 && st.two >= 0) {
             smx_f32_two_shape(st.two, &choice[0], &choice[1]);
             choice[2] = st.two_fz;
             choice[3] = 3;
+            choice[4] = st.two_bal;
         }
     }
     return SMX_OK;

@@ -371,10 +371,13 @@ class Plan:
     def autotune(self, trial_steps=20):
         """Pick the fastest FP32 step kernel for this grid; returns (fx, fy, fz, variant):
         variant 0 = the two in-place sweeps (sizes 0), 1 = fused step staged through
-        registers, 2 = fused step staged by asynchronous copies to LDS."""
-        out = np.zeros(4, dtype=np.int32)
+        registers, 2 = fused step staged by asynchronous copies to LDS, 3 = two steps per
+        pass.  `z_balance` then holds how the two-step kernel cuts z (0 equal chunks,
+        k > 0 chunks balanced with a z-PML stage counted 1 + k/2 times)."""
+        out = np.zeros(5, dtype=np.int32)
         _check(lib().smx_plan_autotune(self._h, int(trial_steps), out))
-        return tuple(int(v) for v in out)
+        self.z_balance = int(out[4])
+        return tuple(int(v) for v in out[:4])
 
     @property
     def kernels(self):

@@ -307,6 +307,10 @@ def test_two_step_equals_one_step_kernels(variant, monkeypatch):
     ("2,8", dict(pml_xy=0, pml_z=3, monitor_gap=0, iBc=(0, 0, 0))),
     ("2,5", dict(pml_xy=2, pml_z=3, iBc=(0, 0, 0))),
     ("2,64", dict(pml_xy=3, pml_z=4, iBc=(1, 1, 0), scale_amp=0.3)),
+    # z chunks balanced for the PML layers (third field: a z-PML stage counts 1 + k/2)
+    ("2,16,2", dict(pml_xy=0, pml_z=3, iBc=(1, -1, 0))),
+    ("2,8,1", dict(pml_xy=2, pml_z=3, monitor_gap=0, iBc=(0, 0, 0))),
+    ("0,8,2", dict(pml_xy=0, pml_z=3, iBc=(1, -1, 0))),
     # 16-byte copies
     ("3,16", dict(pml_xy=0, pml_z=3, iBc=(1, -1, 0))),
     ("3,8", dict(pml_xy=0, pml_z=3, monitor_gap=0, iBc=(0, 0, 0))),
@@ -320,7 +324,7 @@ def test_two_step_pass_with_pml(two, kw, monkeypatch):
     cells inside."""
     monkeypatch.setenv("SMX_F32_KERNEL", "fused")
     monkeypatch.setenv("SMX_F32_TWO", two)
-    v, fz = (int(x) for x in two.split(","))
+    v, fz = (int(x) for x in two.split(",")[:2])
     L0 = 44 if TWO_VARIANTS[v][2] else 37
     prep = casegen.prepare(small_waveguide(seed=16, L0=L0, L1=22, L2=44, outer=9, metal_cells=0,
                                            lossy_cells=0, core=(3, 3), **kw))
