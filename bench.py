@@ -130,8 +130,10 @@ def time_plan(plan, steps, warmup, torch):
 
 def batch_run(count, steps, torch, small=False):
     """Independent runs of seeded sizes (L0, L1 in [64, 192], L2 in [64, 256], dielectric
-    boxes, 8-cell PML on every face), FP32, each plan autotuned, advanced together by
-    smx_plan_run_batch; then the same plans one after another, for comparison."""
+    boxes, 8-cell PML on every face), FP32, each plan autotuned.  After the envelope's
+    ramp (untimed), `steps` steps of all plans are timed three ways: smx_plan_run_batch
+    (one captured HIP graph of every plan's segment, replayed), the same batch with
+    every step launched from the host (streams mode), and the plans one after another."""
     from smatranem_amd import casegen, fdtd
     cases = casegen.batch_cases(count, seed=2025)
     if small:
@@ -139,36 +141,113 @@ def batch_run(count, steps, torch, small=False):
                                         time_period=(5, 100000, 100010)) for i in range(count)]
     preps = [casegen.prepare(c) for c in cases]
     plans = [pr.plan(fdtd.F32) for pr in preps]
+    warm = max(pr.N * (pr.case.time_period[0] + 1) for pr in preps)
 
     def sync_all():
         for p in plans:
             p.sync()
         torch.cuda.synchronize()
+
+    def timed(fn):
+        sync_all()
+        t0 = time.perf_counter()
+        fn()
+        sync_all()
+        return time.perf_counter() - t0
+    info = {}
     try:
         for p in plans:
             p.autotune(10)
-        fdtd.run_batch(plans, 0, 20)
-        sync_all()
-        t0 = time.perf_counter()
-        fdtd.run_batch(plans, 20, steps)
-        sync_all()
-        t_batch = time.perf_counter() - t0
-        t0 = time.perf_counter()
-        for p in plans:
-            p.run_steps(20 + steps, steps)
-        sync_all()
-        t_seq = time.perf_counter() - t0
+        fdtd.run_batch(plans, 0, warm, mode=fdtd.BATCH_STREAMS)
+        k = warm
+        t_graph = timed(lambda: fdtd.run_batch(plans, k, steps, info=info))
+        k += steps
+        t_streams = timed(lambda: fdtd.run_batch(plans, k, steps, mode=fdtd.BATCH_STREAMS))
+        k += steps
+
+        def seq():
+            for p in plans:
+                p.run_steps(k, steps)
+        t_seq = timed(seq)
     finally:
         for p in plans:
             p.close()
     cells = sum(c.n for c in cases)
     alg = sum(c.n * bytes_per_cell(pr.pml_fraction, 4, 4) for c, pr in zip(cases, preps))
     return dict(workload="%d independent runs, L0, L1 in [64, 192], L2 in [64, 256], dielectric boxes, "
-                         "8-cell PML on all faces, FP32, smx_plan_run_batch" % count,
-                grids=[list(c.L[:3]) for c in cases], cells=cells, steps=steps,
-                value=round(cells * steps / t_batch / 1e9, 4), unit="Gcell-updates/s",
-                roofline_frac=round(alg * steps / t_batch / 1e9 / HBM_PEAK_GBS, 4),
+                         "8-cell PML on all faces, FP32, smx_plan_run_batch (HIP graph replay)" % count,
+                grids=[list(c.L[:3]) for c in cases], cells=cells, steps=steps, first_step=warm,
+                value=round(cells * steps / t_graph / 1e9, 4), unit="Gcell-updates/s",
+                roofline_frac=round(alg * steps / t_graph / 1e9 / HBM_PEAK_GBS, 4),
+                graph_launches=info.get("graph_launches"),
+                streams_mode=round(cells * steps / t_streams / 1e9, 4),
                 one_after_another=round(cells * steps / t_seq / 1e9, 4))
+
+
+def adjoint_run(torch, small=False, fd_cells=0):
+    """The gradient workload: a 96 x 96 x 128 design region of per-cell eps in [2.1, 12.1]
+    inside a 192 x 192 x 256 grid with 10-cell PML on every face; forward 4000 FP32 steps
+    with the full state checkpointed every 200 steps (smx_adjoint_forward), then the
+    reverse-time adjoint with recomputation (smx_adjoint_backward).  Objective: DFT power
+    of the tangential E at the source frequency on a plane 3 cells below the top PML.
+    fd_cells > 0 also checks dJ/d eps at that many of the largest-gradient cells against
+    central differences (delta eps = 1e-3) of the FP64 forward."""
+    import numpy as np
+    from smatranem_amd import casegen, fdtd
+    if small:
+        L, pml, K, C = (48, 48, 64), 6, 400, 50
+    else:
+        L, pml, K, C = (192, 192, 256), 10, 4000, 200
+    case, box = casegen.adjoint_case(L, pml=pml)
+    prep = casegen.prepare(case)
+    zT = L[2] - pml - 3
+    w = 2 * np.pi / prep.N
+    with prep.plan(fdtd.F32) as p:
+        p.autotune(10)
+        kern = p.kernels
+        p.sync()
+        t0 = time.perf_counter()
+        J = p.adjoint_forward(K, C, zT, w, box)
+        t1 = time.perf_counter()
+        g = p.adjoint_backward()
+        t2 = time.perf_counter()
+    n = case.n
+    nbox = (box[1] - box[0]) * (box[3] - box[2]) * (box[5] - box[4])
+    ckpt_gb = 12 * 4 * n * ((K + C - 1) // C) / 1e9
+    out = dict(workload="adjoint gradient, %dx%dx%d grid, %d-cell PML, design box %s (eps U[2.1, 12.1]), "
+                        "%d FP32 steps, checkpoints every %d" % (L[0], L[1], L[2], pml, list(box), K, C),
+               cells=n, design_cells=nbox, steps=K, ckpt_every=C, step_kernel=kern,
+               forward_s=round(t1 - t0, 3), backward_s=round(t2 - t1, 3),
+               forward_gcells=round(n * K / (t1 - t0) / 1e9, 3),
+               backward_gcells=round(n * K / (t2 - t1) / 1e9, 3),
+               checkpoint_GB=round(ckpt_gb, 2), J=J, grad_l2=float(np.linalg.norm(g)))
+    if fd_cells > 0:
+        def J64(eps):
+            c2 = casegen.Case(**{**case.__dict__, "eps": eps})
+            pr = casegen.prepare(c2)
+            with pr.plan(fdtd.F64) as q:
+                q.set_dft([zT], [w], chunk=32)
+                q.run_steps(0, K)
+                D = q.read_dft()[0]
+            return float(np.sum(np.abs(D[2:4, 0, :]) ** 2))
+        x0, x1, y0, y1, z0, z1 = box
+        order = np.argsort(np.abs(g).reshape(-1))[::-1][:fd_cells]
+        checks = []
+        for idx in order:
+            c, zz, yy, xx = (int(v) for v in np.unravel_index(idx, g.shape))
+            cell = (x0 + xx) + L[0] * ((y0 + yy) + L[1] * (z0 + zz))
+            h = 1e-3
+            ep, em = case.eps.copy(), case.eps.copy()
+            ep[3 * cell + c] += h
+            em[3 * cell + c] -= h
+            fd = (J64(ep) - J64(em)) / (2 * h)
+            checks.append(dict(cell=[x0 + xx, y0 + yy, z0 + zz], comp=c, adjoint=float(g[c, zz, yy, xx]), fd=fd,
+                               rel_err=abs(float(g[c, zz, yy, xx]) - fd) / abs(fd)))
+            print(f"[bench] adjoint vs FD at {checks[-1]['cell']} c={c}: {checks[-1]['rel_err']:.2e}",
+                  file=sys.stderr, flush=True)
+        out["fd_check"] = checks
+        out["fd_max_rel_err"] = max(ch["rel_err"] for ch in checks)
+    return out
 
 
 def cpu_baseline(prep, seconds):
@@ -225,8 +304,11 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 leg")
     ap.add_argument("--no-box", action="store_true", help="skip the all-faces-PML variant")
     ap.add_argument("--no-batch", action="store_true", help="skip the batch of independent runs")
-    ap.add_argument("--batch-count", type=int, default=8)
-    ap.add_argument("--batch-steps", type=int, default=400)
+    ap.add_argument("--batch-count", type=int, default=24)
+    ap.add_argument("--batch-steps", type=int, default=1000)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint gradient workload")
+    ap.add_argument("--adjoint-fd", type=int, default=0, metavar="CELLS",
+                    help="check the adjoint gradient at CELLS cells against FP64 central differences")
     ap.add_argument("--small", action="store_true", help="32x32x48 grid (quick check)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device copies of MIB MiB (scale for rocprofv3 byte counters)")
@@ -333,8 +415,15 @@ def main():
     batch = None
     if not args.no_batch:
         batch = batch_run(args.batch_count, args.batch_steps, torch, args.small)
-        print(f"[bench] batch of {args.batch_count}: {batch['value']:.2f} Gcell/s together, "
+        print(f"[bench] batch of {args.batch_count}: {batch['value']:.2f} Gcell/s graph replay "
+              f"({batch['graph_launches']} launches), {batch['streams_mode']:.2f} streams, "
               f"{batch['one_after_another']:.2f} one after another", file=sys.stderr, flush=True)
+
+    adjoint = None
+    if not args.no_adjoint:
+        adjoint = adjoint_run(torch, args.small, args.adjoint_fd)
+        print(f"[bench] adjoint: forward {adjoint['forward_gcells']:.2f} Gcell/s, backward "
+              f"{adjoint['backward_gcells']:.2f} Gcell/s", file=sys.stderr, flush=True)
 
     copy_gbs = copy_bandwidth(torch)
     print(f"[bench] device copy: {copy_gbs:.0f} GB/s", file=sys.stderr, flush=True)
@@ -377,6 +466,7 @@ def main():
         "err_vs_fp64": err,
         "pml_all_faces": box,
         "batch": batch,
+        "adjoint": adjoint,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -155,6 +155,26 @@ int smx_plan_run_steps(smx_plan *plan, long long step0, long long nsteps, double
  */
 int smx_plan_run_batch(smx_plan *const *plans, int nplans, long long step0, long long nsteps, double *sums);
 
+/*
+ * Batch with an explicit mode.  smx_plan_run_batch is mode SMX_BATCH_GRAPH.
+ *   SMX_BATCH_STREAMS: every step (or two-step pass) of every plan launched from the
+ *     host, step-major, each plan on its own stream.
+ *   SMX_BATCH_GRAPH: as STREAMS until every plan has entered a stretch where the
+ *     per-step source and monitor scalars repeat (the CW periods of the envelope); then
+ *     one segment of every plan (G steps, G a multiple of its N_2pi and of 4) is
+ *     captured from all plan streams into one HIP graph, and that graph is replayed for
+ *     as long as the scalars of every plan's next segment equal the captured ones bit
+ *     for bit; the remaining steps run as in STREAMS.  Plans with a DFT set (or a
+ *     segment longer than 1024 steps, or fewer than two segments to run) run as in
+ *     STREAMS.  Results are those of STREAMS exactly.
+ * graph_launches (may be NULL) receives the number of graph launches.  The batch
+ * allocates its graph, stream and events per call and frees them before returning.
+ * No counterpart in the reference.
+ */
+enum { SMX_BATCH_STREAMS = 0, SMX_BATCH_GRAPH = 1 };
+int smx_plan_run_batch_mode(smx_plan *const *plans, int nplans, long long step0, long long nsteps,
+                            double *sums, int mode, int *graph_launches);
+
 /* Wait for all queued work of the plan. */
 int smx_plan_sync(smx_plan *plan);
 

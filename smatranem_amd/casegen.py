@@ -128,7 +128,7 @@ def sheet_source(L0, L1, profile, n_eff=1.0):
 def waveguide_case(L0, L1, L2, outer, pml_xy, pml_z, *, omega=2 * np.pi / 20.0, eps_core=12.1,
                    eps_clad=2.1, core=(8, 6), scale_amp=0.0, n_modes=8, pec_fraction=0.0,
                    seed=0, iBc=(0, 0, 0), dt_factor=1.0, time_period=(2, 1, 8), monitor_gap=2,
-                   metal_cells=0, lossy_cells=0, blocks=0):
+                   metal_cells=0, lossy_cells=0, blocks=0, design=None):
     """A straight dielectric core along z in cladding, on an L0 x L1 x L2 grid with
     ``outer`` port slices at each end (B0.., T..), PML layers of ``pml_xy`` cells on the
     x/y faces and ``pml_z`` on the z faces (PMLcalc.c:188-260), optional curvilinear scale
@@ -137,7 +137,9 @@ def waveguide_case(L0, L1, L2, outer, pml_xy, pml_z, *, omega=2 * np.pi / 20.0, 
     position, MixedData/BindData.c:223-228), seeded PEC slabs covering about
     ``pec_fraction`` of the E edges of the scatterer, and optional metal ('e'/'h'/'b')
     and lossy cells, and ``blocks`` seeded dielectric boxes (eps uniform in [1, 12.1]) in
-    the scatterer.  Monitor planes sit between the z PML and the scatterer, the source
+    the scatterer, and an optional design region ``design`` = ((x0, x1, y0, y1, z0, z1),
+    lo, hi) whose cells each get one seeded eps, uniform in [lo, hi], for all three
+    components.  Monitor planes sit between the z PML and the scatterer, the source
     plane just above the bottom monitor."""
     rng = np.random.default_rng(seed)
     n = L0 * L1 * L2
@@ -164,6 +166,11 @@ def waveguide_case(L0, L1, L2, outer, pml_xy, pml_z, *, omega=2 * np.pi / 20.0, 
         dx = rng.integers(2, max(3, (L0 - 2 * pml_xy) // 3))
         eps3[bz:min(bz + dz, L2 - outer), by:min(by + dy, L1 - pml_xy), bx:min(bx + dx, L0 - pml_xy), :] = \
             rng.uniform(1.0, 12.1)
+
+    if design is not None:
+        (x0, x1, y0, y1, z0, z1), lo, hi = design
+        cell_eps = rng.uniform(lo, hi, size=(z1 - z0, y1 - y0, x1 - x0))
+        eps3[z0:z1, y0:y1, x0:x1, :] = cell_eps[..., None]
 
     # curvilinear scale factors
     if scale_amp:
@@ -266,6 +273,24 @@ def batch_cases(count, seed=0, pml=8, time_period=(5, 100000, 100010)):
                                   omega=2 * np.pi / 20.0, core=(max(L0 // 8, 2), max(L1 // 10, 2)),
                                   seed=seed * 1000 + i, blocks=6, time_period=time_period))
     return out
+
+
+def adjoint_case(L=(192, 192, 256), pml=10, design_frac=0.5, seed=2026, eps_range=(2.1, 12.1),
+                 time_period=(5, 100000, 100010)):
+    """The gradient workload: a design region of per-cell eps (seeded, uniform in
+    ``eps_range``) of half the grid along each axis (96 x 96 x 128 in a 192 x 192 x 256
+    grid), centred, around a Si-like core in eps 2.1 cladding, ``pml``-cell PML on every
+    face, modal sheet source.  Returns (case, design box)."""
+    L0, L1, L2 = L
+    box = []
+    for n_ in (L0, L1, L2):
+        w = int(round(n_ * design_frac))
+        lo = (n_ - w) // 2
+        box += [lo, lo + w]
+    case = waveguide_case(L0, L1, L2, outer=max(L2 // 8, pml + 4), pml_xy=pml, pml_z=pml,
+                          omega=2 * np.pi / 20.0, core=(max(L0 // 8, 2), max(L1 // 10, 2)), seed=seed,
+                          time_period=time_period, design=(tuple(box), *eps_range))
+    return case, tuple(box)
 
 
 @dataclass

@@ -2565,6 +2565,15 @@ static void two_tiles_x(fused_args &a, int ox, int fx, int L0)
     a.gx = (a.xhi - a.xorg + fx - 1) / fx;
 }
 
+hipError_t smx_f32_reserve(smx_f32_state *st, hipStream_t stream)
+{
+    if (st->two < 0 || !st->has_pml || kTwo[st->two].split || st->tmpT) return hipSuccess;
+    hipError_t e;
+    if ((e = hipMalloc(&st->tmpT, sizeof(float) * 6 * st->n)) != hipSuccess) return e;
+    if ((e = hipMalloc(&st->tmpS, sizeof(float) * 6 * st->n)) != hipSuccess) return e;
+    return hipMemsetAsync(st->tmpS, 0, sizeof(float) * 6 * st->n, stream);
+}
+
 hipError_t smx_f32_step2(smx_f32_state *st, const smx_f32_launch_args *la, hipStream_t stream)
 {
     if (!smx_f32_two_ok(st, st->two, st->two_fz)) return hipErrorInvalidValue;
@@ -2656,10 +2665,8 @@ hipError_t smx_f32_step2(smx_f32_state *st, const smx_f32_launch_args *la, hipSt
      * the second step's +x/+y/+z neighbours of P).
      */
     if (!st->tmpT) {
-        hipError_t e;
-        if ((e = hipMalloc(&st->tmpT, sizeof(float) * 6 * st->n)) != hipSuccess) return e;
-        if ((e = hipMalloc(&st->tmpS, sizeof(float) * 6 * st->n)) != hipSuccess) return e;
-        if ((e = hipMemsetAsync(st->tmpS, 0, sizeof(float) * 6 * st->n, stream)) != hipSuccess) return e;
+        const hipError_t e = smx_f32_reserve(st, stream);
+        if (e != hipSuccess) return e;
     }
     float *tT[6], *tS[6];
     for (int i = 0; i < 6; i++) {

@@ -105,6 +105,10 @@ void smx_f32_free(smx_f32_state *st);
 hipError_t smx_f32_step(smx_f32_state *st, const smx_f32_launch_args *a, hipStream_t stream);
 /* Two steps in one pass (scalars a->s, then a->s2), when smx_f32_two_ok(st, variant, fz). */
 hipError_t smx_f32_step2(smx_f32_state *st, const smx_f32_launch_args *a, hipStream_t stream);
+/* Allocate every buffer the current kernel choice allocates on first use (the scratch
+ * level of the two-step pass with region sweeps), so that the steps that follow make
+ * no allocation and can be captured into a graph. */
+hipError_t smx_f32_reserve(smx_f32_state *st, hipStream_t stream);
 int smx_f32_num_two(void);
 int smx_f32_two_ok(const smx_f32_state *st, int variant, int fz);
 /* tile of two-step variant `variant`: fx x fy cells, one thread each */

@@ -586,8 +586,240 @@ int smx_plan_run_steps(smx_plan *p, long long step0, long long nsteps, double su
     return SMX_OK;
 }
 
-int smx_plan_run_batch(smx_plan *const *plans, int nplans, long long step0, long long nsteps, double *sums)
+} // extern "C"
+
+/* ---------------------------------------------------------------------------------- */
+/* Batch replay through a HIP graph                                                    */
+/* ---------------------------------------------------------------------------------- */
+
+namespace {
+
+/* Scalars and envelope values (fH, fE per step) of steps [k0, k0 + G) of plan p;
+ * false where the envelope is undefined (past the stop period). */
+bool segment_scalars(const smx_plan *p, long long k0, int G, std::vector<smx_step_scalars> &s,
+                     std::vector<double> &env)
 {
+    s.resize((size_t)G);
+    env.resize(2 * (size_t)G);
+    for (int j = 0; j < G; j++) {
+        const long long k = k0 + j;
+        if (step_scalars(p, (int)(k / p->N), (int)(k % p->N), &s[(size_t)j], &env[2 * (size_t)j],
+                         &env[2 * (size_t)j + 1]))
+            return false;
+    }
+    return true;
+}
+
+bool same_scalars(const std::vector<smx_step_scalars> &a, const std::vector<smx_step_scalars> &b)
+{
+    return a.size() == b.size() && std::memcmp(a.data(), b.data(), sizeof(smx_step_scalars) * a.size()) == 0;
+}
+
+/* Host-side state that advance() changes: restored when a capture fails. */
+struct plan_host_state {
+    int cur;
+    long long next_step;
+};
+
+/*
+ * One graph holding, for every plan of the batch, one segment of its source loop: G_p
+ * steps, G_p a multiple of N_2pi and of 4, captured from the plan's own stream.  The
+ * kernels of a step take the step's scalars (envelope, phases) by value, and those
+ * repeat with period N_2pi while the envelope is constant (the CW periods), so the
+ * same graph advances every plan by another G_p steps wherever the scalars of those
+ * steps equal the captured ones bit for bit.  A segment flips the FP32 ping-pong
+ * buffers an even number of times (G_p one-step passes or G_p/2 two-step passes), so
+ * every replay starts from the buffers the capture started from.
+ */
+struct batch_graph {
+    hipStream_t ms = nullptr;              /* capture and launch stream */
+    hipEvent_t fork = nullptr, join_ev = nullptr;
+    std::vector<hipEvent_t> ev;            /* one per plan, for launches */
+    std::vector<hipEvent_t> cap_ev;        /* one per plan, inside the capture */
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<int> G;                    /* steps per segment, per plan */
+    std::vector<int> cur0;                 /* FP32 buffer index at the segment start */
+    std::vector<std::vector<smx_step_scalars>> tpl; /* captured scalars per plan */
+
+    ~batch_graph()
+    {
+        /* the graph's last launch may still run: destroy nothing under it */
+        if (ms) (void)hipStreamSynchronize(ms);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+        for (hipEvent_t e : cap_ev)
+            if (e) (void)hipEventDestroy(e);
+        if (fork) (void)hipEventDestroy(fork);
+        if (join_ev) (void)hipEventDestroy(join_ev);
+        if (ms) (void)hipStreamDestroy(ms);
+    }
+};
+
+long long gcd_ll(long long a, long long b) { return b ? gcd_ll(b, a % b) : a; }
+
+constexpr long long kBatchMaxSegment = 1024; /* longest segment captured, in steps */
+constexpr int kBatchCheckEvery = 8;          /* eager rounds between replay checks */
+
+/* Work already queued on the plans' streams precedes what is launched on ms. */
+int join_into(batch_graph &bg, smx_plan *const *plans, int nplans)
+{
+    for (int i = 0; i < nplans; i++) {
+        SMX_HIP(hipEventRecord(bg.ev[(size_t)i], plans[i]->stream));
+        SMX_HIP(hipStreamWaitEvent(bg.ms, bg.ev[(size_t)i], 0));
+    }
+    return SMX_OK;
+}
+
+/* Work launched on ms precedes what is queued on the plans' streams afterwards. */
+int fork_out(batch_graph &bg, smx_plan *const *plans, int nplans)
+{
+    SMX_HIP(hipEventRecord(bg.join_ev, bg.ms));
+    for (int i = 0; i < nplans; i++) SMX_HIP(hipStreamWaitEvent(plans[i]->stream, bg.join_ev, 0));
+    return SMX_OK;
+}
+
+int graph_launch(batch_graph &bg, smx_plan *const *plans, int nplans)
+{
+    int rc = join_into(bg, plans, nplans);
+    if (rc) return rc;
+    SMX_HIP(hipGraphLaunch(bg.exec, bg.ms));
+    return fork_out(bg, plans, nplans);
+}
+
+/*
+ * Capture one segment of every plan (plan i from step next[i]) into bg and launch it
+ * once.  The host state advances as if the steps had run, which the launch makes
+ * true.  Returns SMX_OK with bg.exec == nullptr when the segment cannot be replayed
+ * (host state restored, nothing launched), an error code on a HIP error.
+ */
+int capture_segment(batch_graph &bg, smx_plan *const *plans, int nplans, std::vector<long long> &next,
+                    double *sums)
+{
+    std::vector<plan_host_state> saved((size_t)nplans);
+    std::vector<double> saved_sums(sums ? 2 * (size_t)nplans : 0);
+    for (int i = 0; i < nplans; i++) {
+        smx_plan *p = plans[i];
+        if (p->g.precision == SMX_F32) SMX_HIP(smx_f32_reserve(&p->f32, p->stream));
+        saved[(size_t)i] = {p->f32.cur, p->next_step};
+    }
+    if (sums) std::memcpy(saved_sums.data(), sums, sizeof(double) * saved_sums.size());
+    bg.cur0.resize((size_t)nplans);
+    bg.tpl.resize((size_t)nplans);
+    std::vector<double> env;
+    for (int i = 0; i < nplans; i++) {
+        bg.cur0[(size_t)i] = plans[i]->f32.cur;
+        if (!segment_scalars(plans[i], next[(size_t)i], bg.G[(size_t)i], bg.tpl[(size_t)i], env))
+            return arg_fail("batch segment beyond the stop period");
+    }
+
+    SMX_HIP(hipStreamBeginCapture(bg.ms, hipStreamCaptureModeThreadLocal));
+    int rc = SMX_OK;
+    hipError_t he = hipEventRecord(bg.fork, bg.ms);
+    for (int i = 0; i < nplans && he == hipSuccess; i++) he = hipStreamWaitEvent(plans[i]->stream, bg.fork, 0);
+    if (he != hipSuccess) rc = hip_fail(he, "batch capture fork");
+    /* step-major within the segment, as the eager loop issues them */
+    std::vector<long long> k((size_t)nplans);
+    for (int i = 0; i < nplans; i++) k[(size_t)i] = next[(size_t)i];
+    for (bool more = true; more && rc == SMX_OK;) {
+        more = false;
+        for (int i = 0; i < nplans && rc == SMX_OK; i++) {
+            const long long end = next[(size_t)i] + bg.G[(size_t)i];
+            if (k[(size_t)i] >= end) continue;
+            int done = 0;
+            rc = advance(plans[i], k[(size_t)i], end - k[(size_t)i], sums ? sums + 2 * i : nullptr, &done);
+            k[(size_t)i] += done;
+            more = more || k[(size_t)i] < end;
+        }
+    }
+    for (int i = 0; i < nplans && rc == SMX_OK; i++) {
+        if ((he = hipEventRecord(bg.cap_ev[(size_t)i], plans[i]->stream)) != hipSuccess ||
+            (he = hipStreamWaitEvent(bg.ms, bg.cap_ev[(size_t)i], 0)) != hipSuccess)
+            rc = hip_fail(he, "batch capture join");
+    }
+    hipGraph_t graph = nullptr;
+    he = hipStreamEndCapture(bg.ms, &graph);
+    if (rc == SMX_OK && he != hipSuccess) rc = hip_fail(he, "hipStreamEndCapture");
+    bool replayable = rc == SMX_OK;
+    for (int i = 0; i < nplans && replayable; i++)
+        replayable = plans[i]->f32.cur == bg.cur0[(size_t)i];
+    if (rc == SMX_OK && replayable) {
+        he = hipGraphInstantiate(&bg.exec, graph, nullptr, nullptr, 0);
+        if (he != hipSuccess) {
+            bg.exec = nullptr;
+            rc = hip_fail(he, "hipGraphInstantiate");
+        }
+    }
+    bg.graph = graph;
+    if (rc != SMX_OK || !replayable) {
+        /* nothing of the segment ran: undo the host-side advance */
+        for (int i = 0; i < nplans; i++) {
+            plans[i]->f32.cur = saved[(size_t)i].cur;
+            plans[i]->next_step = saved[(size_t)i].next_step;
+        }
+        if (sums) std::memcpy(sums, saved_sums.data(), sizeof(double) * saved_sums.size());
+        if (bg.exec) (void)hipGraphExecDestroy(bg.exec);
+        bg.exec = nullptr;
+        return rc;
+    }
+    rc = graph_launch(bg, plans, nplans);
+    if (rc) return rc;
+    for (int i = 0; i < nplans; i++) {
+        next[(size_t)i] += bg.G[(size_t)i];
+        plans[i]->next_step = next[(size_t)i];
+    }
+    return SMX_OK;
+}
+
+/* Replay bg for every plan from next[i] if all their scalars match; *ran = 1 if so. */
+int try_replay(batch_graph &bg, smx_plan *const *plans, int nplans, long long end, std::vector<long long> &next,
+               double *sums, int *ran)
+{
+    *ran = 0;
+    for (int i = 0; i < nplans; i++)
+        if (next[(size_t)i] + bg.G[(size_t)i] > end || plans[i]->f32.cur != bg.cur0[(size_t)i]) return SMX_OK;
+    std::vector<std::vector<double>> env((size_t)nplans);
+    std::vector<smx_step_scalars> s;
+    for (int i = 0; i < nplans; i++) {
+        if (!segment_scalars(plans[i], next[(size_t)i], bg.G[(size_t)i], s, env[(size_t)i]) ||
+            !same_scalars(s, bg.tpl[(size_t)i]))
+            return SMX_OK;
+    }
+    int rc = graph_launch(bg, plans, nplans);
+    if (rc) return rc;
+    for (int i = 0; i < nplans; i++) {
+        if (sums)
+            for (int j = 0; j < bg.G[(size_t)i]; j++) {
+                sums[2 * i] += env[(size_t)i][2 * (size_t)j];
+                sums[2 * i + 1] += env[(size_t)i][2 * (size_t)j + 1];
+            }
+        next[(size_t)i] += bg.G[(size_t)i];
+        plans[i]->next_step = next[(size_t)i];
+    }
+    *ran = 1;
+    return SMX_OK;
+}
+
+/* Whether plan i's segment at next[i] repeats in the G_i steps after it. */
+bool segment_repeats(const smx_plan *p, long long k0, int G, long long end)
+{
+    if (k0 + 2 * (long long)G > end) return false;
+    std::vector<smx_step_scalars> a, b;
+    std::vector<double> env;
+    return segment_scalars(p, k0, G, a, env) && segment_scalars(p, k0 + G, G, b, env) && same_scalars(a, b);
+}
+
+} // namespace
+
+extern "C" {
+
+int smx_plan_run_batch_mode(smx_plan *const *plans, int nplans, long long step0, long long nsteps, double *sums,
+                            int mode, int *graph_launches)
+{
+    if (graph_launches) *graph_launches = 0;
+    if (mode != SMX_BATCH_STREAMS && mode != SMX_BATCH_GRAPH) return arg_fail("batch mode");
     if (!plans || nplans <= 0) return arg_fail("empty batch");
     for (int i = 0; i < nplans; i++) {
         if (!plans[i]) return arg_fail("null plan in batch");
@@ -599,19 +831,86 @@ int smx_plan_run_batch(smx_plan *const *plans, int nplans, long long step0, long
         }
     }
     if (step0 < 0 || nsteps < 0) return arg_fail("negative step");
-    std::vector<long long> next((size_t)nplans, step0);
-    for (long long k = step0; k < step0 + nsteps; k++) {
+    const long long end = step0 + nsteps;
+
+    /* graph replay needs every plan's steps to be plain kernel launches (no DFT
+     * samples, whose fold points depend on host counters) */
+    bool graph = mode == SMX_BATCH_GRAPH;
+    batch_graph bg;
+    if (graph) {
+        bg.G.resize((size_t)nplans);
+        for (int i = 0; i < nplans && graph; i++) {
+            const long long N = plans[i]->N;
+            const long long G = N / gcd_ll(N, 4) * 4;
+            graph = plans[i]->dft_nplanes == 0 && G <= kBatchMaxSegment && 2 * G <= nsteps;
+            bg.G[(size_t)i] = (int)G;
+        }
+    }
+    if (graph) {
+        SMX_HIP(hipStreamCreateWithFlags(&bg.ms, hipStreamNonBlocking));
+        SMX_HIP(hipEventCreateWithFlags(&bg.fork, hipEventDisableTiming));
+        SMX_HIP(hipEventCreateWithFlags(&bg.join_ev, hipEventDisableTiming));
+        bg.ev.assign((size_t)nplans, nullptr);
+        bg.cap_ev.assign((size_t)nplans, nullptr);
         for (int i = 0; i < nplans; i++) {
-            if (next[(size_t)i] != k) continue; /* a two-step pass already took step k */
+            SMX_HIP(hipEventCreateWithFlags(&bg.ev[(size_t)i], hipEventDisableTiming));
+            SMX_HIP(hipEventCreateWithFlags(&bg.cap_ev[(size_t)i], hipEventDisableTiming));
+        }
+    }
+
+    std::vector<long long> next((size_t)nplans, step0);
+    bool captured = false; /* a capture was tried (one per call) */
+    int wait = 0;          /* eager rounds until the next replay or capture check */
+    for (;;) {
+        if (graph && wait-- <= 0) {
+            wait = kBatchCheckEvery;
+            if (bg.exec) {
+                int ran = 0;
+                const int rc = try_replay(bg, plans, nplans, end, next, sums, &ran);
+                if (rc) return rc;
+                if (ran) {
+                    if (graph_launches) ++*graph_launches;
+                    wait = 0;
+                    continue;
+                }
+            } else if (!captured) {
+                /* capture once every plan has entered a stretch of repeating scalars */
+                bool ready = true;
+                for (int i = 0; i < nplans && ready; i++)
+                    ready = segment_repeats(plans[i], next[(size_t)i], bg.G[(size_t)i], end);
+                if (ready) {
+                    captured = true;
+                    const int rc = capture_segment(bg, plans, nplans, next, sums);
+                    if (rc) return rc;
+                    if (bg.exec) {
+                        if (graph_launches) ++*graph_launches;
+                        wait = 0;
+                        continue;
+                    }
+                }
+            }
+        }
+        /* one eager round: every plan that is behind the front advances by one step
+         * or one two-step pass, in step order */
+        long long kmin = end;
+        for (int i = 0; i < nplans; i++) kmin = std::min(kmin, next[(size_t)i]);
+        if (kmin >= end) break;
+        for (int i = 0; i < nplans; i++) {
+            if (next[(size_t)i] != kmin) continue;
             smx_plan *p = plans[i];
             int done = 0;
-            const int rc = advance(p, k, step0 + nsteps - k, sums ? sums + 2 * i : nullptr, &done);
+            const int rc = advance(p, kmin, end - kmin, sums ? sums + 2 * i : nullptr, &done);
             if (rc) return rc;
-            next[(size_t)i] = k + done;
-            p->next_step = k + done;
+            next[(size_t)i] = kmin + done;
+            p->next_step = kmin + done;
         }
     }
     return SMX_OK;
+}
+
+int smx_plan_run_batch(smx_plan *const *plans, int nplans, long long step0, long long nsteps, double *sums)
+{
+    return smx_plan_run_batch_mode(plans, nplans, step0, nsteps, sums, SMX_BATCH_GRAPH, nullptr);
 }
 
 int smx_plan_run(smx_plan *p, int P0, int P1, double sums[2])

@@ -101,6 +101,9 @@ def lib():
     L.smx_plan_run_batch.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_longlong, C.c_longlong,
                                      C.POINTER(C.c_double)]
     L.smx_plan_run_batch.restype = C.c_int
+    L.smx_plan_run_batch_mode.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_longlong, C.c_longlong,
+                                          C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_int)]
+    L.smx_plan_run_batch_mode.restype = C.c_int
     L.smx_plan_autotune.argtypes = [C.c_void_p, C.c_int, _ip]
     L.smx_plan_autotune.restype = C.c_int
     L.smx_plan_sync.argtypes = [C.c_void_p]
@@ -225,13 +228,28 @@ def main_calc(L, Ah, Bh, Ae, Be, iBc, H_t, E_t, iBTIin, Hi, Ei, ExpN2, HB, EB, H
     _check(rc)
 
 
-def run_batch(plans, step0, nsteps):
+BATCH_STREAMS = 0
+BATCH_GRAPH = 1
+
+
+def run_batch(plans, step0, nsteps, mode=None, info=None):
     """smx_plan_run_batch: steps [step0, step0 + nsteps) of independent plans, each on its
-    own stream, issued step-major.  Returns the envelope sums, float64[len(plans), 2]."""
+    own stream.  ``mode`` None is the library default (BATCH_GRAPH: repeating segments
+    replayed from one captured HIP graph); BATCH_STREAMS launches every step from the
+    host.  ``info`` (a dict, optional) receives ``graph_launches``.  Returns the envelope
+    sums, float64[len(plans), 2]."""
     handles = (C.c_void_p * len(plans))(*[p._h.value for p in plans])
     sums = np.zeros((len(plans), 2))
-    _check(lib().smx_plan_run_batch(handles, len(plans), int(step0), int(nsteps),
-                                    sums.ctypes.data_as(C.POINTER(C.c_double))))
+    sp = sums.ctypes.data_as(C.POINTER(C.c_double))
+    if mode is None and info is None:
+        _check(lib().smx_plan_run_batch(handles, len(plans), int(step0), int(nsteps), sp))
+    else:
+        launches = C.c_int(0)
+        _check(lib().smx_plan_run_batch_mode(handles, len(plans), int(step0), int(nsteps), sp,
+                                             BATCH_GRAPH if mode is None else int(mode),
+                                             C.byref(launches)))
+        if info is not None:
+            info["graph_launches"] = launches.value
     return sums
 
 

@@ -102,3 +102,86 @@ def test_batch_errors():
         with pytest.raises(fdtd.SmxError) as e:
             fdtd.run_batch([bare], 0, 1)
         assert e.value.status == 5
+
+
+GRAPH_MEMBERS = [
+    (fdtd.F32, dict(seed=31, L0=12, L1=10, L2=28, metal_cells=0, lossy_cells=0, time_period=(1, 24, 26))),
+    # lossy cells put this member on the one-step kernels; no metal cells, whose stable
+    # form as the reference codes it grows without bound over runs this long
+    (fdtd.F32, dict(seed=32, L0=37, L1=19, L2=41, outer=8, pml_xy=2, pml_z=3, core=(3, 3),
+                    metal_cells=0, lossy_cells=4, time_period=(2, 22, 25))),
+    (fdtd.F64, dict(seed=33, L0=9, L1=14, L2=30, outer=7, pml_xy=0, pml_z=3, metal_cells=0, lossy_cells=4,
+                    time_period=(1, 24, 26))),
+    (fdtd.F32, dict(seed=34, L0=20, L1=16, L2=33, outer=8, pml_xy=0, pml_z=0, iBc=(1, -1, 1),
+                    metal_cells=0, lossy_cells=0, time_period=(1, 24, 25))),
+    (fdtd.F32, dict(seed=35, L0=24, L1=12, L2=30, outer=8, pml_xy=3, pml_z=3, metal_cells=0,
+                    lossy_cells=0, omega=2 * np.pi / 9.0, time_period=(1, 30, 32))),
+]
+
+
+def _graph_preps():
+    return [(prec, casegen.prepare(small_waveguide(**kw))) for prec, kw in GRAPH_MEMBERS]
+
+
+@pytest.mark.parametrize("kernel", [None, "0,16", "2,16", "twopass"])
+def test_batch_graph_replay_equals_streams(kernel, monkeypatch):
+    """The graph mode (one captured segment of every plan, replayed over the CW periods)
+    gives bit for bit what the streams mode and each plan alone give, over ramp, CW
+    and ramp-down, from a step that is not a period start, and it does replay."""
+    if kernel == "twopass":
+        monkeypatch.setenv("SMX_F32_KERNEL", "twopass")
+    elif kernel:
+        monkeypatch.setenv("SMX_F32_TWO", kernel)
+    preps = _graph_preps()
+    assert len({pr.N for _, pr in preps}) > 1, "members should differ in N_2pi"
+    steps = min(pr.case.time_period[2] * pr.N for _, pr in preps) - 3
+    results = {}
+    for mode in (fdtd.BATCH_STREAMS, fdtd.BATCH_GRAPH):
+        plans = [pr.plan(prec) for prec, pr in preps]
+        try:
+            info = {}
+            s1 = fdtd.run_batch(plans, 0, 3, mode=mode)
+            s2 = fdtd.run_batch(plans, 3, steps - 3, mode=mode, info=info)
+            results[mode] = ([_state(p) for p in plans], s1 + s2, info["graph_launches"])
+        finally:
+            for p in plans:
+                p.close()
+    alone = []
+    for prec, pr in preps:
+        with pr.plan(prec) as p:
+            sums = p.run_steps(0, 3) + p.run_steps(3, steps - 3)
+            alone.append(_state(p) + (sums,))
+    st_streams, sums_streams, n_streams = results[fdtd.BATCH_STREAMS]
+    st_graph, sums_graph, n_graph = results[fdtd.BATCH_GRAPH]
+    assert n_streams == 0
+    assert n_graph >= 3, n_graph
+    for i in range(len(preps)):
+        a, b, c = alone[i], st_streams[i], st_graph[i]
+        assert np.abs(a[0]).max() > 0, i
+        for x in (b, c):
+            assert np.array_equal(a[0], x[0]) and np.array_equal(a[1], x[1]), i
+            for ma, mx in zip(a[2], x[2]):
+                assert np.array_equal(ma, mx), i
+        assert np.array_equal(a[3], sums_streams[i]) and np.array_equal(a[3], sums_graph[i]), i
+
+
+def test_batch_graph_off_with_dft():
+    """A member with the multi-frequency DFT set keeps the batch on the streams path
+    (its sample and fold launches depend on host counters), with the same results."""
+    preps = _graph_preps()[:3]
+    steps = min(pr.case.time_period[2] * pr.N for _, pr in preps)
+    out = []
+    for mode in (fdtd.BATCH_STREAMS, fdtd.BATCH_GRAPH):
+        plans = [pr.plan(prec) for prec, pr in preps]
+        try:
+            plans[0].set_dft([preps[0][1].case.L[2] - 4], [0.3, 0.5], chunk=8)
+            info = {}
+            fdtd.run_batch(plans, 0, steps, mode=mode, info=info)
+            out.append(([_state(p) for p in plans], plans[0].read_dft(), info["graph_launches"]))
+        finally:
+            for p in plans:
+                p.close()
+    assert out[1][2] == 0
+    for a, b in zip(out[0][0], out[1][0]):
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert np.array_equal(out[0][1], out[1][1])
