@@ -20,6 +20,9 @@
 
 #include <stdint.h>
 
+#include <cmath>
+#include <cstddef>
+
 #include "smx_dft.h"
 
 namespace {
@@ -70,60 +73,70 @@ __global__ __launch_bounds__(kThreads) void sample_planes_f64(smx_dft_sample_arg
 
 /*
  * acc[m][v][f][q] += sum_t e^{i*w_f*(k0 + t + 1 + h_v)} * series[t][m][v][q],
- * h_v = 0 for H, 1/2 for E.  One thread per (m, v, q); the phase table of the chunk is
- * built once per block in LDS.
+ * h_v = 0 for H, 1/2 for E, computed as rot_f * sum_t tab[h_v][t][f] * series[t] with
+ * tab = e^{i*w_f*(t + 1 + h)} (the plan's table, the same for every chunk) and
+ * rot_f = e^{i*w_f*k0}.  One thread per (m, v, q), all frequencies in one pass over the
+ * chunk (16 at a time); the table is copied to LDS once per block.
  */
 template <typename T>
 __global__ __launch_bounds__(kThreads) void fold_series(smx_dft_fold_args a)
 {
-    __shared__ double ph[2][SMX_DFT_MAX_CHUNK * SMX_DFT_MAX_FREQ][2]; /* [H/E][t*F + f][re, im] */
+    __shared__ double2 ph[2 * SMX_DFT_MAX_CHUNK * SMX_DFT_MAX_FREQ]; /* [h][t][f] */
     const int F = a.nfreq, nt = a.count;
     for (int i = threadIdx.x; i < 2 * nt * F; i += kThreads) {
-        const int e = i / (nt * F), r = i % (nt * F);
-        const int t = r / F, f = r % F;
-        const double arg = a.w[f] * ((double)(a.k0 + t + 1) + 0.5 * e);
-        double s, c;
-        sincos(arg, &s, &c);
-        ph[e][r][0] = c;
-        ph[e][r][1] = s;
+        const int h = i / (nt * F), r = i % (nt * F);
+        ph[i] = a.table[h * a.table_nt * F + r];
     }
     __syncthreads();
     const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
     const int64_t per = a.plane_cells;
     if (i >= per * 4 * a.nplanes) return;
     const int64_t mv = i / per, q = i - mv * per; /* mv = m*4 + v */
-    const int v = (int)(mv % 4);
-    const int e = v >= 2;
+    const int h = (int)(mv % 4) >= 2;
     const T *src = (const T *)a.series + mv * per + q;
     const int64_t tstride = (int64_t)a.nplanes * 4 * per;
     double *acc = a.acc + (mv * F) * 2 * per + 2 * q;
-    for (int f0 = 0; f0 < F; f0 += 8) {
-        double re[8], im[8];
+    const double2 *tab = ph + h * nt * F;
+    for (int f0 = 0; f0 < F; f0 += 16) {
+        double re[16], im[16];
 #pragma unroll
-        for (int j = 0; j < 8; j++) re[j] = im[j] = 0.;
+        for (int j = 0; j < 16; j++) re[j] = im[j] = 0.;
         for (int t = 0; t < nt; t++) {
             const double x = (double)src[t * tstride];
-            const double(*p)[2] = &ph[e][t * F + f0];
+            const double2 *p = tab + t * F + f0;
 #pragma unroll
-            for (int j = 0; j < 8; j++) {
+            for (int j = 0; j < 16; j++) {
                 if (f0 + j < F) {
-                    re[j] += p[j][0] * x;
-                    im[j] += p[j][1] * x;
+                    re[j] += p[j].x * x;
+                    im[j] += p[j].y * x;
                 }
             }
         }
 #pragma unroll
-        for (int j = 0; j < 8; j++) {
+        for (int j = 0; j < 16; j++) {
             if (f0 + j < F) {
+                const double cr = a.rot[f0 + j][0], ci = a.rot[f0 + j][1];
                 double *o = acc + (int64_t)(f0 + j) * 2 * per;
-                o[0] += re[j];
-                o[1] += im[j];
+                o[0] += cr * re[j] - ci * im[j];
+                o[1] += cr * im[j] + ci * re[j];
             }
         }
     }
 }
 
 } // namespace
+
+void smx_dft_table(int nfreq, const double *w, int chunk, double *tab)
+{
+    for (int h = 0; h < 2; h++)
+        for (int t = 0; t < chunk; t++)
+            for (int f = 0; f < nfreq; f++) {
+                const double arg = w[f] * ((double)(t + 1) + 0.5 * h);
+                double *o = tab + 2 * (((size_t)h * chunk + t) * nfreq + f);
+                o[0] = std::cos(arg);
+                o[1] = std::sin(arg);
+            }
+}
 
 extern "C" hipError_t smx_dft_sample(const smx_dft_sample_args *a, int slot, hipStream_t stream)
 {

@@ -431,6 +431,7 @@ constexpr int kDefaultTile = 0, kDefaultFZ = 16;
 
 /* most z chunks of one two-step launch */
 constexpr int kMaxZChunks = 256;
+constexpr int kDftMaxPlanes = 8; /* as SMX_DFT_MAX_PLANES */
 
 struct fused_args {
     int L0, L1, L2;
@@ -472,6 +473,12 @@ struct fused_args {
     int64_t reg_start[7];
     /* region_e_f32 accumulates no monitor for cells inside this box */
     int monskip[6];
+    /* two-step kernel: ring of the multi-frequency DFT ([slot][plane][Hx, Hy, Ex, Ey][q],
+     * floats); the first step's samples go to slot dft_slot, the second's to the next;
+     * dft_n = 0: no DFT */
+    float *dft;
+    int dft_n, dft_slot;
+    int dft_planes[kDftMaxPlanes];
 };
 
 __device__ inline int wrap_index(int v, int L)
@@ -571,6 +578,16 @@ __device__ inline void store3(float *const *dst, int64_t l, const float v[3])
     dst[0][l] = v[0];
     dst[1][l] = v[1];
     dst[2][l] = v[2];
+}
+
+/* Components c0 .. c0+2 of cell l through a group's buffer resource (component stride
+ * cb bytes; the group spans less than 4 GiB). */
+__device__ inline void bstore3(__amdgpu_buffer_rsrc_t r, unsigned cb, int c0, int64_t l, const float v[3])
+{
+    const unsigned off = 4u * (unsigned)l + (unsigned)c0 * cb;
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v[0]), r, off, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v[1]), r, off + cb, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v[2]), r, off + 2 * cb, 0, 0);
 }
 
 /* New H of one cell from preloaded old values: E at the cell (ec) and its -x, -y, -z
@@ -1370,7 +1387,7 @@ __device__ inline void e_regular(const float e[3], const float b[3], const float
     }
 }
 
-template <int FX, int FY, int D, bool SPLIT, bool WIDE>
+template <int FX, int FY, int D, bool SPLIT, bool WIDE, bool DFT>
 __global__ __launch_bounds__(FX * FY) void two_step_f32(fused_args a)
 {
     using Ly = two_layout<FX, FY, SPLIT, WIDE>;
@@ -1385,7 +1402,11 @@ __global__ __launch_bounds__(FX * FY) void two_step_f32(fused_args a)
      * in G4 layout, the chunk's z tables, a flag word */
     constexpr int L_H1 = NSLOT * Ly::SLOT, L_E1 = L_H1 + 3 * N4, L_H2 = L_E1 + 3 * N4;
     constexpr int L_ZT = L_H2 + 3 * N4, L_FLAG = L_ZT + (SPLIT ? 5 * (Ly::kMaxFZ + 4) : 0);
-    __shared__ float lds[L_FLAG + 1];
+    constexpr int L_SC = (L_FLAG + 2) & ~1; /* 20 doubles: the two steps' source and monitor scalars */
+    /* DFT: ring plane of each stage's plane, or -1 (with SPLIT, folded into the z table's
+     * flag word, which keeps the LDS of two blocks within a CU's 160 KiB) */
+    constexpr int L_DZ = L_SC + 2 * 20;
+    __shared__ __attribute__((aligned(16))) float lds[L_DZ + (DFT && !SPLIT ? Ly::kMaxFZ + 4 : 0)];
 
     const int id = xcd_logical(blockIdx.x, a.nblocks);
     const int bxi = id % a.gx, rest = id / a.gx;
@@ -1393,8 +1414,59 @@ __global__ __launch_bounds__(FX * FY) void two_step_f32(fused_args a)
     const int x0 = a.xorg + bxi * FX, y0 = a.box[2] + byi * FY, z0 = a.zc[bzi], z1 = a.zc[bzi + 1];
     const int t = threadIdx.x, lane = t & 63;
     const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
+    /* The per-step scalars are read only on the source and monitor planes: kept in LDS
+     * rather than in scalar registers across the z-march (the first barrier of the
+     * march publishes them). */
+    if (t == 0) {
+        double *d = reinterpret_cast<double *>(lds + L_SC);
+        d[0] = a.srcE_f2, d[1] = a.srcE_re, d[2] = a.srcE_im, d[3] = a.srcH_f2, d[4] = a.srcH_re;
+        d[5] = a.srcH_im, d[6] = a.monH_re, d[7] = a.monH_im, d[8] = a.monE_re, d[9] = a.monE_im;
+        d[10] = a.srcE2_f2, d[11] = a.srcE2_re, d[12] = a.srcE2_im, d[13] = a.srcH2_f2, d[14] = a.srcH2_re;
+        d[15] = a.srcH2_im, d[16] = a.monH2_re, d[17] = a.monH2_im, d[18] = a.monE2_re, d[19] = a.monE2_im;
+    }
+    const double *sc = reinterpret_cast<const double *>(lds + L_SC);
+    /* source profiles Hinc, Einc and the monitors HB, EB, HT, ET are consecutive arrays
+     * of 4*L0*L1 doubles (the host checks) */
+    const double *Hinc = a.Hinc, *Einc = a.Hinc + 4 * a.sz;
+    auto mon = [&](int k) { return a.HB + 4 * k * a.sz; }; /* HB, EB, HT, ET */
     const int ns = z1 - z0 + 4; /* stages: planes z0-2 .. z1+1 */
     auto plane_of = [&](int s) { return wrap_index(z0 - 2 + s, a.L2); };
+    /* DFT planes among this block's own planes: the first two, and whether there are
+     * more (then every plane is checked against the whole list) */
+    /* DFT: for each stage, the ring plane its plane is sampled into (only the block's own
+     * planes [z0, z1), once), published by the first barrier of the march; the ring of
+     * the first step as a buffer resource, the second step's slot dstep bytes further */
+    __amdgpu_buffer_rsrc_t rD = __builtin_amdgcn_make_buffer_rsrc(nullptr, (short)0, 0, 0);
+    unsigned dstep = 0;
+    auto dft_m = [&](int i) { /* ring plane of stage i's plane, -1 if none or not the block's */
+        const int z = z0 - 2 + i;
+        int m = -1;
+        for (int k = 0; k < a.dft_n && z >= z0 && z < z1; k++)
+            if (a.dft_planes[k] == z) m = k;
+        return m;
+    };
+    if constexpr (DFT) {
+        if constexpr (!SPLIT)
+            for (int i = t; i < ns; i += NT) reinterpret_cast<int *>(lds + L_DZ)[i] = dft_m(i);
+        const int64_t slot_words = 4 * (int64_t)a.dft_n * a.sz;
+        rD = __builtin_amdgcn_make_buffer_rsrc((void *)(a.dft + a.dft_slot * slot_words), (short)0,
+                                               (int)(unsigned)(8 * slot_words), (int)kBufWord3);
+        dstep = 4u * (unsigned)slot_words;
+    }
+    /* sample (Hx, Hy, Ex, Ey) of cell q at stage s into the ring (second step: soff = dstep):
+     * the total field, which is the main slot on the lossless grids this kernel takes
+     * outside PML, and the sample taken in PML cells */
+    auto dft_sample = [&](int s_, int64_t q, unsigned soff, float hx, float hy, float ex, float ey) {
+        if constexpr (DFT) {
+            const int m = SPLIT ? ((int)lds[L_ZT + 5 * s_ + 4] >> 1) - 1 : reinterpret_cast<const int *>(lds + L_DZ)[s_];
+            if (m < 0) return;
+            const unsigned o = 4u * (unsigned)(4 * m * a.sz + q), pl = 4u * (unsigned)a.sz;
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(hx), rD, o, soff, 0);
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(hy), rD, o + pl, soff, 0);
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(ex), rD, o + 2 * pl, soff, 0);
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(ey), rD, o + 3 * pl, soff, 0);
+        }
+    };
 
     /* byte offset in a plane of the first word lane `lane` copies in run ch (WIDE: a group
      * of 4 x; the groups past the region repeat the last one, into the run's padding) */
@@ -1474,7 +1546,8 @@ __global__ __launch_bounds__(FX * FY) void two_step_f32(fused_args a)
             lds[L_ZT + 5 * i + 1] = tz[a.L2 + z];
             lds[L_ZT + 5 * i + 2] = tz[2 * a.L2 + z];
             lds[L_ZT + 5 * i + 3] = tz[3 * a.L2 + z];
-            lds[L_ZT + 5 * i + 4] = a.pmlflag[a.L0 + a.L1 + z] ? 1.f : 0.f;
+            /* bit 0: z PML; bits 1..: 1 + the DFT ring plane (DFT variants) */
+            lds[L_ZT + 5 * i + 4] = (float)((a.pmlflag[a.L0 + a.L1 + z] ? 1 : 0) + (DFT ? 2 * (dft_m(i) + 1) : 0));
         }
         /* does any cell of the G4 region lie in x/y PML? (vote through the LDS array) */
         if (t == 0) lds[L_FLAG] = 0.f;
@@ -1484,11 +1557,13 @@ __global__ __launch_bounds__(FX * FY) void two_step_f32(fused_args a)
         tile_pml = lds[L_FLAG] != 0.f;
         retire_loads(); /* the table loads, before the copies start */
     }
-    auto zpml = [&](int s) { return SPLIT && lds[L_ZT + 5 * s + 4] != 0.f; };
+    auto zpml = [&](int s) { return SPLIT && ((int)lds[L_ZT + 5 * s + 4] & 1); };
 
     const __amdgpu_buffer_rsrc_t rT = group_rsrc(a.To[0], a.ncomp), rB = group_rsrc(a.b[0], a.ncomp),
                                  rS = group_rsrc(a.So[0], a.ncomp);
     const unsigned cb = 4u * (unsigned)a.ncomp;
+    /* the new fields and splits: one resource per group, as the copies */
+    const __amdgpu_buffer_rsrc_t rTn = group_rsrc(a.Tn[0], a.ncomp), rSn = group_rsrc(a.Sn[0], a.ncomp);
     auto issue = [&](int s, int z) { /* stage s, plane z = plane_of(s) */
         const unsigned soff = 4u * (unsigned)((int64_t)z * a.sz);
         const bool split = SPLIT && (tile_pml || zpml(s));
@@ -1586,8 +1661,8 @@ __global__ __launch_bounds__(FX * FY) void two_step_f32(fused_args a)
             P[2] = ec[1] - wx * exm[1];
             M[2] = -ec[0] + wy * eym[0];
             if (!pml && zp == a.iI) {
-                P[0] += a.sgn * (float)plane_source(nullptr, a.Einc, a.srcE_f2, a.srcE_re, a.srcE_im, cq[r], 1);
-                P[1] -= a.sgn * (float)plane_source(nullptr, a.Einc, a.srcE_f2, a.srcE_re, a.srcE_im, cq[r], 0);
+                P[0] += a.sgn * (float)plane_source(nullptr, Einc, sc[0], sc[1], sc[2], cq[r], 1);
+                P[1] -= a.sgn * (float)plane_source(nullptr, Einc, sc[0], sc[1], sc[2], cq[r], 0);
                 retire_loads();
             }
             float pa[3], pg[3];
@@ -1616,8 +1691,8 @@ __global__ __launch_bounds__(FX * FY) void two_step_f32(fused_args a)
             P[2] = wx * hxp[1] - hc[1];
             M[2] = -wy * hyp[0] + hc[0];
             if (!pml && zp1 == a.iI) {
-                P[0] += a.sgn * (float)plane_source(nullptr, a.Hinc, a.srcH_f2, a.srcH_re, a.srcH_im, cq[r], 1);
-                P[1] -= a.sgn * (float)plane_source(nullptr, a.Hinc, a.srcH_f2, a.srcH_re, a.srcH_im, cq[r], 0);
+                P[0] += a.sgn * (float)plane_source(nullptr, Hinc, sc[3], sc[4], sc[5], cq[r], 1);
+                P[1] -= a.sgn * (float)plane_source(nullptr, Hinc, sc[3], sc[4], sc[5], cq[r], 0);
                 retire_loads();
             }
             float pa[3], pg[3];
@@ -1629,16 +1704,17 @@ __global__ __launch_bounds__(FX * FY) void two_step_f32(fused_args a)
                  * lane per cell, in step order */
                 for (int m = 0; m < 2; m++) {
                     if (zp1 != (m == 0 ? a.iB : a.iT)) continue;
-                    double *HM = m == 0 ? a.HB : a.HT, *EM = m == 0 ? a.EB : a.ET;
+                    double *HM = mon(2 * m), *EM = mon(2 * m + 1);
                     for (int v = 0; v < 2; v++) {
                         const int64_t o = 2 * (v + 2 * cq[r]);
-                        unsafeAtomicAdd(HM + o, a.monH_re * (double)hc[v]);
-                        unsafeAtomicAdd(HM + o + 1, a.monH_im * (double)hc[v]);
-                        unsafeAtomicAdd(EM + o, a.monE_re * (double)e1n[r][v]);
-                        unsafeAtomicAdd(EM + o + 1, a.monE_im * (double)e1n[r][v]);
+                        unsafeAtomicAdd(HM + o, sc[6] * (double)hc[v]);
+                        unsafeAtomicAdd(HM + o + 1, sc[7] * (double)hc[v]);
+                        unsafeAtomicAdd(EM + o, sc[8] * (double)e1n[r][v]);
+                        unsafeAtomicAdd(EM + o + 1, sc[9] * (double)e1n[r][v]);
                     }
                 }
             }
+            if (DFT && inT[r]) dft_sample(s - 1, cq[r], 0u, hc[0], hc[1], e1n[r][0], e1n[r][1]);
             if (inG2[r]) {
                 lds[L_E1 + 0 * N4 + e] = e1n[r][0];
                 lds[L_E1 + 1 * N4 + e] = e1n[r][1];
@@ -1667,16 +1743,16 @@ __global__ __launch_bounds__(FX * FY) void two_step_f32(fused_args a)
             P[2] = ec[1] - wx * exm[1];
             M[2] = -ec[0] + wy * eym[0];
             if (!pml && zp1 == a.iI) {
-                P[0] += a.sgn * (float)plane_source(nullptr, a.Einc, a.srcE2_f2, a.srcE2_re, a.srcE2_im, cq[r], 1);
-                P[1] -= a.sgn * (float)plane_source(nullptr, a.Einc, a.srcE2_f2, a.srcE2_re, a.srcE2_im, cq[r], 0);
+                P[0] += a.sgn * (float)plane_source(nullptr, Einc, sc[10], sc[11], sc[12], cq[r], 1);
+                P[1] -= a.sgn * (float)plane_source(nullptr, Einc, sc[10], sc[11], sc[12], cq[r], 0);
                 retire_loads();
             }
             float pa[3], pg[3];
             factors(r, s - 1, false, pa, pg);
             h_regular(h1p[r], bhp[r], sh1p[r], P, M, pml, pa, pg, h2n[r], sh2);
             if (inT[r] && p - 1 < z1) {
-                store3(a.Tn, cq[r] + a.sz * zp1, h2n[r]);
-                if (SPLIT && pml) store3(a.Sn, cq[r] + a.sz * zp1, sh2);
+                bstore3(rTn, cb, 0, cq[r] + a.sz * zp1, h2n[r]);
+                if (SPLIT && pml) bstore3(rSn, cb, 0, cq[r] + a.sz * zp1, sh2);
             }
         }
         /* d. E2(p-2) on T: forward curl of H2 (previous plane from LDS, +z from c.) */
@@ -1701,25 +1777,26 @@ __global__ __launch_bounds__(FX * FY) void two_step_f32(fused_args a)
             P[2] = wx * hxp[1] - hc[1];
             M[2] = -wy * hyp[0] + hc[0];
             if (!pml && zp2 == a.iI) {
-                P[0] += a.sgn * (float)plane_source(nullptr, a.Hinc, a.srcH2_f2, a.srcH2_re, a.srcH2_im, cq[r], 1);
-                P[1] -= a.sgn * (float)plane_source(nullptr, a.Hinc, a.srcH2_f2, a.srcH2_re, a.srcH2_im, cq[r], 0);
+                P[0] += a.sgn * (float)plane_source(nullptr, Hinc, sc[13], sc[14], sc[15], cq[r], 1);
+                P[1] -= a.sgn * (float)plane_source(nullptr, Hinc, sc[13], sc[14], sc[15], cq[r], 0);
                 retire_loads();
             }
             float pa[3], pg[3];
             factors(r, s - 2, true, pa, pg);
             e_regular(e1pp[r], bepp[r], se1pp[r], P, M, pml, pa, pg, en, sn);
-            store3(a.Tn + 3, cq[r] + a.sz * zp2, en);
-            if (SPLIT && pml) store3(a.Sn + 3, cq[r] + a.sz * zp2, sn);
+            bstore3(rTn, cb, 3, cq[r] + a.sz * zp2, en);
+            if (SPLIT && pml) bstore3(rSn, cb, 3, cq[r] + a.sz * zp2, sn);
+            if (DFT) dft_sample(s - 2, cq[r], dstep, hc[0], hc[1], en[0], en[1]);
             if (!pml && (zp2 == a.iB || zp2 == a.iT)) { /* second step's monitors */
                 for (int m = 0; m < 2; m++) {
                     if (zp2 != (m == 0 ? a.iB : a.iT)) continue;
-                    double *HM = m == 0 ? a.HB : a.HT, *EM = m == 0 ? a.EB : a.ET;
+                    double *HM = mon(2 * m), *EM = mon(2 * m + 1);
                     for (int v = 0; v < 2; v++) {
                         const int64_t o = 2 * (v + 2 * cq[r]);
-                        unsafeAtomicAdd(HM + o, a.monH2_re * (double)hc[v]);
-                        unsafeAtomicAdd(HM + o + 1, a.monH2_im * (double)hc[v]);
-                        unsafeAtomicAdd(EM + o, a.monE2_re * (double)en[v]);
-                        unsafeAtomicAdd(EM + o + 1, a.monE2_im * (double)en[v]);
+                        unsafeAtomicAdd(HM + o, sc[16] * (double)hc[v]);
+                        unsafeAtomicAdd(HM + o + 1, sc[17] * (double)hc[v]);
+                        unsafeAtomicAdd(EM + o, sc[18] * (double)en[v]);
+                        unsafeAtomicAdd(EM + o + 1, sc[19] * (double)en[v]);
                     }
                 }
             }
@@ -2119,11 +2196,22 @@ constexpr int kNumTwo = sizeof kTwo / sizeof kTwo[0];
 
 int smx_f32_num_two(void) { return kNumTwo; }
 
+int smx_f32_two_full(const smx_f32_state *st, int variant)
+{
+    return variant >= 0 && variant < kNumTwo && (!st->has_pml || kTwo[variant].split);
+}
+
 /* the kernels of kTwo, in its order */
 using two_kernel = void (*)(fused_args);
-constexpr two_kernel kTwoKernel[] = {two_step_f32<32, 8, 1, false, false>, two_step_f32<32, 8, 2, false, false>,
-                                     two_step_f32<32, 8, 1, true, false>, two_step_f32<32, 8, 1, false, true>};
+constexpr two_kernel kTwoKernel[] = {
+    two_step_f32<32, 8, 1, false, false, false>, two_step_f32<32, 8, 2, false, false, false>,
+    two_step_f32<32, 8, 1, true, false, false>, two_step_f32<32, 8, 1, false, true, false>};
 static_assert(sizeof kTwoKernel / sizeof kTwoKernel[0] == kNumTwo, "one kernel per variant");
+/* the same with the DFT samples of both steps (run while a DFT is set) */
+constexpr two_kernel kTwoKernelDft[] = {
+    two_step_f32<32, 8, 1, false, false, true>, two_step_f32<32, 8, 2, false, false, true>,
+    two_step_f32<32, 8, 1, true, false, true>, two_step_f32<32, 8, 1, false, true, true>};
+static_assert(sizeof kTwoKernelDft / sizeof kTwoKernelDft[0] == kNumTwo, "one kernel per variant");
 
 void smx_f32_two_shape(int variant, int *fx, int *fy)
 {
@@ -2600,7 +2688,7 @@ hipError_t smx_f32_step2(smx_f32_state *st, const smx_f32_launch_args *la, hipSt
     a.nblocks = a.gx * a.gy * a.nzc;
     auto launch_two = [&](const fused_args &k) {
         const dim3 grid(k.nblocks), block(tv.fx * tv.fy);
-        hipLaunchKernelGGL(kTwoKernel[st->two], grid, block, 0, stream, k);
+        hipLaunchKernelGGL(k.dft_n ? kTwoKernelDft[st->two] : kTwoKernel[st->two], grid, block, 0, stream, k);
         return hipGetLastError();
     };
     a.sz = (int64_t)g.L0 * g.L1;
@@ -2647,6 +2735,26 @@ hipError_t smx_f32_step2(smx_f32_state *st, const smx_f32_launch_args *la, hipSt
     a.EB = la->EB;
     a.HT = la->HT;
     a.ET = la->ET;
+    /* the kernel reaches Einc, EB, HT, ET and the field components from one base each */
+    const int64_t pl4 = 4 * (int64_t)g.L0 * g.L1;
+    if (la->Einc != la->Hinc + pl4 || la->EB != la->HB + pl4 || la->HT != la->HB + 2 * pl4 ||
+        la->ET != la->HB + 3 * pl4)
+        return hipErrorInvalidValue;
+    for (int i = 1; i < 6; i++)
+        if (st->T[1 - st->cur][i] != st->T[1 - st->cur][0] + (size_t)i * st->n ||
+            st->S[1 - st->cur][i] != st->S[1 - st->cur][0] + (size_t)i * st->n)
+            return hipErrorInvalidValue;
+    if (la->dft_n > 0) {
+        /* the kernel samples only the cells it updates, and keeps a table of the stages of
+         * its z chunk (at most kMaxFZ planes) */
+        if (!smx_f32_two_full(st, st->two) || la->dft_n > kDftMaxPlanes) return hipErrorInvalidValue;
+        for (int i = 0; i < a.nzc; i++)
+            if (a.zc[i + 1] - a.zc[i] > two_layout<32, 8, true, false>::kMaxFZ) return hipErrorInvalidValue;
+        a.dft = la->dft;
+        a.dft_n = la->dft_n;
+        a.dft_slot = la->dft_slot;
+        for (int m = 0; m < la->dft_n; m++) a.dft_planes[m] = la->dft_planes[m];
+    }
     if (!st->has_pml || tv.split) {
         const hipError_t e = launch_two(a);
         if (e != hipSuccess) return e;

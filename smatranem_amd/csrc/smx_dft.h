@@ -26,12 +26,16 @@ struct smx_dft_sample_args {
 
 struct smx_dft_fold_args {
     int nplanes, nfreq, count;
-    long long k0;                   /* run step index of ring slot 0 */
     int64_t plane_cells;
-    double w[SMX_DFT_MAX_FREQ];     /* phase advance per step, omega_f*Dt */
+    int table_nt;                   /* chunk length the table was built for (>= count) */
+    const double2 *table;           /* [2 (H, E)][table_nt][nfreq]: e^{i*w_f*(t + 1 + h/2)} */
+    double rot[SMX_DFT_MAX_FREQ][2]; /* e^{i*w_f*k0}, k0 = run step index of ring slot 0 */
     const void *series;
     double *acc;                    /* [plane][4][freq][L0*L1] complex */
 };
+
+/* The phase table of a plan's DFT (host): tab[h][t][f] = e^{i*w_f*(t + 1 + h/2)}, t < chunk. */
+void smx_dft_table(int nfreq, const double *w, int chunk, double *tab);
 
 extern "C" {
 hipError_t smx_dft_sample(const smx_dft_sample_args *a, int slot, hipStream_t stream);

@@ -96,6 +96,11 @@ struct smx_f32_launch_args {
     const double *srcH_plane, *srcE_plane;
     const double *Hinc, *Einc;
     double *HB, *EB, *HT, *ET;
+    /* smx_f32_step2 only: multi-frequency DFT ring (float [slot][plane][4][L0*L1]); the
+     * two steps' samples go to slots dft_slot and dft_slot + 1; dft_n = 0: none */
+    float *dft;
+    int dft_n, dft_slot;
+    int dft_planes[8];
 };
 
 int smx_f32_create(smx_f32_state *st, const smx_geom *g, double iR, double iI, const double *mu,
@@ -110,6 +115,9 @@ hipError_t smx_f32_step2(smx_f32_state *st, const smx_f32_launch_args *a, hipStr
  * no allocation and can be captured into a graph. */
 hipError_t smx_f32_reserve(smx_f32_state *st, hipStream_t stream);
 int smx_f32_num_two(void);
+/* Whether two-step variant `variant` updates every cell of this grid in its one launch
+ * (no region sweeps), so that it can also take the DFT samples of both steps. */
+int smx_f32_two_full(const smx_f32_state *st, int variant);
 int smx_f32_two_ok(const smx_f32_state *st, int variant, int fz);
 /* tile of two-step variant `variant`: fx x fy cells, one thread each */
 void smx_f32_two_shape(int variant, int *fx, int *fy);

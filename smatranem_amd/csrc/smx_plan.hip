@@ -81,6 +81,7 @@ struct smx_plan {
     long long dft_k0 = 0;
     void *dft_series = nullptr;
     double *dft_acc = nullptr;
+    double *dft_table = nullptr; /* [2][chunk][nfreq] complex phases (kern_dft.hip) */
 
     /* adjoint (kern_adjoint.hip): checkpoints of the forward run, adjoint fields */
     long long adj_K = 0;
@@ -194,7 +195,7 @@ void free_plan(smx_plan *p)
     if (!p) return;
     if (p->stream) (void)hipStreamSynchronize(p->stream);
     void *bufs[] = {p->Ah, p->Bh, p->Ae, p->Be, p->H, p->E, p->Cont, p->addr, p->mon, p->inc, p->src,
-                    p->dft_series, p->dft_acc, p->adj_ckpt, p->adj_D};
+                    p->dft_series, p->dft_acc, p->dft_table, p->adj_ckpt, p->adj_D};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     smx_f32_free(&p->f32);
@@ -273,9 +274,14 @@ int dft_flush(smx_plan *p)
     f.nplanes = p->dft_nplanes;
     f.nfreq = p->dft_nfreq;
     f.count = p->dft_fill;
-    f.k0 = p->dft_k0;
     f.plane_cells = p->plane;
-    std::memcpy(f.w, p->dft_w, sizeof f.w);
+    f.table_nt = p->dft_chunk;
+    f.table = reinterpret_cast<const double2 *>(p->dft_table);
+    for (int i = 0; i < p->dft_nfreq; i++) {
+        const double arg = p->dft_w[i] * (double)p->dft_k0;
+        f.rot[i][0] = std::cos(arg);
+        f.rot[i][1] = std::sin(arg);
+    }
     f.series = p->dft_series;
     f.acc = p->dft_acc;
     SMX_HIP(smx_dft_fold(&f, p->g.precision == SMX_F64, p->stream));
@@ -329,7 +335,7 @@ int launch_step(smx_plan *p, const smx_step_scalars &s, bool explicit_src)
         SMX_HIP(smx_launch_f64_step(&a, p->stream));
         return SMX_OK;
     }
-    smx_f32_launch_args a;
+    smx_f32_launch_args a{};
     a.g = make_geom(p);
     a.s = s;
     a.srcH_plane = explicit_src ? p->src : nullptr;
@@ -517,9 +523,9 @@ int smx_plan_set_source(smx_plan *p, const double *Hinc, const double *Einc, int
     return SMX_OK;
 }
 
-static int launch_step2(smx_plan *p, const smx_step_scalars &s, const smx_step_scalars &s2)
+static int launch_step2(smx_plan *p, const smx_step_scalars &s, const smx_step_scalars &s2, int dft_slot)
 {
-    smx_f32_launch_args a;
+    smx_f32_launch_args a{};
     a.g = make_geom(p);
     a.s = s;
     a.s2 = s2;
@@ -531,20 +537,48 @@ static int launch_step2(smx_plan *p, const smx_step_scalars &s, const smx_step_s
     a.EB = p->mon + 4 * p->plane;
     a.HT = p->mon + 8 * p->plane;
     a.ET = p->mon + 12 * p->plane;
+    if (dft_slot >= 0) {
+        a.dft = (float *)p->dft_series;
+        a.dft_n = p->dft_nplanes;
+        a.dft_slot = dft_slot;
+        std::memcpy(a.dft_planes, p->dft_planes, sizeof(int) * (size_t)p->dft_nplanes);
+    }
     SMX_HIP(smx_f32_step2(&p->f32, &a, p->stream));
     return SMX_OK;
 }
 
+/* Whether steps k and k+1 may run as one two-step pass: an FP32 plan with the two-step
+ * kernel selected, and with the DFT on only where that kernel samples every cell and
+ * the ring holds two slots. */
+static bool pair_ok(const smx_plan *p)
+{
+    if (p->g.precision != SMX_F32 || p->f32.two < 0) return false;
+    return p->dft_nplanes == 0 || (smx_f32_two_full(&p->f32, p->f32.two) && p->dft_chunk >= 2);
+}
+
+/* Ring slot of the pair (k, k+1) of DFT samples, flushing the ring first where the
+ * pair does not continue its chunk or does not fit. */
+static int dft_pair_slot(smx_plan *p, long long k, int *slot)
+{
+    int rc;
+    if (p->dft_fill > 0 && (k != p->dft_k0 + p->dft_fill || p->dft_fill + 2 > p->dft_chunk) &&
+        (rc = dft_flush(p)))
+        return rc;
+    if (p->dft_fill == 0) p->dft_k0 = k;
+    *slot = p->dft_fill;
+    return SMX_OK;
+}
+
 /* Step k of the source loop, or steps k and k+1 in one pass when the plan's two-step
- * kernel is in use, the DFT is off and both steps are to run; *done = steps taken.
- * sums get the envelope values in the same order as step-by-step. */
+ * kernel is in use (pair_ok) and both steps are to run; *done = steps taken.  sums get
+ * the envelope values in the same order as step-by-step. */
 static int advance(smx_plan *p, long long k, long long nleft, double *sums, int *done)
 {
     smx_step_scalars s;
     double fH, fE;
     int rc = step_scalars(p, (int)(k / p->N), (int)(k % p->N), &s, &fH, &fE);
     if (rc) return rc;
-    if (p->g.precision == SMX_F32 && p->f32.two >= 0 && p->dft_nplanes == 0 && nleft >= 2) {
+    if (pair_ok(p) && nleft >= 2) {
         smx_step_scalars s2;
         double fH2, fE2;
         if (step_scalars(p, (int)((k + 1) / p->N), (int)((k + 1) % p->N), &s2, &fH2, &fE2) == SMX_OK) {
@@ -555,7 +589,14 @@ static int advance(smx_plan *p, long long k, long long nleft, double *sums, int 
                 sums[1] += fE2;
             }
             *done = 2;
-            return launch_step2(p, s, s2);
+            int slot = -1;
+            if (p->dft_nplanes && (rc = dft_pair_slot(p, k, &slot))) return rc;
+            if ((rc = launch_step2(p, s, s2, slot))) return rc;
+            if (p->dft_nplanes) {
+                p->dft_fill += 2;
+                if (p->dft_fill == p->dft_chunk) return dft_flush(p);
+            }
+            return SMX_OK;
         }
     }
     if (sums) {
@@ -946,8 +987,10 @@ int smx_plan_set_dft(smx_plan *p, int nplanes, const int *planes, int nfreq, con
     SMX_HIP(hipStreamSynchronize(p->stream));
     if (p->dft_series) SMX_HIP(hipFree(p->dft_series));
     if (p->dft_acc) SMX_HIP(hipFree(p->dft_acc));
+    if (p->dft_table) SMX_HIP(hipFree(p->dft_table));
     p->dft_series = nullptr;
     p->dft_acc = nullptr;
+    p->dft_table = nullptr;
     p->dft_nplanes = 0;
     p->dft_fill = 0;
     if (nplanes == 0) return SMX_OK;
@@ -963,6 +1006,11 @@ int smx_plan_set_dft(smx_plan *p, int nplanes, const int *planes, int nfreq, con
     std::memset(p->dft_w, 0, sizeof p->dft_w);
     std::memcpy(p->dft_planes, planes, sizeof(int) * nplanes);
     std::memcpy(p->dft_w, w_per_step, sizeof(double) * nfreq);
+    std::vector<double> tab(4 * (size_t)chunk * (size_t)nfreq);
+    smx_dft_table(nfreq, w_per_step, chunk, tab.data());
+    SMX_HIP(hipMalloc(&p->dft_table, sizeof(double) * tab.size()));
+    SMX_HIP(hipMemcpyAsync(p->dft_table, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice,
+                           p->stream));
     SMX_HIP(hipStreamSynchronize(p->stream));
     return SMX_OK;
 }
@@ -1239,14 +1287,16 @@ int smx_plan_autotune(smx_plan *p, int trial_steps, int choice[5])
             for (int ci = 0; ci < 4 * 3; ci++) {
                 const int fz = chunks[ci % 4], bal = ci / 4;
                 if (!smx_f32_two_ok(&st, v, fz) || (bal > 0 && !zpml)) continue;
+                /* with the DFT on, only a variant that samples both steps itself pairs steps */
+                if (p->dft_nplanes && (!smx_f32_two_full(&st, v) || p->dft_chunk < 2)) continue;
                 st.two = v;
                 st.two_fz = fz;
                 st.two_bal = bal;
                 int rc;
-                if ((rc = launch_step2(p, s, s))) return rc;
+                if ((rc = launch_step2(p, s, s, -1))) return rc;
                 SMX_HIP(hipEventRecord(e0, p->stream));
                 for (int k = 0; k < pairs; k++)
-                    if ((rc = launch_step2(p, s, s))) return rc;
+                    if ((rc = launch_step2(p, s, s, -1))) return rc;
                 SMX_HIP(hipEventRecord(e1, p->stream));
                 SMX_HIP(hipEventSynchronize(e1));
                 float ms = 0.f;

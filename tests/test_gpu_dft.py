@@ -95,3 +95,34 @@ def test_dft_argument_checks():
         p._dft = (1, 1)
         with pytest.raises(fdtd.SmxError):
             p.read_dft()
+
+
+LOSSLESS = dict(pml_xy=0, pml_z=3, iBc=(1, -1, 0), metal_cells=0, lossy_cells=0)
+
+
+@pytest.mark.parametrize("two,chunk", [(None, 16), ("2,16", 16), ("2,16", 7), ("2,8", 1), ("0,16", 16),
+                                       ("3,16", 16)])
+def test_dft_with_two_step_passes(two, chunk, monkeypatch):
+    """With the two-step kernel selected the DFT samples of both steps of a pass come
+    from the kernel itself (split-slot variant) or the steps run one at a time (the
+    variants with region sweeps, or a ring of one slot); an odd step count ends on a
+    single step.  Against a direct DFT of the oracle's per-step samples."""
+    if two:
+        monkeypatch.setenv("SMX_F32_TWO", two)
+    prep = casegen.prepare(small_waveguide(seed=8, **LOSSLESS))
+    c = prep.case
+    nsteps = 3 * prep.N + 5
+    ws = np.array([0.1, 2 * np.pi / prep.N, 0.9])
+    S = _oracle_samples(prep, nsteps)
+    k = np.arange(nsteps)
+    ref = np.zeros((2, 4, len(ws), c.plane), dtype=np.complex128)
+    for fi, w in enumerate(ws):
+        for v in range(4):
+            ph = np.exp(1j * w * (k + 1 + (0.5 if v >= 2 else 0.0)))
+            ref[:, v, fi, :] = np.einsum("k,kmq->mq", ph, S[:, :, v, :])
+    with prep.plan(fdtd.F32) as p:
+        p.set_dft([c.iB, c.iT], ws, chunk=chunk)
+        p.run_steps(0, 7)
+        p.run_steps(7, nsteps - 7)
+        D = p.read_dft()
+    assert rel_l2(D, ref) < 2e-4
