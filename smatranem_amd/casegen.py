@@ -275,6 +275,30 @@ def batch_cases(count, seed=0, pml=8, time_period=(5, 100000, 100010)):
     return out
 
 
+def block_in_vacuum_case():
+    """The reference-runnable workload: a 48 x 48 x 64 uniform grid (unit spacing), eps = 1
+    with a 16 x 16 x 24 block of eps = 12.1 at [16, 32) x [16, 32) x [20, 44), 8-cell z PML
+    and PEC x/y walls (the layout the reference's monitors accept), omega = 2*pi*0.1 with
+    the time step at 1/0.99 of the CFL factor (N_2pi = 18), envelope (10, 80, 111) periods
+    = 1998 steps, a point-like sheet source on plane 10 and the monitors on planes 9 and
+    54."""
+    return waveguide_case(48, 48, 64, outer=12, pml_xy=0, pml_z=8, omega=2 * np.pi * 0.1, eps_core=1.0,
+                          eps_clad=1.0, core=(1, 1), seed=1, iBc=(0, 0, 0), dt_factor=1 / 0.99,
+                          time_period=(10, 80, 111), monitor_gap=1, design=((16, 32, 16, 32, 20, 44), 12.1, 12.1))
+
+
+def curvilinear_waveguide_case(L=(160, 160, 320), pml=10, pml_xy=0, time_period=(5, 100000, 100010)):
+    """The headline workload: a curvilinear L0 x L1 x L2 grid (scale factors
+    h_j = 1 + 0.3 g_j, g_j a seeded sum of 8 low-order Fourier modes), a Si-like core
+    eps = 12.1 in eps = 2.1 cladding, PEC on ~5% of the scatterer's E edges as seeded slabs,
+    a ``pml``-cell z PML and PEC x/y walls (``pml_xy`` > 0 puts PML on the x/y faces too,
+    with the reference monitors off), a modal sheet source."""
+    L0, L1, L2 = L
+    return waveguide_case(L0, L1, L2, outer=L2 // 8, pml_xy=pml_xy, pml_z=pml, omega=2 * np.pi / 20.0,
+                          eps_core=12.1, eps_clad=2.1, core=(L0 // 8, L1 // 10), scale_amp=0.3, n_modes=8,
+                          pec_fraction=0.05, seed=2024, iBc=(0, 0, 0), time_period=time_period)
+
+
 def adjoint_case(L=(192, 192, 256), pml=10, design_frac=0.5, seed=2026, eps_range=(2.1, 12.1),
                  time_period=(5, 100000, 100010)):
     """The gradient workload: a design region of per-cell eps (seeded, uniform in

@@ -1,0 +1,137 @@
+"""The benchmark configurations at their own sizes.
+
+C1: 48 x 48 x 64 uniform grid, eps = 1 with a 16 x 16 x 24 block of eps = 12.1, 8-cell z
+PML, omega = 2*pi*0.1, Dt factor 1/0.99 (N_2pi = 18), envelope (10, 80, 111): 1998 steps,
+the reference monitors on planes 9 and 54 and a 4-frequency DFT on plane 54.  FP64 on
+the GPU is bit-identical to the oracle; FP32 |S| within 1e-3 of FP64.
+C2: 160 x 160 x 320 curvilinear grid with PEC edges and 10-cell z PML (the bench
+workload), FP32 against FP64 on the GPU over 3000 steps: |S| of the reference monitors
+and of the 16-frequency DFT within 1e-3.
+C3: three runs drawn from the batch generator (sizes in [64, 192]^2 x [64, 256], 8-cell
+PML on all faces), advanced together with graph replay over CW periods, equal to each
+run alone.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from smatranem_amd import casegen, fdtd
+from tests.cases import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+S_TOL = 1e-3  # |S| error of FP32 against FP64 (absolute, |S| of the source mode ~ 1)
+
+
+def overlap(case, H, E):
+    """Projection of monitor phasors on the source profile (S_calc's overlap,
+    FDTDcalc/FDTDsplitfieldMetal.c:437-445, with the incident profile as the mode)."""
+    w = np.concatenate([case.Hinc, case.Einc])
+    v = np.concatenate([H, E])
+    return complex(np.vdot(w, v) / np.vdot(w, w).real)
+
+
+def dft_phasors(D, m, f):
+    """(H, E) complex[2*L0*L1], (x, y) interleaved, of DFT plane m and frequency f."""
+    plane = D.shape[-1]
+    H = np.empty(2 * plane, dtype=np.complex128)
+    E = np.empty_like(H)
+    H[0::2], H[1::2], E[0::2], E[1::2] = D[m, 0, f], D[m, 1, f], D[m, 2, f], D[m, 3, f]
+    return H, E
+
+
+def test_c1_fp64_bitwise_and_fp32_s_error():
+    case = casegen.block_in_vacuum_case()
+    prep = casegen.prepare(case)
+    assert prep.N == 18 and prep.N * case.time_period[2] == 1998
+    assert (case.iB, case.iT, case.iI) == (9, 54, 10)
+    ws = 2 * np.pi / prep.N * np.array([0.9, 1.0, 1.1, 1.2])
+    out = {}
+    for prec in (fdtd.F64, fdtd.F32):
+        with prep.plan(prec) as p:
+            p.set_dft([case.iT], ws, chunk=32)
+            sums = p.run(0, case.time_period[2])
+            H, E = p.read_fields()
+            out[prec] = (H, E, p.read_monitors(), p.read_dft(), sums)
+    N, iR, iI, _ = oracle.set_tau(case.omega, case.min_dt / case.dt_factor)
+    st = oracle.setup_from_media(case.L, case.mu, case.eps, iR=iR, iI=iI)
+    Ho, Eo, mo, so = oracle.run(case.L, st, case.iBc, (case.iB, case.iT, case.iI, case.inc_mode), case.Hinc,
+                                case.Einc, N, case.time_period, 0, case.time_period[2])
+    H, E, mons, D64, sums = out[fdtd.F64]
+    assert np.abs(Ho).max() > 0
+    assert np.array_equal(H, Ho) and np.array_equal(E, Eo)
+    for a, b in zip(mons, mo):
+        assert np.array_equal(a.view(np.float64), b)
+    assert np.array_equal(sums, so)
+    # the DFT at the source frequency is the reference's top monitor
+    HT, ET = mons[2], mons[3]
+    ref = np.stack([HT[0::2], HT[1::2], ET[0::2], ET[1::2]])
+    assert rel_l2(D64[0, :, 1, :], ref) < 1e-12
+    # FP32 against FP64: |S| of both monitors and of every DFT frequency
+    H32, E32, m32, D32, _ = out[fdtd.F32]
+    # fields: FP32 rounding accumulated over 1998 steps (2.5e-4 measured); the bound that
+    # matters is the one on |S| below
+    assert rel_l2(H32, H) < 1e-3 and rel_l2(E32, E) < 1e-3
+    errs = [abs(abs(overlap(case, m32[2 * k], m32[2 * k + 1])) - abs(overlap(case, mons[2 * k], mons[2 * k + 1])))
+            for k in range(2)]
+    errs += [abs(abs(overlap(case, *dft_phasors(D32, 0, f))) - abs(overlap(case, *dft_phasors(D64, 0, f))))
+             for f in range(len(ws))]
+    assert max(errs) <= S_TOL, errs
+
+
+def test_c2_fp32_vs_fp64_s_error():
+    case = casegen.curvilinear_waveguide_case()
+    prep = casegen.prepare(case)
+    ws = 2 * np.pi / prep.N * np.linspace(0.85, 1.15, 16)
+    steps = 3000
+    out = {}
+    for prec in (fdtd.F32, fdtd.F64):
+        with prep.plan(prec) as p:
+            p.set_dft([case.iB, case.iT], ws, chunk=32)
+            if prec == fdtd.F32:
+                p.autotune(10)
+            p.run_steps(0, steps)
+            out[prec] = (p.read_monitors(), p.read_dft())
+    m32, D32 = out[fdtd.F32]
+    m64, D64 = out[fdtd.F64]
+    errs, mags = [], []
+    for k in range(2):
+        s32, s64 = overlap(case, m32[2 * k], m32[2 * k + 1]), overlap(case, m64[2 * k], m64[2 * k + 1])
+        errs.append(abs(abs(s32) - abs(s64)))
+        mags.append(abs(s64))
+        for f in range(len(ws)):
+            s32 = overlap(case, *dft_phasors(D32, k, f))
+            s64 = overlap(case, *dft_phasors(D64, k, f))
+            errs.append(abs(abs(s32) - abs(s64)))
+            mags.append(abs(s64))
+    assert max(mags) > 0
+    assert max(errs) <= S_TOL, max(errs)
+    # and relative to the largest |S| seen (the run is still in its first periods)
+    assert max(errs) <= 1e-4 * max(mags), (max(errs), max(mags))
+
+
+def test_c3_batch_members_graph_equals_alone(monkeypatch):
+    """The two-step pass with split slots, chosen explicitly (the autotuner's timing-based
+    choice could differ between the members alone and batched)."""
+    monkeypatch.setenv("SMX_F32_TWO", "2,32,1")
+    cases = casegen.batch_cases(3, seed=2025, time_period=(1, 12, 14))
+    preps = [casegen.prepare(c) for c in cases]
+    steps = min(pr.N * pr.case.time_period[2] for pr in preps)
+    alone = []
+    for pr in preps:
+        with pr.plan(fdtd.F32) as p:
+            assert p.kernels == "two_step_f32<32,8,1,true,false>"
+            p.run_steps(0, steps)
+            alone.append(p.read_fields())
+    plans = [pr.plan(fdtd.F32) for pr in preps]
+    try:
+        info = {}
+        fdtd.run_batch(plans, 0, steps, info=info)
+        together = [p.read_fields() for p in plans]
+    finally:
+        for p in plans:
+            p.close()
+    assert info["graph_launches"] >= 1
+    for a, b in zip(alone, together):
+        assert np.abs(a[0]).max() > 0
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
