@@ -128,7 +128,7 @@ def sheet_source(L0, L1, profile, n_eff=1.0):
 def waveguide_case(L0, L1, L2, outer, pml_xy, pml_z, *, omega=2 * np.pi / 20.0, eps_core=12.1,
                    eps_clad=2.1, core=(8, 6), scale_amp=0.0, n_modes=8, pec_fraction=0.0,
                    seed=0, iBc=(0, 0, 0), dt_factor=1.0, time_period=(2, 1, 8), monitor_gap=2,
-                   metal_cells=0, lossy_cells=0, blocks=0, design=None):
+                   metal_cells=0, lossy_cells=0, blocks=0, design=None, layers=0, grating=False):
     """A straight dielectric core along z in cladding, on an L0 x L1 x L2 grid with
     ``outer`` port slices at each end (B0.., T..), PML layers of ``pml_xy`` cells on the
     x/y faces and ``pml_z`` on the z faces (PMLcalc.c:188-260), optional curvilinear scale
@@ -139,7 +139,10 @@ def waveguide_case(L0, L1, L2, outer, pml_xy, pml_z, *, omega=2 * np.pi / 20.0, 
     and lossy cells, and ``blocks`` seeded dielectric boxes (eps uniform in [1, 12.1]) in
     the scatterer, and an optional design region ``design`` = ((x0, x1, y0, y1, z0, z1),
     lo, hi) whose cells each get one seeded eps, uniform in [lo, hi], for all three
-    components.  Monitor planes sit between the z PML and the scatterer, the source
+    components; ``layers`` > 0 fills the scatterer slices with that many z-layers of
+    seeded eps in [1, 12.1] (replacing core and cladding there), and ``grating`` puts a PEC
+    grating (x-period 16, 8 cells wide, clear of the x/y PML) on the middle scatterer
+    plane.  Monitor planes sit between the z PML and the scatterer, the source
     plane just above the bottom monitor."""
     rng = np.random.default_rng(seed)
     n = L0 * L1 * L2
@@ -167,6 +170,11 @@ def waveguide_case(L0, L1, L2, outer, pml_xy, pml_z, *, omega=2 * np.pi / 20.0, 
         eps3[bz:min(bz + dz, L2 - outer), by:min(by + dy, L1 - pml_xy), bx:min(bx + dx, L0 - pml_xy), :] = \
             rng.uniform(1.0, 12.1)
 
+    if layers:
+        bounds = np.linspace(outer, L2 - outer, layers + 1).round().astype(int)
+        for i in range(layers):
+            eps3[bounds[i]:bounds[i + 1], :, :, :] = rng.uniform(1.0, 12.1)
+
     if design is not None:
         (x0, x1, y0, y1, z0, z1), lo, hi = design
         cell_eps = rng.uniform(lo, hi, size=(z1 - z0, y1 - y0, x1 - x0))
@@ -179,11 +187,15 @@ def waveguide_case(L0, L1, L2, outer, pml_xy, pml_z, *, omega=2 * np.pi / 20.0, 
                   for _ in range(n_modes)] for _ in range(3)]
 
         def hfield(d, pos):
-            X, Y, Z = x + pos[0], y + pos[1], z + pos[2]
-            g = np.zeros((L2, L1, L0))
-            for (k, ph, a) in modes[d]:
-                g = g + a * np.cos(2 * np.pi * (k[0] * X / L0 + k[1] * Y / L1 + k[2] * Z / L2) + ph)
-            return g
+            # sum_m a_m cos(2 pi (k0 X/L0 + k1 Y/L1 + k2 Z/L2) + ph_m): each mode is a product
+            # of 1-D exponentials, so the sum over modes is one complex matrix product
+            kk = np.array([k for k, _, _ in modes[d]], dtype=np.float64)          # (M, 3)
+            amp = np.array([a * np.exp(1j * ph) for _, ph, a in modes[d]])       # (M,)
+            ex = np.exp(2j * np.pi * np.outer(kk[:, 0], (np.arange(L0) + pos[0]) / L0))  # (M, L0)
+            ey = np.exp(2j * np.pi * np.outer(kk[:, 1], (np.arange(L1) + pos[1]) / L1))
+            ez = np.exp(2j * np.pi * np.outer(kk[:, 2], (np.arange(L2) + pos[2]) / L2))
+            A = (amp[:, None, None] * ez[:, :, None] * ey[:, None, :]).reshape(len(amp), L2 * L1)
+            return np.real(A.T @ ex).reshape(L2, L1, L0)
         gmax = [max(np.abs(hfield(d, (0, 0, 0))).max(), 1e-12) for d in range(3)]
 
         def fx_at(pos, c):
@@ -199,7 +211,8 @@ def waveguide_case(L0, L1, L2, outer, pml_xy, pml_z, *, omega=2 * np.pi / 20.0, 
         target = pec_fraction * (L2 - 2 * outer) * L1 * L0
         zlo, zhi = outer + 2, L2 - outer - 2
         tries = 0
-        while pec_mask.sum() < target and tries < 10000:
+        count = 0
+        while count < target and tries < 10000:
             tries += 1
             sz = rng.integers(1, 4)
             sy = rng.integers(2, max(3, L1 // 6))
@@ -207,7 +220,16 @@ def waveguide_case(L0, L1, L2, outer, pml_xy, pml_z, *, omega=2 * np.pi / 20.0, 
             z0 = rng.integers(zlo, max(zlo + 1, zhi - sz))
             y0 = rng.integers(pml_xy, max(pml_xy + 1, L1 - pml_xy - sy))
             x0 = rng.integers(pml_xy, max(pml_xy + 1, L0 - pml_xy - sx))
-            pec_mask[z0:z0 + sz, y0:y0 + sy, x0:x0 + sx] = True
+            slab = pec_mask[z0:z0 + sz, y0:y0 + sy, x0:x0 + sx]
+            count += slab.size - int(np.count_nonzero(slab))
+            slab[...] = True
+        eps3[pec_mask, :] = PEC_EPS
+
+    if grating:
+        zg = L2 // 2
+        stripes = (np.arange(L0) % 16) < 8
+        stripes[:pml_xy] = stripes[L0 - pml_xy:] = False
+        pec_mask[zg, pml_xy:L1 - pml_xy, :] |= stripes[None, :]
         eps3[pec_mask, :] = PEC_EPS
 
     # optional metal / lossy cells (scatterer interior, away from PML and planes)
@@ -297,6 +319,17 @@ def curvilinear_waveguide_case(L=(160, 160, 320), pml=10, pml_xy=0, time_period=
     return waveguide_case(L0, L1, L2, outer=L2 // 8, pml_xy=pml_xy, pml_z=pml, omega=2 * np.pi / 20.0,
                           eps_core=12.1, eps_clad=2.1, core=(L0 // 8, L1 // 10), scale_amp=0.3, n_modes=8,
                           pec_fraction=0.05, seed=2024, iBc=(0, 0, 0), time_period=time_period)
+
+
+def layered_stack_case(L=(768, 768, 256), pml=12, layers=12, seed=2027, time_period=(5, 100000, 100010)):
+    """The large-grid workload: the headline's curvilinear scale factors on an
+    L0 x L1 x L2 grid, a seeded stack of ``layers`` dielectric z-layers (eps in [1, 12.1])
+    with a PEC grating in its middle plane, ``pml``-cell PML on every face (the reference
+    monitors off), a modal sheet source near the bottom."""
+    L0, L1, L2 = L
+    return waveguide_case(L0, L1, L2, outer=max(L2 // 8, pml + 4), pml_xy=pml, pml_z=pml, omega=2 * np.pi / 20.0,
+                          eps_core=12.1, eps_clad=2.1, core=(L0 // 8, L1 // 10), scale_amp=0.3, n_modes=8,
+                          seed=seed, iBc=(0, 0, 0), time_period=time_period, layers=layers, grating=True)
 
 
 def adjoint_case(L=(192, 192, 256), pml=10, design_frac=0.5, seed=2026, eps_range=(2.1, 12.1),
