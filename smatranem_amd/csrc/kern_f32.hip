@@ -1438,7 +1438,7 @@ __global__ __launch_bounds__(FX * FY) void two_step_f32(fused_args a)
      * the first step as a buffer resource, the second step's slot dstep bytes further */
     __amdgpu_buffer_rsrc_t rD = __builtin_amdgcn_make_buffer_rsrc(nullptr, (short)0, 0, 0);
     unsigned dstep = 0;
-    auto dft_m = [&](int i) { /* ring plane of stage i's plane, -1 if none or not the block's */
+    auto dft_plane_of = [&](int i) { /* ring plane of stage i's plane, -1 if none or not the block's */
         const int z = z0 - 2 + i;
         int m = -1;
         for (int k = 0; k < a.dft_n && z >= z0 && z < z1; k++)
@@ -1447,7 +1447,7 @@ __global__ __launch_bounds__(FX * FY) void two_step_f32(fused_args a)
     };
     if constexpr (DFT) {
         if constexpr (!SPLIT)
-            for (int i = t; i < ns; i += NT) reinterpret_cast<int *>(lds + L_DZ)[i] = dft_m(i);
+            for (int i = t; i < ns; i += NT) reinterpret_cast<int *>(lds + L_DZ)[i] = dft_plane_of(i);
         const int64_t slot_words = 4 * (int64_t)a.dft_n * a.sz;
         rD = __builtin_amdgcn_make_buffer_rsrc((void *)(a.dft + a.dft_slot * slot_words), (short)0,
                                                (int)(unsigned)(8 * slot_words), (int)kBufWord3);
@@ -1456,9 +1456,14 @@ __global__ __launch_bounds__(FX * FY) void two_step_f32(fused_args a)
     /* sample (Hx, Hy, Ex, Ey) of cell q at stage s into the ring (second step: soff = dstep):
      * the total field, which is the main slot on the lossless grids this kernel takes
      * outside PML, and the sample taken in PML cells */
-    auto dft_sample = [&](int s_, int64_t q, unsigned soff, float hx, float hy, float ex, float ey) {
+    /* ring plane of stage s_ (-1: none), a wave-uniform value */
+    auto dft_m = [&](int s_) {
+        if constexpr (!DFT) return -1;
+        const int m = SPLIT ? ((int)lds[L_ZT + 5 * s_ + 4] >> 1) - 1 : reinterpret_cast<const int *>(lds + L_DZ)[s_];
+        return __builtin_amdgcn_readfirstlane(m);
+    };
+    auto dft_sample = [&](int m, int64_t q, unsigned soff, float hx, float hy, float ex, float ey) {
         if constexpr (DFT) {
-            const int m = SPLIT ? ((int)lds[L_ZT + 5 * s_ + 4] >> 1) - 1 : reinterpret_cast<const int *>(lds + L_DZ)[s_];
             if (m < 0) return;
             const unsigned o = 4u * (unsigned)(4 * m * a.sz + q), pl = 4u * (unsigned)a.sz;
             __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(hx), rD, o, soff, 0);
@@ -1547,7 +1552,7 @@ __global__ __launch_bounds__(FX * FY) void two_step_f32(fused_args a)
             lds[L_ZT + 5 * i + 2] = tz[2 * a.L2 + z];
             lds[L_ZT + 5 * i + 3] = tz[3 * a.L2 + z];
             /* bit 0: z PML; bits 1..: 1 + the DFT ring plane (DFT variants) */
-            lds[L_ZT + 5 * i + 4] = (float)((a.pmlflag[a.L0 + a.L1 + z] ? 1 : 0) + (DFT ? 2 * (dft_m(i) + 1) : 0));
+            lds[L_ZT + 5 * i + 4] = (float)((a.pmlflag[a.L0 + a.L1 + z] ? 1 : 0) + (DFT ? 2 * (dft_plane_of(i) + 1) : 0));
         }
         /* does any cell of the G4 region lie in x/y PML? (vote through the LDS array) */
         if (t == 0) lds[L_FLAG] = 0.f;
@@ -1629,6 +1634,7 @@ __global__ __launch_bounds__(FX * FY) void two_step_f32(fused_args a)
         }
         const float *S = lds + (s % NSLOT) * Ly::SLOT;
         const bool zs = zpml(s), zs1 = s >= 1 && zpml(s - 1), zs2 = s >= 2 && zpml(s - 2);
+        const int dm1 = s >= 1 ? dft_m(s - 1) : -1, dm2 = s >= 2 ? dft_m(s - 2) : -1; /* DFT ring planes */
 
         float h1n[CELLS][3], sh1n[CELLS][3], e1n[CELLS][3], se1n[CELLS][3], h2n[CELLS][3];
 #pragma unroll
@@ -1714,7 +1720,7 @@ __global__ __launch_bounds__(FX * FY) void two_step_f32(fused_args a)
                     }
                 }
             }
-            if (DFT && inT[r]) dft_sample(s - 1, cq[r], 0u, hc[0], hc[1], e1n[r][0], e1n[r][1]);
+            if (DFT && inT[r]) dft_sample(dm1, cq[r], 0u, hc[0], hc[1], e1n[r][0], e1n[r][1]);
             if (inG2[r]) {
                 lds[L_E1 + 0 * N4 + e] = e1n[r][0];
                 lds[L_E1 + 1 * N4 + e] = e1n[r][1];
@@ -1786,7 +1792,7 @@ __global__ __launch_bounds__(FX * FY) void two_step_f32(fused_args a)
             e_regular(e1pp[r], bepp[r], se1pp[r], P, M, pml, pa, pg, en, sn);
             bstore3(rTn, cb, 3, cq[r] + a.sz * zp2, en);
             if (SPLIT && pml) bstore3(rSn, cb, 3, cq[r] + a.sz * zp2, sn);
-            if (DFT) dft_sample(s - 2, cq[r], dstep, hc[0], hc[1], en[0], en[1]);
+            if (DFT) dft_sample(dm2, cq[r], dstep, hc[0], hc[1], en[0], en[1]);
             if (!pml && (zp2 == a.iB || zp2 == a.iT)) { /* second step's monitors */
                 for (int m = 0; m < 2; m++) {
                     if (zp2 != (m == 0 ? a.iB : a.iT)) continue;
@@ -2183,12 +2189,14 @@ struct two_variant {
     int fx, fy, d;
     bool split, wide;
     int ox; /* columns of the stage region left of a tile */
-    const char *name, *with_regions;
+    const char *name, *with_regions, *name_dft; /* as profilers list them; the last with the DFT */
 };
 #define SMX_TWO(FX, FY, D, SP, WI)                                                                         \
-    {FX, FY, D, SP, WI, two_layout<FX, FY, SP, WI>::OX, "two_step_f32<" #FX "," #FY "," #D "," #SP "," #WI ">", \
-     "region_h_f32 + region_e_f32 + two_step_f32<" #FX "," #FY "," #D "," #SP "," #WI                      \
-     "> + region_h_f32 + region_e_f32"}
+    {FX, FY, D, SP, WI, two_layout<FX, FY, SP, WI>::OX,                                                   \
+     "two_step_f32<" #FX "," #FY "," #D "," #SP "," #WI ",false>",                                       \
+     "region_h_f32 + region_e_f32 + two_step_f32<" #FX "," #FY "," #D "," #SP "," #WI                     \
+     ",false> + region_h_f32 + region_e_f32",                                                             \
+     "two_step_f32<" #FX "," #FY "," #D "," #SP "," #WI ",true>"}
 constexpr two_variant kTwo[] = {SMX_TWO(32, 8, 1, false, false), SMX_TWO(32, 8, 2, false, false),
                                 SMX_TWO(32, 8, 1, true, false), SMX_TWO(32, 8, 1, false, true)};
 #undef SMX_TWO
@@ -2234,7 +2242,7 @@ int smx_f32_two_ok(const smx_f32_state *st, int variant, int fz)
     return !st->has_pml || st->split2_ok;
 }
 
-const char *smx_f32_kernel_name(const smx_f32_state *st)
+const char *smx_f32_kernel_name(const smx_f32_state *st, int dft)
 {
     static const char *const fused[kNumTiles] = {
         "fused_step_f32<32,8>", "fused_step_f32<64,4>", "fused_step_f32<16,16>",
@@ -2248,8 +2256,10 @@ const char *smx_f32_kernel_name(const smx_f32_state *st)
         "region_h_f32 + fused_dma_f32<32,8,4,false> + region_e_f32",
         "region_h_f32 + fused_dma_f32<32,8,2,false> + region_e_f32",
         "region_h_f32 + fused_dma_f32<32,8,1,false> + region_e_f32"};
-    if (st->two >= 0 && st->two < kNumTwo)
-        return st->has_pml && !kTwo[st->two].split ? kTwo[st->two].with_regions : kTwo[st->two].name;
+    if (st->two >= 0 && st->two < kNumTwo) {
+        if (st->has_pml && !kTwo[st->two].split) return kTwo[st->two].with_regions;
+        return dft ? kTwo[st->two].name_dft : kTwo[st->two].name;
+    }
     if (!st->fused) return st->bx == 64 ? "h_sweep_f32<64> + e_sweep_f32<64>" : "h_sweep_f32<32> + e_sweep_f32<32>";
     if (st->tile < 0 || st->tile >= kNumTiles) return "";
     return st->tile >= kFirstNoSplitTile && st->has_pml ? split[st->tile] : fused[st->tile];

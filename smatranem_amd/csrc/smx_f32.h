@@ -127,7 +127,8 @@ const char *smx_f32_last_error(void);
 /* Number of fused tile variants, and whether (tile, z chunk) can run on this grid. */
 int smx_f32_num_tiles(void);
 void smx_f32_tile_shape(int tile, int *fx, int *fy);
-/* Kernels of one step in the current choice, " + "-separated (the lossy/metal passes
- * of the fused variants not included). */
-const char *smx_f32_kernel_name(const smx_f32_state *st);
+/* Kernels of one step (one pass of the two-step kernel) in the current choice,
+ * " + "-separated (the lossy/metal passes of the fused variants not included); dft != 0:
+ * the two-step kernel that also samples the DFT planes. */
+const char *smx_f32_kernel_name(const smx_f32_state *st, int dft);
 int smx_f32_tile_ok(const smx_f32_state *st, int tile, int fz);

@@ -971,7 +971,7 @@ const char *smx_plan_kernels(const smx_plan *p)
 {
     if (!p) return "";
     if (p->g.precision == SMX_F64) return "h_sweep_f64 + e_sweep_f64";
-    return smx_f32_kernel_name(&p->f32);
+    return smx_f32_kernel_name(&p->f32, p->dft_nplanes > 0 && pair_ok(p));
 }
 
 int smx_plan_set_dft(smx_plan *p, int nplanes, const int *planes, int nfreq, const double *w_per_step,

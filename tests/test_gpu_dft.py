@@ -122,6 +122,9 @@ def test_dft_with_two_step_passes(two, chunk, monkeypatch):
             ref[:, v, fi, :] = np.einsum("k,kmq->mq", ph, S[:, :, v, :])
     with prep.plan(fdtd.F32) as p:
         p.set_dft([c.iB, c.iT], ws, chunk=chunk)
+        # the split-slot variant with a ring of two or more slots samples in the kernel
+        fused = two is not None and two.startswith("2,") and chunk >= 2
+        assert p.kernels.endswith(",true>") == fused, p.kernels
         p.run_steps(0, 7)
         p.run_steps(7, nsteps - 7)
         D = p.read_dft()

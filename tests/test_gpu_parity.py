@@ -241,7 +241,7 @@ def test_region_sweeps_carry_monitor_plane(tile, monkeypatch):
 
 # two-step variants: (copies in flight, split slots, 16-byte copies); tiles are 32 x 8
 TWO_VARIANTS = ((1, False, False), (2, False, False), (1, True, False), (1, False, True))
-TWO_NAMES = tuple(f"two_step_f32<32,8,{d},{str(sp).lower()},{str(wi).lower()}>" for d, sp, wi in TWO_VARIANTS)
+TWO_NAMES = tuple(f"two_step_f32<32,8,{d},{str(sp).lower()},{str(wi).lower()},false>" for d, sp, wi in TWO_VARIANTS)
 TWO_SPLIT = tuple(v[1] for v in TWO_VARIANTS)
 
 
