@@ -146,9 +146,10 @@ int smx_plan_run_steps(smx_plan *plan, long long step0, long long nsteps, double
 /*
  * Batch: global steps [step0, step0 + nsteps) of every plan in plans[0 .. nplans-1]
  * (independent runs, each with its own grid, source and monitors), with the same
- * results as smx_plan_run_steps on each plan in turn.  The steps are issued in
- * step-major order, each plan on its own stream, so kernels of different plans run
- * concurrently and small grids share the device.  sums (may be NULL) is double[2*nplans],
+ * results as smx_plan_run_steps on each plan in turn.  Each plan runs on its own stream,
+ * so kernels of different plans run concurrently and small grids share the device; over
+ * the CW periods one captured HIP graph of every plan's steps is replayed
+ * (smx_plan_run_batch_mode, SMX_BATCH_GRAPH).  sums (may be NULL) is double[2*nplans],
  * the envelope sums of each plan.  On an error return the plans may have advanced by
  * different numbers of steps.  No counterpart in the reference (its runs are separate
  * processes, run_all.sh).
@@ -185,7 +186,8 @@ int smx_plan_sync(smx_plan *plan);
  * smx_plan_run_steps, or by successive smx_plan_step calls) adds
  * e^{i*w*(k+1)}*(Hx, Hy) and e^{i*w*(k+1.5)}*(Ex, Ey) of the main slots (the total
  * field in PML cells); with w = 2*pi/N_2pi this is the reference's monitor.  Samples
- * are buffered and folded into FP64 accumulators every `chunk` (1..32) steps.
+ * are buffered and folded into FP64 accumulators every `chunk` (1..32) steps; FP32 plans
+ * running two steps per pass take the samples of both steps inside the pass (chunk >= 2).
  * nplanes = 0 switches the DFT off.  No counterpart in the reference.
  */
 int smx_plan_set_dft(smx_plan *plan, int nplanes, const int *planes, int nfreq, const double *w_per_step,
@@ -218,17 +220,18 @@ int smx_adjoint_backward(smx_plan *plan, double *grad);
  * and restores fields and monitors.  choice (may be NULL) receives {tile x, tile y,
  * z chunk, variant, z balance}: variant 0 = in-place sweeps (sizes 0, also for FP64
  * plans), 1 = register-staged fused step, 2 = LDS-staged fused step, 3 = two steps per
- * pass (grids without lossy/metal cells; used for pairs of steps of the source loop
- * while the DFT is off, single steps take the best one-step kernel); z balance (variant
- * 3 only) 0 = z chunks of equal size, k > 0 = chunks balanced with a z-PML stage
- * counted 1 + k/2 times.  trial_steps = 0 only reports.  No counterpart in the
- * reference.
+ * pass (grids without lossy/metal cells; used for pairs of steps of the source loop,
+ * single steps take the best one-step kernel; with a DFT set, only the variants that
+ * sample the DFT planes of both steps themselves are considered); z balance (variant 3
+ * only) 0 = z chunks of equal size, k > 0 = chunks balanced with a z-PML stage counted
+ * 1 + k/2 times.  trial_steps = 0 only reports.  No counterpart in the reference.
  */
 int smx_plan_autotune(smx_plan *plan, int trial_steps, int choice[5]);
 
-/* Names of the kernels one step of the plan launches with its current kernel choice,
- * " + "-separated, as profilers list them (lossy/metal passes not included); a static
- * string.  No counterpart in the reference. */
+/* Names of the kernels one step (one pass, with two steps per pass) of the plan launches
+ * with its current kernel choice and DFT setting, " + "-separated, as profilers list
+ * them (lossy/metal passes and the DFT fold not included); a static string.  No
+ * counterpart in the reference. */
 const char *smx_plan_kernels(const smx_plan *plan);
 
 /* The plan's HIP stream (a hipStream_t), for callers that order or time their own
