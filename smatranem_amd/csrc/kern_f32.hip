@@ -2688,6 +2688,14 @@ hipError_t smx_f32_step2(smx_f32_state *st, const smx_f32_launch_args *la, hipSt
     a.iT = g.iT;
     a.iI = g.iI;
     const two_variant tv = kTwo[st->two];
+    if (la->dft_n > 0) {
+        /* the kernel samples only the cells it updates */
+        if (!smx_f32_two_full(st, st->two) || la->dft_n > kDftMaxPlanes) return hipErrorInvalidValue;
+        a.dft = la->dft;
+        a.dft_n = la->dft_n;
+        a.dft_slot = la->dft_slot;
+        for (int m = 0; m < la->dft_n; m++) a.dft_planes[m] = la->dft_planes[m];
+    }
     two_tiles_x(a, tv.ox, tv.fx, g.L0);
     a.gy = (g.L1 + tv.fy - 1) / tv.fy;
     const int maxlen = tv.split ? two_layout<32, 8, true, false>::kMaxFZ : g.L2;
@@ -2754,17 +2762,10 @@ hipError_t smx_f32_step2(smx_f32_state *st, const smx_f32_launch_args *la, hipSt
         if (st->T[1 - st->cur][i] != st->T[1 - st->cur][0] + (size_t)i * st->n ||
             st->S[1 - st->cur][i] != st->S[1 - st->cur][0] + (size_t)i * st->n)
             return hipErrorInvalidValue;
-    if (la->dft_n > 0) {
-        /* the kernel samples only the cells it updates, and keeps a table of the stages of
-         * its z chunk (at most kMaxFZ planes) */
-        if (!smx_f32_two_full(st, st->two) || la->dft_n > kDftMaxPlanes) return hipErrorInvalidValue;
+    if (la->dft_n > 0)
+        /* the kernel keeps a table of the stages of its z chunk (at most kMaxFZ planes) */
         for (int i = 0; i < a.nzc; i++)
             if (a.zc[i + 1] - a.zc[i] > two_layout<32, 8, true, false>::kMaxFZ) return hipErrorInvalidValue;
-        a.dft = la->dft;
-        a.dft_n = la->dft_n;
-        a.dft_slot = la->dft_slot;
-        for (int m = 0; m < la->dft_n; m++) a.dft_planes[m] = la->dft_planes[m];
-    }
     if (!st->has_pml || tv.split) {
         const hipError_t e = launch_two(a);
         if (e != hipSuccess) return e;

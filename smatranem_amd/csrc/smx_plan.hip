@@ -1063,6 +1063,109 @@ static smx_adj_args adj_args(smx_plan *p)
     return a;
 }
 
+} // extern "C"
+
+namespace {
+
+/* The DFT state of a plan (set by smx_plan_set_dft), parked while the adjoint forward
+ * runs its own. */
+struct dft_state {
+    int nplanes, nfreq, chunk, fill;
+    int planes[SMX_DFT_MAX_PLANES];
+    double w[SMX_DFT_MAX_FREQ];
+    long long k0;
+    void *series;
+    double *acc, *table;
+};
+
+dft_state park_dft(smx_plan *p)
+{
+    dft_state d;
+    d.nplanes = p->dft_nplanes, d.nfreq = p->dft_nfreq, d.chunk = p->dft_chunk, d.fill = p->dft_fill;
+    std::memcpy(d.planes, p->dft_planes, sizeof d.planes);
+    std::memcpy(d.w, p->dft_w, sizeof d.w);
+    d.k0 = p->dft_k0;
+    d.series = p->dft_series, d.acc = p->dft_acc, d.table = p->dft_table;
+    p->dft_nplanes = 0, p->dft_fill = 0;
+    p->dft_series = nullptr, p->dft_acc = nullptr, p->dft_table = nullptr;
+    return d;
+}
+
+void restore_dft(smx_plan *p, const dft_state &d)
+{
+    p->dft_nplanes = d.nplanes, p->dft_nfreq = d.nfreq, p->dft_chunk = d.chunk, p->dft_fill = d.fill;
+    std::memcpy(p->dft_planes, d.planes, sizeof d.planes);
+    std::memcpy(p->dft_w, d.w, sizeof d.w);
+    p->dft_k0 = d.k0;
+    p->dft_series = d.series, p->dft_acc = d.acc, p->dft_table = d.table;
+}
+
+/* Whether the forward pass can take two steps per pass: the plan's two-step kernel
+ * samples the objective plane itself (split slots, or no PML) and checkpoints fall on
+ * pass boundaries. */
+bool adjoint_forward_paired(const smx_plan *p, int ckpt_every)
+{
+    return p->f32.two >= 0 && smx_f32_two_full(&p->f32, p->f32.two) && ckpt_every % 2 == 0;
+}
+
+/*
+ * The forward pass with two steps per pass: D_v(q) is the plan's DFT of E_v on plane zT
+ * at w, whose E phase e^{i*w*(k+1.5)} is the objective's, taken by the two-step kernel
+ * inside each pass; the caller's DFT, if any, is parked meanwhile and left as it was.
+ */
+int adjoint_forward_two_step(smx_plan *p, long long nsteps, int ckpt_every, int zT, double w, double *J)
+{
+    const int64_t n = p->n;
+    const dft_state saved = park_dft(p);
+    int rc = smx_plan_set_dft(p, 1, &zT, 1, &w, SMX_DFT_MAX_CHUNK);
+    for (long long k = 0; k < nsteps && rc == SMX_OK;) {
+        if (k % ckpt_every == 0) {
+            float *dst = p->adj_ckpt + (size_t)(k / ckpt_every) * 12 * (size_t)n;
+            hipError_t e = hipSuccess;
+            for (int i = 0; i < 6 && e == hipSuccess; i++) {
+                e = hipMemcpyAsync(dst + (size_t)i * n, p->f32.T[p->f32.cur][i], sizeof(float) * n,
+                                   hipMemcpyDeviceToDevice, p->stream);
+                if (e == hipSuccess)
+                    e = hipMemcpyAsync(dst + (size_t)(6 + i) * n, p->f32.S[p->f32.cur][i], sizeof(float) * n,
+                                       hipMemcpyDeviceToDevice, p->stream);
+            }
+            if (e != hipSuccess) {
+                rc = hip_fail(e, "adjoint checkpoint");
+                break;
+            }
+        }
+        /* up to the next checkpoint, so that a pair never straddles one */
+        const long long stop = std::min(nsteps, (k / ckpt_every + 1) * (long long)ckpt_every);
+        int done = 0;
+        rc = advance(p, k, stop - k, nullptr, &done);
+        k += done;
+    }
+    if (rc == SMX_OK) rc = dft_flush(p);
+    if (rc == SMX_OK) {
+        /* acc[plane 0][Ex, Ey][freq 0][q]: the objective's D, laid out as adj_D */
+        const size_t len = 4 * (size_t)p->plane;
+        hipError_t e = hipMemcpyAsync(p->adj_D, p->dft_acc + len, sizeof(double) * len, hipMemcpyDeviceToDevice,
+                                      p->stream);
+        std::vector<double> D(len);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(D.data(), p->dft_acc + len, sizeof(double) * len, hipMemcpyDeviceToHost, p->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+        if (e != hipSuccess) rc = hip_fail(e, "adjoint objective");
+        double acc = 0.;
+        for (size_t i = 0; i < D.size(); i++) acc += D[i] * D[i];
+        *J = acc;
+    }
+    (void)hipStreamSynchronize(p->stream);
+    for (void *b : {(void *)p->dft_series, (void *)p->dft_acc, (void *)p->dft_table})
+        if (b) (void)hipFree(b);
+    restore_dft(p, saved);
+    return rc;
+}
+
+} // namespace
+
+extern "C" {
+
 int smx_adjoint_forward(smx_plan *p, long long nsteps, int ckpt_every, int zT, double w, const int box[6],
                         double *J)
 {
@@ -1105,6 +1208,11 @@ int smx_adjoint_forward(smx_plan *p, long long nsteps, int ckpt_every, int zT, d
             if (p->f32.T[k][i]) SMX_HIP(hipMemsetAsync(p->f32.T[k][i], 0, sizeof(float) * n, p->stream));
             if (p->f32.S[k][i]) SMX_HIP(hipMemsetAsync(p->f32.S[k][i], 0, sizeof(float) * n, p->stream));
         }
+    if (adjoint_forward_paired(p, ckpt_every)) {
+        const int rc = adjoint_forward_two_step(p, nsteps, ckpt_every, zT, w, J);
+        if (rc == SMX_OK) p->adj_ready = true;
+        return rc;
+    }
     smx_adj_args a = adj_args(p);
     for (long long k = 0; k < nsteps; k++) {
         if (k % ckpt_every == 0) {

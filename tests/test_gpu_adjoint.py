@@ -21,11 +21,17 @@ def _box(case):
     return (3, L0 - 3, 3, L1 - 3, L2 // 2 - 3, L2 // 2 + 3)
 
 
-@pytest.mark.parametrize("kernel,ckpt", [("fused", 7), ("twopass", 5), ("fused", 1000), ("staged", 7)])
+@pytest.mark.parametrize("kernel,ckpt", [("fused", 7), ("twopass", 5), ("fused", 1000), ("staged", 7),
+                                         ("two-step", 10), ("two-step", 7), ("two-step", 1000)])
 def test_adjoint_matches_oracle(kernel, ckpt, monkeypatch):
+    """With the two-step kernel selected and an even checkpoint interval the forward pass
+    runs two steps per pass and takes the objective's samples inside the pass; with an
+    odd interval it runs one step at a time."""
     monkeypatch.setenv("SMX_F32_KERNEL", "twopass" if kernel == "twopass" else "fused")
     if kernel == "staged":
         monkeypatch.setenv("SMX_F32_TILE", "3,16")
+    if kernel == "two-step":
+        monkeypatch.setenv("SMX_F32_TWO", "2,8")
     case = _case()
     prep = casegen.prepare(case)
     zT = case.L[2] - 5
@@ -83,3 +89,23 @@ def test_adjoint_vs_finite_differences():
         eps_m[3 * cell + c] -= h
         fd = (J64(eps_p) - J64(eps_m)) / (2 * h)
         assert abs(fd - g[c, zz, yy, xx]) <= 1e-3 * abs(fd) + 1e-3 * flat.max(), (fd, g[c, zz, yy, xx])
+
+
+def test_adjoint_forward_keeps_caller_dft(monkeypatch):
+    """The paired forward pass runs its own DFT of the objective plane; a DFT the caller
+    set stays set, with its accumulators untouched, and the forward's J is the one of
+    the step-by-step forward."""
+    monkeypatch.setenv("SMX_F32_TWO", "2,8")
+    case = _case()
+    prep = casegen.prepare(case)
+    zT = case.L[2] - 5
+    w = 0.9 * 2 * np.pi / prep.N
+    K = case.time_period[2] * prep.N
+    box = _box(case)
+    with prep.plan(fdtd.F32) as p:
+        p.set_dft([zT - 1, zT], [w, 1.1 * w], chunk=8)
+        J2 = p.adjoint_forward(K, 10, zT, w, box)
+        D = p.read_dft()
+        assert D.shape == (2, 4, 2, case.plane) and not np.any(D)
+        J1 = p.adjoint_forward(K, 7, zT, w, box)  # odd interval: one step at a time
+    assert abs(J2 - J1) <= 1e-5 * J1
