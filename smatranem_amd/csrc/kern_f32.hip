@@ -1456,11 +1456,11 @@ __global__ __launch_bounds__(FX * FY) void two_step_f32(fused_args a)
     /* sample (Hx, Hy, Ex, Ey) of cell q at stage s into the ring (second step: soff = dstep):
      * the total field, which is the main slot on the lossless grids this kernel takes
      * outside PML, and the sample taken in PML cells */
-    /* ring plane of stage s_ (-1: none), a wave-uniform value */
+    /* ring plane of stage s_ (-1: none); read one stage ahead of its use, so that the
+     * next barrier's LDS wait covers the read */
     auto dft_m = [&](int s_) {
         if constexpr (!DFT) return -1;
-        const int m = SPLIT ? ((int)lds[L_ZT + 5 * s_ + 4] >> 1) - 1 : reinterpret_cast<const int *>(lds + L_DZ)[s_];
-        return __builtin_amdgcn_readfirstlane(m);
+        return SPLIT ? ((int)lds[L_ZT + 5 * s_ + 4] >> 1) - 1 : reinterpret_cast<const int *>(lds + L_DZ)[s_];
     };
     auto dft_sample = [&](int m, int64_t q, unsigned soff, float hx, float hy, float ex, float ey) {
         if constexpr (DFT) {
@@ -1617,6 +1617,7 @@ __global__ __launch_bounds__(FX * FY) void two_step_f32(fused_args a)
                 bepp[r][c] = 0.f;
 
     int zq0 = plane_of(0), zq1 = plane_of(-1), zq2 = plane_of(-2); /* planes of stages s, s-1, s-2 */
+    int dm_s1 = -1, dm_s2 = -1; /* DFT ring planes of stages s-1, s-2 (read in earlier iterations) */
     for (int s = 0; s < ns; s++) {
         const int p = z0 - 2 + s;     /* unwrapped plane of this stage */
         const int zp = zq0, zp1 = zq1, zp2 = zq2; /* wrapped planes of stages s, s-1, s-2 */
@@ -1634,7 +1635,8 @@ __global__ __launch_bounds__(FX * FY) void two_step_f32(fused_args a)
         }
         const float *S = lds + (s % NSLOT) * Ly::SLOT;
         const bool zs = zpml(s), zs1 = s >= 1 && zpml(s - 1), zs2 = s >= 2 && zpml(s - 2);
-        const int dm1 = s >= 1 ? dft_m(s - 1) : -1, dm2 = s >= 2 ? dft_m(s - 2) : -1; /* DFT ring planes */
+        const int dm1 = __builtin_amdgcn_readfirstlane(dm_s1), dm2 = __builtin_amdgcn_readfirstlane(dm_s2);
+        const int dm0 = dft_m(s); /* this stage's, used in the next two iterations */
 
         float h1n[CELLS][3], sh1n[CELLS][3], e1n[CELLS][3], se1n[CELLS][3], h2n[CELLS][3];
 #pragma unroll
@@ -1808,6 +1810,8 @@ __global__ __launch_bounds__(FX * FY) void two_step_f32(fused_args a)
             }
         }
         /* carry to the next plane */
+        dm_s2 = dm_s1;
+        dm_s1 = dm0;
         const bool split_s = SPLIT && (tile_pml || zs);
 #pragma unroll
         for (int r = 0; r < CELLS; r++)
