@@ -259,7 +259,7 @@ def large_run(torch, steps, small=False):
     out = {}
     for prec in (fdtd.F32, fdtd.F64):
         with prep.plan(prec) as p:
-            p.set_dft(planes, ws, chunk=32)
+            p.set_dft(planes, ws, chunk=64)
             kern = p.kernels
             if prec == fdtd.F32:
                 p.autotune(6)
@@ -401,7 +401,7 @@ def main():
     # FP32 path: the autotuner picks the step kernel (fused tile shape or the two sweeps)
     with prep.plan(fdtd.F32) as p32:
         if dft_planes:
-            p32.set_dft(dft_planes, dft_w, chunk=32)
+            p32.set_dft(dft_planes, dft_w, chunk=64)
         choice = p32.autotune(20)
         fx, fy, fz, variant = choice
         step_kernels = p32.kernels.split(" + ")
@@ -426,7 +426,7 @@ def main():
     if not args.no_fp64:
         with prep.plan(fdtd.F64) as p64:
             if dft_planes:
-                p64.set_dft(dft_planes, dft_w, chunk=32)
+                p64.set_dft(dft_planes, dft_w, chunk=64)
             wall64, ev64 = time_plan(p64, K, W, torch)
             H64, E64 = p64.read_fields()
             m64 = p64.read_monitors()

@@ -81,7 +81,7 @@ __global__ __launch_bounds__(kThreads) void sample_planes_f64(smx_dft_sample_arg
 template <typename T>
 __global__ __launch_bounds__(kThreads) void fold_series(smx_dft_fold_args a)
 {
-    __shared__ double2 ph[2 * SMX_DFT_MAX_CHUNK * SMX_DFT_MAX_FREQ]; /* [h][t][f] */
+    extern __shared__ double2 ph[]; /* [h][t][f], 2 * count * nfreq entries */
     const int F = a.nfreq, nt = a.count;
     for (int i = threadIdx.x; i < 2 * nt * F; i += kThreads) {
         const int h = i / (nt * F), r = i % (nt * F);
@@ -152,7 +152,8 @@ extern "C" hipError_t smx_dft_fold(const smx_dft_fold_args *a, int f64, hipStrea
     if (a->count <= 0) return hipSuccess;
     const int64_t total = a->plane_cells * 4 * a->nplanes;
     const dim3 grid((unsigned)((total + kThreads - 1) / kThreads)), block(kThreads);
-    if (f64) hipLaunchKernelGGL(fold_series<double>, grid, block, 0, stream, *a);
-    else hipLaunchKernelGGL(fold_series<float>, grid, block, 0, stream, *a);
+    const size_t lds = sizeof(double2) * 2 * (size_t)a->count * (size_t)a->nfreq; /* <= 64 KiB */
+    if (f64) hipLaunchKernelGGL(fold_series<double>, grid, block, lds, stream, *a);
+    else hipLaunchKernelGGL(fold_series<float>, grid, block, lds, stream, *a);
     return hipGetLastError();
 }

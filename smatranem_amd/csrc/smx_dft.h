@@ -8,7 +8,7 @@
 
 #define SMX_DFT_MAX_PLANES 8
 #define SMX_DFT_MAX_FREQ 32
-#define SMX_DFT_MAX_CHUNK 32
+#define SMX_DFT_MAX_CHUNK 64
 
 struct smx_dft_sample_args {
     int L0, L1, L2;

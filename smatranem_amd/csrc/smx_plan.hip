@@ -981,7 +981,7 @@ int smx_plan_set_dft(smx_plan *p, int nplanes, const int *planes, int nfreq, con
     if (nplanes < 0 || nplanes > SMX_DFT_MAX_PLANES) return arg_fail("DFT planes: 0..8");
     if (nplanes > 0 && (!planes || !w_per_step)) return arg_fail("null pointer");
     if (nplanes > 0 && (nfreq < 1 || nfreq > SMX_DFT_MAX_FREQ)) return arg_fail("DFT frequencies: 1..32");
-    if (nplanes > 0 && (chunk < 1 || chunk > SMX_DFT_MAX_CHUNK)) return arg_fail("DFT chunk: 1..32");
+    if (nplanes > 0 && (chunk < 1 || chunk > SMX_DFT_MAX_CHUNK)) return arg_fail("DFT chunk: 1..64");
     for (int m = 0; m < nplanes; m++)
         if (planes[m] < 0 || planes[m] >= p->g.L[2]) return arg_fail("DFT plane outside the grid");
     SMX_HIP(hipStreamSynchronize(p->stream));

@@ -16,7 +16,7 @@ def _prep():
     return casegen.prepare(small_waveguide(seed=6, **CASE))
 
 
-@pytest.mark.parametrize("chunk", [1, 7, 32])
+@pytest.mark.parametrize("chunk", [1, 7, 32, 64])
 def test_dft_at_omega_equals_reference_monitors(chunk):
     prep = _prep()
     c = prep.case
@@ -100,7 +100,7 @@ def test_dft_argument_checks():
 LOSSLESS = dict(pml_xy=0, pml_z=3, iBc=(1, -1, 0), metal_cells=0, lossy_cells=0)
 
 
-@pytest.mark.parametrize("two,chunk", [(None, 16), ("2,16", 16), ("2,16", 7), ("2,8", 1), ("0,16", 16),
+@pytest.mark.parametrize("two,chunk", [(None, 16), ("2,16", 16), ("2,16", 7), ("2,16", 64), ("2,8", 1), ("0,16", 16),
                                        ("3,16", 16)])
 def test_dft_with_two_step_passes(two, chunk, monkeypatch):
     """With the two-step kernel selected the DFT samples of both steps of a pass come
