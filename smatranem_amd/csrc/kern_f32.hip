@@ -1337,11 +1337,36 @@ struct two_layout {
 
 /* Copies of one stage dealt to wave W: arrays E0, H0, bH, bE (and SH0, SE0 when `split`),
  * each 3 components x NC runs of CH words of the G4 region. */
-template <int FX, int FY, bool SPLIT, bool WIDE, int W>
-__device__ inline void two_issue(float *slot, __amdgpu_buffer_rsrc_t rT, __amdgpu_buffer_rsrc_t rB,
-                                 __amdgpu_buffer_rsrc_t rS, unsigned cb, unsigned soff, bool split, const unsigned *vq)
+/* Where a stage's copies read: one buffer resource per group of 6 component arrays
+ * (component c at c*cb bytes, the plane at soff), or -- with BIG, for groups of 4 GiB and
+ * more -- a resource built per copy on one array's plane (base + comp*ncomp + zoff
+ * floats, plane_bytes long), so that every offset stays below 2^32. */
+struct two_src {
+    __amdgpu_buffer_rsrc_t rT, rB, rS;
+    unsigned cb, soff;
+    const float *pT, *pB, *pS;
+    int64_t ncomp, zoff;
+    unsigned plane_bytes;
+};
+
+template <bool BIG>
+__device__ inline __amdgpu_buffer_rsrc_t two_res(const two_src &q, int group, unsigned comp, unsigned &voff)
+{
+    if constexpr (BIG) {
+        const float *base = group == 0 ? q.pT : group == 1 ? q.pB : q.pS;
+        return __builtin_amdgcn_make_buffer_rsrc((void *)(base + comp * q.ncomp + q.zoff), (short)0,
+                                                 (int)q.plane_bytes, (int)kBufWord3);
+    } else {
+        voff += comp * q.cb;
+        return group == 0 ? q.rT : group == 1 ? q.rB : q.rS;
+    }
+}
+
+template <int FX, int FY, bool SPLIT, bool WIDE, bool BIG, int W>
+__device__ inline void two_issue(float *slot, const two_src &q, bool split, const unsigned *vq)
 {
     using Ly = two_layout<FX, FY, SPLIT, WIDE>;
+    const unsigned so = BIG ? 0u : q.soff;
     /* the copy's size must be a literal */
     auto copy = [](__amdgpu_buffer_rsrc_t r, float *dst, unsigned voff, unsigned so) {
         if constexpr (WIDE) __builtin_amdgcn_raw_ptr_buffer_load_lds(r, SMX_LDS(dst), 16, voff, so, 0, 0);
@@ -1353,14 +1378,18 @@ __device__ inline void two_issue(float *slot, __amdgpu_buffer_rsrc_t rT, __amdgp
         /* E0 = T comp 3+c, H0 = T comp c, bH = b comp c, bE = b comp 3+c */
         const unsigned comp = (arr == 0 || arr == 3) ? 3 + c : c;
         float *dst = slot + arr * Ly::ARR + (c * Ly::NC + ch) * Ly::CH;
-        copy(arr < 2 ? rT : rB, dst, vq[ch] + comp * cb, soff);
+        unsigned voff = vq[ch];
+        const __amdgpu_buffer_rsrc_t res = two_res<BIG>(q, arr < 2 ? 0 : 1, comp, voff);
+        copy(res, dst, voff, so);
     }
     if (SPLIT && split) {
 #pragma unroll
         for (int j = W; j < Ly::NSPLIT; j += Ly::NW) {
             const int arr = j / (3 * Ly::NC), r = j % (3 * Ly::NC), c = r / Ly::NC, ch = r % Ly::NC;
             float *dst = slot + (4 + arr) * Ly::ARR + (c * Ly::NC + ch) * Ly::CH; /* SH0 = S comp c, SE0 = 3+c */
-            copy(rS, dst, vq[ch] + (arr * 3 + c) * cb, soff);
+            unsigned voff = vq[ch];
+            const __amdgpu_buffer_rsrc_t res = two_res<BIG>(q, 2, arr * 3 + c, voff);
+            copy(res, dst, voff, so);
         }
     }
 }
@@ -1387,7 +1416,7 @@ __device__ inline void e_regular(const float e[3], const float b[3], const float
     }
 }
 
-template <int FX, int FY, int D, bool SPLIT, bool WIDE, bool DFT>
+template <int FX, int FY, int D, bool SPLIT, bool WIDE, bool DFT, bool BIG = false>
 __global__ __launch_bounds__(FX * FY) void two_step_f32(fused_args a)
 {
     using Ly = two_layout<FX, FY, SPLIT, WIDE>;
@@ -1567,29 +1596,45 @@ __global__ __launch_bounds__(FX * FY) void two_step_f32(fused_args a)
     const __amdgpu_buffer_rsrc_t rT = group_rsrc(a.To[0], a.ncomp), rB = group_rsrc(a.b[0], a.ncomp),
                                  rS = group_rsrc(a.So[0], a.ncomp);
     const unsigned cb = 4u * (unsigned)a.ncomp;
-    /* the new fields and splits: one resource per group, as the copies */
+    /* the new fields and splits: one resource per group, as the copies (BIG: 64-bit
+     * addresses from the groups' first arrays) */
     const __amdgpu_buffer_rsrc_t rTn = group_rsrc(a.Tn[0], a.ncomp), rSn = group_rsrc(a.Sn[0], a.ncomp);
+    auto store = [&](bool split_group, int c0, int64_t l, const float v[3]) {
+        if constexpr (BIG) {
+            float *base = (split_group ? a.Sn[0] : a.Tn[0]) + (int64_t)c0 * a.ncomp + l;
+            base[0] = v[0];
+            base[a.ncomp] = v[1];
+            base[2 * a.ncomp] = v[2];
+        } else {
+            bstore3(split_group ? rSn : rTn, cb, c0, l, v);
+        }
+    };
+    two_src src;
+    src.rT = rT, src.rB = rB, src.rS = rS, src.cb = cb, src.soff = 0;
+    src.pT = a.To[0], src.pB = a.b[0], src.pS = a.So[0], src.ncomp = a.ncomp, src.zoff = 0;
+    src.plane_bytes = 4u * (unsigned)a.sz;
     auto issue = [&](int s, int z) { /* stage s, plane z = plane_of(s) */
-        const unsigned soff = 4u * (unsigned)((int64_t)z * a.sz);
+        src.soff = 4u * (unsigned)((int64_t)z * a.sz);
+        src.zoff = (int64_t)z * a.sz;
         const bool split = SPLIT && (tile_pml || zpml(s));
         float *slot = lds + (s % NSLOT) * Ly::SLOT;
         if constexpr (Ly::NW == 4) {
             switch (wv) {
-            case 0: two_issue<FX, FY, SPLIT, WIDE, 0>(slot, rT, rB, rS, cb, soff, split, vq); break;
-            case 1: two_issue<FX, FY, SPLIT, WIDE, 1>(slot, rT, rB, rS, cb, soff, split, vq); break;
-            case 2: two_issue<FX, FY, SPLIT, WIDE, 2>(slot, rT, rB, rS, cb, soff, split, vq); break;
-            default: two_issue<FX, FY, SPLIT, WIDE, 3>(slot, rT, rB, rS, cb, soff, split, vq); break;
+            case 0: two_issue<FX, FY, SPLIT, WIDE, BIG, 0>(slot, src, split, vq); break;
+            case 1: two_issue<FX, FY, SPLIT, WIDE, BIG, 1>(slot, src, split, vq); break;
+            case 2: two_issue<FX, FY, SPLIT, WIDE, BIG, 2>(slot, src, split, vq); break;
+            default: two_issue<FX, FY, SPLIT, WIDE, BIG, 3>(slot, src, split, vq); break;
             }
         } else {
             switch (wv) {
-            case 0: two_issue<FX, FY, SPLIT, WIDE, 0>(slot, rT, rB, rS, cb, soff, split, vq); break;
-            case 1: two_issue<FX, FY, SPLIT, WIDE, 1>(slot, rT, rB, rS, cb, soff, split, vq); break;
-            case 2: two_issue<FX, FY, SPLIT, WIDE, 2>(slot, rT, rB, rS, cb, soff, split, vq); break;
-            case 3: two_issue<FX, FY, SPLIT, WIDE, 3>(slot, rT, rB, rS, cb, soff, split, vq); break;
-            case 4: two_issue<FX, FY, SPLIT, WIDE, 4>(slot, rT, rB, rS, cb, soff, split, vq); break;
-            case 5: two_issue<FX, FY, SPLIT, WIDE, 5>(slot, rT, rB, rS, cb, soff, split, vq); break;
-            case 6: two_issue<FX, FY, SPLIT, WIDE, 6>(slot, rT, rB, rS, cb, soff, split, vq); break;
-            default: two_issue<FX, FY, SPLIT, WIDE, 7>(slot, rT, rB, rS, cb, soff, split, vq); break;
+            case 0: two_issue<FX, FY, SPLIT, WIDE, BIG, 0>(slot, src, split, vq); break;
+            case 1: two_issue<FX, FY, SPLIT, WIDE, BIG, 1>(slot, src, split, vq); break;
+            case 2: two_issue<FX, FY, SPLIT, WIDE, BIG, 2>(slot, src, split, vq); break;
+            case 3: two_issue<FX, FY, SPLIT, WIDE, BIG, 3>(slot, src, split, vq); break;
+            case 4: two_issue<FX, FY, SPLIT, WIDE, BIG, 4>(slot, src, split, vq); break;
+            case 5: two_issue<FX, FY, SPLIT, WIDE, BIG, 5>(slot, src, split, vq); break;
+            case 6: two_issue<FX, FY, SPLIT, WIDE, BIG, 6>(slot, src, split, vq); break;
+            default: two_issue<FX, FY, SPLIT, WIDE, BIG, 7>(slot, src, split, vq); break;
             }
         }
     };
@@ -1759,8 +1804,8 @@ __global__ __launch_bounds__(FX * FY) void two_step_f32(fused_args a)
             factors(r, s - 1, false, pa, pg);
             h_regular(h1p[r], bhp[r], sh1p[r], P, M, pml, pa, pg, h2n[r], sh2);
             if (inT[r] && p - 1 < z1) {
-                bstore3(rTn, cb, 0, cq[r] + a.sz * zp1, h2n[r]);
-                if (SPLIT && pml) bstore3(rSn, cb, 0, cq[r] + a.sz * zp1, sh2);
+                store(false, 0, cq[r] + a.sz * zp1, h2n[r]);
+                if (SPLIT && pml) store(true, 0, cq[r] + a.sz * zp1, sh2);
             }
         }
         /* d. E2(p-2) on T: forward curl of H2 (previous plane from LDS, +z from c.) */
@@ -1792,8 +1837,8 @@ __global__ __launch_bounds__(FX * FY) void two_step_f32(fused_args a)
             float pa[3], pg[3];
             factors(r, s - 2, true, pa, pg);
             e_regular(e1pp[r], bepp[r], se1pp[r], P, M, pml, pa, pg, en, sn);
-            bstore3(rTn, cb, 3, cq[r] + a.sz * zp2, en);
-            if (SPLIT && pml) bstore3(rSn, cb, 3, cq[r] + a.sz * zp2, sn);
+            store(false, 3, cq[r] + a.sz * zp2, en);
+            if (SPLIT && pml) store(true, 3, cq[r] + a.sz * zp2, sn);
             if (DFT) dft_sample(dm2, cq[r], dstep, hc[0], hc[1], en[0], en[1]);
             if (!pml && (zp2 == a.iB || zp2 == a.iT)) { /* second step's monitors */
                 for (int m = 0; m < 2; m++) {
@@ -2197,16 +2242,19 @@ struct two_variant {
 };
 #define SMX_TWO(FX, FY, D, SP, WI)                                                                         \
     {FX, FY, D, SP, WI, two_layout<FX, FY, SP, WI>::OX,                                                   \
-     "two_step_f32<" #FX "," #FY "," #D "," #SP "," #WI ",false>",                                       \
+     "two_step_f32<" #FX "," #FY "," #D "," #SP "," #WI ",false,false>",                                 \
      "region_h_f32 + region_e_f32 + two_step_f32<" #FX "," #FY "," #D "," #SP "," #WI                     \
-     ",false> + region_h_f32 + region_e_f32",                                                             \
-     "two_step_f32<" #FX "," #FY "," #D "," #SP "," #WI ",true>"}
+     ",false,false> + region_h_f32 + region_e_f32",                                                       \
+     "two_step_f32<" #FX "," #FY "," #D "," #SP "," #WI ",true,false>"}
 constexpr two_variant kTwo[] = {SMX_TWO(32, 8, 1, false, false), SMX_TWO(32, 8, 2, false, false),
                                 SMX_TWO(32, 8, 1, true, false), SMX_TWO(32, 8, 1, false, true)};
 #undef SMX_TWO
 constexpr int kNumTwo = sizeof kTwo / sizeof kTwo[0];
 
 int smx_f32_num_two(void) { return kNumTwo; }
+
+/* Whether the two-step pass runs in its 64-bit form (groups of 4 GiB and more, or forced). */
+static bool two_big(const smx_f32_state *st) { return st->force_big || 24 * st->n >= ((int64_t)1 << 32); }
 
 int smx_f32_two_full(const smx_f32_state *st, int variant)
 {
@@ -2224,6 +2272,10 @@ constexpr two_kernel kTwoKernelDft[] = {
     two_step_f32<32, 8, 1, false, false, true>, two_step_f32<32, 8, 2, false, false, true>,
     two_step_f32<32, 8, 1, true, false, true>, two_step_f32<32, 8, 1, false, true, true>};
 static_assert(sizeof kTwoKernelDft / sizeof kTwoKernelDft[0] == kNumTwo, "one kernel per variant");
+/* the split-slot variant for groups of 4 GiB and more (24 n >= 2^32), without and with the DFT */
+constexpr int kTwoBigVariant = 2;
+constexpr two_kernel kTwoKernelBig[2] = {two_step_f32<32, 8, 1, true, false, false, true>,
+                                         two_step_f32<32, 8, 1, true, false, true, true>};
 
 void smx_f32_two_shape(int variant, int *fx, int *fy)
 {
@@ -2238,7 +2290,9 @@ int smx_f32_two_ok(const smx_f32_state *st, int variant, int fz)
      * split slots need the PML outside the kernel's box, those with them keep the
      * chunk's z tables in LDS; 16-byte copies need x groups of 4 that never straddle
      * the x wrap (the two-step box starts at a multiple of 4) */
-    if (variant < 0 || variant >= kNumTwo || fz < 1 || st->nspecial != 0 || 24 * st->n >= ((int64_t)1 << 32))
+    if (variant < 0 || variant >= kNumTwo || fz < 1 || st->nspecial != 0) return 0;
+    /* groups of 4 GiB and more: the split-slot variant's 64-bit form, one array below 4 GiB */
+    if ((24 * st->n >= ((int64_t)1 << 32) || st->force_big) && (variant != kTwoBigVariant || 4 * st->n >= ((int64_t)1 << 32)))
         return 0;
     if (kTwo[variant].wide && st->L[0] % 4 != 0) return 0;
     if ((st->L[2] + fz - 1) / fz > kMaxZChunks) return 0;
@@ -2261,6 +2315,8 @@ const char *smx_f32_kernel_name(const smx_f32_state *st, int dft)
         "region_h_f32 + fused_dma_f32<32,8,2,false> + region_e_f32",
         "region_h_f32 + fused_dma_f32<32,8,1,false> + region_e_f32"};
     if (st->two >= 0 && st->two < kNumTwo) {
+        if (two_big(st))
+            return dft ? "two_step_f32<32,8,1,true,false,true,true>" : "two_step_f32<32,8,1,true,false,false,true>";
         if (st->has_pml && !kTwo[st->two].split) return kTwo[st->two].with_regions;
         return dft ? kTwo[st->two].name_dft : kTwo[st->two].name;
     }
@@ -2419,6 +2475,8 @@ int smx_f32_create(smx_f32_state *st, const smx_geom *g, double iR, double iI, c
     st->two = -1;
     st->two_fz = kDefaultFZ;
     st->two_bal = 0;
+    /* the two-step pass's 64-bit form on any grid ("1"), for tests of it on small grids */
+    st->force_big = getenv("SMX_F32_BIG") && !strcmp(getenv("SMX_F32_BIG"), "1");
     if (const char *t = getenv("SMX_F32_TILE")) { /* "<variant>,<z chunk>", e.g. "1,32" */
         int ti = kDefaultTile, zc = kDefaultFZ;
         if (sscanf(t, "%d,%d", &ti, &zc) >= 1 && smx_f32_tile_ok(st, ti, zc)) {
@@ -2710,7 +2768,9 @@ hipError_t smx_f32_step2(smx_f32_state *st, const smx_f32_launch_args *la, hipSt
     a.nblocks = a.gx * a.gy * a.nzc;
     auto launch_two = [&](const fused_args &k) {
         const dim3 grid(k.nblocks), block(tv.fx * tv.fy);
-        hipLaunchKernelGGL(k.dft_n ? kTwoKernelDft[st->two] : kTwoKernel[st->two], grid, block, 0, stream, k);
+        const two_kernel kern = two_big(st) ? kTwoKernelBig[k.dft_n ? 1 : 0]
+                                : k.dft_n ? kTwoKernelDft[st->two] : kTwoKernel[st->two];
+        hipLaunchKernelGGL(kern, grid, block, 0, stream, k);
         return hipGetLastError();
     };
     a.sz = (int64_t)g.L0 * g.L1;

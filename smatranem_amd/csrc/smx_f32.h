@@ -83,6 +83,7 @@ struct smx_f32_state {
     int two_bal; /* z chunks of the two-step kernel: 0 uniform (two_fz planes), k > 0
                     balanced with a stage of a z-PML plane counted as 1 + k/2 */
     unsigned char *zpml_host; /* z-PML flag of every plane (host copy) */
+    int force_big; /* the two-step pass in its 64-bit form on any grid (tests) */
     /* host copies for the reference-layout field import/export */
     char *cont_host;
     int *addr_host;

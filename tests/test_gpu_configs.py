@@ -120,7 +120,7 @@ def test_c3_batch_members_graph_equals_alone(monkeypatch):
     alone = []
     for pr in preps:
         with pr.plan(fdtd.F32) as p:
-            assert p.kernels == "two_step_f32<32,8,1,true,false,false>"
+            assert p.kernels == "two_step_f32<32,8,1,true,false,false,false>"
             p.run_steps(0, steps)
             alone.append(p.read_fields())
     plans = [pr.plan(fdtd.F32) for pr in preps]

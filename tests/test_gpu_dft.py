@@ -124,7 +124,7 @@ def test_dft_with_two_step_passes(two, chunk, monkeypatch):
         p.set_dft([c.iB, c.iT], ws, chunk=chunk)
         # the split-slot variant with a ring of two or more slots samples in the kernel
         fused = two is not None and two.startswith("2,") and chunk >= 2
-        assert p.kernels.endswith(",true>") == fused, p.kernels
+        assert p.kernels.endswith(",true,false>") == fused, p.kernels
         p.run_steps(0, 7)
         p.run_steps(7, nsteps - 7)
         D = p.read_dft()

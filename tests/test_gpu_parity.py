@@ -241,7 +241,7 @@ def test_region_sweeps_carry_monitor_plane(tile, monkeypatch):
 
 # two-step variants: (copies in flight, split slots, 16-byte copies); tiles are 32 x 8
 TWO_VARIANTS = ((1, False, False), (2, False, False), (1, True, False), (1, False, True))
-TWO_NAMES = tuple(f"two_step_f32<32,8,{d},{str(sp).lower()},{str(wi).lower()},false>" for d, sp, wi in TWO_VARIANTS)
+TWO_NAMES = tuple(f"two_step_f32<32,8,{d},{str(sp).lower()},{str(wi).lower()},false,false>" for d, sp, wi in TWO_VARIANTS)
 TWO_SPLIT = tuple(v[1] for v in TWO_VARIANTS)
 
 
@@ -378,3 +378,25 @@ def test_autotune_keeps_state():
         H, E = p.read_fields()
     Ho, Eo, _, _ = _oracle_run(prep, 0, prep.case.time_period[2])
     assert rel_l2(H, Ho) < 2e-4 and rel_l2(E, Eo) < 2e-4
+
+
+@pytest.mark.parametrize("kw", [dict(pml_xy=3, pml_z=3, iBc=(0, 0, 0)), dict(pml_xy=0, pml_z=3, iBc=(1, -1, 0))])
+def test_two_step_64bit_form(kw, monkeypatch):
+    """The two-step pass's form for grids of 4 GiB component groups and more (a buffer
+    resource per array and plane, 64-bit stores), forced on a small grid: fields, monitors
+    and DFT are bit for bit those of the usual form."""
+    prep = casegen.prepare(small_waveguide(seed=19, L0=40, L1=24, L2=36, outer=8, metal_cells=0, lossy_cells=0,
+                                           **kw))
+    ws = [0.3, 2 * np.pi / prep.N]
+    out = []
+    for big in ("0", "1"):
+        monkeypatch.setenv("SMX_F32_TWO", "2,16")
+        monkeypatch.setenv("SMX_F32_BIG", big)
+        with prep.plan(fdtd.F32) as p:
+            p.set_dft([10, 20], ws, chunk=8)
+            assert p.kernels.endswith(",true>" if big == "1" else ",false>"), p.kernels
+            p.run(0, prep.case.time_period[2])
+            out.append(p.read_fields() + tuple(p.read_monitors()) + (p.read_dft(),))
+    assert np.abs(out[0][0]).max() > 0
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
