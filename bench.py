@@ -241,24 +241,31 @@ def adjoint_run(torch, small=False, fd_cells=0):
     return out
 
 
-def large_run(torch, steps, small=False):
-    """The large-grid workload: 768 x 768 x 256 cells (151 M; a quarter of the depth of a
-    768 x 768 x 1024 run), curvilinear scale factors, 12 dielectric layers with a PEC
+def large_run(torch, steps, depth=256, small=False):
+    """The large-grid workload: 768 x 768 x `depth` cells (1024: the full 604 M-cell grid;
+    256: a quarter of it), curvilinear scale factors, 12 dielectric layers with a PEC
     grating, 12-cell PML on every face, an 8-frequency DFT (FP64 accumulators) on two
-    planes.  FP32 (autotuned) and FP64 over the same `steps` steps from zero fields;
-    |S| error of FP32 against FP64 from the DFT phasors projected on the source profile."""
+    planes.  FP32 (autotuned) over `steps` steps from zero fields; up to 200 M cells also
+    FP64 over the same steps, with the |S| error of FP32 against FP64 from the DFT phasors
+    projected on the source profile (beyond, the FP64 plan's host coefficient arrays alone
+    would take 70 GB and more)."""
     import numpy as np
     from smatranem_amd import casegen, fdtd
-    L, pml = ((96, 96, 64), 6) if small else ((768, 768, 256), 12)
+    L, pml = ((96, 96, 64), 6) if small else ((768, 768, depth), 12)
     t0 = time.perf_counter()
-    case = casegen.layered_stack_case(L, pml=pml)
-    prep = casegen.prepare(case)
-    print(f"[bench] large grid {L}: setup {time.perf_counter() - t0:.0f} s", file=sys.stderr, flush=True)
+
+    def say(msg):
+        print(f"[bench] large grid {L}: {msg} ({time.perf_counter() - t0:.0f} s)", file=sys.stderr, flush=True)
+    case = casegen.layered_stack_case(L, pml=pml, progress=say)
+    prep = casegen.prepare(case, progress=say)
+    say("setup done")
     planes = [pml + 3, L[2] - pml - 4]
     ws = [float(x) for x in 2 * np.pi / prep.N * np.linspace(0.9, 1.1, 8)]
     out = {}
-    for prec in (fdtd.F32, fdtd.F64):
+    with_fp64 = case.n <= 200_000_000
+    for prec in (fdtd.F32, fdtd.F64) if with_fp64 else (fdtd.F32,):
         with prep.plan(prec) as p:
+            say("%s plan created" % ("FP32" if prec == fdtd.F32 else "FP64"))
             p.set_dft(planes, ws, chunk=64)
             kern = p.kernels
             if prec == fdtd.F32:
@@ -269,6 +276,15 @@ def large_run(torch, steps, small=False):
         print(f"[bench] large grid {'FP32' if prec == fdtd.F32 else 'FP64'}: {out[prec][0]:.3f} ms/step",
               file=sys.stderr, flush=True)
     n, phi = case.n, prep.pml_fraction
+    g32 = n / (out[fdtd.F32][0] * 1e-3) / 1e9
+    res = dict(workload="layered stack %dx%dx%d: curvilinear scale factors, 12 dielectric layers, PEC "
+                        "grating, %d-cell PML on all faces, 8-frequency DFT on 2 planes" % (*L, pml),
+               cells=n, pml_fraction=round(phi, 4), steps=steps, kernel=out[fdtd.F32][2],
+               fp32_gcells=round(g32, 3), fp32_ms_per_step=round(out[fdtd.F32][0], 4),
+               fp32_roofline_frac=round(bytes_per_cell(phi, 4, 4) * g32 / HBM_PEAK_GBS, 4),
+               fp32_device_GB=round(30 * 4 * n / 1e9, 1))
+    if not with_fp64:
+        return res
     errs = []
     for m in range(len(planes)):
         for f in range(len(ws)):
@@ -279,14 +295,8 @@ def large_run(torch, steps, small=False):
                 return H, E
             s32, s64 = overlap(prep, *ph(out[fdtd.F32][1])), overlap(prep, *ph(out[fdtd.F64][1]))
             errs.append((abs(abs(s32) - abs(s64)), abs(s64)))
-    g32 = n / (out[fdtd.F32][0] * 1e-3) / 1e9
     g64 = n / (out[fdtd.F64][0] * 1e-3) / 1e9
-    return dict(workload="layered stack %dx%dx%d: curvilinear scale factors, 12 dielectric layers, PEC "
-                         "grating, %d-cell PML on all faces, 8-frequency DFT on 2 planes" % (*L, pml),
-                cells=n, pml_fraction=round(phi, 4), steps=steps, kernel=out[fdtd.F32][2],
-                fp32_gcells=round(g32, 3), fp32_ms_per_step=round(out[fdtd.F32][0], 4),
-                fp32_roofline_frac=round(bytes_per_cell(phi, 4, 4) * g32 / HBM_PEAK_GBS, 4),
-                fp64_gcells=round(g64, 3),
+    return dict(res, fp64_gcells=round(g64, 3),
                 fp64_roofline_frac=round(bytes_per_cell(phi, 8, 8) * g64 / HBM_PEAK_GBS, 4),
                 speedup_fp32_over_fp64=round(g32 / g64, 3),
                 S_abs_err_max=max(e for e, _ in errs), S_abs_fp64_max=max(r for _, r in errs),
@@ -352,7 +362,10 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint gradient workload")
     ap.add_argument("--no-dft", action="store_true", help="leave the 16-frequency port DFT off")
     ap.add_argument("--large", type=int, default=0, metavar="STEPS",
-                    help="also run the 768x768x256 layered-stack grid for STEPS steps (FP32 and FP64)")
+                    help="also run the 768x768xDEPTH layered-stack grid for STEPS steps (FP32, and FP64 up to "
+                         "200 M cells)")
+    ap.add_argument("--large-depth", type=int, default=256, metavar="DEPTH",
+                    help="z cells of the large grid (1024: 768x768x1024, 604 M cells)")
     ap.add_argument("--adjoint-fd", type=int, default=0, metavar="CELLS",
                     help="check the adjoint gradient at CELLS cells against FP64 central differences")
     ap.add_argument("--small", action="store_true", help="32x32x48 grid (quick check)")
@@ -493,7 +506,7 @@ def main():
 
     large = None
     if args.large:
-        large = large_run(torch, args.large, args.small)
+        large = large_run(torch, args.large, args.large_depth, args.small)
 
     adjoint = None
     if not args.no_adjoint:

@@ -128,7 +128,7 @@ def sheet_source(L0, L1, profile, n_eff=1.0):
 def waveguide_case(L0, L1, L2, outer, pml_xy, pml_z, *, omega=2 * np.pi / 20.0, eps_core=12.1,
                    eps_clad=2.1, core=(8, 6), scale_amp=0.0, n_modes=8, pec_fraction=0.0,
                    seed=0, iBc=(0, 0, 0), dt_factor=1.0, time_period=(2, 1, 8), monitor_gap=2,
-                   metal_cells=0, lossy_cells=0, blocks=0, design=None, layers=0, grating=False):
+                   metal_cells=0, lossy_cells=0, blocks=0, design=None, layers=0, grating=False, progress=None):
     """A straight dielectric core along z in cladding, on an L0 x L1 x L2 grid with
     ``outer`` port slices at each end (B0.., T..), PML layers of ``pml_xy`` cells on the
     x/y faces and ``pml_z`` on the z faces (PMLcalc.c:188-260), optional curvilinear scale
@@ -143,8 +143,10 @@ def waveguide_case(L0, L1, L2, outer, pml_xy, pml_z, *, omega=2 * np.pi / 20.0, 
     seeded eps in [1, 12.1] (replacing core and cladding there), and ``grating`` puts a PEC
     grating (x-period 16, 8 cells wide, clear of the x/y PML) on the middle scatterer
     plane.  Monitor planes sit between the z PML and the scatterer, the source
-    plane just above the bottom monitor."""
+    plane just above the bottom monitor.  ``progress`` (optional) is called with a short
+    text after each stage, for callers that report on long generations."""
     rng = np.random.default_rng(seed)
+    say = progress if progress is not None else (lambda msg: None)
     n = L0 * L1 * L2
     z = np.arange(L2)[:, None, None]
     y = np.arange(L1)[None, :, None]
@@ -180,6 +182,7 @@ def waveguide_case(L0, L1, L2, outer, pml_xy, pml_z, *, omega=2 * np.pi / 20.0, 
         cell_eps = rng.uniform(lo, hi, size=(z1 - z0, y1 - y0, x1 - x0))
         eps3[z0:z1, y0:y1, x0:x1, :] = cell_eps[..., None]
 
+    say("materials")
     # curvilinear scale factors
     if scale_amp:
         # one smooth field per axis, evaluated at every staggered position
@@ -204,6 +207,7 @@ def waveguide_case(L0, L1, L2, outer, pml_xy, pml_z, *, omega=2 * np.pi / 20.0, 
         for c in range(3):
             mu3[..., c] *= fx_at(MU_POS[c], c)
             eps3[..., c] *= fx_at(EPS_POS[c], c)
+            say("scale factors of component %d" % c)
 
     # PEC slabs in the scatterer region
     pec_mask = np.zeros((L2, L1, L0), dtype=bool)
@@ -253,6 +257,7 @@ def waveguide_case(L0, L1, L2, outer, pml_xy, pml_z, *, omega=2 * np.pi / 20.0, 
         for zz, yy, xx in zip(zs, ys, xs):
             eps_c[zz, yy, xx, :] = eps_c[zz, yy, xx, :].real + 0.3j
 
+    say("PEC, metal and lossy cells")
     min_dt = cfl_min_dt(mu3.reshape(-1)[np.isfinite(mu3.reshape(-1))],
                         np.where(eps3 >= PEC_EPS, eps_clad, eps3).reshape(-1))
 
@@ -262,6 +267,7 @@ def waveguide_case(L0, L1, L2, outer, pml_xy, pml_z, *, omega=2 * np.pi / 20.0, 
     sig_x = pml_profile(L0, pml_xy, pml_xy, omega)
     sig_y = pml_profile(L1, pml_xy, pml_xy, omega)
     sig_z = pml_profile(L2, pml_z, pml_z, omega)
+    say("CFL time step")
     mu_s, eps_s = apply_stretch((L0, L1, L2), mu_c.reshape(-1), eps_c.reshape(-1),
                                 (sig_x, sig_y, sig_z))
 
@@ -321,7 +327,8 @@ def curvilinear_waveguide_case(L=(160, 160, 320), pml=10, pml_xy=0, time_period=
                           pec_fraction=0.05, seed=2024, iBc=(0, 0, 0), time_period=time_period)
 
 
-def layered_stack_case(L=(768, 768, 256), pml=12, layers=12, seed=2027, time_period=(5, 100000, 100010)):
+def layered_stack_case(L=(768, 768, 256), pml=12, layers=12, seed=2027, time_period=(5, 100000, 100010),
+                       progress=None):
     """The large-grid workload: the headline's curvilinear scale factors on an
     L0 x L1 x L2 grid, a seeded stack of ``layers`` dielectric z-layers (eps in [1, 12.1])
     with a PEC grating in its middle plane, ``pml``-cell PML on every face (the reference
@@ -329,7 +336,8 @@ def layered_stack_case(L=(768, 768, 256), pml=12, layers=12, seed=2027, time_per
     L0, L1, L2 = L
     return waveguide_case(L0, L1, L2, outer=max(L2 // 8, pml + 4), pml_xy=pml, pml_z=pml, omega=2 * np.pi / 20.0,
                           eps_core=12.1, eps_clad=2.1, core=(L0 // 8, L1 // 10), scale_amp=0.3, n_modes=8,
-                          seed=seed, iBc=(0, 0, 0), time_period=time_period, layers=layers, grating=True)
+                          seed=seed, iBc=(0, 0, 0), time_period=time_period, layers=layers, grating=True,
+                          progress=progress)
 
 
 def adjoint_case(L=(192, 192, 256), pml=10, design_frac=0.5, seed=2026, eps_range=(2.1, 12.1),
@@ -382,12 +390,16 @@ class Prepared:
         return float(np.count_nonzero(self.Cont != ord("n"))) / self.Cont.size
 
 
-def prepare(case: Case) -> Prepared:
-    """set_sigma, remove_sigma, set_address, set_tau (F:63-104) on the case."""
+def prepare(case: Case, progress=None) -> Prepared:
+    """set_sigma, remove_sigma, set_address, set_tau (F:63-104) on the case; ``progress``
+    as in waveguide_case."""
+    say = progress if progress is not None else (lambda msg: None)
     L = case.L
     sigma = fdtd.set_sigma(L, case.mu, case.eps)
     mu, eps = fdtd.remove_sigma(L, case.mu.copy(), case.eps.copy(), sigma)
+    say("set_sigma, remove_sigma")
     cont, addr, lnotn = fdtd.set_address(L, sigma, mu, eps)
+    say("set_address")
     N, iR, iI, dt = fdtd.set_tau(case.omega, case.min_dt / case.dt_factor)
     return Prepared(case=case, sigma=sigma, mu=mu, eps=eps, Cont=cont, addr=addr, Lnotn=lnotn,
                     N=N, invtau_R=iR, invtau_I=iI, dt=dt)
