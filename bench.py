@@ -119,14 +119,14 @@ def time_plan(plan, steps, warmup, torch):
     return (t1 - t0) * 1e3, ev0.elapsed_time(ev1)
 
 
-def batch_run(count, steps, torch, small=False):
-    """Independent runs of seeded sizes (L0, L1 in [64, 192], L2 in [64, 256], dielectric
-    boxes, 8-cell PML on every face), FP32, each plan autotuned.  After the envelope's
-    ramp (untimed), `steps` steps of all plans are timed three ways: smx_plan_run_batch
-    (one captured HIP graph of every plan's segment, replayed), the same batch with
-    every step launched from the host (streams mode), and the plans one after another."""
+def batch_run(count, steps, torch, small=False, dims=((64, 192), (64, 192), (64, 256)), pml=8, seed=2025):
+    """Independent runs of seeded sizes (L0, L1, L2 uniform in `dims`, dielectric boxes,
+    `pml`-cell PML on every face), FP32, each plan autotuned.  After the envelope's ramp
+    (untimed), `steps` steps of all plans are timed three ways: smx_plan_run_batch (one
+    captured HIP graph of every plan's segment, replayed), the same batch with every step
+    launched from the host (streams mode), and the plans one after another."""
     from smatranem_amd import casegen, fdtd
-    cases = casegen.batch_cases(count, seed=2025)
+    cases = casegen.batch_cases(count, seed=seed, pml=pml, dims=dims)
     if small:
         cases = [casegen.waveguide_case(24 + 8 * i, 20, 40, outer=8, pml_xy=4, pml_z=4, seed=i, blocks=2,
                                         time_period=(5, 100000, 100010)) for i in range(count)]
@@ -165,8 +165,9 @@ def batch_run(count, steps, torch, small=False):
             p.close()
     cells = sum(c.n for c in cases)
     alg = sum(c.n * bytes_per_cell(pr.pml_fraction, 4, 4) for c, pr in zip(cases, preps))
-    return dict(workload="%d independent runs, L0, L1 in [64, 192], L2 in [64, 256], dielectric boxes, "
-                         "8-cell PML on all faces, FP32, smx_plan_run_batch (HIP graph replay)" % count,
+    return dict(workload="%d independent runs, L0 in [%d, %d], L1 in [%d, %d], L2 in [%d, %d], dielectric boxes, "
+                         "%d-cell PML on all faces, FP32, smx_plan_run_batch (HIP graph replay)"
+                         % (count, *dims[0], *dims[1], *dims[2], pml),
                 grids=[list(c.L[:3]) for c in cases], cells=cells, steps=steps, first_step=warm,
                 value=round(cells * steps / t_graph / 1e9, 4), unit="Gcell-updates/s",
                 roofline_frac=round(alg * steps / t_graph / 1e9 / HBM_PEAK_GBS, 4),
@@ -497,12 +498,18 @@ def main():
                    roofline_frac=round(bb * bcase.n / (evb / kb * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
         print(f"[bench] box-PML fp32: {box['value']:.2f} Gcell/s", file=sys.stderr, flush=True)
 
-    batch = None
+    batch = batch_small = None
     if not args.no_batch:
         batch = batch_run(args.batch_count, args.batch_steps, torch, args.small)
         print(f"[bench] batch of {args.batch_count}: {batch['value']:.2f} Gcell/s graph replay "
               f"({batch['graph_launches']} launches), {batch['streams_mode']:.2f} streams, "
               f"{batch['one_after_another']:.2f} one after another", file=sys.stderr, flush=True)
+        # small grids, where launching every step from the host is the cost
+        batch_small = batch_run(args.batch_count, 4 * args.batch_steps, torch, args.small,
+                                dims=((24, 48), (24, 48), (32, 64)), pml=4, seed=2026)
+        print(f"[bench] batch of {args.batch_count} small grids: {batch_small['value']:.2f} Gcell/s graph replay "
+              f"({batch_small['graph_launches']} launches), {batch_small['streams_mode']:.2f} streams, "
+              f"{batch_small['one_after_another']:.2f} one after another", file=sys.stderr, flush=True)
 
     large = None
     if args.large:
@@ -555,6 +562,7 @@ def main():
         "err_vs_fp64": err,
         "pml_all_faces": box,
         "batch": batch,
+        "batch_small_grids": batch_small,
         "adjoint": adjoint,
         "large_grid": large,
         "cpu_baseline": cpu,

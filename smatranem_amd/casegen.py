@@ -288,15 +288,16 @@ def waveguide_case(L0, L1, L2, outer, pml_xy, pml_z, *, omega=2 * np.pi / 20.0, 
                 meta=dict(pec_edges=int(3 * pec_mask.sum()), seed=seed))
 
 
-def batch_cases(count, seed=0, pml=8, time_period=(5, 100000, 100010)):
-    """``count`` independent runs of seeded sizes: L0, L1 uniform in [64, 192], L2 in
-    [64, 256], a waveguide core with 6 seeded dielectric boxes (eps uniform in [1, 12.1]),
-    ``pml``-cell PML on every face, modal sheet source."""
+def batch_cases(count, seed=0, pml=8, time_period=(5, 100000, 100010), dims=((64, 192), (64, 192), (64, 256))):
+    """``count`` independent runs of seeded sizes: L0, L1, L2 uniform in the ranges ``dims``
+    (by default [64, 192], [64, 192], [64, 256]), a waveguide core with 6 seeded dielectric
+    boxes (eps uniform in [1, 12.1]), ``pml``-cell PML on every face, modal sheet source."""
     rng = np.random.default_rng(seed)
     out = []
     for i in range(count):
-        L0, L1 = (int(v) for v in rng.integers(64, 193, size=2))
-        L2 = int(rng.integers(64, 257))
+        L0 = int(rng.integers(dims[0][0], dims[0][1] + 1))
+        L1 = int(rng.integers(dims[1][0], dims[1][1] + 1))
+        L2 = int(rng.integers(dims[2][0], dims[2][1] + 1))
         out.append(waveguide_case(L0, L1, L2, outer=max(L2 // 8, pml + 4), pml_xy=pml, pml_z=pml,
                                   omega=2 * np.pi / 20.0, core=(max(L0 // 8, 2), max(L1 // 10, 2)),
                                   seed=seed * 1000 + i, blocks=6, time_period=time_period))
