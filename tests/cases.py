@@ -77,3 +77,68 @@ def rel_l2(a, b):
     b = np.asarray(b)
     den = np.linalg.norm(b)
     return float(np.linalg.norm(a - b) / (den if den > 0 else 1.0))
+
+
+# z column with periodic x/y wrap, a uniform E_x/H_y sheet source and z PML at both ends:
+# the fields stay uniform in x and y and the scheme is the 1-D Yee recurrence.
+COLUMN_OMEGA = 2 * np.pi / 20.0       # 20 cells per vacuum wavelength
+COLUMN_PML = 32
+COLUMN_L2 = 224
+COLUMN_UP = COLUMN_PML + 8            # reflection plane, upstream of the source
+COLUMN_SRC = COLUMN_UP + 2            # source plane
+COLUMN_DOWN = COLUMN_PML + 18         # first plane downstream of the source
+COLUMN_SLAB = (COLUMN_PML + 38, COLUMN_PML + 45)   # slab planes [a, b)
+COLUMN_TOP = COLUMN_L2 - COLUMN_PML - 20           # transmission plane
+
+
+def column_case(eps_slab=None, iB=COLUMN_UP, iT=COLUMN_TOP, L01=2, time_period=(4, 4, 30)):
+    """A Case on an L01 x L01 x 224 column; eps_slab (if given) fills every component of
+    the slab planes.  PML: 32 cells, cubic, R = 1e-12 (its discrete reflection is about
+    1e-6 at 20 cells per wavelength; 16 cells with R = 1e-8 give about 1e-4)."""
+    from smatranem_amd import casegen
+    L0 = L1 = L01
+    L2 = COLUMN_L2
+    eps3 = np.ones((L2, L1, L0, 3), dtype=np.complex128)
+    if eps_slab is not None:
+        eps3[COLUMN_SLAB[0]:COLUMN_SLAB[1]] = eps_slab
+    mu3 = np.ones_like(eps3)
+    min_dt = casegen.cfl_min_dt(mu3.real.reshape(-1), eps3.real.reshape(-1))
+    sig_z = casegen.pml_profile(L2, COLUMN_PML, COLUMN_PML, COLUMN_OMEGA, R=1e-12)
+    mu_s, eps_s = casegen.apply_stretch((L0, L1, L2), mu3.reshape(-1), eps3.reshape(-1),
+                                        (np.zeros((L0, 2)), np.zeros((L1, 2)), sig_z))
+    Hinc, Einc = casegen.sheet_source(L0, L1, np.ones(L0 * L1))
+    return casegen.Case(L=(L0, L1, L2, 1), mu=mu_s, eps=eps_s, omega=COLUMN_OMEGA, min_dt=min_dt,
+                        iBc=(1, 1, 0), iB=iB, iT=iT, iI=COLUMN_SRC, inc_mode=1,
+                        time_period=tuple(time_period), Hinc=Hinc, Einc=Einc)
+
+
+def column_theta(eps=1.0):
+    """Discrete propagation constant per cell: 4 sin^2(theta/2) = omega^2 eps (mu = 1)."""
+    return 2.0 * np.arcsin(COLUMN_OMEGA * np.sqrt(eps) / 2.0)
+
+
+def column_exact_slab(eps_slab):
+    """(r, t) of E[k] = e^{i th k} + r e^{-i th k} left of the slab and t e^{i th k} right
+    of it, from E[k+1] + E[k-1] = (2 - omega^2 eps_k) E[k] marched leftwards."""
+    th = column_theta()
+    a, b = COLUMN_SLAB
+    E = {j: np.exp(1j * th * j) for j in (b, b + 1)}
+    for j in range(b, a - 2, -1):
+        eps_j = eps_slab if a <= j < b else 1.0
+        E[j - 1] = (2.0 - COLUMN_OMEGA ** 2 * eps_j) * E[j] - E[j + 1]
+    # E[k] = al e^{i th k} + be e^{-i th k} on two vacuum planes left of the slab
+    p, q = a - 2, a - 1
+    M = np.array([[np.exp(1j * th * p), np.exp(-1j * th * p)],
+                  [np.exp(1j * th * q), np.exp(-1j * th * q)]])
+    al, be = np.linalg.solve(M, np.array([E[p], E[q]]))
+    return be / al, 1.0 / al
+
+
+def column_r_t(vacuum, slab):
+    """(r, t) from the E_x phasors (at COLUMN_UP, at COLUMN_TOP) of a vacuum run and a
+    slab run with the same source.  The vacuum run gives the incident wave and the
+    source's backward leak, which is the same in both runs and cancels."""
+    th = column_theta()
+    (eb0, et0), (eb1, et1) = vacuum, slab
+    inc0 = et0 * np.exp(-1j * th * COLUMN_TOP)          # incident amplitude at plane 0
+    return (eb1 - eb0) * np.exp(1j * th * COLUMN_UP) / inc0, et1 / et0
