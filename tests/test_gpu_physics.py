@@ -4,9 +4,10 @@ FP64: the monitor phasors are bit for bit the oracle's, so the oracle's pins car
 FP32: every kernel family (the two sweeps, the fused staged kernel with split slots, the
 two-step pass with split slots, and that pass writing the multi-frequency DFT) reproduces
 the exact discrete solution -- dispersion between two planes, the reflection and
-transmission of lossless and lossy slabs, |r|^2 + |t|^2 = 1 -- within 2e-4, which
-leaves room for the FP32 rounding of the fields over 1,050 steps on top of the 2.2e-6
-of the PML and the run's end (each test prints its residues)."""
+transmission of lossless and lossy slabs, |r|^2 + |t|^2 = 1 -- within 5e-5.  Measured on
+an MI355X, the FP32 residues are 1.1e-7 to 2.3e-6 for every family: the 2.2e-6 of the
+PML and the run's end, as in FP64, with the FP32 rounding far below it (each test
+prints its residues)."""
 import numpy as np
 import pytest
 
@@ -17,7 +18,7 @@ from tests import cases as C
 pytestmark = pytest.mark.gpu
 
 L01 = 4
-F32_BOUND = 2e-4
+F32_BOUND = 5e-5
 
 # environment of each FP32 kernel family
 KERNELS = {
