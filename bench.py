@@ -362,6 +362,7 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint gradient workload")
     ap.add_argument("--no-dft", action="store_true", help="leave the 16-frequency port DFT off")
+    ap.add_argument("--dft-chunk", type=int, default=256, help="steps between folds of the DFT samples (1..256)")
     ap.add_argument("--large", type=int, default=0, metavar="STEPS",
                     help="also run the 768x768xDEPTH layered-stack grid for STEPS steps (FP32, and FP64 up to "
                          "200 M cells)")
@@ -407,6 +408,7 @@ def main():
     # 16 frequencies around the source's on the two port planes (FP64 accumulators)
     dft_planes = [] if args.no_dft else [case.iB, case.iT]
     dft_w = [float(x) for x in 2 * np.pi / prep.N * np.linspace(0.85, 1.15, 16)]
+    dft_chunk = args.dft_chunk
     setup_s = time.perf_counter() - t_setup
     print(f"[bench] grid {case.L[:3]} n={n} N_2pi={prep.N} phi={phi:.4f} setup {setup_s:.1f}s",
           file=sys.stderr, flush=True)
@@ -415,7 +417,7 @@ def main():
     # FP32 path: the autotuner picks the step kernel (fused tile shape or the two sweeps)
     with prep.plan(fdtd.F32) as p32:
         if dft_planes:
-            p32.set_dft(dft_planes, dft_w, chunk=64)
+            p32.set_dft(dft_planes, dft_w, chunk=dft_chunk)
         choice = p32.autotune(20)
         fx, fy, fz, variant = choice
         step_kernels = p32.kernels.split(" + ")
@@ -440,7 +442,7 @@ def main():
     if not args.no_fp64:
         with prep.plan(fdtd.F64) as p64:
             if dft_planes:
-                p64.set_dft(dft_planes, dft_w, chunk=64)
+                p64.set_dft(dft_planes, dft_w, chunk=dft_chunk)
             wall64, ev64 = time_plan(p64, K, W, torch)
             H64, E64 = p64.read_fields()
             m64 = p64.read_monitors()
@@ -472,7 +474,7 @@ def main():
                 for f in range(len(dft_w)):
                     s32, s64 = overlap(prep, *phasors(d32, m, f)), overlap(prep, *phasors(d64, m, f))
                     dft_err.append((abs(abs(s32) - abs(s64)), abs(s64)))
-            S["dft_16_frequencies"] = dict(planes=dft_planes, w_per_step=[round(x, 6) for x in dft_w],
+            S["dft_16_frequencies"] = dict(planes=dft_planes, w_per_step=[round(x, 6) for x in dft_w], chunk=dft_chunk,
                                            abs_err=max(e for e, _ in dft_err),
                                            abs_fp64_max=max(r for _, r in dft_err),
                                            err_rel_to_max=max(e for e, _ in dft_err) /

@@ -186,7 +186,7 @@ int smx_plan_sync(smx_plan *plan);
  * smx_plan_run_steps, or by successive smx_plan_step calls) adds
  * e^{i*w*(k+1)}*(Hx, Hy) and e^{i*w*(k+1.5)}*(Ex, Ey) of the main slots (the total
  * field in PML cells); with w = 2*pi/N_2pi this is the reference's monitor.  Samples
- * are buffered and folded into FP64 accumulators every `chunk` (1..64) steps; FP32 plans
+ * are buffered and folded into FP64 accumulators every `chunk` (1..256) steps; FP32 plans
  * running two steps per pass take the samples of both steps inside the pass (chunk >= 2).
  * nplanes = 0 switches the DFT off.  No counterpart in the reference.
  */

@@ -76,42 +76,50 @@ __global__ __launch_bounds__(kThreads) void sample_planes_f64(smx_dft_sample_arg
  * h_v = 0 for H, 1/2 for E, computed as rot_f * sum_t tab[h_v][t][f] * series[t] with
  * tab = e^{i*w_f*(t + 1 + h)} (the plan's table, the same for every chunk) and
  * rot_f = e^{i*w_f*k0}.  One thread per (m, v, q), all frequencies in one pass over the
- * chunk (16 at a time); the table is copied to LDS once per block.
+ * chunk (16 at a time).  Grid: x over q, y over (m, v), so the table row a wave reads
+ * is the same in every lane: it is read through the scalar cache (constant address
+ * space), and its rows are padded with zeros to a multiple of 16 frequencies, so the
+ * inner loop has no per-frequency test.  The samples are loaded 8 slots ahead of use.
  */
+typedef const double2 __attribute__((address_space(4))) *const_table_ptr;
+
+template <typename T>
+__device__ inline void fold_slot(double (&re)[16], double (&im)[16], const_table_ptr p, double x)
+{
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        re[j] += p[j].x * x;
+        im[j] += p[j].y * x;
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kThreads) void fold_series(smx_dft_fold_args a)
 {
-    extern __shared__ double2 ph[]; /* [h][t][f], 2 * count * nfreq entries */
-    const int F = a.nfreq, nt = a.count;
-    for (int i = threadIdx.x; i < 2 * nt * F; i += kThreads) {
-        const int h = i / (nt * F), r = i % (nt * F);
-        ph[i] = a.table[h * a.table_nt * F + r];
-    }
-    __syncthreads();
-    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    const int F = a.nfreq, nt = a.count, FR = smx_dft_table_row(a.nfreq);
     const int64_t per = a.plane_cells;
-    if (i >= per * 4 * a.nplanes) return;
-    const int64_t mv = i / per, q = i - mv * per; /* mv = m*4 + v */
-    const int h = (int)(mv % 4) >= 2;
+    const int64_t q = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (q >= per) return;
+    const int mv = blockIdx.y; /* m*4 + v */
+    const int h = (mv % 4) >= 2;
     const T *src = (const T *)a.series + mv * per + q;
     const int64_t tstride = (int64_t)a.nplanes * 4 * per;
-    double *acc = a.acc + (mv * F) * 2 * per + 2 * q;
-    const double2 *tab = ph + h * nt * F;
+    double *acc = a.acc + ((int64_t)mv * F) * 2 * per + 2 * q;
+    const const_table_ptr tab = (const_table_ptr)a.table + h * a.table_nt * FR;
+    constexpr int kAhead = 8;
+    const int nt8 = nt - nt % kAhead;
     for (int f0 = 0; f0 < F; f0 += 16) {
         double re[16], im[16];
 #pragma unroll
         for (int j = 0; j < 16; j++) re[j] = im[j] = 0.;
-        for (int t = 0; t < nt; t++) {
-            const double x = (double)src[t * tstride];
-            const double2 *p = tab + t * F + f0;
+        for (int t0 = 0; t0 < nt8; t0 += kAhead) {
+            T xs[kAhead];
 #pragma unroll
-            for (int j = 0; j < 16; j++) {
-                if (f0 + j < F) {
-                    re[j] += p[j].x * x;
-                    im[j] += p[j].y * x;
-                }
-            }
+            for (int u = 0; u < kAhead; u++) xs[u] = src[(int64_t)(t0 + u) * tstride];
+#pragma unroll
+            for (int u = 0; u < kAhead; u++) fold_slot<T>(re, im, tab + (t0 + u) * FR + f0, (double)xs[u]);
         }
+        for (int t = nt8; t < nt; t++) fold_slot<T>(re, im, tab + t * FR + f0, (double)src[(int64_t)t * tstride]);
 #pragma unroll
         for (int j = 0; j < 16; j++) {
             if (f0 + j < F) {
@@ -128,13 +136,14 @@ __global__ __launch_bounds__(kThreads) void fold_series(smx_dft_fold_args a)
 
 void smx_dft_table(int nfreq, const double *w, int chunk, double *tab)
 {
+    const int row = smx_dft_table_row(nfreq);
     for (int h = 0; h < 2; h++)
         for (int t = 0; t < chunk; t++)
-            for (int f = 0; f < nfreq; f++) {
-                const double arg = w[f] * ((double)(t + 1) + 0.5 * h);
-                double *o = tab + 2 * (((size_t)h * chunk + t) * nfreq + f);
-                o[0] = std::cos(arg);
-                o[1] = std::sin(arg);
+            for (int f = 0; f < row; f++) {
+                const double arg = f < nfreq ? w[f] * ((double)(t + 1) + 0.5 * h) : 0.;
+                double *o = tab + 2 * (((size_t)h * chunk + t) * row + f);
+                o[0] = f < nfreq ? std::cos(arg) : 0.;
+                o[1] = f < nfreq ? std::sin(arg) : 0.;
             }
 }
 
@@ -150,10 +159,9 @@ extern "C" hipError_t smx_dft_sample(const smx_dft_sample_args *a, int slot, hip
 extern "C" hipError_t smx_dft_fold(const smx_dft_fold_args *a, int f64, hipStream_t stream)
 {
     if (a->count <= 0) return hipSuccess;
-    const int64_t total = a->plane_cells * 4 * a->nplanes;
-    const dim3 grid((unsigned)((total + kThreads - 1) / kThreads)), block(kThreads);
-    const size_t lds = sizeof(double2) * 2 * (size_t)a->count * (size_t)a->nfreq; /* <= 64 KiB */
-    if (f64) hipLaunchKernelGGL(fold_series<double>, grid, block, lds, stream, *a);
-    else hipLaunchKernelGGL(fold_series<float>, grid, block, lds, stream, *a);
+    const dim3 grid((unsigned)((a->plane_cells + kThreads - 1) / kThreads), (unsigned)(4 * a->nplanes)),
+        block(kThreads);
+    if (f64) hipLaunchKernelGGL(fold_series<double>, grid, block, 0, stream, *a);
+    else hipLaunchKernelGGL(fold_series<float>, grid, block, 0, stream, *a);
     return hipGetLastError();
 }

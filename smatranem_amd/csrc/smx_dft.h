@@ -8,7 +8,7 @@
 
 #define SMX_DFT_MAX_PLANES 8
 #define SMX_DFT_MAX_FREQ 32
-#define SMX_DFT_MAX_CHUNK 64
+#define SMX_DFT_MAX_CHUNK 256
 
 struct smx_dft_sample_args {
     int L0, L1, L2;
@@ -28,13 +28,18 @@ struct smx_dft_fold_args {
     int nplanes, nfreq, count;
     int64_t plane_cells;
     int table_nt;                   /* chunk length the table was built for (>= count) */
-    const double2 *table;           /* [2 (H, E)][table_nt][nfreq]: e^{i*w_f*(t + 1 + h/2)} */
+    const double2 *table;           /* [2 (H, E)][table_nt][row]: e^{i*w_f*(t + 1 + h/2)}, zero for f >= nfreq */
     double rot[SMX_DFT_MAX_FREQ][2]; /* e^{i*w_f*k0}, k0 = run step index of ring slot 0 */
     const void *series;
     double *acc;                    /* [plane][4][freq][L0*L1] complex */
 };
 
-/* The phase table of a plan's DFT (host): tab[h][t][f] = e^{i*w_f*(t + 1 + h/2)}, t < chunk. */
+/* Entries per row of the phase table: nfreq rounded up to a multiple of 16. */
+__host__ __device__ inline int smx_dft_table_row(int nfreq) { return (nfreq + 15) / 16 * 16; }
+
+/* The phase table of a plan's DFT (host): tab[h][t][f] = e^{i*w_f*(t + 1 + h/2)} for
+ * t < chunk, f < nfreq, and 0 for nfreq <= f < smx_dft_table_row(nfreq);
+ * 4 * chunk * smx_dft_table_row(nfreq) doubles. */
 void smx_dft_table(int nfreq, const double *w, int chunk, double *tab);
 
 extern "C" {

@@ -81,7 +81,7 @@ struct smx_plan {
     long long dft_k0 = 0;
     void *dft_series = nullptr;
     double *dft_acc = nullptr;
-    double *dft_table = nullptr; /* [2][chunk][nfreq] complex phases (kern_dft.hip) */
+    double *dft_table = nullptr; /* [2][chunk][row] complex phases (smx_dft_table) */
 
     /* adjoint (kern_adjoint.hip): checkpoints of the forward run, adjoint fields */
     long long adj_K = 0;
@@ -981,7 +981,7 @@ int smx_plan_set_dft(smx_plan *p, int nplanes, const int *planes, int nfreq, con
     if (nplanes < 0 || nplanes > SMX_DFT_MAX_PLANES) return arg_fail("DFT planes: 0..8");
     if (nplanes > 0 && (!planes || !w_per_step)) return arg_fail("null pointer");
     if (nplanes > 0 && (nfreq < 1 || nfreq > SMX_DFT_MAX_FREQ)) return arg_fail("DFT frequencies: 1..32");
-    if (nplanes > 0 && (chunk < 1 || chunk > SMX_DFT_MAX_CHUNK)) return arg_fail("DFT chunk: 1..64");
+    if (nplanes > 0 && (chunk < 1 || chunk > SMX_DFT_MAX_CHUNK)) return arg_fail("DFT chunk: 1..256");
     for (int m = 0; m < nplanes; m++)
         if (planes[m] < 0 || planes[m] >= p->g.L[2]) return arg_fail("DFT plane outside the grid");
     SMX_HIP(hipStreamSynchronize(p->stream));
@@ -1006,7 +1006,7 @@ int smx_plan_set_dft(smx_plan *p, int nplanes, const int *planes, int nfreq, con
     std::memset(p->dft_w, 0, sizeof p->dft_w);
     std::memcpy(p->dft_planes, planes, sizeof(int) * nplanes);
     std::memcpy(p->dft_w, w_per_step, sizeof(double) * nfreq);
-    std::vector<double> tab(4 * (size_t)chunk * (size_t)nfreq);
+    std::vector<double> tab(4 * (size_t)chunk * (size_t)smx_dft_table_row(nfreq));
     smx_dft_table(nfreq, w_per_step, chunk, tab.data());
     SMX_HIP(hipMalloc(&p->dft_table, sizeof(double) * tab.size()));
     SMX_HIP(hipMemcpyAsync(p->dft_table, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice,
@@ -1117,7 +1117,7 @@ int adjoint_forward_two_step(smx_plan *p, long long nsteps, int ckpt_every, int 
 {
     const int64_t n = p->n;
     const dft_state saved = park_dft(p);
-    int rc = smx_plan_set_dft(p, 1, &zT, 1, &w, SMX_DFT_MAX_CHUNK);
+    int rc = smx_plan_set_dft(p, 1, &zT, 1, &w, 64);
     for (long long k = 0; k < nsteps && rc == SMX_OK;) {
         if (k % ckpt_every == 0) {
             float *dst = p->adj_ckpt + (size_t)(k / ckpt_every) * 12 * (size_t)n;

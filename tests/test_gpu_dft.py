@@ -16,7 +16,7 @@ def _prep():
     return casegen.prepare(small_waveguide(seed=6, **CASE))
 
 
-@pytest.mark.parametrize("chunk", [1, 7, 32, 64])
+@pytest.mark.parametrize("chunk", [1, 7, 32, 64, 200, 256])
 def test_dft_at_omega_equals_reference_monitors(chunk):
     prep = _prep()
     c = prep.case
@@ -89,7 +89,7 @@ def test_dft_multifrequency_vs_direct(prec, tol, monkeypatch):
 def test_dft_argument_checks():
     prep = _prep()
     with prep.plan(fdtd.F32) as p:
-        for planes, ws, chunk in (([prep.case.L[2]], [0.1], 4), ([0], [0.1] * 40, 4), ([0], [0.1], 0)):
+        for planes, ws, chunk in (([prep.case.L[2]], [0.1], 4), ([0], [0.1] * 40, 4), ([0], [0.1], 0), ([0], [0.1], 257)):
             with pytest.raises(fdtd.SmxError):
                 p.set_dft(planes, ws, chunk)
         p._dft = (1, 1)
@@ -101,7 +101,7 @@ LOSSLESS = dict(pml_xy=0, pml_z=3, iBc=(1, -1, 0), metal_cells=0, lossy_cells=0)
 
 
 @pytest.mark.parametrize("two,chunk", [(None, 16), ("2,16", 16), ("2,16", 7), ("2,16", 64), ("2,8", 1), ("0,16", 16),
-                                       ("3,16", 16)])
+                                       ("3,16", 16), ("2,16", 30), (None, 256), ("2,16", 256)])
 def test_dft_with_two_step_passes(two, chunk, monkeypatch):
     """With the two-step kernel selected the DFT samples of both steps of a pass come
     from the kernel itself (split-slot variant) or the steps run one at a time (the
