@@ -188,6 +188,11 @@ int smx_plan_sync(smx_plan *plan);
  * field in PML cells); with w = 2*pi/N_2pi this is the reference's monitor.  Samples
  * are buffered and folded into FP64 accumulators every `chunk` (1..256) steps; FP32 plans
  * running two steps per pass take the samples of both steps inside the pass (chunk >= 2).
+ * While the DFT is on, an FP32 plan also buffers the monitor planes (up to 8 planes in
+ * all) and folds the reference monitors from those samples with the reference's phases
+ * instead of accumulating them per step (steps run by smx_plan_step, with the caller's
+ * ExpN2, still accumulate per step); smx_plan_read_monitors folds what is pending first.
+ * The monitors then differ from per-step accumulation only in the order of the sums.
  * nplanes = 0 switches the DFT off.  No counterpart in the reference.
  */
 int smx_plan_set_dft(smx_plan *plan, int nplanes, const int *planes, int nfreq, const double *w_per_step,

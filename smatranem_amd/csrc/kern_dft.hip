@@ -103,7 +103,7 @@ __global__ __launch_bounds__(kThreads) void fold_series(smx_dft_fold_args a)
     const int mv = blockIdx.y; /* m*4 + v */
     const int h = (mv % 4) >= 2;
     const T *src = (const T *)a.series + mv * per + q;
-    const int64_t tstride = (int64_t)a.nplanes * 4 * per;
+    const int64_t tstride = (int64_t)a.ring * 4 * per;
     double *acc = a.acc + ((int64_t)mv * F) * 2 * per + 2 * q;
     const const_table_ptr tab = (const_table_ptr)a.table + h * a.table_nt * FR;
     constexpr int kAhead = 8;
@@ -132,6 +132,33 @@ __global__ __launch_bounds__(kThreads) void fold_series(smx_dft_fold_args a)
     }
 }
 
+/* One thread per (monitor, component, q); grid y = 2 monitors x 4 components. */
+template <typename T>
+__global__ __launch_bounds__(kThreads) void fold_monitors(smx_dft_mon_args a)
+{
+    const int64_t per = a.plane_cells;
+    const int64_t q = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    const int m = blockIdx.y / 4, c = blockIdx.y % 4;
+    if (q >= per || a.mon[m] < 0) return;
+    const int h = c >= 2; /* E: the half-step phase */
+    const T *src = (const T *)a.series + ((int64_t)a.mon[m] * 4 + c) * per + q;
+    const int64_t tstride = (int64_t)a.ring * 4 * per;
+    const const_table_ptr ph = (const_table_ptr)a.expn + h;
+    double re = 0., im = 0.;
+    int j = a.j0;
+    for (int t = 0; t < a.count; t++) {
+        if ((a.mask[t / 64] >> (t % 64)) & 1u) {
+            const double x = (double)src[(int64_t)t * tstride];
+            re += ph[2 * j].x * x;
+            im += ph[2 * j].y * x;
+        }
+        if (++j == a.N) j = 0;
+    }
+    double *dst = a.monitors + (int64_t)(2 * m + h) * 4 * per + 2 * ((c % 2) + 2 * q);
+    dst[0] += re;
+    dst[1] += im;
+}
+
 } // namespace
 
 void smx_dft_table(int nfreq, const double *w, int chunk, double *tab)
@@ -153,6 +180,15 @@ extern "C" hipError_t smx_dft_sample(const smx_dft_sample_args *a, int slot, hip
     const dim3 grid((unsigned)((total + kThreads - 1) / kThreads)), block(kThreads);
     if (a->H64) hipLaunchKernelGGL(sample_planes_f64, grid, block, 0, stream, *a, slot);
     else hipLaunchKernelGGL(sample_planes_f32<float>, grid, block, 0, stream, *a, slot);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t smx_dft_fold_monitors(const smx_dft_mon_args *a, int f64, hipStream_t stream)
+{
+    if (a->count <= 0 || (a->mon[0] < 0 && a->mon[1] < 0)) return hipSuccess;
+    const dim3 grid((unsigned)((a->plane_cells + kThreads - 1) / kThreads), 8), block(kThreads);
+    if (f64) hipLaunchKernelGGL(fold_monitors<double>, grid, block, 0, stream, *a);
+    else hipLaunchKernelGGL(fold_monitors<float>, grid, block, 0, stream, *a);
     return hipGetLastError();
 }
 

@@ -25,13 +25,30 @@ struct smx_dft_sample_args {
 };
 
 struct smx_dft_fold_args {
-    int nplanes, nfreq, count;
+    int nplanes, nfreq, count;      /* folded planes: the first nplanes of each ring slot */
+    int ring;                       /* planes per ring slot (>= nplanes) */
     int64_t plane_cells;
     int table_nt;                   /* chunk length the table was built for (>= count) */
     const double2 *table;           /* [2 (H, E)][table_nt][row]: e^{i*w_f*(t + 1 + h/2)}, zero for f >= nfreq */
     double rot[SMX_DFT_MAX_FREQ][2]; /* e^{i*w_f*k0}, k0 = run step index of ring slot 0 */
     const void *series;
     double *acc;                    /* [plane][4][freq][L0*L1] complex */
+};
+
+/* The reference monitors (F:318-328) taken from the ring instead of per step: for the
+ * ring slots whose bit is set in mask, HB/HT += e^{i*dphi*j}*(Hx, Hy) and EB/ET +=
+ * e^{i*dphi*(j + 1/2)}*(Ex, Ey) of the monitor planes' samples, j = (run step + 1) mod N,
+ * with the phases of the reference's table (expn, F:93-99). */
+struct smx_dft_mon_args {
+    int ring;                       /* planes per ring slot */
+    int mon[2];                     /* ring plane of the bottom / top monitor; -1: none */
+    int count;                      /* slots to fold */
+    int N, j0;                      /* period of the table; j of slot 0 */
+    int64_t plane_cells;
+    uint64_t mask[SMX_DFT_MAX_CHUNK / 64];
+    const double *expn;             /* [N][4]: e^{i*dphi*j}, e^{i*dphi*(j + 1/2)} */
+    const void *series;
+    double *monitors;               /* HB, EB, HT, ET: complex[2*L0*L1] each, in this order */
 };
 
 /* Entries per row of the phase table: nfreq rounded up to a multiple of 16. */
@@ -45,4 +62,5 @@ void smx_dft_table(int nfreq, const double *w, int chunk, double *tab);
 extern "C" {
 hipError_t smx_dft_sample(const smx_dft_sample_args *a, int slot, hipStream_t stream);
 hipError_t smx_dft_fold(const smx_dft_fold_args *a, int f64, hipStream_t stream);
+hipError_t smx_dft_fold_monitors(const smx_dft_mon_args *a, int f64, hipStream_t stream);
 }

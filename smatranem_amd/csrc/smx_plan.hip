@@ -82,6 +82,16 @@ struct smx_plan {
     void *dft_series = nullptr;
     double *dft_acc = nullptr;
     double *dft_table = nullptr; /* [2][chunk][row] complex phases (smx_dft_table) */
+    /* FP32 plans with the DFT on take the reference monitors from the same ring: its slots
+     * hold the DFT planes, then the monitor planes not among them (dft_ring planes in
+     * all); dft_mon = ring plane of each monitor (-1: the kernels accumulate the
+     * monitors per step, as FP64 plans always do); a slot's monitors are folded at
+     * flush time when its bit in dft_mon_mask is set (steps run by smx_plan_step keep
+     * the caller's phases and accumulate in the kernel). */
+    int dft_ring = 0;
+    int dft_mon[2] = {-1, -1};
+    uint64_t dft_mon_mask[SMX_DFT_MAX_CHUNK / 64] = {0};
+    double *expn_dev = nullptr; /* the phase table on the device */
 
     /* adjoint (kern_adjoint.hip): checkpoints of the forward run, adjoint fields */
     long long adj_K = 0;
@@ -195,7 +205,7 @@ void free_plan(smx_plan *p)
     if (!p) return;
     if (p->stream) (void)hipStreamSynchronize(p->stream);
     void *bufs[] = {p->Ah, p->Bh, p->Ae, p->Be, p->H, p->E, p->Cont, p->addr, p->mon, p->inc, p->src,
-                    p->dft_series, p->dft_acc, p->dft_table, p->adj_ckpt, p->adj_D};
+                    p->dft_series, p->dft_acc, p->dft_table, p->adj_ckpt, p->adj_D, p->expn_dev};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     smx_f32_free(&p->f32);
@@ -282,15 +292,40 @@ int dft_flush(smx_plan *p)
         f.rot[i][0] = std::cos(arg);
         f.rot[i][1] = std::sin(arg);
     }
+    f.ring = p->dft_ring;
     f.series = p->dft_series;
     f.acc = p->dft_acc;
     SMX_HIP(smx_dft_fold(&f, p->g.precision == SMX_F64, p->stream));
+    bool any = false;
+    for (uint64_t w : p->dft_mon_mask) any |= w != 0;
+    if (any) {
+        smx_dft_mon_args m;
+        std::memset(&m, 0, sizeof m);
+        m.ring = p->dft_ring;
+        m.mon[0] = p->dft_mon[0];
+        m.mon[1] = p->dft_mon[1];
+        m.count = p->dft_fill;
+        m.N = p->N;
+        m.j0 = (int)((p->dft_k0 + 1) % p->N);
+        m.plane_cells = p->plane;
+        std::memcpy(m.mask, p->dft_mon_mask, sizeof m.mask);
+        m.expn = p->expn_dev;
+        m.series = p->dft_series;
+        m.monitors = p->mon;
+        SMX_HIP(smx_dft_fold_monitors(&m, p->g.precision == SMX_F64, p->stream));
+        std::memset(p->dft_mon_mask, 0, sizeof p->dft_mon_mask);
+    }
     p->dft_fill = 0;
     return SMX_OK;
 }
 
-/* Sample the DFT planes after run step k (steps must come in order within a chunk). */
-int dft_after_step(smx_plan *p, long long k)
+bool mon_ring(const smx_plan *p) { return p->dft_mon[0] >= 0 || p->dft_mon[1] >= 0; }
+
+void mark_mon_slot(smx_plan *p, int slot) { p->dft_mon_mask[slot / 64] |= (uint64_t)1 << (slot % 64); }
+
+/* Sample the ring's planes after run step k (steps must come in order within a chunk);
+ * mon: the ring carries this step's monitors. */
+int dft_after_step(smx_plan *p, long long k, bool mon)
 {
     if (!p->dft_nplanes) return SMX_OK;
     int rc;
@@ -301,7 +336,7 @@ int dft_after_step(smx_plan *p, long long k)
     a.L0 = p->g.L[0];
     a.L1 = p->g.L[1];
     a.L2 = p->g.L[2];
-    a.nplanes = p->dft_nplanes;
+    a.nplanes = p->dft_ring;
     std::memcpy(a.planes, p->dft_planes, sizeof a.planes);
     a.plane_cells = p->plane;
     if (p->g.precision == SMX_F64) {
@@ -319,11 +354,13 @@ int dft_after_step(smx_plan *p, long long k)
     }
     a.series = p->dft_series;
     SMX_HIP(smx_dft_sample(&a, p->dft_fill, p->stream));
+    if (mon) mark_mon_slot(p, p->dft_fill);
     if (++p->dft_fill == p->dft_chunk) return dft_flush(p);
     return SMX_OK;
 }
 
-int launch_step(smx_plan *p, const smx_step_scalars &s, bool explicit_src)
+/* One step; mon_kernel = false leaves the monitors to the ring (FP32 only). */
+int launch_step(smx_plan *p, const smx_step_scalars &s, bool explicit_src, bool mon_kernel = true)
 {
     if (p->g.precision == SMX_F64) {
         smx_f64_args a = f64_args(p);
@@ -337,6 +374,7 @@ int launch_step(smx_plan *p, const smx_step_scalars &s, bool explicit_src)
     }
     smx_f32_launch_args a{};
     a.g = make_geom(p);
+    if (!mon_kernel) a.g.iB = a.g.iT = -1;
     a.s = s;
     a.srcH_plane = explicit_src ? p->src : nullptr;
     a.srcE_plane = explicit_src ? p->src + 2 * p->plane : nullptr;
@@ -465,6 +503,10 @@ int smx_plan_read_fields(const smx_plan *p, double *H_t, double *E_t)
 int smx_plan_read_monitors(const smx_plan *p, double *HB, double *EB, double *HT, double *ET)
 {
     if (!p || !HB || !EB || !HT || !ET) return arg_fail("null pointer");
+    /* fold the ring's pending samples first: the monitors they carry belong to steps
+     * already run (folding early changes only the order of the sums) */
+    int rc = dft_flush(const_cast<smx_plan *>(p));
+    if (rc) return rc;
     double *dst[4] = {HB, EB, HT, ET};
     for (int i = 0; i < 4; i++)
         SMX_HIP(hipMemcpyAsync(dst[i], p->mon + 4 * p->plane * i, sizeof(double) * 4 * p->plane,
@@ -477,6 +519,8 @@ int smx_plan_write_monitors(smx_plan *p, const double *HB, const double *EB, con
                             const double *ET)
 {
     if (!p) return arg_fail("null pointer");
+    int rc = dft_flush(p); /* pending monitor samples belong to the values being replaced */
+    if (rc) return rc;
     const double *src[4] = {HB, EB, HT, ET};
     for (int i = 0; i < 4; i++) {
         double *d = p->mon + 4 * p->plane * i;
@@ -500,7 +544,7 @@ int smx_plan_step(smx_plan *p, const double *Hi, const double *Ei, const double 
     s.monE_re = ExpN2[2];
     s.monE_im = ExpN2[3];
     int rc = launch_step(p, s, true);
-    if (rc || (rc = dft_after_step(p, p->next_step))) return rc;
+    if (rc || (rc = dft_after_step(p, p->next_step, false))) return rc;
     p->next_step++;
     SMX_HIP(hipStreamSynchronize(p->stream));
     return SMX_OK;
@@ -514,19 +558,27 @@ int smx_plan_set_source(smx_plan *p, const double *Hinc, const double *Einc, int
     SMX_HIP(hipMemcpyAsync(p->inc, Hinc, sizeof(double) * 4 * p->plane, hipMemcpyHostToDevice, p->stream));
     SMX_HIP(hipMemcpyAsync(p->inc + 4 * p->plane, Einc, sizeof(double) * 4 * p->plane, hipMemcpyHostToDevice,
                            p->stream));
+    int rc = dft_flush(p); /* pending monitor samples use the old phase table */
+    if (rc) return rc;
     SMX_HIP(hipStreamSynchronize(p->stream));
     p->N = N_2pi;
     std::memcpy(p->tp, time_period, sizeof p->tp);
     p->expn.assign(4 * (size_t)N_2pi, 0.);
     smx_expn_table(N_2pi, p->expn.data());
+    if (p->expn_dev) SMX_HIP(hipFree(p->expn_dev));
+    p->expn_dev = nullptr;
+    SMX_HIP(hipMalloc(&p->expn_dev, sizeof(double) * p->expn.size()));
+    SMX_HIP(hipMemcpy(p->expn_dev, p->expn.data(), sizeof(double) * p->expn.size(), hipMemcpyHostToDevice));
     p->have_source = true;
     return SMX_OK;
 }
 
-static int launch_step2(smx_plan *p, const smx_step_scalars &s, const smx_step_scalars &s2, int dft_slot)
+static int launch_step2(smx_plan *p, const smx_step_scalars &s, const smx_step_scalars &s2, int dft_slot,
+                        bool mon_kernel)
 {
     smx_f32_launch_args a{};
     a.g = make_geom(p);
+    if (!mon_kernel) a.g.iB = a.g.iT = -1;
     a.s = s;
     a.s2 = s2;
     a.srcH_plane = nullptr;
@@ -539,9 +591,9 @@ static int launch_step2(smx_plan *p, const smx_step_scalars &s, const smx_step_s
     a.ET = p->mon + 12 * p->plane;
     if (dft_slot >= 0) {
         a.dft = (float *)p->dft_series;
-        a.dft_n = p->dft_nplanes;
+        a.dft_n = p->dft_ring;
         a.dft_slot = dft_slot;
-        std::memcpy(a.dft_planes, p->dft_planes, sizeof(int) * (size_t)p->dft_nplanes);
+        std::memcpy(a.dft_planes, p->dft_planes, sizeof(int) * (size_t)p->dft_ring);
     }
     SMX_HIP(smx_f32_step2(&p->f32, &a, p->stream));
     return SMX_OK;
@@ -591,8 +643,12 @@ static int advance(smx_plan *p, long long k, long long nleft, double *sums, int 
             *done = 2;
             int slot = -1;
             if (p->dft_nplanes && (rc = dft_pair_slot(p, k, &slot))) return rc;
-            if ((rc = launch_step2(p, s, s2, slot))) return rc;
+            if ((rc = launch_step2(p, s, s2, slot, !mon_ring(p)))) return rc;
             if (p->dft_nplanes) {
+                if (mon_ring(p)) {
+                    mark_mon_slot(p, slot);
+                    mark_mon_slot(p, slot + 1);
+                }
                 p->dft_fill += 2;
                 if (p->dft_fill == p->dft_chunk) return dft_flush(p);
             }
@@ -604,8 +660,8 @@ static int advance(smx_plan *p, long long k, long long nleft, double *sums, int 
         sums[1] += fE;
     }
     *done = 1;
-    if ((rc = launch_step(p, s, false))) return rc;
-    return dft_after_step(p, k);
+    if ((rc = launch_step(p, s, false, !mon_ring(p)))) return rc;
+    return dft_after_step(p, k, mon_ring(p));
 }
 
 int smx_plan_run_steps(smx_plan *p, long long step0, long long nsteps, double sums[2])
@@ -984,6 +1040,9 @@ int smx_plan_set_dft(smx_plan *p, int nplanes, const int *planes, int nfreq, con
     if (nplanes > 0 && (chunk < 1 || chunk > SMX_DFT_MAX_CHUNK)) return arg_fail("DFT chunk: 1..256");
     for (int m = 0; m < nplanes; m++)
         if (planes[m] < 0 || planes[m] >= p->g.L[2]) return arg_fail("DFT plane outside the grid");
+    /* samples still in the ring: their monitor contributions are kept */
+    int rc = dft_flush(p);
+    if (rc) return rc;
     SMX_HIP(hipStreamSynchronize(p->stream));
     if (p->dft_series) SMX_HIP(hipFree(p->dft_series));
     if (p->dft_acc) SMX_HIP(hipFree(p->dft_acc));
@@ -992,11 +1051,37 @@ int smx_plan_set_dft(smx_plan *p, int nplanes, const int *planes, int nfreq, con
     p->dft_acc = nullptr;
     p->dft_table = nullptr;
     p->dft_nplanes = 0;
+    p->dft_ring = 0;
     p->dft_fill = 0;
+    p->dft_mon[0] = p->dft_mon[1] = -1;
+    std::memset(p->dft_mon_mask, 0, sizeof p->dft_mon_mask);
     if (nplanes == 0) return SMX_OK;
+    /* ring planes: the DFT planes, then (FP32) the monitor planes not among them */
+    int ring[SMX_DFT_MAX_PLANES];
+    std::memcpy(ring, planes, sizeof(int) * (size_t)nplanes);
+    int nring = nplanes, mon[2] = {-1, -1};
+    bool fits = true;
+    for (int m = 0; m < 2 && p->g.precision == SMX_F32; m++) {
+        const int z = p->g.iBTIin[m];
+        if (z < 0) continue;
+        int r = 0;
+        while (r < nring && ring[r] != z) r++;
+        if (r == nring) {
+            if (nring == SMX_DFT_MAX_PLANES) {
+                fits = false;
+                break;
+            }
+            ring[nring++] = z;
+        }
+        mon[m] = r;
+    }
+    if (!fits) { /* no room for the monitor planes: the kernels keep accumulating them */
+        nring = nplanes;
+        mon[0] = mon[1] = -1;
+    }
     const size_t word = p->g.precision == SMX_F64 ? sizeof(double) : sizeof(float);
     const size_t cells = (size_t)nplanes * 4 * (size_t)p->plane;
-    SMX_HIP(hipMalloc(&p->dft_series, word * cells * (size_t)chunk));
+    SMX_HIP(hipMalloc(&p->dft_series, word * (size_t)nring * 4 * (size_t)p->plane * (size_t)chunk));
     SMX_HIP(hipMalloc(&p->dft_acc, sizeof(double) * 2 * cells * (size_t)nfreq));
     SMX_HIP(hipMemsetAsync(p->dft_acc, 0, sizeof(double) * 2 * cells * (size_t)nfreq, p->stream));
     p->dft_nplanes = nplanes;
@@ -1004,8 +1089,11 @@ int smx_plan_set_dft(smx_plan *p, int nplanes, const int *planes, int nfreq, con
     p->dft_chunk = chunk;
     std::memset(p->dft_planes, 0, sizeof p->dft_planes);
     std::memset(p->dft_w, 0, sizeof p->dft_w);
-    std::memcpy(p->dft_planes, planes, sizeof(int) * nplanes);
+    std::memcpy(p->dft_planes, ring, sizeof(int) * nring);
     std::memcpy(p->dft_w, w_per_step, sizeof(double) * nfreq);
+    p->dft_ring = nring;
+    p->dft_mon[0] = mon[0];
+    p->dft_mon[1] = mon[1];
     std::vector<double> tab(4 * (size_t)chunk * (size_t)smx_dft_table_row(nfreq));
     smx_dft_table(nfreq, w_per_step, chunk, tab.data());
     SMX_HIP(hipMalloc(&p->dft_table, sizeof(double) * tab.size()));
@@ -1070,7 +1158,9 @@ namespace {
 /* The DFT state of a plan (set by smx_plan_set_dft), parked while the adjoint forward
  * runs its own. */
 struct dft_state {
-    int nplanes, nfreq, chunk, fill;
+    int nplanes, nfreq, chunk, fill, ring;
+    int mon[2];
+    uint64_t mon_mask[SMX_DFT_MAX_CHUNK / 64];
     int planes[SMX_DFT_MAX_PLANES];
     double w[SMX_DFT_MAX_FREQ];
     long long k0;
@@ -1082,11 +1172,15 @@ dft_state park_dft(smx_plan *p)
 {
     dft_state d;
     d.nplanes = p->dft_nplanes, d.nfreq = p->dft_nfreq, d.chunk = p->dft_chunk, d.fill = p->dft_fill;
+    d.ring = p->dft_ring, d.mon[0] = p->dft_mon[0], d.mon[1] = p->dft_mon[1];
+    std::memcpy(d.mon_mask, p->dft_mon_mask, sizeof d.mon_mask);
     std::memcpy(d.planes, p->dft_planes, sizeof d.planes);
     std::memcpy(d.w, p->dft_w, sizeof d.w);
     d.k0 = p->dft_k0;
     d.series = p->dft_series, d.acc = p->dft_acc, d.table = p->dft_table;
-    p->dft_nplanes = 0, p->dft_fill = 0;
+    p->dft_nplanes = 0, p->dft_fill = 0, p->dft_ring = 0;
+    p->dft_mon[0] = p->dft_mon[1] = -1;
+    std::memset(p->dft_mon_mask, 0, sizeof p->dft_mon_mask);
     p->dft_series = nullptr, p->dft_acc = nullptr, p->dft_table = nullptr;
     return d;
 }
@@ -1094,6 +1188,8 @@ dft_state park_dft(smx_plan *p)
 void restore_dft(smx_plan *p, const dft_state &d)
 {
     p->dft_nplanes = d.nplanes, p->dft_nfreq = d.nfreq, p->dft_chunk = d.chunk, p->dft_fill = d.fill;
+    p->dft_ring = d.ring, p->dft_mon[0] = d.mon[0], p->dft_mon[1] = d.mon[1];
+    std::memcpy(p->dft_mon_mask, d.mon_mask, sizeof d.mon_mask);
     std::memcpy(p->dft_planes, d.planes, sizeof d.planes);
     std::memcpy(p->dft_w, d.w, sizeof d.w);
     p->dft_k0 = d.k0;
@@ -1367,10 +1463,10 @@ int smx_plan_autotune(smx_plan *p, int trial_steps, int choice[5])
             }
             int rc;
             for (int k = 0; k < 2; k++)
-                if ((rc = launch_step(p, s, false))) return rc;
+                if ((rc = launch_step(p, s, false, !mon_ring(p)))) return rc;
             SMX_HIP(hipEventRecord(e0, p->stream));
             for (int k = 0; k < trial_steps; k++)
-                if ((rc = launch_step(p, s, false))) return rc;
+                if ((rc = launch_step(p, s, false, !mon_ring(p)))) return rc;
             SMX_HIP(hipEventRecord(e1, p->stream));
             SMX_HIP(hipEventSynchronize(e1));
             float ms = 0.f;
@@ -1401,10 +1497,10 @@ int smx_plan_autotune(smx_plan *p, int trial_steps, int choice[5])
                 st.two_fz = fz;
                 st.two_bal = bal;
                 int rc;
-                if ((rc = launch_step2(p, s, s, -1))) return rc;
+                if ((rc = launch_step2(p, s, s, -1, !mon_ring(p)))) return rc;
                 SMX_HIP(hipEventRecord(e0, p->stream));
                 for (int k = 0; k < pairs; k++)
-                    if ((rc = launch_step2(p, s, s, -1))) return rc;
+                    if ((rc = launch_step2(p, s, s, -1, !mon_ring(p)))) return rc;
                 SMX_HIP(hipEventRecord(e1, p->stream));
                 SMX_HIP(hipEventSynchronize(e1));
                 float ms = 0.f;

@@ -129,3 +129,45 @@ def test_dft_with_two_step_passes(two, chunk, monkeypatch):
         p.run_steps(7, nsteps - 7)
         D = p.read_dft()
     assert rel_l2(D, ref) < 2e-4
+
+
+@pytest.mark.parametrize("kernel,planes", [
+    ({"SMX_F32_TWO": "2,16"}, "other"),          # two-step pass, monitor planes appended to the ring
+    ({"SMX_F32_TWO": "2,16"}, "monitors"),       # DFT planes are the monitor planes
+    ({"SMX_F32_TWO": "2,16"}, "one"),            # one monitor plane among the DFT planes
+    ({"SMX_F32_KERNEL": "fused", "SMX_F32_TILE": "3,16"}, "other"),   # one step per pass
+])
+def test_monitors_from_ring(kernel, planes, monkeypatch):
+    """FP32 plans with the DFT on take the reference monitors from the DFT ring, folded
+    with the reference's phases (e^{i dphi ((k+1) mod N)}, and + 1/2 for E): against the
+    same plan without the DFT, whose kernels accumulate them per step.  The run mixes
+    smx_plan_run_steps, one smx_plan_step with the caller's phases (accumulated in the
+    kernel, not folded twice), a read in the middle of a ring chunk, a write of the
+    monitors, and an odd step count; fields agree bit for bit, monitors to rounding."""
+    for k, v in kernel.items():
+        monkeypatch.setenv(k, v)
+    prep = casegen.prepare(small_waveguide(seed=8, **LOSSLESS))
+    c = prep.case
+    dft_planes = {"other": [c.iB + 1, c.iT - 1], "monitors": [c.iB, c.iT], "one": [c.iT, c.iB + 3]}[planes]
+    rng = np.random.default_rng(3)
+    Hi, Ei = rng.normal(size=2 * c.plane), rng.normal(size=2 * c.plane)
+    ExpN2 = np.exp(1j * np.array([0.3, 0.7]))
+    nsteps = 3 * prep.N + 5
+    out = []
+    for dft in (False, True):
+        with prep.plan(fdtd.F32) as p:
+            if dft:
+                p.set_dft(dft_planes, [0.2, 2 * np.pi / prep.N], chunk=12)
+            p.run_steps(0, 9)
+            p.step(Hi, Ei, ExpN2)                  # step 9 with the caller's phases
+            p.run_steps(10, 11)
+            mid = p.read_monitors()               # inside a ring chunk
+            p.write_monitors(*[m * 0.5 for m in mid])
+            p.run_steps(21, nsteps - 21)
+            out.append((p.read_fields(), mid, p.read_monitors(), p.kernels))
+    (F0, mid0, mon0, k0), (F1, mid1, mon1, k1) = out
+    print(k0, "|", k1, flush=True)
+    assert np.array_equal(F0[0], F1[0]) and np.array_equal(F0[1], F1[1])
+    for a, b in list(zip(mid0, mid1)) + list(zip(mon0, mon1)):
+        assert np.abs(a).max() > 0
+        assert np.abs(a - b).max() <= 1e-12 * np.abs(a).max(), np.abs(a - b).max() / np.abs(a).max()
