@@ -146,14 +146,24 @@ __global__ __launch_bounds__(kThreads) void fold_monitors(smx_dft_mon_args a)
     const const_table_ptr ph = (const_table_ptr)a.expn + h;
     double re = 0., im = 0.;
     int j = a.j0;
-    for (int t = 0; t < a.count; t++) {
+    auto add = [&](int t, double x) {
         if ((a.mask[t / 64] >> (t % 64)) & 1u) {
-            const double x = (double)src[(int64_t)t * tstride];
             re += ph[2 * j].x * x;
             im += ph[2 * j].y * x;
         }
         if (++j == a.N) j = 0;
+    };
+    /* 16 slots loaded before any is used: one memory latency per 16 slots */
+    constexpr int kAhead = 16;
+    const int n16 = a.count - a.count % kAhead;
+    for (int t0 = 0; t0 < n16; t0 += kAhead) {
+        T xs[kAhead];
+#pragma unroll
+        for (int u = 0; u < kAhead; u++) xs[u] = src[(int64_t)(t0 + u) * tstride];
+#pragma unroll
+        for (int u = 0; u < kAhead; u++) add(t0 + u, (double)xs[u]);
     }
+    for (int t = n16; t < a.count; t++) add(t, (double)src[(int64_t)t * tstride]);
     double *dst = a.monitors + (int64_t)(2 * m + h) * 4 * per + 2 * ((c % 2) + 2 * q);
     dst[0] += re;
     dst[1] += im;
