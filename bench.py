@@ -176,14 +176,17 @@ def batch_run(count, steps, torch, small=False, dims=((64, 192), (64, 192), (64,
                 one_after_another=round(cells * steps / t_seq / 1e9, 4))
 
 
-def adjoint_run(torch, small=False, fd_cells=0):
+def adjoint_run(torch, small=False, fd_cells=0, fd_seed=None):
     """The gradient workload: a 96 x 96 x 128 design region of per-cell eps in [2.1, 12.1]
     inside a 192 x 192 x 256 grid with 10-cell PML on every face; forward 4000 FP32 steps
     with the full state checkpointed every 200 steps (smx_adjoint_forward), then the
     reverse-time adjoint with recomputation (smx_adjoint_backward).  Objective: DFT power
     of the tangential E at the source frequency on a plane 3 cells below the top PML.
-    fd_cells > 0 also checks dJ/d eps at that many of the largest-gradient cells against
-    central differences (delta eps = 1e-3) of the FP64 forward."""
+    fd_cells > 0 also checks dJ/d eps at that many cells against central differences
+    (delta eps = 1e-3) of the FP64 forward: the largest-gradient cells, or with fd_seed
+    cells (and components) drawn uniformly from the design box with that seed.  Each
+    check reports the error relative to the cell's FD value and to the largest |dJ/d eps|
+    (FP32 rounding sets a floor on the second)."""
     import numpy as np
     from smatranem_amd import casegen, fdtd
     if small:
@@ -223,7 +226,11 @@ def adjoint_run(torch, small=False, fd_cells=0):
                 D = q.read_dft()[0]
             return float(np.sum(np.abs(D[2:4, 0, :]) ** 2))
         x0, x1, y0, y1, z0, z1 = box
-        order = np.argsort(np.abs(g).reshape(-1))[::-1][:fd_cells]
+        if fd_seed is None:
+            order = np.argsort(np.abs(g).reshape(-1))[::-1][:fd_cells]
+        else:
+            order = np.random.default_rng(fd_seed).choice(g.size, size=fd_cells, replace=False)
+        gmax = float(np.abs(g).max())
         checks = []
         for idx in order:
             c, zz, yy, xx = (int(v) for v in np.unravel_index(idx, g.shape))
@@ -233,12 +240,15 @@ def adjoint_run(torch, small=False, fd_cells=0):
             ep[3 * cell + c] += h
             em[3 * cell + c] -= h
             fd = (J64(ep) - J64(em)) / (2 * h)
+            err = abs(float(g[c, zz, yy, xx]) - fd)
             checks.append(dict(cell=[x0 + xx, y0 + yy, z0 + zz], comp=c, adjoint=float(g[c, zz, yy, xx]), fd=fd,
-                               rel_err=abs(float(g[c, zz, yy, xx]) - fd) / abs(fd)))
-            print(f"[bench] adjoint vs FD at {checks[-1]['cell']} c={c}: {checks[-1]['rel_err']:.2e}",
-                  file=sys.stderr, flush=True)
+                               rel_err=err / abs(fd), err_rel_to_max=err / gmax))
+            print(f"[bench] adjoint vs FD at {checks[-1]['cell']} c={c}: {checks[-1]['rel_err']:.2e} of the cell's "
+                  f"value, {checks[-1]['err_rel_to_max']:.2e} of max |dJ/d eps|", file=sys.stderr, flush=True)
         out["fd_check"] = checks
+        out["fd_cells"] = "largest |dJ/d eps|" if fd_seed is None else "seeded uniform draw, seed %d" % fd_seed
         out["fd_max_rel_err"] = max(ch["rel_err"] for ch in checks)
+        out["fd_max_err_rel_to_max"] = max(ch["err_rel_to_max"] for ch in checks)
     return out
 
 
@@ -370,6 +380,8 @@ def main():
                     help="z cells of the large grid (1024: 768x768x1024, 604 M cells)")
     ap.add_argument("--adjoint-fd", type=int, default=0, metavar="CELLS",
                     help="check the adjoint gradient at CELLS cells against FP64 central differences")
+    ap.add_argument("--adjoint-fd-seed", type=int, default=None, metavar="SEED",
+                    help="draw the --adjoint-fd cells uniformly from the design box with this seed")
     ap.add_argument("--small", action="store_true", help="32x32x48 grid (quick check)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device copies of MIB MiB (scale for rocprofv3 byte counters)")
@@ -519,7 +531,7 @@ def main():
 
     adjoint = None
     if not args.no_adjoint:
-        adjoint = adjoint_run(torch, args.small, args.adjoint_fd)
+        adjoint = adjoint_run(torch, args.small, args.adjoint_fd, args.adjoint_fd_seed)
         print(f"[bench] adjoint: forward {adjoint['forward_gcells']:.2f} Gcell/s, backward "
               f"{adjoint['backward_gcells']:.2f} Gcell/s", file=sys.stderr, flush=True)
 
