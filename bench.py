@@ -314,18 +314,39 @@ def large_run(torch, steps, depth=256, small=False):
                 S_err_rel_to_max=max(e for e, _ in errs) / max(max(r for _, r in errs), 1e-300))
 
 
-def cpu_baseline(prep, seconds):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(prep, seconds, all_cores=False):
     """The reference's main_calc (compiled from its source, oracle/_ref) or, where that
     is not built, the oracle's restatement, single-threaded on this host, over whole
-    steps of the same grid until ~`seconds` have elapsed."""
+    steps of the same grid until ~`seconds` have elapsed.  all_cores: the oracle's
+    restatement with its sweeps on OpenMP threads (OMP_NUM_THREADS of them, else one per
+    host core), bit for bit the same fields; None when that build is absent."""
     import numpy as np
     import oracle
     c = prep.case
     N, iR, iI, _ = oracle.set_tau(c.omega, c.min_dt / c.dt_factor)
     st = oracle.setup_from_media(c.L, c.mu, c.eps, iR=iR, iI=iI)
-    ref = oracle.load_ref()
-    kind = "reference" if ref is not None else "port"
-    fn = ref.main_calc if ref is not None else oracle.load_oracle().orc_main_calc
+    threads = 1
+    if all_cores:
+        omp = oracle.load_oracle_omp()
+        if omp is None:
+            return None
+        kind, fn = "port", omp.orc_main_calc
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    else:
+        ref = oracle.load_ref()
+        kind = "reference" if ref is not None else "port"
+        fn = ref.main_calc if ref is not None else oracle.load_oracle().orc_main_calc
     n, plane = c.n, c.plane
     flen = 3 * (n + st["Lnotn"] + 1)
     H, E = np.zeros(flen), np.zeros(flen)
@@ -353,9 +374,10 @@ def cpu_baseline(prep, seconds):
         el = time.perf_counter() - t0
         if el >= seconds:
             break
-    return dict(value=round(n * steps / el / 1e9, 6), unit="Gcell-updates/s", cores=1, kind=kind,
+    return dict(value=round(n * steps / el / 1e9, 6), unit="Gcell-updates/s", cores=threads, kind=kind,
                 sample=f"{steps} full steps of the same {c.L[0]}x{c.L[1]}x{c.L[2]} grid, "
-                       f"single-threaded, {el:.1f} s")
+                       + (f"{threads} OpenMP threads" if all_cores else "single-threaded") + f", {el:.1f} s",
+                cpu_model=cpu_model())
 
 
 def main():
@@ -539,10 +561,14 @@ def main():
     print(f"[bench] device copy: {copy_gbs:.0f} GB/s", file=sys.stderr, flush=True)
     traffic = None if args.small else measured_traffic(step_kernels, 2 if variant == 3 else 1)
 
-    cpu = None
+    cpu = cpu_all = None
     if not args.no_cpu:
         cpu = cpu_baseline(prep, args.cpu_seconds)
         print(f"[bench] cpu baseline: {cpu['value']} Gcell/s ({cpu['kind']})", file=sys.stderr, flush=True)
+        cpu_all = cpu_baseline(prep, args.cpu_seconds / 2, all_cores=True)
+        if cpu_all:
+            print(f"[bench] cpu baseline, {cpu_all['cores']} threads: {cpu_all['value']} Gcell/s "
+                  f"({cpu_all['kind']})", file=sys.stderr, flush=True)
 
     line = {
         "metric": METRIC,
@@ -580,6 +606,7 @@ def main():
         "adjoint": adjoint,
         "large_grid": large,
         "cpu_baseline": cpu,
+        "cpu_baseline_all_cores": cpu_all,
     }
     print(json.dumps(line), flush=True)
 

@@ -32,6 +32,7 @@ _MAIN_CALC_ARGS = [_ip, _dp, _dp, _dp, _dp, _ip, _dp, _dp, _ip, _dp, _dp, _dp,
 
 _oracle = None
 _ref = None
+_oracle_omp = None
 
 
 def load_oracle():
@@ -67,6 +68,21 @@ def load_oracle():
         lib.orc_adjoint_run.restype = C.c_double
         _oracle = lib
     return _oracle
+
+
+def load_oracle_omp():
+    """liboracle_omp.so, the oracle's main_calc with its sweeps on OpenMP threads (the
+    all-cores CPU baseline); None when it has not been built."""
+    global _oracle_omp
+    if _oracle_omp is None:
+        path = os.path.join(os.path.dirname(ORACLE_SO), "liboracle_omp.so")
+        if not os.path.exists(path):
+            return None
+        lib = C.CDLL(path)
+        lib.orc_main_calc.argtypes = _MAIN_CALC_ARGS
+        lib.orc_main_calc.restype = C.c_int
+        _oracle_omp = lib
+    return _oracle_omp
 
 
 def load_ref():

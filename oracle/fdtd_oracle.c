@@ -166,11 +166,15 @@ int orc_main_calc(const int L[4], const double *Ah, const double *Bh, const doub
     const int iI = iBTIin[2];
     const double sgn = (double)(iBTIin[3] / abs(iBTIin[3])); /* F:253 */
     double *Hs = H_t + 3 * n, *Es = E_t + 3 * n;          /* split region */
-    double W[3];
-    int j[3];
 
-    /* H half step (F:255-286): reads E only, so the cell order is immaterial. */
-    for (j[2] = 0; j[2] < L2; j[2]++)
+    /* H half step (F:255-286): reads E only, so the cell order is immaterial; the OpenMP
+     * build of this file (liboracle_omp.so, a CPU baseline) runs the planes on several
+     * threads with the same per-cell arithmetic. */
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < (int)L2; z++) {
+        double W[3];
+        int j[3];
+        j[2] = z;
         for (j[1] = 0; j[1] < L1; j[1]++)
             for (j[0] = 0; j[0] < L0; j[0]++) {
                 const int64_t l = j[0] + L0 * j[1] + L0 * L1 * j[2];
@@ -206,9 +210,14 @@ int orc_main_calc(const int L[4], const double *Ah, const double *Bh, const doub
                     for (int c = 0; c < 3; c++) H_t[c + 3 * l] += 0.5 * H_t[s + c];
                 }
             }
+    }
 
     /* E half step (F:287-317): reads H only. */
-    for (j[2] = 0; j[2] < L2; j[2]++)
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < (int)L2; z++) {
+        double W[3];
+        int j[3];
+        j[2] = z;
         for (j[1] = 0; j[1] < L1; j[1]++)
             for (j[0] = 0; j[0] < L0; j[0]++) {
                 const int64_t l = j[0] + L0 * j[1] + L0 * L1 * j[2];
@@ -246,6 +255,7 @@ int orc_main_calc(const int L[4], const double *Ah, const double *Bh, const doub
                     for (int c = 0; c < 3; c++) E_t[c + 3 * l] += 0.5 * E_t[s + c];
                 }
             }
+    }
 
     /* Running single-frequency DFT on the two monitor planes, main slots (F:318-328). */
     /* A plane index of -1 switches that monitor off (an extension of this build, used

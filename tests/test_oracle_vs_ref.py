@@ -168,3 +168,19 @@ def test_s_calc_bitwise(tmp_path):
     orc.orc_s_project(L, len(cols), np.ascontiguousarray(tcols).view(np.float64).reshape(-1), HB, EB, S_o)
     S_r = SB.view(np.complex128)[cols]
     assert _eq(S_r.view(np.float64), S_o)
+
+
+@pytest.mark.parametrize("seed,L", [(31, (9, 7, 11)), (32, (5, 6, 17))])
+def test_openmp_build_bitwise(seed, L):
+    """The OpenMP build of the oracle (the all-cores CPU baseline) gives the serial
+    oracle's fields and monitors bit for bit: each cell's arithmetic is unchanged."""
+    omp = oracle.load_oracle_omp()
+    if omp is None:
+        pytest.skip("liboracle_omp.so not built")
+    st = random_state(seed, L=L)
+    Ha, Ea, ma, rca = call_main_calc(omp.orc_main_calc, st)
+    Hb, Eb, mb, rcb = call_main_calc(oracle.load_oracle().orc_main_calc, st)
+    assert rca == rcb == 0
+    assert np.array_equal(Ha, Hb) and np.array_equal(Ea, Eb)
+    for a, b in zip(ma, mb):
+        assert np.array_equal(a, b)
