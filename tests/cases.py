@@ -112,20 +112,20 @@ def column_case(eps_slab=None, iB=COLUMN_UP, iT=COLUMN_TOP, L01=2, time_period=(
                         time_period=tuple(time_period), Hinc=Hinc, Einc=Einc)
 
 
-def column_theta(eps=1.0):
+def column_theta(eps=1.0, omega=COLUMN_OMEGA):
     """Discrete propagation constant per cell: 4 sin^2(theta/2) = omega^2 eps (mu = 1)."""
-    return 2.0 * np.arcsin(COLUMN_OMEGA * np.sqrt(eps) / 2.0)
+    return 2.0 * np.arcsin(omega * np.sqrt(eps) / 2.0)
 
 
-def column_exact_slab(eps_slab):
+def column_exact_slab(eps_slab, omega=COLUMN_OMEGA):
     """(r, t) of E[k] = e^{i th k} + r e^{-i th k} left of the slab and t e^{i th k} right
     of it, from E[k+1] + E[k-1] = (2 - omega^2 eps_k) E[k] marched leftwards."""
-    th = column_theta()
+    th = column_theta(omega=omega)
     a, b = COLUMN_SLAB
     E = {j: np.exp(1j * th * j) for j in (b, b + 1)}
     for j in range(b, a - 2, -1):
         eps_j = eps_slab if a <= j < b else 1.0
-        E[j - 1] = (2.0 - COLUMN_OMEGA ** 2 * eps_j) * E[j] - E[j + 1]
+        E[j - 1] = (2.0 - omega ** 2 * eps_j) * E[j] - E[j + 1]
     # E[k] = al e^{i th k} + be e^{-i th k} on two vacuum planes left of the slab
     p, q = a - 2, a - 1
     M = np.array([[np.exp(1j * th * p), np.exp(-1j * th * p)],
@@ -134,11 +134,11 @@ def column_exact_slab(eps_slab):
     return be / al, 1.0 / al
 
 
-def column_r_t(vacuum, slab):
+def column_r_t(vacuum, slab, omega=COLUMN_OMEGA):
     """(r, t) from the E_x phasors (at COLUMN_UP, at COLUMN_TOP) of a vacuum run and a
     slab run with the same source.  The vacuum run gives the incident wave and the
     source's backward leak, which is the same in both runs and cancels."""
-    th = column_theta()
+    th = column_theta(omega=omega)
     (eb0, et0), (eb1, et1) = vacuum, slab
     inc0 = et0 * np.exp(-1j * th * COLUMN_TOP)          # incident amplitude at plane 0
     return (eb1 - eb0) * np.exp(1j * th * COLUMN_UP) / inc0, et1 / et0

@@ -2,7 +2,8 @@
 
 FP64: the monitor phasors are bit for bit the oracle's, so the oracle's pins carry over.
 FP32: every kernel family (the two sweeps, the fused staged kernel with split slots, the
-two-step pass with split slots, and that pass writing the multi-frequency DFT) reproduces
+two-step pass with split slots, and that pass writing the multi-frequency DFT, at the drive
+frequency and, through the discrete time derivative, at four others) reproduces
 the exact discrete solution -- dispersion between two planes, the reflection and
 transmission of lossless and lossy slabs, |r|^2 + |t|^2 = 1 -- within 5e-5.  Measured on
 an MI355X, the FP32 residues are 1.1e-7 to 2.3e-6 for every family: the 2.2e-6 of the
@@ -101,3 +102,34 @@ def test_f32_column_dft_matches_monitors(monkeypatch):
     r_x, t_x = C.column_exact_slab(4.0)
     print("dft |dr| %.1e |dt| %.1e" % (abs(r - r_x), abs(t - t_x)), flush=True)
     assert abs(r - r_x) < F32_BOUND and abs(t - t_x) < F32_BOUND
+
+
+def test_f32_column_dft_off_drive_frequencies(monkeypatch):
+    """At a DFT phase step w other than the drive's 2 pi / N, the leapfrog in lossless cells
+    is the same recurrence with omega replaced by omega_eff = 2 invtau_R sin(w/2) (the
+    discrete time derivative at w; at w = 2 pi / N it is omega, set_tau's exactness).  The
+    two-step pass's DFT of a lossless slab at five frequencies around the drive gives that
+    exact solution's r and t, and |r|^2 + |t|^2 = 1, within 2e-5 (measured on an MI355X:
+    2.2e-6 and below).  The continuum omega = w / Dt instead of omega_eff moves r by 2.9e-3
+    to 7.2e-3 at these frequencies: far outside the bound."""
+    for k, v in KERNELS["two_step_split"].items():
+        monkeypatch.setenv(k, v)
+    prep = casegen.prepare(C.column_case(L01=L01))
+    w0 = 2 * np.pi / prep.N
+    ws = [w0 * s for s in (0.92, 0.96, 1.0, 1.04, 1.08)]
+    out = []
+    for eps_slab in (None, 4.0):
+        pr = casegen.prepare(C.column_case(eps_slab, L01=L01))
+        with pr.plan(fdtd.F32) as p:
+            p.set_dft([C.COLUMN_UP, C.COLUMN_TOP], ws, chunk=64)
+            p.run(0, pr.case.time_period[2])
+            D = p.read_dft()
+        out.append(D[:, 2, :, 0])   # E_x phasors [plane][f]
+    for f, w in enumerate(ws):
+        om = 2 * prep.invtau_R * np.sin(w / 2)
+        r, t = C.column_r_t((out[0][0, f], out[0][1, f]), (out[1][0, f], out[1][1, f]), omega=om)
+        r_x, t_x = C.column_exact_slab(4.0, omega=om)
+        print("w/w0 %.2f omega_eff/omega %.5f |dr| %.1e |dt| %.1e" % (w / w0, om / C.COLUMN_OMEGA,
+              abs(r - r_x), abs(t - t_x)), flush=True)
+        assert abs(r - r_x) < 2e-5 and abs(t - t_x) < 2e-5, (w / w0, r, r_x, t, t_x)
+        assert abs(abs(r) ** 2 + abs(t) ** 2 - 1.0) < 2e-5
