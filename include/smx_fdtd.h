@@ -59,6 +59,26 @@ int smx_main_calc(const int L[4], const double *Ah, const double *Bh, const doub
                   double *HB, double *EB, double *HT, double *ET, const char *Cont,
                   const int *address_notn);
 
+/*
+ * No counterpart in the reference: smx_main_calc for callers that cannot allocate
+ * device memory (code that forbids hipMalloc/hipFree, such as tight serving loops or
+ * harnesses that own all device memory).  Same arguments, semantics and results as
+ * smx_main_calc (bit-identical), but the arrays are staged in a fixed device array of
+ * SMX_NOALLOC_MAX_BYTES that the library holds on each device it runs on until the
+ * process ends, instead of buffers allocated per call.  A grid whose staged arrays
+ * (4 coefficient and 2 field arrays, Cont, address_notn, source planes and monitors,
+ * each rounded up to 256 bytes) exceed it is refused with SMX_ERR_ARG before any device
+ * work.  Calls on one device share that array; the library serialises them with a
+ * per-device lock, and work the caller queued on other streams is not ordered against
+ * them.
+ */
+#define SMX_NOALLOC_MAX_BYTES ((size_t)64 << 20)
+int smx_main_calc_noalloc(const int L[4], const double *Ah, const double *Bh, const double *Ae,
+                          const double *Be, const int iBc[3], double *H_t, double *E_t,
+                          const int iBTIin[4], const double *Hi, const double *Ei,
+                          const double *ExpN2, double *HB, double *EB, double *HT, double *ET,
+                          const char *Cont, const int *address_notn);
+
 /* ------------------------------------------------------------------------------------
  * (ii) Production plan: device-resident state, many steps per call.
  *

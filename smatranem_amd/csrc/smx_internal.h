@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "smx_fdtd.h"
+
 /* Per-step scalars of the modal sheet source (F:155-158) and the monitor phases
  * (F:165, F:322-325), computed on the host for every step. */
 struct smx_step_scalars {
@@ -42,3 +44,11 @@ struct smx_f64_args {
 extern "C" {
 hipError_t smx_launch_f64_step(const smx_f64_args *a, hipStream_t stream);
 }
+
+/* Host-side checks and error text shared by the entry points (smx_plan.hip):
+ * smx_check_grid validates a grid and its class map as smx_plan_create does (SMX_OK,
+ * SMX_ERR_ARG or SMX_ERR_MONITOR_PML); smx_note_hip / smx_note_arg record the text
+ * smx_last_error returns and give the status code. */
+int smx_check_grid(const smx_grid *g, const char *Cont);
+int smx_note_hip(hipError_t e, const char *what);
+int smx_note_arg(const char *what);

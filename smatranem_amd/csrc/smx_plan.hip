@@ -390,6 +390,10 @@ int launch_step(smx_plan *p, const smx_step_scalars &s, bool explicit_src, bool 
 
 } // namespace
 
+int smx_check_grid(const smx_grid *g, const char *Cont) { return validate_grid(g, Cont); }
+int smx_note_hip(hipError_t e, const char *what) { return hip_fail(e, what); }
+int smx_note_arg(const char *what) { return arg_fail(what); }
+
 /* ==================================================================================== */
 /* C ABI                                                                                */
 /* ==================================================================================== */

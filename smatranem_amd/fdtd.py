@@ -62,6 +62,8 @@ def lib():
     L.smx_main_calc.argtypes = [_ip, _dp, _dp, _dp, _dp, _ip, _dp, _dp, _ip, _dp, _dp, _dp,
                                 _dp, _dp, _dp, _dp, _cp, _ip]
     L.smx_main_calc.restype = C.c_int
+    L.smx_main_calc_noalloc.argtypes = L.smx_main_calc.argtypes
+    L.smx_main_calc_noalloc.restype = C.c_int
     L.smx_plan_create.argtypes = [C.POINTER(_Grid), _dp, _dp, _dp, _cp, _ip, C.POINTER(C.c_void_p)]
     L.smx_plan_create.restype = C.c_int
     L.smx_plan_create_coeffs.argtypes = [C.POINTER(_Grid), _dp, _dp, _dp, _dp, _cp, _ip,
@@ -226,6 +228,20 @@ def main_calc(L, Ah, Bh, Ae, Be, iBc, H_t, E_t, iBTIin, Hi, Ei, ExpN2, HB, EB, H
                              _i32(iBTIin), _f64(Hi), _f64(Ei), _cplx(ExpN2), HB, EB, HT, ET,
                              _cont(Cont), _i32(address_notn))
     _check(rc)
+
+
+def main_calc_noalloc(L, Ah, Bh, Ae, Be, iBc, H_t, E_t, iBTIin, Hi, Ei, ExpN2, HB, EB, HT, ET,
+                      Cont, address_notn):
+    """main_calc (F:242) on the GPU through smx_main_calc_noalloc: the arrays are staged
+    in the library's fixed device scratch (NOALLOC_MAX_BYTES) instead of buffers
+    allocated per call; same results as :func:`main_calc`."""
+    rc = lib().smx_main_calc_noalloc(_i32(L), _f64(Ah), _f64(Bh), _f64(Ae), _f64(Be), _i32(iBc), H_t,
+                                     E_t, _i32(iBTIin), _f64(Hi), _f64(Ei), _cplx(ExpN2), HB, EB, HT,
+                                     ET, _cont(Cont), _i32(address_notn))
+    _check(rc)
+
+
+NOALLOC_MAX_BYTES = 64 << 20
 
 
 BATCH_STREAMS = 0
