@@ -27,11 +27,15 @@
  *   orc_set_sigma        bitwise vs reference set_sigma (F:858-926); round trip of the
  *   orc_remove_sigma     PML stretch (PMLset/PMLcalc.c:230) through both functions.
  *   orc_set_address      bitwise vs reference set_address (F:188-233); brute-force counts.
- *   orc_build_coeffs     parity unpinned against the reference (the loop is inline in
- *                        main, F:106-143, which needs BLAS to run); pinned by closed
- *                        forms: the single-cell scheme reproduces the frequency-domain
- *                        response at omega exactly, A=1/B=1/(iR*mu) in lossless cells,
- *                        and the metal stable form of check_files/StableForm.pdf p.3.
+ *   orc_build_coeffs     not run against the reference (the loop is inline in main,
+ *                        F:106-143, which needs BLAS to run); pinned by closed forms
+ *                        of every branch: one cell of each class, driven at omega through
+ *                        orc_main_calc, settles to the frequency-domain response the
+ *                        scheme is exact for -- lossy eps and lossy mu ('n'), the main
+ *                        and split slots of both families in PML with distinct sigma
+ *                        samples per axis ('a'), and the eps- and mu-metal stable forms
+ *                        of check_files/StableForm.pdf p.3 ('e', 'h', 'b') -- and
+ *                        A = 1, B = 1/(iR*eps) in lossless cells.
  *   orc_run              composition of the above in the order of F:153-167.
  *   orc_s_project        bitwise vs reference S_calc (F:419-448) reading the same files.
  *   orc_step_adjoint,    no reference counterpart (this build's adjoint): the transpose

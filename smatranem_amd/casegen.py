@@ -128,7 +128,8 @@ def sheet_source(L0, L1, profile, n_eff=1.0):
 def waveguide_case(L0, L1, L2, outer, pml_xy, pml_z, *, omega=2 * np.pi / 20.0, eps_core=12.1,
                    eps_clad=2.1, core=(8, 6), scale_amp=0.0, n_modes=8, pec_fraction=0.0,
                    seed=0, iBc=(0, 0, 0), dt_factor=1.0, time_period=(2, 1, 8), monitor_gap=2,
-                   metal_cells=0, lossy_cells=0, blocks=0, design=None, layers=0, grating=False, progress=None):
+                   metal_cells=0, lossy_cells=0, lossy_mu_cells=0, blocks=0, design=None, layers=0, grating=False,
+                   progress=None):
     """A straight dielectric core along z in cladding, on an L0 x L1 x L2 grid with
     ``outer`` port slices at each end (B0.., T..), PML layers of ``pml_xy`` cells on the
     x/y faces and ``pml_z`` on the z faces (PMLcalc.c:188-260), optional curvilinear scale
@@ -136,7 +137,8 @@ def waveguide_case(L0, L1, L2, outer, pml_xy, pml_z, *, omega=2 * np.pi / 20.0, 
     normalised to max|g| = 1; fx_c = h_{c+1} h_{c+2} / h_c at each component's
     position, MixedData/BindData.c:223-228), seeded PEC slabs covering about
     ``pec_fraction`` of the E edges of the scatterer, and optional metal ('e'/'h'/'b')
-    and lossy cells, and ``blocks`` seeded dielectric boxes (eps uniform in [1, 12.1]) in
+    cells, cells of lossy eps (``lossy_cells``) and of lossy mu (``lossy_mu_cells``, mu =
+    Re mu + 0.25i, class 'n'), and ``blocks`` seeded dielectric boxes (eps uniform in [1, 12.1]) in
     the scatterer, and an optional design region ``design`` = ((x0, x1, y0, y1, z0, z1),
     lo, hi) whose cells each get one seeded eps, uniform in [lo, hi], for all three
     components; ``layers`` > 0 fills the scatterer slices with that many z-layers of
@@ -256,6 +258,11 @@ def waveguide_case(L0, L1, L2, outer, pml_xy, pml_z, *, omega=2 * np.pi / 20.0, 
         zs, ys, xs = pick(lossy_cells)
         for zz, yy, xx in zip(zs, ys, xs):
             eps_c[zz, yy, xx, :] = eps_c[zz, yy, xx, :].real + 0.3j
+    if lossy_mu_cells:
+        zs, ys, xs = pick(lossy_mu_cells)
+        for zz, yy, xx in zip(zs, ys, xs):
+            if mu_c[zz, yy, xx, 0].real > 0:
+                mu_c[zz, yy, xx, :] = mu_c[zz, yy, xx, :].real + 0.25j
 
     say("PEC, metal and lossy cells")
     min_dt = cfl_min_dt(mu3.reshape(-1)[np.isfinite(mu3.reshape(-1))],

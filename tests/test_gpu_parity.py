@@ -61,6 +61,8 @@ CASES = {
     "box": dict(pml_xy=2, pml_z=3, iBc=(0, 0, 0), metal_cells=6, lossy_cells=4),
     "curvilinear": dict(pml_xy=0, pml_z=3, iBc=(1, 1, 0), scale_amp=0.3, pec_fraction=0.03,
                         metal_cells=0, lossy_cells=0),
+    # lossy mu in 'n' cells next to lossy eps and metal: every H coefficient branch
+    "lossy_mu": dict(pml_xy=2, pml_z=3, iBc=(0, 0, 0), metal_cells=3, lossy_cells=3, lossy_mu_cells=8),
 }
 
 

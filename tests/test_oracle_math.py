@@ -130,91 +130,142 @@ def _one_cell(cls, mu, eps, sig6, iR, iI):
     return L, cont, addr, lnotn, co
 
 
-def _drive_e(cls, mu, eps, sig6, omega=2 * np.pi / 16, min_dt=0.5, periods=400):
-    """Drive a one-cell grid (PEC wrap, so W[c] = H[c1] - H[c2] for the forward curl and
-    H[c2] - H[c1] for the backward one) with H held at a sinusoid; return the steady E
-    phasor per unit forcing phasor, fitted over the last periods."""
+# Distinct sigma samples per axis and per integer/half position, so that a coefficient
+# taking another axis's sample, or the half sample instead of the integer one, changes
+# the response.  Order: (x int, x half, y int, y half, z int, z half).
+SIG6 = np.array([0.11, 0.23, 0.37, 0.41, 0.53, 0.67])
+# Distinct drive phasors of the three components of the held field.
+DRIVE = np.array([1.0 + 0.0j, 0.5j, -0.7 + 0.3j])
+
+
+def _drive(cls, mu, eps, sig6, field, drive=DRIVE, omega=2 * np.pi / 16, min_dt=0.5, periods=400):
+    """Drive a one-cell grid (PEC wrap, so every neighbour term of a curl vanishes) with
+    the other field held at the sinusoids Re(drive[c] e^{-i omega t}), and return the
+    steady phasors of the driven field's main slots [0:3] and split/aux slots [3:6],
+    fitted over the last 8 periods, with (iR, iI, omega).
+
+    field 'H': E is set to its values at t = k dt before step k, E's own update is
+    discarded, and H after step k is H at (k + 1/2) dt.  field 'E': H is set to its
+    values at (k + 1/2) dt before step k, and E after step k is E at (k + 1) dt.  The
+    held field's split/aux slots are zero, so the curls see the drive as the total."""
     N, iR, iI, dt = oracle.set_tau(omega, min_dt)
     L, cont, addr, lnotn, (Ah, Bh, Ae, Be) = _one_cell(cls, mu, eps, sig6, iR, iI)
+    held_h = field == "E"
+    if held_h:
+        # the H sweep runs before the E sweep of a step: A = 1, B = 0 keeps the held H
+        Ah, Bh = np.ones_like(Ah), np.zeros_like(Bh)
     flen = 3 * (1 + lnotn + 1)
     H = np.zeros(flen)
     E = np.zeros(flen)
     z2 = np.zeros(2)
     mons = [np.zeros(4) for _ in range(4)]
-    Bh0 = np.zeros_like(Bh)          # H is set by the test, not updated
-    Ah1 = np.ones_like(Ah)
-    if cls in "hb":
-        Bh0[3:] = 0.0
-        Ah1[3:] = 1.0
+    t_held = 0.5 if held_h else 0.0
+    t_out = 1.0 if held_h else 0.5
     hist = []
     total = periods * N
     for k in range(total):
-        # H at time k+1/2 (the E update of step k uses H^{k+1/2}); H_y carries the drive
-        hval = np.cos(omega * (k + 0.5) * dt)
-        H[:3] = 0.0
-        H[1] = hval
-        oracle.main_calc(orc.orc_main_calc, L, Ah1, Bh0, Ae, Be, oracle.i32([0, 0, 0]), H, E,
-                         oracle.i32([0, 0, 5, 1]), z2, z2, np.zeros(4), *mons, cont, addr)
-        hist.append(E[0])
-    t = np.arange(total - 8 * N, total)
+        th = omega * (k + t_held) * dt
+        vals = drive.real * np.cos(th) + drive.imag * np.sin(th)
+        held = H if held_h else E
+        held[:] = 0.0
+        held[:3] = vals
+        oracle.main_calc(orc.orc_main_calc, L, Ah, Bh, Ae, Be, oracle.i32([0, 0, 0]), H, E,
+                         oracle.i32([-1, -1, 5, 1]), z2, z2, np.zeros(4), *mons, cont, addr)
+        out = E if held_h else H
+        hist.append(out[:3 * (1 + lnotn)].copy())
+    t = np.arange(total - 8 * N, total) + t_out
     y = np.array(hist[-8 * N:])
-    # E^n = Re(Ehat e^{-i omega n dt}) + c: least squares over the last 8 periods
-    A = np.stack([np.cos(omega * (t + 1) * dt), np.sin(omega * (t + 1) * dt), np.ones_like(t, dtype=float)], 1)
-    cr, ci, _ = np.linalg.lstsq(A, y, rcond=None)[0]
-    Ehat = cr + 1j * ci
-    return Ehat, iR, iI, omega
+    A = np.stack([np.cos(omega * t * dt), np.sin(omega * t * dt), np.ones_like(t)], 1)
+    coef = np.linalg.lstsq(A, y, rcond=None)[0]
+    ph = coef[0] + 1j * coef[1]
+    if lnotn == 0:
+        ph = np.concatenate([ph, np.zeros(3)])
+    return ph, iR, iI, omega
+
+
+def _curl_halves(drive, backward):
+    """The '+' and '-' halves (F:372-413, js = 0 and 1) of the one-cell curl of a field
+    with component phasors `drive`: every neighbour term vanishes, so the forward curl
+    is (-F[c2], +F[c1]) and the backward curl (+F[c2], -F[c1]), c1 = c+1, c2 = c+2."""
+    plus = np.zeros(3, complex)
+    minus = np.zeros(3, complex)
+    for c in range(3):
+        c1, c2 = (c + 1) % 3, (c + 2) % 3
+        plus[c] = drive[c2] if backward else -drive[c2]
+        minus[c] = -drive[c1] if backward else drive[c1]
+    return plus, minus
 
 
 def test_lossy_dielectric_cell_matches_frequency_domain():
-    """E cell, class 'n', eps = 2 + 0.5i: -i*omega*eps*Ehat = What (What the forcing
-    phasor of W_x = H_y - H_z = H_y at time n + 1/2)."""
+    """E cell, class 'n', eps = 2 + 0.5i: -i omega eps Ehat = What, What the phasor of
+    the forward curl of H (F:291-299)."""
     eps = 2.0 + 0.5j
-    Ehat, iR, iI, omega = _drive_e("n", 1.0 + 0j, eps, np.zeros(6))
-    W = 1.0  # W_x = H[c1=y] - H[c2=z] for the forward curl with zero wrap weights
-    assert abs(-1j * omega * eps * Ehat - W) < 1e-9
+    ph, iR, iI, omega = _drive("n", 1.0 + 0j, eps, np.zeros(6), "E")
+    plus, minus = _curl_halves(DRIVE, backward=False)
+    W = plus + minus
+    np.testing.assert_allclose(-1j * omega * eps * ph[:3], W, rtol=0, atol=1e-9)
 
 
-def test_pml_cell_matches_stretched_response():
-    """E cell, class 'a', sigma on the main slot's axis (y for E_x; half samples):
-    the main slot follows -i*omega*eps*(1 + i*sigma)*Ehat_main = What_+ with
-    What_+ = H_z part -> here the '+' half carries -H[c2] = 0 and the '-' half +H[c1]."""
-    sig = 0.7
-    sig6 = [0, 0, 0, sig, 0, 0]  # y half sample
-    # E_x main slot: '+' half = w*H_z(l+e_y) - H_z(l) = 0 (H_z = 0): no drive;
-    # split slot (sigma of axis z = 0) gets the '-' half = -w*H_y(l+e_z) + H_y(l) = H_y.
-    # The total E_x = main + split therefore follows the unstretched response.
-    N, iR, iI, dt = oracle.set_tau(2 * np.pi / 16, 0.5)
-    L, cont, addr, lnotn, (Ah, Bh, Ae, Be) = _one_cell("a", 1.0 + 0j, 2.0 + 0j, sig6, iR, iI)
-    # closed forms of the two slots' coefficients from the scheme that is exact at omega
-    e_main = 2.0 * (1 + 1j * sig)
-    a_m, b_m = Ae[0], Be[0]
-    a_s, b_s = Ae[3], Be[3]
-    B_exp = lambda er, s: 1.0 / (iR * er + iI * er * s)
-    A_exp = lambda er, s: (iR * er - iI * er * s) * B_exp(er, s)
-    assert b_m == pytest.approx(B_exp(2.0, sig), rel=1e-15) and a_m == pytest.approx(A_exp(2.0, sig), rel=1e-15)
-    assert b_s == pytest.approx(B_exp(2.0, 0.0), rel=1e-15) and a_s == pytest.approx(1.0, abs=2e-16)
-    # and the slot scheme X^{n+1} = A X^n + B W is exact at omega for eps*(1 + i*sigma):
-    omega = 2 * np.pi / 16
-    z = np.exp(-1j * omega * dt)
-    # X(z - A) = B * W z^{1/2}  =>  X/W = B z^{1/2} / (z - A) = 1/(-i omega eps_main)
-    ratio = b_m * np.exp(-0.5j * omega * dt) / (z - a_m)
-    assert abs(ratio * (-1j * omega * e_main) - 1.0) < 1e-12
+def test_lossy_magnetic_cell_matches_frequency_domain():
+    """H cell, class 'n', mu = 2 + 0.5i: H <- A H + B (-W) with W the backward curl of E
+    (F:259-267) is exact at omega for -i omega mu Hhat = -What.  A flipped sign of Im mu
+    in A or B, or a dropped term, moves the response by O(Im mu / |mu|)."""
+    mu = 2.0 + 0.5j
+    ph, iR, iI, omega = _drive("n", mu, 1.0 + 0j, np.zeros(6), "H")
+    plus, minus = _curl_halves(DRIVE, backward=True)
+    W = plus + minus
+    np.testing.assert_allclose(-1j * omega * mu * ph[:3], -W, rtol=0, atol=1e-9)
 
 
-def test_metal_cell_stable_form_response():
-    """E cell, class 'e', eps = -10 + 1.5i: the stable form of check_files/StableForm.pdf
-    p.3 as the reference codes it (trapezoid E(l) = E(l-1) + (dE(l)+dE(l-1))/2 with no
-    time step factor) settles to -i*omega*(eps/iR)*Ehat = What, where What is the
-    forcing phasor of the backward curl W_x = H_z - H_y = -H_y taken at time n - 1/2.
-    A dropped or misplaced term in either coefficient changes this response."""
+@pytest.mark.parametrize("family", ["E", "H"])
+def test_pml_slots_match_stretched_response(family):
+    """Class 'a' cell with lossy mu and eps and distinct sigma samples on every axis.
+    The main slot of component c takes the '+' half of the curl, stretched along
+    d1 = c+1; the split slot the '-' half, along d2 = c+2 (F:118-127, F:268-276,
+    F:300-307).  H uses the integer sigma samples, E the half samples.  Each slot is
+    exact at omega for the stretched medium m_eff = Re m (1 + i sigma) + i Im m:
+    -i omega eps_eff Ehat = What (E) and -i omega mu_eff Hhat = -What (H)."""
+    mu, eps = 1.5 + 0.3j, 2.5 + 0.4j
+    ph, iR, iI, omega = _drive("a", mu, eps, SIG6, family)
+    plus, minus = _curl_halves(DRIVE, backward=family == "H")
+    m = mu if family == "H" else eps
+    half = 0 if family == "H" else 1
+    sgn = -1.0 if family == "H" else 1.0
+    for c in range(3):
+        d1, d2 = (c + 1) % 3, (c + 2) % 3
+        m_main = m.real * (1 + 1j * SIG6[half + 2 * d1]) + 1j * m.imag
+        m_split = m.real * (1 + 1j * SIG6[half + 2 * d2]) + 1j * m.imag
+        assert abs(-1j * omega * m_main * ph[c] - sgn * plus[c]) < 1e-9, (family, c, "main")
+        assert abs(-1j * omega * m_split * ph[3 + c] - sgn * minus[c]) < 1e-9, (family, c, "split")
+
+
+@pytest.mark.parametrize("cls", ["e", "b"])
+def test_metal_cell_stable_form_response(cls):
+    """E in a class 'e' (or 'b') cell, eps = -10 + 1.5i: the stable form of
+    check_files/StableForm.pdf p.3 as the reference codes it (trapezoid
+    E(l) = E(l-1) + (dE(l)+dE(l-1))/2 with no time step factor; F:129-134, F:308-314)
+    settles to -i omega (eps/iR) Ehat = What, What the phasor of the BACKWARD curl of H
+    (Q1) at the same half step.  A dropped or misplaced term in either coefficient
+    changes this response."""
     eps = -10.0 + 1.5j
-    Ehat, iR, iI, omega = _drive_e("e", 1.0 + 0j, eps, np.zeros(6))
-    N, iR2, iI2, dt = oracle.set_tau(omega, 0.5)
-    # W^{k} uses H^{k+1/2} (set before step k) -> phasor -1 at time k+1/2 relative to the
-    # E sample at k+1; the stable-form relation pairs W with E at the same half step.
-    W = -1.0
-    lhs = -1j * omega * (eps / iR) * Ehat * np.exp(0.0j)
-    assert abs(lhs - W) < 1e-9 * abs(W), (lhs, W)
+    ph, iR, iI, omega = _drive(cls, -3.0 + 0.8j, eps, np.zeros(6), "E")
+    plus, minus = _curl_halves(DRIVE, backward=True)
+    W = plus + minus
+    np.testing.assert_allclose(-1j * omega * (eps / iR) * ph[:3], W, rtol=0, atol=1e-9)
+
+
+@pytest.mark.parametrize("cls", ["h", "b"])
+def test_magnetic_metal_cell_stable_form_response(cls):
+    """H in a class 'h' (or 'b') cell, mu = -3 + 0.8i: the aux slot X takes
+    X <- A X + B (-W) with the stable-form coefficients (F:135-139) and the main slot
+    H += (X_old + X_new)/2 (F:277-283).  With (z+1) iI = omega z^{1/2} and
+    (z-1) iR = -i omega z^{1/2}, z = e^{-i omega dt}, this settles to
+    -i omega (mu/iR) Hhat = -What.  A flipped sign of Im mu in the aux A or B moves it."""
+    mu = -3.0 + 0.8j
+    ph, iR, iI, omega = _drive(cls, mu, -10.0 + 1.5j, np.zeros(6), "H")
+    plus, minus = _curl_halves(DRIVE, backward=True)
+    W = plus + minus
+    np.testing.assert_allclose(-1j * omega * (mu / iR) * ph[:3], -W, rtol=0, atol=1e-9)
 
 
 def test_lossless_cell_coefficients():

@@ -74,7 +74,7 @@ def test_set_env_and_expn_match_oracle():
 @pytest.mark.parametrize("seed,scale", [(0, 0.0), (3, 0.3)])
 def test_setup_chain_matches_oracle(seed, scale):
     case = casegen.waveguide_case(10, 8, 20, 5, 2, 2, core=(4, 3), metal_cells=5, lossy_cells=3,
-                                  seed=seed, scale_amp=scale)
+                                  lossy_mu_cells=4, seed=seed, scale_amp=scale)
     prep = casegen.prepare(case)
     N, iR, iI, _ = oracle.set_tau(case.omega, case.min_dt)
     ref = oracle.setup_from_media(case.L, case.mu, case.eps, iR=iR, iI=iI)
@@ -91,6 +91,8 @@ def test_setup_chain_matches_oracle(seed, scale):
         np.testing.assert_array_equal(a, b)
     kinds = set(np.unique(prep.Cont).tobytes())
     assert {ord("n"), ord("a")} <= kinds and kinds & {ord("e"), ord("h"), ord("b")}
+    mu_n = prep.mu.reshape(-1, 3)[prep.Cont == ord("n")]
+    assert np.count_nonzero(mu_n.imag) >= 3, "no lossy-mu 'n' cell in the case"
 
 
 def test_plan_without_device_fails_loudly():
