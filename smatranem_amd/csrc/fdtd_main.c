@@ -12,6 +12,9 @@
  *  - check_Xi (F:495-496, a BLAS residual printed only when > 1e-15) is not computed.
  *  - SMX_FDTD_PRECISION=f32 in the environment selects the FP32 plan (default f64,
  *    bit-identical to the reference).
+ *  - SMX_FDTD_SETUP_DUMP=<file> is a dry run that needs no GPU: after the plane indices
+ *    are reported, the program writes what it read from the files to <file> and exits
+ *    with status 0 (layout at dump_setup).
  * The file formats are the reference's (and its writers', Medium3DSi/Med3DSi.c:210-258,
  * PMLset/PMLcalc.c:331, ModeCalc/Mcalc.c:191-228), so the scan formats below follow
  * them.
@@ -403,6 +406,23 @@ static void check(int rc)
     exit(EXIT_FAILURE);
 }
 
+/* The dry run's file: int32 L[4], BoundaryCondition[3], periods[3], incident mode, iB, iT,
+ * iI; float64 Dt_factor, omega, min_Dt; then mu and eps as read from the Med files
+ * (complex128[3n] each, fx applied, before set_sigma), all little-endian. */
+static void dump_setup(const char *path, const int L[4], const run_input *in, const int planes[3], double omega,
+                       double min_dt, const double complex *mu, const double complex *eps)
+{
+    FILE *f = fopen(path, "wb");
+    if (!f) die("open error for SMX_FDTD_SETUP_DUMP\n");
+    const int ints[14] = {L[0],     L[1],     L[2],     L[3],        in->bc[0], in->bc[1], in->bc[2],
+                          in->tp[0], in->tp[1], in->tp[2], in->inc_mode, planes[0], planes[1], planes[2]};
+    const double dbl[3] = {in->dt_factor, omega, min_dt};
+    const size_t n3 = 3 * (size_t)L[0] * L[1] * L[2];
+    if (fwrite(ints, sizeof ints, 1, f) != 1 || fwrite(dbl, sizeof dbl, 1, f) != 1 ||
+        fwrite(mu, sizeof *mu, n3, f) != n3 || fwrite(eps, sizeof *eps, n3, f) != n3 || fclose(f) != 0)
+        die("write error for SMX_FDTD_SETUP_DUMP\n");
+}
+
 int main(int argc, char **argv)
 {
     time_t now = time(0);
@@ -435,6 +455,16 @@ int main(int argc, char **argv)
         if (z == 0 || min_dt > sqrt(sq)) min_dt = sqrt(sq);
     }
 
+    const char *dump = getenv("SMX_FDTD_SETUP_DUMP");
+    double complex *mu_read = NULL, *eps_read = NULL;
+    if (dump && *dump) {
+        mu_read = malloc(3 * (size_t)n * sizeof *mu_read);
+        eps_read = malloc(3 * (size_t)n * sizeof *eps_read);
+        if (!mu_read || !eps_read) die("out of memory for SMX_FDTD_SETUP_DUMP\n");
+        memcpy(mu_read, mu, 3 * (size_t)n * sizeof *mu);
+        memcpy(eps_read, eps, 3 * (size_t)n * sizeof *eps);
+    }
+
     const int nsig = 2 * (L[0] + L[1] + L[2]);
     double *sigma = calloc((size_t)nsig, sizeof *sigma);
     smx_set_sigma(L, (const double *)mu, (const double *)eps, sigma);
@@ -462,6 +492,11 @@ int main(int argc, char **argv)
     }
     fprintf(stderr, "Start = %d for %s, Incident point = %d for %s, Incident mode = %+d, Target_T = %d for %s\n", iB,
             in.plane_b, iI, in.plane_inc, in.inc_mode, iT, in.plane_t);
+    if (mu_read) {
+        const int planes[3] = {iB, iT, iI};
+        dump_setup(dump, L, &in, planes, omega, min_dt, mu_read, eps_read);
+        return 0;
+    }
 
     double complex *Hinc = calloc(2 * (size_t)P, sizeof *Hinc), *Einc = calloc(2 * (size_t)P, sizeof *Einc);
     incident_profile(in.prefix, L, iB, iT, iI, in.inc_mode, Hinc, Einc);

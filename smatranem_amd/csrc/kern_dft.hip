@@ -13,8 +13,8 @@
  * e^{i*w*(k+1)} for H and e^{i*w*(k+1.5)} for E, w = omega_f*Dt; at w = 2*pi/N_2pi this
  * is the reference's monitor.
  *
- * Sample of a cell = the reference's main slot (total minus split/aux); in PML cells,
- * which the reference refuses on monitor planes, the total field.
+ * Sample of a cell = the reference's main slot (total minus split/aux); in class 'a'
+ * (PML) cells, which the reference refuses on monitor planes, the total field.
  */
 #include <hip/hip_runtime.h>
 
@@ -40,13 +40,18 @@ __global__ __launch_bounds__(kThreads) void sample_planes_f32(smx_dft_sample_arg
     const int64_t q = i - m * per_plane;
     const int64_t l = q + per_plane * a.planes[m];
     const int x = (int)(q % a.L0), y = (int)(q / a.L0), z = a.planes[m];
-    const bool pml = a.pmlflag && (a.pmlflag[x] | a.pmlflag[a.L0 + y] | a.pmlflag[a.L0 + a.L1 + z]);
+    /* the total field in class 'a' cells, as sample_planes_f64: the separable PML flag
+     * for regular cells, the table's class for lossy and metal cells (a metal cell
+     * inside a PML layer is class 'e'/'h'/'b', F:188-233, and samples its main slot) */
+    bool total = a.pmlflag && (a.pmlflag[x] | a.pmlflag[a.L0 + y] | a.pmlflag[a.L0 + a.L1 + z]);
+    const float b0 = a.bH0[l];
+    if (__builtin_isnan(b0)) total = a.special[(int64_t)__float_as_uint(a.bH1[l]) * a.special_stride] == 'a';
     T *dst = (T *)a.series + ((int64_t)slot * a.nplanes + m) * 4 * per_plane + q;
     const int comp[4] = {0, 1, 3, 4};
 #pragma unroll
     for (int v = 0; v < 4; v++) {
         const float t = a.T32[comp[v]][l];
-        dst[v * per_plane] = pml ? t : t - a.S32[comp[v]][l];
+        dst[v * per_plane] = total ? t : t - a.S32[comp[v]][l];
     }
 }
 

@@ -17,6 +17,10 @@ struct smx_dft_sample_args {
     int64_t plane_cells;            /* L0*L1 */
     const float *T32[6], *S32[6];   /* FP32 plan: total and split (current buffer) */
     const unsigned char *pmlflag;   /* FP32 plan: per-axis PML flags */
+    const float *bH0, *bH1;         /* FP32 plan: coefficient words bH_x, bH_y (NaN in bH_x marks a
+                                       lossy/metal cell, whose bH_y bits index the special table) */
+    const int *special;             /* FP32 plan: special table, the class char first in each entry */
+    int special_stride;             /* ints per special-table entry */
     const double *H64, *E64;        /* FP64 plan: reference layout */
     const char *cont;               /* FP64 plan: cell classes ('a' cells add the split slot) */
     const int *addr;

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <algorithm>
 #include <cstring>
@@ -351,6 +352,12 @@ int dft_after_step(smx_plan *p, long long k, bool mon)
             a.S32[i] = p->f32.S[p->f32.cur][i];
         }
         a.pmlflag = p->f32.pmlflag;
+        a.bH0 = p->f32.b[0];
+        a.bH1 = p->f32.b[1];
+        static_assert(offsetof(smx_special_f32, cls) == 0 && sizeof(smx_special_f32) % sizeof(int) == 0,
+                      "the DFT sample reads the class as the first int of a special-table entry");
+        a.special = reinterpret_cast<const int *>(p->f32.special);
+        a.special_stride = (int)(sizeof(smx_special_f32) / sizeof(int));
     }
     a.series = p->dft_series;
     SMX_HIP(smx_dft_sample(&a, p->dft_fill, p->stream));

@@ -82,9 +82,9 @@ def _expected(case, Xi, n_real):
     return lines
 
 
-def _write(case, tmp_path, n_real=3):
+def _write(case, tmp_path, n_real=3, **kw):
     case.meta["omega0"] = case.omega
-    inp, Xi = casegen.write_case_files(case, str(tmp_path / "Data"), n_real=n_real)
+    inp, Xi = casegen.write_case_files(case, str(tmp_path / "Data"), n_real=n_real, **kw)
     return inp, Xi
 
 
@@ -100,6 +100,20 @@ def test_fdtd_report_matches_reference(tmp_path, inc):
     assert rc == 0, "\n".join(got)
     exp = _expected(case, Xi, 3)
     assert got == exp
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("corr", [False, True])
+def test_fdtd_report_curvilinear(tmp_path, corr):
+    """Curvilinear scale factors: the Med lines carry per-component fx != 1 (the
+    MixedData/BindData.c layout), with CorrFactor's fx/(1 + eta) or without; lossy-mu
+    cells too.  The report equals the reference's to the last digit."""
+    case = _case(scale_amp=0.3, lossy_mu_cells=3)
+    assert case.fx_mu is not None
+    inp, Xi = _write(case, tmp_path, corr_factor=corr)
+    rc, got = _run(inp)
+    assert rc == 0, "\n".join(got)
+    assert got == _expected(case, Xi, 3)
 
 
 @pytest.mark.gpu

@@ -171,3 +171,32 @@ def test_monitors_from_ring(kernel, planes, monkeypatch):
     for a, b in list(zip(mid0, mid1)) + list(zip(mon0, mon1)):
         assert np.abs(a).max() > 0
         assert np.abs(a - b).max() <= 1e-12 * np.abs(a).max(), np.abs(a - b).max() / np.abs(a).max()
+
+
+def test_dft_f32_samples_metal_cell_in_pml_as_f64():
+    """A metal cell in a PML layer is class 'e' (the metal class overrides 'a', F:188-233)
+    and its DFT sample is the main slot, not main + aux: the FP32 sample follows the
+    cell's class, as the FP64 one does.  The cell sits on the first cell of the +x PML
+    layer, where sigma_x is zero at the integer sample and positive at the half one, so
+    remove_sigma (F:822-853) accepts a lossy metal eps_x there."""
+    case = small_waveguide(seed=6, pml_xy=2, pml_z=3, iBc=(0, 0, 0), metal_cells=0, lossy_cells=0)
+    L0, L1, L2 = (int(v) for v in case.L[:3])
+    zd, x, y = L2 // 2, L0 - 2, L1 // 2
+    q = x + L0 * y
+    l = q + L0 * L1 * zd
+    case.eps[3 * l] = -10.0 + 1.5j
+    prep = casegen.prepare(case)
+    assert chr(prep.Cont[l]) == "e"
+    assert prep.sigma[2 * x] == 0.0 and prep.sigma[2 * x + 1] > 0.0
+    w = 2 * np.pi / prep.N
+    D = {}
+    for prec in (fdtd.F64, fdtd.F32):
+        with prep.plan(prec) as p:
+            p.set_dft([zd], [w, 0.7 * w], chunk=16)
+            p.run(0, case.time_period[2])
+            D[prec] = p.read_dft()
+    ref = D[fdtd.F64]
+    scale = np.abs(ref).max()
+    assert scale > 0 and np.abs(ref[..., q]).max() > 1e-3 * scale
+    assert rel_l2(D[fdtd.F32], ref) < 2e-4
+    assert np.abs(D[fdtd.F32][..., q] - ref[..., q]).max() <= 2e-4 * scale
