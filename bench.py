@@ -11,10 +11,11 @@ monitors on two port planes.  Inputs are synthetic and resident in HBM before th
 timed region.
 
 Prints one JSON line: FP32 Gcell-updates/s (value; the step kernel chosen by the
-plan's autotuner), the roofline of that kernel against MI355X HBM peak, the plain FP64 path on the same run, the
-FP32-vs-FP64 |S| error from the monitor projections, and the reference's own
-main_calc timed on this host's cores (oracle/_ref; the restatement in oracle/ where
-that is not built).
+plan's autotuner), the roofline of that kernel against MI355X HBM peak, the plain FP64
+path on the same grid, the FP32-vs-FP64 |S| error (a separate run of the ramp plus 3000
+CW steps from zero fields, whatever --steps is), and the reference's own main_calc
+timed on this host's cores (oracle/_ref; the restatement in oracle/ where that is not
+built).
 """
 from __future__ import annotations
 
@@ -61,7 +62,7 @@ def measured_traffic(kernels, steps_per_pass=1):
         else:
             table[w[0]] = (float(w[1]), float(w[2]))
     if scale is None or any(k not in table for k in kernels):
-        return None
+        return None  # the committed profile does not list the kernel this run chose: stale
     return sum((table[k][0] * scale[0] + table[k][1] * scale[1]) * 1024 for k in kernels) / steps_per_pass
 
 
@@ -96,6 +97,60 @@ def overlap(prep, H, E):
     return complex(np.vdot(w, v) / np.vdot(w, w).real)
 
 
+ACC_CW_STEPS = 3000  # CW steps of the accuracy run after the envelope's ramp
+
+
+def run_from_zero(plan, dft_planes, dft_w, chunk, steps):
+    """Zero the plan's fields, monitors and DFT, run steps [0, steps) of the source loop
+    and return (monitors, DFT or None, envelope sums)."""
+    import numpy as np
+    if dft_planes:
+        plan.set_dft(dft_planes, dft_w, chunk=chunk)
+    z = np.zeros(2 * plan.grid.plane, dtype=np.complex128)
+    plan.write_monitors(z, z, z, z)
+    plan.write_fields(np.zeros(plan.flen), np.zeros(plan.flen))
+    sums = plan.run_steps(0, steps)
+    return plan.read_monitors(), (plan.read_dft() if dft_planes else None), sums, plan.read_fields()
+
+
+def s_comparison(prep, r32, r64, dft_planes):
+    """FP32 against FP64 on the |S| values FDTD would print: the monitor phasors and every
+    DFT (plane, frequency), projected on the source profile and divided by the envelope
+    sum of the steps run (sum00, F:159-168), which makes a partial run's |S| the size of a
+    whole run's.  Raises when an FP64 value is zero (a monitor the wave has not reached
+    compares nothing)."""
+    import numpy as np
+    (m32, d32, _, f32), (m64, d64, sums, f64) = r32, r64
+    env = float(sums[0])
+    vals = {}
+    for name, k in (("bottom", 0), ("top", 1)):
+        if prep.case.iB >= 0:
+            vals[name] = (overlap(prep, m32[2 * k], m32[2 * k + 1]), overlap(prep, m64[2 * k], m64[2 * k + 1]))
+    if dft_planes:
+        def phasors(d, m, f):
+            H = np.empty(2 * prep.case.plane, dtype=np.complex128)
+            E = np.empty_like(H)
+            H[0::2], H[1::2], E[0::2], E[1::2] = d[m, 0, f], d[m, 1, f], d[m, 2, f], d[m, 3, f]
+            return H, E
+        for m in range(len(dft_planes)):
+            for f in range(d64.shape[2]):
+                vals[f"dft_plane{m}_f{f}"] = (overlap(prep, *phasors(d32, m, f)), overlap(prep, *phasors(d64, m, f)))
+    errs = {}
+    for key, (a, b) in vals.items():
+        a, b = abs(a) / env, abs(b) / env
+        if not b > 0:
+            raise RuntimeError(f"|S| comparison: FP64 value '{key}' is zero; the wave has not reached it")
+        errs[key] = (abs(a - b), b)
+    worst = max(errs, key=lambda k: errs[k][0] / errs[k][1])
+    return dict(S_rel_err_max=max(e / b for e, b in errs.values()), S_rel_err_at=worst,
+                S_abs_err_max=max(e for e, _ in errs.values()), S_abs_min=min(b for _, b in errs.values()),
+                S_abs_max=max(b for _, b in errs.values()), compared=len(errs),
+                monitors={k: dict(abs_fp64=errs[k][1], abs_err=errs[k][0]) for k in ("bottom", "top") if k in errs},
+                envelope_sum=env,
+                fields_rel_l2=float(np.linalg.norm(np.concatenate([f32[0] - f64[0], f32[1] - f64[1]])) /
+                                    np.linalg.norm(np.concatenate(f64))))
+
+
 def build_workload(small=False, box=False):
     from smatranem_amd import casegen
     L, pml = ((32, 32, 48), 4) if small else ((160, 160, 320), 10)
@@ -112,6 +167,7 @@ def time_plan(plan, steps, warmup, torch):
     ev0.record(stream)
     t0 = time.perf_counter()
     plan.run_steps(warmup, steps)
+    plan.flush()  # the DFT and monitor samples still in the ring are folded inside the region
     ev1.record(stream)
     plan.sync()
     torch.cuda.synchronize()
@@ -315,14 +371,24 @@ def large_run(torch, steps, depth=256, small=False):
 
 
 def cpu_model():
+    """Model name, family/model/stepping and logical CPU counts of this host."""
+    info = {}
     try:
         with open("/proc/cpuinfo") as f:
             for ln in f:
-                if ln.startswith("model name"):
-                    return ln.split(":", 1)[1].strip()
+                k, _, v = ln.partition(":")
+                k = k.strip()
+                if k in ("model name", "cpu family", "model", "stepping") and k not in info:
+                    info[k] = v.strip()
+                if k == "processor":
+                    info["logical_cpus"] = info.get("logical_cpus", 0) + 1
     except OSError:
+        return None
+    try:
+        info["usable_cpus"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
         pass
-    return None
+    return ", ".join(f"{k} {v}" for k, v in info.items())
 
 
 def cpu_baseline(prep, seconds, all_cores=False):
@@ -447,7 +513,9 @@ def main():
     print(f"[bench] grid {case.L[:3]} n={n} N_2pi={prep.N} phi={phi:.4f} setup {setup_s:.1f}s",
           file=sys.stderr, flush=True)
 
-    out = {}
+    # the accuracy run: the envelope's ramp, then ACC_CW_STEPS CW steps, from zero fields,
+    # whatever --steps is (the wave crosses the grid; a few steps would compare zeros)
+    acc_steps = case.time_period[0] * prep.N + ACC_CW_STEPS
     # FP32 path: the autotuner picks the step kernel (fused tile shape or the two sweeps)
     with prep.plan(fdtd.F32) as p32:
         if dft_planes:
@@ -460,9 +528,7 @@ def main():
             kernel += " (chunks balanced, z-PML stage counted %g)" % (1 + p32.z_balance / 2)
         print(f"[bench] autotuner: {kernel}", file=sys.stderr, flush=True)
         wall_ms, ev_ms = time_plan(p32, K, W, torch)
-        H32, E32 = p32.read_fields()
-        m32 = p32.read_monitors()
-        d32 = p32.read_dft() if dft_planes else None
+        acc32 = None if args.no_fp64 else run_from_zero(p32, dft_planes, dft_w, dft_chunk, acc_steps)
     ms_step = wall_ms / K
     gcells = n / (ms_step * 1e-3) / 1e9
     b32 = bytes_per_cell(phi, 4, 4)
@@ -478,9 +544,7 @@ def main():
             if dft_planes:
                 p64.set_dft(dft_planes, dft_w, chunk=dft_chunk)
             wall64, ev64 = time_plan(p64, K, W, torch)
-            H64, E64 = p64.read_fields()
-            m64 = p64.read_monitors()
-            d64 = p64.read_dft() if dft_planes else None
+            acc64 = run_from_zero(p64, dft_planes, dft_w, dft_chunk, acc_steps)
         ms64 = wall64 / K
         g64 = n / (ms64 * 1e-3) / 1e9
         b64 = bytes_per_cell(phi, 8, 8)
@@ -490,49 +554,44 @@ def main():
                                   unit="GB/s", frac=round(ach64 / HBM_PEAK_GBS, 4),
                                   bytes_per_cell=round(b64, 2)),
                     speedup_fp32_over_fp64=round(gcells / g64, 3))
-        S = {}
-        for name, (h32, e32, h64, e64) in {"bottom": (m32[0], m32[1], m64[0], m64[1]),
-                                           "top": (m32[2], m32[3], m64[2], m64[3])}.items():
-            s32, s64 = overlap(prep, h32, e32), overlap(prep, h64, e64)
-            S[name] = dict(abs_fp32=abs(s32), abs_fp64=abs(s64), abs_err=abs(abs(s32) - abs(s64)),
-                           rel_err=abs(abs(s32) - abs(s64)) / max(abs(s64), 1e-300))
-        # the 16-frequency DFT on the two port planes, projected the same way
-        if dft_planes:
-            def phasors(d, m, f):
-                H = np.empty(2 * case.plane, dtype=np.complex128)
-                E = np.empty_like(H)
-                H[0::2], H[1::2], E[0::2], E[1::2] = d[m, 0, f], d[m, 1, f], d[m, 2, f], d[m, 3, f]
-                return H, E
-            dft_err = []
-            for m in range(len(dft_planes)):
-                for f in range(len(dft_w)):
-                    s32, s64 = overlap(prep, *phasors(d32, m, f)), overlap(prep, *phasors(d64, m, f))
-                    dft_err.append((abs(abs(s32) - abs(s64)), abs(s64)))
-            S["dft_16_frequencies"] = dict(planes=dft_planes, w_per_step=[round(x, 6) for x in dft_w], chunk=dft_chunk,
-                                           abs_err=max(e for e, _ in dft_err),
-                                           abs_fp64_max=max(r for _, r in dft_err),
-                                           err_rel_to_max=max(e for e, _ in dft_err) /
-                                           max(max(r for _, r in dft_err), 1e-300))
-        err = dict(S_abs_err_max=max(v["abs_err"] for v in S.values()), S=S,
-                   fields_rel_l2=float(np.linalg.norm(np.concatenate([H32 - H64, E32 - E64])) /
-                                       np.linalg.norm(np.concatenate([H64, E64]))))
-        print(f"[bench] fp64: {ms64:.4f} ms/step, {g64:.2f} Gcell/s; |S| err "
-              f"{err['S_abs_err_max']:.2e}", file=sys.stderr, flush=True)
+        err = s_comparison(prep, acc32, acc64, dft_planes)
+        err.update(steps=acc_steps, ramp_steps=case.time_period[0] * prep.N, bar=1e-3,
+                   passes=bool(err["S_rel_err_max"] <= 1e-3 and err["S_abs_err_max"] <= 1e-3),
+                   dft=dict(planes=dft_planes, w_per_step=[round(x, 6) for x in dft_w], chunk=dft_chunk)
+                   if dft_planes else None,
+                   definition="|S| = |overlap with the source profile| / envelope sum of the steps run; "
+                              "error = max over the reference monitors and every DFT (plane, frequency) of "
+                              "| |S_fp32| - |S_fp64| |, relative = that / |S_fp64|")
+        print(f"[bench] fp64: {ms64:.4f} ms/step, {g64:.2f} Gcell/s; |S| err {err['S_abs_err_max']:.2e} "
+              f"abs, {err['S_rel_err_max']:.2e} rel over {err['compared']} values ({acc_steps} steps)",
+              file=sys.stderr, flush=True)
 
     box = None
     if not args.no_box:
+        # the headline grid as the survey sizes it: 10-cell PML on all six faces (the
+        # reference monitors off: every plane holds PML cells) and the 16-frequency DFT on
+        # the two port planes
         bcase = build_workload(args.small, box=True)
         bprep = casegen.prepare(bcase)
+        bpml = 4 if args.small else 10
+        bplanes = [] if args.no_dft else [bpml + 2, int(bcase.L[2]) - 1 - bpml - 2]
         with bprep.plan(fdtd.F32) as pb:
+            if bplanes:
+                pb.set_dft(bplanes, dft_w, chunk=dft_chunk)
             pb.autotune(20)
+            bkern = pb.kernels
             kb = max(K // 5, 100)
             wb, evb = time_plan(pb, kb, min(W, 100), torch)
         bphi = bprep.pml_fraction
         bb = bytes_per_cell(bphi, 4, 4)
-        box = dict(workload="same grid, 10-cell PML on all six faces, reference monitors off",
-                   steps=kb, value=round(bcase.n / (wb / kb * 1e-3) / 1e9, 4), pml_fraction=round(bphi, 4),
+        box = dict(workload="same grid, 10-cell PML on all six faces, reference monitors off, "
+                            + ("16-frequency DFT on planes %s" % bplanes if bplanes else "no DFT"),
+                   kernel=bkern, steps=kb, value=round(bcase.n / (wb / kb * 1e-3) / 1e9, 4),
+                   pml_fraction=round(bphi, 4), bytes_per_cell=round(bb, 2),
+                   kernel_ms_per_step=round(evb / kb, 5),
                    roofline_frac=round(bb * bcase.n / (evb / kb * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
-        print(f"[bench] box-PML fp32: {box['value']:.2f} Gcell/s", file=sys.stderr, flush=True)
+        print(f"[bench] box-PML fp32: {box['value']:.2f} Gcell/s, {box['roofline_frac']:.3f} of peak ({bkern})",
+              file=sys.stderr, flush=True)
 
     batch = batch_small = None
     if not args.no_batch:
@@ -592,8 +651,18 @@ def main():
                    "small": bool(args.small)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "achieved_is": "algorithmic bytes (B_alg per cell x cells) / kernel time per step; the "
+                                    "two-step pass reads and writes once per two steps, so this effective "
+                                    "rate can exceed the device copy rate",
                      "traffic": None if traffic is None else round(traffic),
                      "traffic_unit": "B per step (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-scaled)",
+                     "traffic_source": os.path.relpath(PMC_SUMMARY, ROOT) + " (committed counter run of "
+                                       "the same kernel; null when it does not list this run's kernel)",
+                     "measured_GBps": None if traffic is None else round(traffic / (kern_ms * 1e-3) / 1e9, 1),
+                     "measured_frac": None if traffic is None else
+                     round(traffic / (kern_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "limiter": "L1-to-L2 read requests in flight x L2 latency (TCP_TCC_READ_REQ), not "
+                                "HBM bandwidth" if variant == 3 else "HBM",
                      "algorithmic_bytes_per_step": round(b32 * n),
                      "kernel": kernel + " (one step)", "copy_GBps": round(copy_gbs, 1),
                      "frac_of_copy": round(achieved / copy_gbs, 4),

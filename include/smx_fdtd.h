@@ -199,6 +199,14 @@ int smx_plan_run_batch_mode(smx_plan *const *plans, int nplans, long long step0,
 /* Wait for all queued work of the plan. */
 int smx_plan_sync(smx_plan *plan);
 
+/* Queue the folds of the DFT and monitor samples the plan still holds in its ring
+ * (smx_plan_set_dft) on the plan's stream, without waiting; a no-op without a DFT.
+ * Reads of the DFT or monitors do this themselves; a caller that times a run up to an
+ * event on the plan's stream calls it before recording the event.  Folding earlier than
+ * the chunk boundary changes only how the samples are grouped in the FP64 sums.  No
+ * counterpart in the reference. */
+int smx_plan_flush(smx_plan *plan);
+
 /*
  * Extension: running DFT of the tangential fields on up to 8 z-planes at up to 32
  * frequencies (the reference's monitors, F:318-328, sample one frequency on two

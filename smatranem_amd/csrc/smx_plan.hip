@@ -1136,6 +1136,12 @@ int smx_plan_sync(smx_plan *p)
     return SMX_OK;
 }
 
+int smx_plan_flush(smx_plan *p)
+{
+    if (!p) return arg_fail("null pointer");
+    return dft_flush(p);
+}
+
 /* ---------------------------------------------------------------------------------- */
 /* Adjoint gradient (extension)                                                        */
 /* ---------------------------------------------------------------------------------- */

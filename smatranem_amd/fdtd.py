@@ -110,6 +110,8 @@ def lib():
     L.smx_plan_autotune.restype = C.c_int
     L.smx_plan_sync.argtypes = [C.c_void_p]
     L.smx_plan_sync.restype = C.c_int
+    L.smx_plan_flush.argtypes = [C.c_void_p]
+    L.smx_plan_flush.restype = C.c_int
     L.smx_set_tau.argtypes = [C.c_double, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double),
                               C.POINTER(C.c_double)]
     L.smx_set_tau.restype = C.c_int
@@ -425,3 +427,7 @@ class Plan:
 
     def sync(self):
         _check(lib().smx_plan_sync(self._h))
+
+    def flush(self):
+        """Queue the folds of the DFT and monitor samples still in the ring (no wait)."""
+        _check(lib().smx_plan_flush(self._h))
