@@ -229,6 +229,22 @@ int smx_plan_set_dft(smx_plan *plan, int nplanes, const int *planes, int nfreq, 
 int smx_plan_read_dft(smx_plan *plan, double *out);
 
 /*
+ * The mode projection of S_calc (F:437-445) on the device: for each column t of tilXi
+ * given (modes: complex[nmodes][4*L0*L1], rows in tilXi's order, F:598-628), the sum
+ * over q of conj(t[q]) Hx[q] + conj(t[q+P]) Hy[q] + conj(t[q+2P]) (-Ey[q]) +
+ * conj(t[q+3P]) Ex[q], P = L0*L1, of
+ *   S_mon: complex[2][nmodes] -- the monitors (HB, EB) then (HT, ET); a monitor that
+ *          is off gives zeros;
+ *   S_dft: complex[nplanes][nfreq][nmodes] -- every plane and frequency of the DFT set
+ *          by smx_plan_set_dft (SMX_ERR_STATE without one).
+ * Either output may be NULL, not both.  Pending samples are folded first.  FP64 with a
+ * fixed summation order (rows in chunks of 256 reduced by a tree, chunks in order): it
+ * differs from the reference's sequential sum only by rounding.  Replaces the loop of
+ * S_calc, F:437-445, for device-resident monitors.
+ */
+int smx_plan_project(smx_plan *plan, int nmodes, const double *modes, double *S_mon, double *S_dft);
+
+/*
  * Extension: adjoint gradient of a port objective (FP32 plans without lossy or metal
  * cells).  No counterpart in the reference.
  *   J = sum_{v = x, y} sum_q |D_v(q)|^2,  D_v(q) = sum_k e^{i*w*(k+1.5)} E_v(q, zT) after step k

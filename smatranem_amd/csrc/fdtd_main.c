@@ -12,6 +12,9 @@
  *  - check_Xi (F:495-496, a BLAS residual printed only when > 1e-15) is not computed.
  *  - SMX_FDTD_PRECISION=f32 in the environment selects the FP32 plan (default f64,
  *    bit-identical to the reference).
+ *  - SMX_FDTD_PROJECT=gpu projects the monitors on the mode columns on the device
+ *    (smx_plan_project) instead of on the host in the reference's order; the |S|^2
+ *    printed then differ from the reference's by rounding only.
  *  - SMX_FDTD_SETUP_DUMP=<file> is a dry run that needs no GPU: after the plane indices
  *    are reported, the program writes what it read from the files to <file> and exits
  *    with status 0 (layout at dump_setup).
@@ -395,6 +398,57 @@ static int project(const char *prefix, const int L[4], int plane, const double c
     return n_real;
 }
 
+typedef struct {
+    int NA, n_real;
+    double complex *modes; /* [2*n_real][2NA]: tilXi columns j < n_real, then j + NA */
+} cols_ctx;
+
+/* Block j < n_real gives tilXi columns j and j+NA (F:617-619): rows rotated by NA, the
+ * second column negated (as project_block forms them). */
+static void collect_block(void *p, int j, int row, double complex a, double complex b)
+{
+    cols_ctx *c = p;
+    if (j >= c->n_real) return;
+    const int r = (row + c->NA) % (2 * c->NA);
+    c->modes[(size_t)j * 2 * c->NA + r] = a;
+    c->modes[(size_t)(c->n_real + j) * 2 * c->NA + r] = -b;
+}
+
+static void check(int rc);
+
+/* S_calc (F:419-448) for the printed columns of one port, projected on the device from
+ * the plan's monitors (smx_plan_project); port 0 bottom, 1 top. */
+static int project_gpu(smx_plan *plan, const char *prefix, const int L[4], int plane, int port,
+                       double complex **S_out)
+{
+    char xi[LINE], md[LINE];
+    if (plane >= 0 && plane < L[3]) {
+        snprintf(xi, sizeof xi, "%s_bXi.dat", prefix);
+        snprintf(md, sizeof md, "%s_bMode.dat", prefix);
+    } else if (plane >= L[2] - L[3] && plane < L[2]) {
+        snprintf(xi, sizeof xi, "%s_tXi.dat", prefix);
+        snprintf(md, sizeof md, "%s_tMode.dat", prefix);
+    } else {
+        fprintf(stderr, "iB or iT = %d \n", plane);
+        exit(EXIT_FAILURE);
+    }
+    const int NA = 2 * L[0] * L[1];
+    const int n_real = real_mode_count(md, L);
+    double complex *S = calloc(2 * (size_t)(n_real + 1), sizeof *S);
+    if (n_real > 0) {
+        cols_ctx c = {NA, n_real, calloc((size_t)2 * n_real * 2 * (size_t)NA, sizeof(double complex))};
+        double complex *smon = calloc(2 * (size_t)2 * n_real, sizeof *smon);
+        if (!c.modes || !smon) die("out of memory in S_calc\n");
+        stream_xi(xi, NA, n_real, collect_block, &c);
+        check(smx_plan_project(plan, 2 * n_real, (const double *)c.modes, (double *)smon, NULL));
+        memcpy(S, smon + (size_t)port * 2 * n_real, sizeof *S * 2 * (size_t)n_real);
+        free(c.modes);
+        free(smon);
+    }
+    *S_out = S;
+    return n_real;
+}
+
 static void check(int rc)
 {
     if (rc == SMX_OK) return;
@@ -540,13 +594,20 @@ int main(int argc, char **argv)
     double complex *mon[4];
     for (int k = 0; k < 4; k++) mon[k] = calloc(2 * (size_t)P, sizeof(double complex));
     check(smx_plan_read_monitors(plan, (double *)mon[0], (double *)mon[1], (double *)mon[2], (double *)mon[3]));
+    double complex *SB = NULL, *ST = NULL;
+    int nB, nT;
+    const char *proj = getenv("SMX_FDTD_PROJECT");
+    if (proj && !strcmp(proj, "gpu")) { /* on the device, from the plan's monitors */
+        nB = project_gpu(plan, in.prefix, L, iB, 0, &SB);
+        nT = project_gpu(plan, in.prefix, L, iT, 1, &ST);
+    } else { /* on the host, in the reference's summation order */
+        nB = project(in.prefix, L, iB, mon[0], mon[1], &SB);
+        nT = project(in.prefix, L, iT, mon[2], mon[3], &ST);
+    }
     smx_plan_destroy(plan);
     free(Hinc);
     free(Einc);
 
-    double complex *SB = NULL, *ST = NULL;
-    const int nB = project(in.prefix, L, iB, mon[0], mon[1], &SB);
-    const int nT = project(in.prefix, L, iT, mon[2], mon[3], &ST);
     fprintf(stderr, "Bottom |S_(m,%+d)|^2 :\n", in.inc_mode);
     for (int m = 0; m < nB; m++)
         fprintf(stderr, "%+d, %.20e, %+d, %.20e\n", m + 1, creal(conj(SB[m]) * SB[m]), -(m + 1),

@@ -22,6 +22,7 @@
 #include "smx_adjoint.h"
 #include "smx_dft.h"
 #include "smx_f32.h"
+#include "smx_project.h"
 
 namespace {
 
@@ -1140,6 +1141,52 @@ int smx_plan_flush(smx_plan *p)
 {
     if (!p) return arg_fail("null pointer");
     return dft_flush(p);
+}
+
+int smx_plan_project(smx_plan *p, int nmodes, const double *modes, double *S_mon, double *S_dft)
+{
+    if (!p || !modes || (!S_mon && !S_dft)) return arg_fail("null pointer");
+    if (nmodes < 1) return arg_fail("nmodes must be >= 1");
+    if (S_dft && !p->dft_nplanes) {
+        snprintf(g_last_error, sizeof g_last_error, "smx_plan_project: no DFT set");
+        return SMX_ERR_STATE;
+    }
+    int rc = dft_flush(p); /* pending samples into the monitors and the DFT accumulators */
+    if (rc) return rc;
+    smx_project_args a;
+    std::memset(&a, 0, sizeof a);
+    a.P = p->plane;
+    a.nmodes = nmodes;
+    a.nmon = S_mon ? 2 : 0;
+    a.nfreq = S_dft ? p->dft_nfreq : 0;
+    a.nvec = a.nmon + (S_dft ? p->dft_nplanes * p->dft_nfreq : 0);
+    a.nchunks = smx_project_chunks(a.P);
+    a.mon = reinterpret_cast<const double2 *>(p->mon);
+    a.dft = reinterpret_cast<const double2 *>(p->dft_acc);
+    const size_t mbytes = sizeof(double2) * (size_t)nmodes * 4 * (size_t)a.P;
+    const size_t pbytes = sizeof(double2) * (size_t)a.nvec * nmodes * (size_t)a.nchunks;
+    const size_t sbytes = sizeof(double2) * (size_t)a.nvec * nmodes;
+    void *dm = nullptr, *dp = nullptr, *ds = nullptr;
+    hipError_t e = hipMalloc(&dm, mbytes);
+    if (e == hipSuccess) e = hipMalloc(&dp, pbytes);
+    if (e == hipSuccess) e = hipMalloc(&ds, sbytes);
+    if (e == hipSuccess) e = hipMemcpyAsync(dm, modes, mbytes, hipMemcpyHostToDevice, p->stream);
+    if (e == hipSuccess) {
+        a.modes = static_cast<const double2 *>(dm);
+        a.partial = static_cast<double2 *>(dp);
+        a.S = static_cast<double2 *>(ds);
+        e = smx_project_launch(&a, p->stream);
+    }
+    std::vector<double> S(2 * (size_t)a.nvec * nmodes);
+    if (e == hipSuccess) e = hipMemcpyAsync(S.data(), ds, sbytes, hipMemcpyDeviceToHost, p->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+    for (void *b : {dm, dp, ds})
+        if (b) (void)hipFree(b);
+    if (e != hipSuccess) return hip_fail(e, "smx_plan_project");
+    const size_t per = 2 * (size_t)nmodes;
+    if (S_mon) std::memcpy(S_mon, S.data(), sizeof(double) * 2 * per);
+    if (S_dft) std::memcpy(S_dft, S.data() + a.nmon * per, sizeof(double) * (a.nvec - a.nmon) * per);
+    return SMX_OK;
 }
 
 /* ---------------------------------------------------------------------------------- */

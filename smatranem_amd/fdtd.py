@@ -112,6 +112,8 @@ def lib():
     L.smx_plan_sync.restype = C.c_int
     L.smx_plan_flush.argtypes = [C.c_void_p]
     L.smx_plan_flush.restype = C.c_int
+    L.smx_plan_project.argtypes = [C.c_void_p, C.c_int, _dp, C.c_void_p, C.c_void_p]
+    L.smx_plan_project.restype = C.c_int
     L.smx_set_tau.argtypes = [C.c_double, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double),
                               C.POINTER(C.c_double)]
     L.smx_set_tau.restype = C.c_int
@@ -381,6 +383,22 @@ class Plan:
         self._dft = (len(planes), len(w_per_step))
         _check(lib().smx_plan_set_dft(self._h, len(planes), _i32(planes), len(w_per_step),
                                       _f64(w_per_step), int(chunk)))
+
+    def project(self, modes, monitors=True, dft=False):
+        """smx_plan_project: the S_calc overlap (F:437-445) of each mode column (complex
+        [nmodes, 4*L0*L1], tilXi's row order) with the monitors and/or every DFT plane
+        and frequency, on the device.  Returns (S_mon [2, nmodes] or None,
+        S_dft [nplanes, nfreq, nmodes] or None)."""
+        modes = np.ascontiguousarray(np.asarray(modes, dtype=np.complex128))
+        nm = modes.shape[0]
+        smon = np.zeros((2, nm), dtype=np.complex128) if monitors else None
+        sdft = None
+        if dft:
+            m, f = getattr(self, "_dft", (1, 1))  # no DFT set: the library refuses (status 5)
+            sdft = np.zeros((m, f, nm), dtype=np.complex128)
+        ptr = lambda a: None if a is None else a.ctypes.data_as(C.c_void_p)  # noqa: E731
+        _check(lib().smx_plan_project(self._h, nm, modes.view(np.float64).reshape(-1), ptr(smon), ptr(sdft)))
+        return smon, sdft
 
     def read_dft(self):
         """complex128[nplanes, 4 (Hx, Hy, Ex, Ey), nfreq, L0*L1]."""

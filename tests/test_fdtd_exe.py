@@ -117,6 +117,29 @@ def test_fdtd_report_curvilinear(tmp_path, corr):
 
 
 @pytest.mark.gpu
+def test_fdtd_report_gpu_projection(tmp_path):
+    """SMX_FDTD_PROJECT=gpu: the |S|^2 come from smx_plan_project on the device; every
+    other line equals the reference's and the |S|^2 agree to 1e-12 relative."""
+    case = _case(scale_amp=0.3)
+    inp, Xi = _write(case, tmp_path)
+    rc, got = _run(inp, {"SMX_FDTD_PROJECT": "gpu"})
+    assert rc == 0, "\n".join(got)
+    exp = _expected(case, Xi, 3)
+    assert len(got) == len(exp)
+    n_s = 0
+    for a, b in zip(got, exp):
+        if a.startswith(("+", "-")):
+            va = [float(v) for v in a.split(",")[1::2]]
+            vb = [float(v) for v in b.split(",")[1::2]]
+            for x, y in zip(va, vb):
+                assert abs(x - y) <= 1e-12 * max(abs(y), 1e-300), (a, b)
+            n_s += 1
+        else:
+            assert a == b
+    assert n_s == 6
+
+
+@pytest.mark.gpu
 def test_fdtd_report_f32_close(tmp_path):
     case = _case()
     inp, Xi = _write(case, tmp_path)
