@@ -232,7 +232,7 @@ def batch_run(count, steps, torch, small=False, dims=((64, 192), (64, 192), (64,
                 one_after_another=round(cells * steps / t_seq / 1e9, 4))
 
 
-def adjoint_run(torch, small=False, fd_cells=0, fd_seed=None):
+def adjoint_run(torch, small=False, fd_cells=0, fd_seed=None, steps=None):
     """The gradient workload: a 96 x 96 x 128 design region of per-cell eps in [2.1, 12.1]
     inside a 192 x 192 x 256 grid with 10-cell PML on every face; forward 4000 FP32 steps
     with the full state checkpointed every 200 steps (smx_adjoint_forward), then the
@@ -249,29 +249,46 @@ def adjoint_run(torch, small=False, fd_cells=0, fd_seed=None):
         L, pml, K, C = (48, 48, 64), 6, 400, 50
     else:
         L, pml, K, C = (192, 192, 256), 10, 4000, 200
+    if steps:
+        K, C = steps, min(C, steps)
     case, box = casegen.adjoint_case(L, pml=pml)
     prep = casegen.prepare(case)
     zT = L[2] - pml - 3
     w = 2 * np.pi / prep.N
+    n = case.n
+    nbox = (box[1] - box[0]) * (box[3] - box[2]) * (box[5] - box[4])
+    modes = {}
+    g = None
     with prep.plan(fdtd.F32) as p:
         p.autotune(10)
         kern = p.kernels
-        p.sync()
-        t0 = time.perf_counter()
-        J = p.adjoint_forward(K, C, zT, w, box)
-        t1 = time.perf_counter()
-        g = p.adjoint_backward()
-        t2 = time.perf_counter()
-    n = case.n
-    nbox = (box[1] - box[0]) * (box[3] - box[2]) * (box[5] - box[4])
-    ckpt_gb = 12 * 4 * n * ((K + C - 1) // C) / 1e9
+        # checkpoints every C steps with recomputation; then the design box's E stored
+        # after every step (ckpt_every = 0), no recomputation
+        for ck in (C, 0):
+            p.sync()
+            t0 = time.perf_counter()
+            J = p.adjoint_forward(K, ck, zT, w, box)
+            t1 = time.perf_counter()
+            gk = p.adjoint_backward()
+            t2 = time.perf_counter()
+            if g is None:
+                g = gk
+            mem = 12 * 4 * n * ((K + ck - 1) // ck) if ck else 3 * 4 * nbox * (K + 1)
+            modes["checkpoint_recompute" if ck else "box_history"] = dict(
+                ckpt_every=ck, forward_s=round(t1 - t0, 3), backward_s=round(t2 - t1, 3),
+                forward_gcells=round(n * K / (t1 - t0) / 1e9, 3), backward_gcells=round(n * K / (t2 - t1) / 1e9, 3),
+                stored_GB=round(mem / 1e9, 2), J=J,
+                grad_rel_diff_vs_checkpoint=float(np.linalg.norm(gk - g) / np.linalg.norm(g)))
+            print(f"[bench] adjoint {'every %d' % ck if ck else 'box history'}: forward "
+                  f"{modes[list(modes)[-1]]['forward_gcells']} Gcell/s, backward "
+                  f"{modes[list(modes)[-1]]['backward_gcells']} Gcell/s", file=sys.stderr, flush=True)
+    ref = modes["checkpoint_recompute"]
     out = dict(workload="adjoint gradient, %dx%dx%d grid, %d-cell PML, design box %s (eps U[2.1, 12.1]), "
                         "%d FP32 steps, checkpoints every %d" % (L[0], L[1], L[2], pml, list(box), K, C),
                cells=n, design_cells=nbox, steps=K, ckpt_every=C, step_kernel=kern,
-               forward_s=round(t1 - t0, 3), backward_s=round(t2 - t1, 3),
-               forward_gcells=round(n * K / (t1 - t0) / 1e9, 3),
-               backward_gcells=round(n * K / (t2 - t1) / 1e9, 3),
-               checkpoint_GB=round(ckpt_gb, 2), J=J, grad_l2=float(np.linalg.norm(g)))
+               forward_s=ref["forward_s"], backward_s=ref["backward_s"], forward_gcells=ref["forward_gcells"],
+               backward_gcells=ref["backward_gcells"], checkpoint_GB=ref["stored_GB"], J=ref["J"],
+               grad_l2=float(np.linalg.norm(g)), modes=modes)
     if fd_cells > 0:
         def J64(eps):
             c2 = casegen.Case(**{**case.__dict__, "eps": eps})
@@ -466,6 +483,8 @@ def main():
                          "200 M cells)")
     ap.add_argument("--large-depth", type=int, default=256, metavar="DEPTH",
                     help="z cells of the large grid (1024: 768x768x1024, 604 M cells)")
+    ap.add_argument("--adjoint-steps", type=int, default=0, metavar="K",
+                    help="steps of the adjoint workload (default 4000, checkpoints every 200)")
     ap.add_argument("--adjoint-fd", type=int, default=0, metavar="CELLS",
                     help="check the adjoint gradient at CELLS cells against FP64 central differences")
     ap.add_argument("--adjoint-fd-seed", type=int, default=None, metavar="SEED",
@@ -612,9 +631,7 @@ def main():
 
     adjoint = None
     if not args.no_adjoint:
-        adjoint = adjoint_run(torch, args.small, args.adjoint_fd, args.adjoint_fd_seed)
-        print(f"[bench] adjoint: forward {adjoint['forward_gcells']:.2f} Gcell/s, backward "
-              f"{adjoint['backward_gcells']:.2f} Gcell/s", file=sys.stderr, flush=True)
+        adjoint = adjoint_run(torch, args.small, args.adjoint_fd, args.adjoint_fd_seed, args.adjoint_steps)
 
     copy_gbs = copy_bandwidth(torch)
     print(f"[bench] device copy: {copy_gbs:.0f} GB/s", file=sys.stderr, flush=True)

@@ -28,6 +28,7 @@
 #include "smx_f32.h"
 #include "smx_fdtd.h"
 #include "smx_setup.h"
+#include "smx_stage.h"
 
 namespace {
 
@@ -43,7 +44,6 @@ thread_local char f32_err[512] = "";
     } while (0)
 
 constexpr int kThreads = 256;
-constexpr int kXCD = 8; /* MI355X: 8 accelerator complex dies, blocks dealt round-robin */
 
 struct f32_args {
     int L0, L1, L2;
@@ -67,13 +67,6 @@ struct f32_args {
     double monH_re, monH_im, monE_re, monE_im;
 };
 
-/* Hardware block h runs on XCD h % 8; give each XCD a contiguous run of logical ids. */
-__device__ inline int xcd_logical(int h, int nblocks)
-{
-    const int q = nblocks / kXCD, r = nblocks % kXCD;
-    const int x = h % kXCD, s = h / kXCD;
-    return x * q + (x < r ? x : r) + s;
-}
 
 struct cellpos {
     int j0, j1, j2;
@@ -481,11 +474,6 @@ struct fused_args {
     int dft_planes[kDftMaxPlanes];
 };
 
-__device__ inline int wrap_index(int v, int L)
-{
-    const int r = v % L;
-    return r < 0 ? r + L : r;
-}
 
 __device__ inline double plane_source(const double *plane, const double *inc, double f2, double re, double im,
                                       int64_t q, int v)
@@ -913,18 +901,6 @@ struct dma_layout {
     static constexpr int kMaxFZ = 64;
 };
 
-constexpr unsigned kBufWord3 = 0x00020000u; /* raw 32-bit buffer resource (gfx9 family) */
-
-/* Buffer resource over one group of 6 component arrays (6*ncomp floats). */
-__device__ inline __amdgpu_buffer_rsrc_t group_rsrc(const float *base, int64_t ncomp)
-{
-    return __builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, (int)(unsigned)(24 * ncomp), (int)kBufWord3);
-}
-
-/* LDS address of a generic pointer into the kernel's LDS array: its low 32 bits (a
- * plain cast would add a null test, which the code generator cannot emit when the
- * pointer folds to the array's base) */
-#define SMX_LDS(p) ((__attribute__((address_space(3))) void *)(size_t)(unsigned)(size_t)(p))
 
 /* Copies of stage slot `slot` (plane byte offset soff) dealt to wave W.  Component c of
  * a group starts c*cb bytes into the group's buffer. */
@@ -970,29 +946,6 @@ __device__ inline void dma_issue(float *slot, __amdgpu_buffer_rsrc_t rT, __amdgp
     }
 }
 
-/* s_waitcnt vmcnt(VM) lgkmcnt(0), then s_barrier; the empty asm statements keep the
- * compiler from moving LDS accesses across it. */
-template <int VM>
-__device__ inline void wait_vm_barrier()
-{
-    static_assert(VM >= 0 && VM < 64, "vmcnt field");
-    asm volatile("" ::: "memory");
-    __builtin_amdgcn_s_waitcnt((VM & 0xF) | ((VM >> 4) << 14) | (0x7 << 4));
-    __builtin_amdgcn_s_barrier();
-    asm volatile("" ::: "memory");
-}
-
-/* Wait for every outstanding vector memory operation of this wave.  Used right after
- * the rare direct loads inside the staged loop (source planes, monitors, lossy/metal
- * cells): the compiler does not count the asynchronous copies in its own vmcnt
- * bookkeeping, so a load whose result it still thinks pending at a join would make it
- * wait for vmcnt(0) -- every copy in flight -- on the common path too. */
-__device__ inline void retire_loads()
-{
-    asm volatile("" ::: "memory");
-    __builtin_amdgcn_s_waitcnt((0x7 << 4) | (0xF << 8)); /* vmcnt(0) */
-    asm volatile("" ::: "memory");
-}
 
 /* New H of one regular cell from its curl halves: lossless X -= b*(P + M), or the PML
  * main/split pair with the axis factors pa (A) and pg (B/b) of the cell. */

@@ -22,7 +22,33 @@ struct smx_adj_args {
     double *D;                 /* [2][L0*L1] complex: the objective's DFT */
 };
 
+/* One fused, LDS-staged transposed step (adj_fused_f32): lambda^{k+1} -> lambda^k over
+ * the whole grid.  lambda buffers are groups of 6 component arrays, ncomp floats apart
+ * (24*ncomp < 2^32); T = total, S = split. */
+struct smx_adj_fused_args {
+    int L0, L1, L2;
+    int64_t sz, ncomp;
+    float w[3];
+    int gx, gy, fz;            /* tiles of kAdjFX x kAdjFY cells, z chunks of fz planes */
+    const int *blocks;         /* device: block ids bx + gx*(by + gy*bz), bit 30 set when the
+                                  block's reach holds a PML cell */
+    int n_pml, n_free;         /* blocks with and without PML in reach */
+    const float *lT, *lS;      /* lambda^{k+1} (T group, S group) */
+    float *nT, *nS;            /* lambda^k */
+    const float *b;            /* coefficient group: bH x y z, bE x y z */
+    const float *pml;
+    const unsigned char *pmlflag;
+    int has_pml;
+};
+
+constexpr int kAdjFX = 32, kAdjFY = 8, kAdjFZ = 32;
+
 extern "C" {
+/* Launch one transposed step over the block list. */
+hipError_t smx_adj_launch_fused(const smx_adj_fused_args *a, hipStream_t s);
+/* G_c += lambda_TE^{k+1}(box) * (E^{k+1} - E^k)(box), the box arrays [3][nbox]. */
+hipError_t smx_adj_launch_grad(const smx_adj_args *a, const float *lTE, const float *cur, const float *next,
+                               hipStream_t s);
 hipError_t smx_adj_launch_a(const smx_adj_args *a, int finish_h, const float *cur, const float *next, hipStream_t s);
 hipError_t smx_adj_launch_b(const smx_adj_args *a, int inject, double cs, double sn, hipStream_t s);
 hipError_t smx_adj_launch_inject(const smx_adj_args *a, double cs, double sn, hipStream_t s);

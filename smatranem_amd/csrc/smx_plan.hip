@@ -13,6 +13,7 @@
 #include <cstddef>
 #include <cstdio>
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -101,6 +102,7 @@ struct smx_plan {
     double adj_w = 0.;
     bool adj_ready = false;
     float *adj_ckpt = nullptr; /* [ceil(K/C)][12][n]: T[6], S[6] at steps 0, C, 2C, ... */
+    float *adj_hist = nullptr; /* C = 0: [K+1][3][nbox], E of the design box after every step */
     double *adj_D = nullptr;   /* [2][L0*L1] complex */
 };
 
@@ -207,7 +209,7 @@ void free_plan(smx_plan *p)
     if (!p) return;
     if (p->stream) (void)hipStreamSynchronize(p->stream);
     void *bufs[] = {p->Ah, p->Bh, p->Ae, p->Be, p->H, p->E, p->Cont, p->addr, p->mon, p->inc, p->src,
-                    p->dft_series, p->dft_acc, p->dft_table, p->adj_ckpt, p->adj_D, p->expn_dev};
+                    p->dft_series, p->dft_acc, p->dft_table, p->adj_ckpt, p->adj_D, p->expn_dev, p->adj_hist};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     smx_f32_free(&p->f32);
@@ -1336,7 +1338,7 @@ int smx_adjoint_forward(smx_plan *p, long long nsteps, int ckpt_every, int zT, d
         snprintf(g_last_error, sizeof g_last_error, "smx_adjoint_forward before smx_plan_set_source");
         return SMX_ERR_STATE;
     }
-    if (nsteps < 1 || ckpt_every < 1) return arg_fail("nsteps and ckpt_every must be >= 1");
+    if (nsteps < 1 || ckpt_every < 0) return arg_fail("nsteps must be >= 1 and ckpt_every >= 0");
     if (zT < 0 || zT >= p->g.L[2]) return arg_fail("objective plane outside the grid");
     for (int d = 0; d < 3; d++)
         if (box[2 * d] < 0 || box[2 * d + 1] > p->g.L[d] || box[2 * d] >= box[2 * d + 1])
@@ -1347,14 +1349,19 @@ int smx_adjoint_forward(smx_plan *p, long long nsteps, int ckpt_every, int zT, d
                 if (p->f32.cont_host[x + (int64_t)p->g.L[0] * (y + (int64_t)p->g.L[1] * z)] != 'n')
                     return arg_fail("design box holds a PML or special cell");
     const int64_t n = p->n;
-    const long long nck = (nsteps + ckpt_every - 1) / ckpt_every;
+    const bool hist = ckpt_every == 0;
+    const long long nck = hist ? 0 : (nsteps + ckpt_every - 1) / ckpt_every;
     SMX_HIP(hipStreamSynchronize(p->stream));
     if (p->adj_ckpt) SMX_HIP(hipFree(p->adj_ckpt));
     if (p->adj_D) SMX_HIP(hipFree(p->adj_D));
+    if (p->adj_hist) SMX_HIP(hipFree(p->adj_hist));
     p->adj_ckpt = nullptr;
     p->adj_D = nullptr;
+    p->adj_hist = nullptr;
     p->adj_ready = false;
-    SMX_HIP(hipMalloc(&p->adj_ckpt, sizeof(float) * 12 * (size_t)n * (size_t)nck));
+    const int64_t nbox = (int64_t)(box[1] - box[0]) * (box[3] - box[2]) * (box[5] - box[4]);
+    if (hist) SMX_HIP(hipMalloc(&p->adj_hist, sizeof(float) * 3 * (size_t)nbox * (size_t)(nsteps + 1)));
+    else SMX_HIP(hipMalloc(&p->adj_ckpt, sizeof(float) * 12 * (size_t)n * (size_t)nck));
     SMX_HIP(hipMalloc(&p->adj_D, sizeof(double) * 4 * (size_t)p->plane));
     SMX_HIP(hipMemsetAsync(p->adj_D, 0, sizeof(double) * 4 * (size_t)p->plane, p->stream));
     p->adj_K = nsteps;
@@ -1368,14 +1375,20 @@ int smx_adjoint_forward(smx_plan *p, long long nsteps, int ckpt_every, int zT, d
             if (p->f32.T[k][i]) SMX_HIP(hipMemsetAsync(p->f32.T[k][i], 0, sizeof(float) * n, p->stream));
             if (p->f32.S[k][i]) SMX_HIP(hipMemsetAsync(p->f32.S[k][i], 0, sizeof(float) * n, p->stream));
         }
-    if (adjoint_forward_paired(p, ckpt_every)) {
+    if (!hist && adjoint_forward_paired(p, ckpt_every)) {
         const int rc = adjoint_forward_two_step(p, nsteps, ckpt_every, zT, w, J);
         if (rc == SMX_OK) p->adj_ready = true;
         return rc;
     }
     smx_adj_args a = adj_args(p);
+    auto save_hist = [&](long long slot) {
+        float **T = p->f32.T[p->f32.cur];
+        return smx_adj_launch_save_box(&a, T[3], T[4], T[5], p->adj_hist + (size_t)slot * 3 * (size_t)nbox,
+                                       p->stream);
+    };
+    if (hist) SMX_HIP(save_hist(0));
     for (long long k = 0; k < nsteps; k++) {
-        if (k % ckpt_every == 0) {
+        if (!hist && k % ckpt_every == 0) {
             float *dst = p->adj_ckpt + (size_t)(k / ckpt_every) * 12 * (size_t)n;
             for (int i = 0; i < 6; i++) {
                 SMX_HIP(hipMemcpyAsync(dst + (size_t)i * n, p->f32.T[p->f32.cur][i], sizeof(float) * n,
@@ -1391,6 +1404,7 @@ int smx_adjoint_forward(smx_plan *p, long long nsteps, int ckpt_every, int zT, d
         const double th = w * ((double)k + 1.5);
         SMX_HIP(smx_adj_launch_sample(&a, p->f32.T[p->f32.cur][3], p->f32.T[p->f32.cur][4], std::cos(th),
                                       std::sin(th), p->stream));
+        if (hist) SMX_HIP(save_hist(k + 1));
     }
     std::vector<double> D(4 * (size_t)p->plane);
     SMX_HIP(hipMemcpyAsync(D.data(), p->adj_D, sizeof(double) * D.size(), hipMemcpyDeviceToHost, p->stream));
@@ -1411,7 +1425,13 @@ int smx_adjoint_backward(smx_plan *p, double *grad)
     }
     const int64_t n = p->n;
     smx_adj_args a = adj_args(p);
-    const long long K = p->adj_K, C = p->adj_C, nck = (K + C - 1) / C;
+    const long long K = p->adj_K, C = p->adj_C;
+    const bool hist = C == 0;
+    const long long seglen = hist ? K : C, nck = (K + seglen - 1) / seglen;
+    /* the fused transposed step addresses a group of 6 arrays with 32-bit offsets; larger
+     * grids take the two gather kernels (in place) */
+    const char *gather = getenv("SMX_ADJ_GATHER"); /* "1": the two gather kernels on any grid (tests) */
+    const bool fused = 24 * n < ((int64_t)1 << 32) && !(gather && !strcmp(gather, "1"));
     float *lam = nullptr, *seg = nullptr;
     double *G = nullptr, *gdev = nullptr;
     int rc = SMX_OK;
@@ -1420,19 +1440,66 @@ int smx_adjoint_backward(smx_plan *p, double *grad)
         return rc;
     };
     hipError_t e;
-    if ((e = hipMalloc(&lam, sizeof(float) * 12 * (size_t)n)) != hipSuccess) return fail(e, "hipMalloc(lambda)");
-    if ((e = hipMalloc(&seg, sizeof(float) * 3 * (size_t)a.nbox * (size_t)(C + 1))) != hipSuccess ||
+    const size_t lam_words = (fused ? 24 : 12) * (size_t)n;
+    if ((e = hipMalloc(&lam, sizeof(float) * lam_words)) != hipSuccess) return fail(e, "hipMalloc(lambda)");
+    if ((!hist && (e = hipMalloc(&seg, sizeof(float) * 3 * (size_t)a.nbox * (size_t)(C + 1))) != hipSuccess) ||
         (e = hipMalloc(&G, sizeof(double) * 3 * (size_t)a.nbox)) != hipSuccess ||
         (e = hipMalloc(&gdev, sizeof(double) * 3 * (size_t)a.nbox)) != hipSuccess) {
         fail(e, "hipMalloc(adjoint)");
     }
-    if (rc == SMX_OK) {
+    float *cur = lam, *nxt = lam + 12 * (size_t)n; /* lambda^{k+1}, lambda^k (fused) */
+    auto point = [&](float *buf) {
         for (int i = 0; i < 6; i++) {
-            a.lT[i] = lam + (size_t)i * n;
-            a.lS[i] = lam + (size_t)(6 + i) * n;
+            a.lT[i] = buf + (size_t)i * n;
+            a.lS[i] = buf + (size_t)(6 + i) * n;
         }
+    };
+    smx_adj_fused_args f;
+    std::memset(&f, 0, sizeof f);
+    f.L0 = a.L0, f.L1 = a.L1, f.L2 = a.L2, f.sz = a.sz, f.ncomp = n;
+    for (int d = 0; d < 3; d++) f.w[d] = a.w[d];
+    f.b = p->f32.b[0];
+    f.pml = p->f32.pml;
+    f.pmlflag = p->f32.pmlflag;
+    f.has_pml = p->f32.has_pml;
+    int *blocks_dev = nullptr;
+    if (fused && rc == SMX_OK) {
+        /* blocks whose reach -- x in [x0-1, x0+FX], y in [y0-1, y0+FY], z in [z0-1, z1] --
+         * holds a PML cell take the split instantiation; the list puts them first */
+        f.gx = (a.L0 + kAdjFX - 1) / kAdjFX;
+        f.gy = (a.L1 + kAdjFY - 1) / kAdjFY;
+        f.fz = kAdjFZ;
+        const int gz = (a.L2 + kAdjFZ - 1) / kAdjFZ;
+        std::vector<unsigned char> flag(a.L0 + a.L1 + a.L2);
+        if ((e = hipMemcpy(flag.data(), p->f32.pmlflag, flag.size(), hipMemcpyDeviceToHost)) != hipSuccess)
+            fail(e, "read PML flags");
+        auto any = [&](int off, int L, int lo, int hi) {
+            for (int v = lo; v <= hi; v++)
+                if (flag[off + ((v % L) + L) % L]) return true;
+            return false;
+        };
+        std::vector<int> ids;
+        for (int bz = 0; bz < gz; bz++)
+            for (int by = 0; by < f.gy; by++)
+                for (int bx = 0; bx < f.gx; bx++) {
+                    const int x0 = bx * kAdjFX, y0 = by * kAdjFY, z0 = bz * kAdjFZ;
+                    const int z1 = std::min(z0 + kAdjFZ, a.L2);
+                    const bool near = f.has_pml && (any(0, a.L0, x0 - 1, x0 + kAdjFX) ||
+                                                    any(a.L0, a.L1, y0 - 1, y0 + kAdjFY) ||
+                                                    any(a.L0 + a.L1, a.L2, z0 - 1, z1));
+                    ids.push_back((bx + f.gx * (by + f.gy * bz)) | (near ? 1 << 30 : 0));
+                    (near ? f.n_pml : f.n_free)++;
+                }
+        if (rc == SMX_OK && ((e = hipMalloc(&blocks_dev, sizeof(int) * ids.size())) != hipSuccess ||
+                             (e = hipMemcpy(blocks_dev, ids.data(), sizeof(int) * ids.size(),
+                                            hipMemcpyHostToDevice)) != hipSuccess))
+            fail(e, "adjoint block list");
+        f.blocks = blocks_dev;
+    }
+    if (rc == SMX_OK) {
+        point(cur);
         a.G = G;
-        (void)hipMemsetAsync(lam, 0, sizeof(float) * 12 * (size_t)n, p->stream);
+        (void)hipMemsetAsync(lam, 0, sizeof(float) * lam_words, p->stream);
         (void)hipMemsetAsync(G, 0, sizeof(double) * 3 * (size_t)a.nbox, p->stream);
         const double th_last = p->adj_w * ((double)(K - 1) + 1.5);
         if ((e = smx_adj_launch_inject(&a, std::cos(th_last), std::sin(th_last), p->stream)) != hipSuccess)
@@ -1440,36 +1507,62 @@ int smx_adjoint_backward(smx_plan *p, double *grad)
     }
     bool first = true;
     for (long long s = nck - 1; s >= 0 && rc == SMX_OK; s--) {
-        const long long ks = s * C, ke = std::min(ks + C, K);
-        const float *src = p->adj_ckpt + (size_t)s * 12 * (size_t)n;
-        for (int i = 0; i < 6 && rc == SMX_OK; i++) {
-            if ((e = hipMemcpyAsync(p->f32.T[p->f32.cur][i], src + (size_t)i * n, sizeof(float) * n,
-                                    hipMemcpyDeviceToDevice, p->stream)) != hipSuccess ||
-                (e = hipMemcpyAsync(p->f32.S[p->f32.cur][i], src + (size_t)(6 + i) * n, sizeof(float) * n,
-                                    hipMemcpyDeviceToDevice, p->stream)) != hipSuccess)
-                fail(e, "restore checkpoint");
-        }
-        auto save = [&](long long slot) {
-            float **T = p->f32.T[p->f32.cur];
-            hipError_t ee = smx_adj_launch_save_box(&a, T[3], T[4], T[5], seg + (size_t)slot * 3 * a.nbox, p->stream);
-            if (ee != hipSuccess) fail(ee, "save design box");
+        const long long ks = s * seglen, ke = std::min(ks + seglen, K);
+        /* E of the design box after step k - 1 (state k) */
+        auto box_state = [&](long long k) -> const float * {
+            return hist ? p->adj_hist + (size_t)k * 3 * a.nbox : seg + (size_t)(k - ks) * 3 * a.nbox;
         };
-        if (rc == SMX_OK) save(0);
-        for (long long k = ks; k < ke && rc == SMX_OK; k++) {
-            smx_step_scalars sc;
-            double fH, fE;
-            if ((rc = step_scalars(p, (int)(k / p->N), (int)(k % p->N), &sc, &fH, &fE))) break;
-            if ((rc = launch_step(p, sc, false))) break;
-            save(k - ks + 1);
+        if (!hist) { /* recompute the segment from its checkpoint, saving the box after every step */
+            const float *src = p->adj_ckpt + (size_t)s * 12 * (size_t)n;
+            for (int i = 0; i < 6 && rc == SMX_OK; i++) {
+                if ((e = hipMemcpyAsync(p->f32.T[p->f32.cur][i], src + (size_t)i * n, sizeof(float) * n,
+                                        hipMemcpyDeviceToDevice, p->stream)) != hipSuccess ||
+                    (e = hipMemcpyAsync(p->f32.S[p->f32.cur][i], src + (size_t)(6 + i) * n, sizeof(float) * n,
+                                        hipMemcpyDeviceToDevice, p->stream)) != hipSuccess)
+                    fail(e, "restore checkpoint");
+            }
+            auto save = [&](long long slot) {
+                float **T = p->f32.T[p->f32.cur];
+                hipError_t ee =
+                    smx_adj_launch_save_box(&a, T[3], T[4], T[5], seg + (size_t)slot * 3 * a.nbox, p->stream);
+                if (ee != hipSuccess) fail(ee, "save design box");
+            };
+            if (rc == SMX_OK) save(0);
+            for (long long k = ks; k < ke && rc == SMX_OK; k++) {
+                smx_step_scalars sc;
+                double fH, fE;
+                if ((rc = step_scalars(p, (int)(k / p->N), (int)(k % p->N), &sc, &fH, &fE))) break;
+                if ((rc = launch_step(p, sc, false))) break;
+                save(k - ks + 1);
+            }
         }
         for (long long k = ke - 1; k >= ks && rc == SMX_OK; k--) {
-            const float *cur = seg + (size_t)(k - ks) * 3 * a.nbox, *next = cur + 3 * a.nbox;
-            if ((e = smx_adj_launch_a(&a, first ? 0 : 1, cur, next, p->stream)) != hipSuccess) {
+            const float *bcur = box_state(k), *bnext = box_state(k + 1);
+            const double th = p->adj_w * ((double)(k - 1) + 1.5);
+            if (fused) {
+                /* gradient with lambda_TE^{k+1}, then lambda^k, then the objective's sample of state k */
+                if ((e = smx_adj_launch_grad(&a, cur + 3 * (size_t)n, bcur, bnext, p->stream)) != hipSuccess) {
+                    fail(e, "adjoint gradient");
+                    break;
+                }
+                f.lT = cur, f.lS = cur + 6 * (size_t)n, f.nT = nxt, f.nS = nxt + 6 * (size_t)n;
+                if ((e = smx_adj_launch_fused(&f, p->stream)) != hipSuccess) {
+                    fail(e, "adjoint step");
+                    break;
+                }
+                std::swap(cur, nxt);
+                point(cur);
+                if (k >= 1 && (e = smx_adj_launch_inject(&a, std::cos(th), std::sin(th), p->stream)) != hipSuccess) {
+                    fail(e, "adjoint inject");
+                    break;
+                }
+                continue;
+            }
+            if ((e = smx_adj_launch_a(&a, first ? 0 : 1, bcur, bnext, p->stream)) != hipSuccess) {
                 fail(e, "adjoint kernel A");
                 break;
             }
             first = false;
-            const double th = p->adj_w * ((double)(k - 1) + 1.5);
             if ((e = smx_adj_launch_b(&a, k >= 1, std::cos(th), std::sin(th), p->stream)) != hipSuccess) {
                 fail(e, "adjoint kernel B");
                 break;
@@ -1488,6 +1581,7 @@ int smx_adjoint_backward(smx_plan *p, double *grad)
     if (seg) (void)hipFree(seg);
     if (G) (void)hipFree(G);
     if (gdev) (void)hipFree(gdev);
+    if (blocks_dev) (void)hipFree(blocks_dev);
     return rc;
 }
 

@@ -21,20 +21,34 @@ def _box(case):
     return (3, L0 - 3, 3, L1 - 3, L2 // 2 - 3, L2 // 2 + 3)
 
 
-@pytest.mark.parametrize("kernel,ckpt", [("fused", 7), ("twopass", 5), ("fused", 1000), ("staged", 7),
-                                         ("two-step", 10), ("two-step", 7), ("two-step", 1000)])
-def test_adjoint_matches_oracle(kernel, ckpt, monkeypatch):
+GRIDS = {
+    "small": {},
+    # several x/y tiles and z chunks of the fused transposed step, Bloch-periodic walls
+    "tiles": dict(L0=40, L1=20, L2=48, outer=8, pml_xy=0, pml_z=4, iBc=(1, -1, 0), time_period=(1, 2, 4)),
+    # z chunks [32, 64) and [64, 96) reach no PML plane: the split-free instantiation
+    "long": dict(L0=40, L1=20, L2=120, outer=20, pml_xy=0, pml_z=4, iBc=(1, -1, 0), time_period=(1, 8, 10)),
+}
+
+
+@pytest.mark.parametrize("kernel,ckpt,grid", [("fused", 7, "small"), ("twopass", 5, "small"),
+                                              ("fused", 1000, "small"), ("staged", 7, "small"),
+                                              ("two-step", 10, "small"), ("two-step", 7, "small"),
+                                              ("two-step", 1000, "small"), ("fused", 0, "small"),
+                                              ("two-step", 0, "tiles"), ("fused", 9, "tiles"),
+                                              ("two-step", 0, "long")])
+def test_adjoint_matches_oracle(kernel, ckpt, grid, monkeypatch):
     """With the two-step kernel selected and an even checkpoint interval the forward pass
     runs two steps per pass and takes the objective's samples inside the pass; with an
-    odd interval it runs one step at a time."""
+    odd interval it runs one step at a time.  ckpt = 0: the design box's E stored after
+    every step, no recomputation."""
     monkeypatch.setenv("SMX_F32_KERNEL", "twopass" if kernel == "twopass" else "fused")
     if kernel == "staged":
         monkeypatch.setenv("SMX_F32_TILE", "3,16")
     if kernel == "two-step":
         monkeypatch.setenv("SMX_F32_TWO", "2,8")
-    case = _case()
+    case = _case(**GRIDS[grid])
     prep = casegen.prepare(case)
-    zT = case.L[2] - 5
+    zT = min(case.L[2] - 5, case.iI + 50)
     w = 0.9 * 2 * np.pi / prep.N
     K = case.time_period[2] * prep.N
     box = _box(case)
@@ -57,7 +71,7 @@ def test_adjoint_vs_finite_differences():
     where the gradient is large enough to resolve."""
     case = _case(seed=5)
     prep = casegen.prepare(case)
-    zT = case.L[2] - 5
+    zT = min(case.L[2] - 5, case.iI + 50)
     w = 0.9 * 2 * np.pi / prep.N
     K = case.time_period[2] * prep.N
     box = _box(case)
@@ -98,7 +112,7 @@ def test_adjoint_forward_keeps_caller_dft(monkeypatch):
     monkeypatch.setenv("SMX_F32_TWO", "2,8")
     case = _case()
     prep = casegen.prepare(case)
-    zT = case.L[2] - 5
+    zT = min(case.L[2] - 5, case.iI + 50)
     w = 0.9 * 2 * np.pi / prep.N
     K = case.time_period[2] * prep.N
     box = _box(case)
@@ -109,3 +123,24 @@ def test_adjoint_forward_keeps_caller_dft(monkeypatch):
         assert D.shape == (2, 4, 2, case.plane) and not np.any(D)
         J1 = p.adjoint_forward(K, 7, zT, w, box)  # odd interval: one step at a time
     assert abs(J2 - J1) <= 1e-5 * J1
+
+
+@pytest.mark.parametrize("grid", list(GRIDS))
+def test_fused_transposed_step_equals_gather_kernels(grid, monkeypatch):
+    """The fused LDS-staged transposed step against the two gather kernels it replaces:
+    the same gradient up to FP32 rounding order (1e-5 relative)."""
+    case = _case(**GRIDS[grid])
+    prep = casegen.prepare(case)
+    zT = min(case.L[2] - 5, case.iI + 50)
+    w = 0.9 * 2 * np.pi / prep.N
+    K = case.time_period[2] * prep.N
+    box = _box(case)
+    out = []
+    for gather in ("0", "1"):
+        monkeypatch.setenv("SMX_ADJ_GATHER", gather)
+        with prep.plan(fdtd.F32) as p:
+            p.adjoint_forward(K, 0, zT, w, box)
+            out.append(p.adjoint_backward())
+    assert np.abs(out[1]).max() > 0
+    err = np.linalg.norm(out[0] - out[1]) / np.linalg.norm(out[1])
+    assert err <= 1e-5, err
