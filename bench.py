@@ -178,9 +178,10 @@ def time_plan(plan, steps, warmup, torch):
 def batch_run(count, steps, torch, small=False, dims=((64, 192), (64, 192), (64, 256)), pml=8, seed=2025):
     """Independent runs of seeded sizes (L0, L1, L2 uniform in `dims`, dielectric boxes,
     `pml`-cell PML on every face), FP32, each plan autotuned.  After the envelope's ramp
-    (untimed), `steps` steps of all plans are timed three ways: smx_plan_run_batch (one
-    captured HIP graph of every plan's segment, replayed), the same batch with every step
-    launched from the host (streams mode), and the plans one after another."""
+    (untimed), `steps` steps of all plans are timed four ways: the fused batch mode (one
+    launch of two_step_multi per two-step pass of all plans; `value`), smx_plan_run_batch
+    (one captured HIP graph of every plan's segment, replayed), the same batch with every
+    step launched from the host (streams mode), and the plans one after another."""
     from smatranem_amd import casegen, fdtd
     cases = casegen.batch_cases(count, seed=seed, pml=pml, dims=dims)
     if small:
@@ -201,12 +202,17 @@ def batch_run(count, steps, torch, small=False, dims=((64, 192), (64, 192), (64,
         fn()
         sync_all()
         return time.perf_counter() - t0
-    info = {}
+    info, info_f = {}, {}
     try:
         for p in plans:
             p.autotune(10)
         fdtd.run_batch(plans, 0, warm, mode=fdtd.BATCH_STREAMS)
         k = warm
+        fdtd.run_batch(plans, k, 20, mode=fdtd.BATCH_FUSED)  # first launch of the batch kernel, untimed
+        k += 20
+        t_fused = timed(lambda: fdtd.run_batch(plans, k, steps, mode=fdtd.BATCH_FUSED, info=info_f))
+        assert info_f["graph_launches"] == steps // 2, "the batch did not take the fused launch"
+        k += steps
         t_graph = timed(lambda: fdtd.run_batch(plans, k, steps, info=info))
         k += steps
         t_streams = timed(lambda: fdtd.run_batch(plans, k, steps, mode=fdtd.BATCH_STREAMS))
@@ -222,11 +228,14 @@ def batch_run(count, steps, torch, small=False, dims=((64, 192), (64, 192), (64,
     cells = sum(c.n for c in cases)
     alg = sum(c.n * bytes_per_cell(pr.pml_fraction, 4, 4) for c, pr in zip(cases, preps))
     return dict(workload="%d independent runs, L0 in [%d, %d], L1 in [%d, %d], L2 in [%d, %d], dielectric boxes, "
-                         "%d-cell PML on all faces, FP32, smx_plan_run_batch (HIP graph replay)"
+                         "%d-cell PML on all faces, FP32, smx_plan_run_batch_mode SMX_BATCH_FUSED (one "
+                         "two_step_multi launch per two-step pass of all plans)"
                          % (count, *dims[0], *dims[1], *dims[2], pml),
                 grids=[list(c.L[:3]) for c in cases], cells=cells, steps=steps, first_step=warm,
-                value=round(cells * steps / t_graph / 1e9, 4), unit="Gcell-updates/s",
-                roofline_frac=round(alg * steps / t_graph / 1e9 / HBM_PEAK_GBS, 4),
+                value=round(cells * steps / t_fused / 1e9, 4), unit="Gcell-updates/s",
+                roofline_frac=round(alg * steps / t_fused / 1e9 / HBM_PEAK_GBS, 4),
+                fused_launches=info_f.get("graph_launches"),
+                graph_replay=round(cells * steps / t_graph / 1e9, 4),
                 graph_launches=info.get("graph_launches"),
                 streams_mode=round(cells * steps / t_streams / 1e9, 4),
                 one_after_another=round(cells * steps / t_seq / 1e9, 4))
@@ -615,13 +624,15 @@ def main():
     batch = batch_small = None
     if not args.no_batch:
         batch = batch_run(args.batch_count, args.batch_steps, torch, args.small)
-        print(f"[bench] batch of {args.batch_count}: {batch['value']:.2f} Gcell/s graph replay "
+        print(f"[bench] batch of {args.batch_count}: {batch['value']:.2f} Gcell/s fused, "
+              f"{batch['graph_replay']:.2f} graph replay "
               f"({batch['graph_launches']} launches), {batch['streams_mode']:.2f} streams, "
               f"{batch['one_after_another']:.2f} one after another", file=sys.stderr, flush=True)
         # small grids, where launching every step from the host is the cost
         batch_small = batch_run(args.batch_count, 4 * args.batch_steps, torch, args.small,
                                 dims=((24, 48), (24, 48), (32, 64)), pml=4, seed=2026)
-        print(f"[bench] batch of {args.batch_count} small grids: {batch_small['value']:.2f} Gcell/s graph replay "
+        print(f"[bench] batch of {args.batch_count} small grids: {batch_small['value']:.2f} Gcell/s fused, "
+              f"{batch_small['graph_replay']:.2f} graph replay "
               f"({batch_small['graph_launches']} launches), {batch_small['streams_mode']:.2f} streams, "
               f"{batch_small['one_after_another']:.2f} one after another", file=sys.stderr, flush=True)
 

@@ -188,11 +188,18 @@ int smx_plan_run_batch(smx_plan *const *plans, int nplans, long long step0, long
  *     for bit; the remaining steps run as in STREAMS.  Plans with a DFT set (or a
  *     segment longer than 1024 steps, or fewer than two segments to run) run as in
  *     STREAMS.  Results are those of STREAMS exactly.
- * graph_launches (may be NULL) receives the number of graph launches.  The batch
- * allocates its graph, stream and events per call and frees them before returning.
- * No counterpart in the reference.
+ *   SMX_BATCH_FUSED: every two-step pass of all plans in ONE launch (two_step_multi, a
+ *     grid of every plan's tiles), one launch per pass; an odd last step runs as in
+ *     STREAMS.  Each plan runs the split-slot two-step pass (SMX_F32_TWO variant 2)
+ *     whatever kernel it had selected, so the results are bit for bit those of each
+ *     plan run alone with that kernel.  Needs FP32 plans without lossy/metal cells,
+ *     without a DFT set and under 4 GiB per field group; otherwise the batch runs as
+ *     SMX_BATCH_GRAPH.
+ * graph_launches (may be NULL) receives the number of graph launches (SMX_BATCH_FUSED:
+ * of two_step_multi launches).  The batch allocates its graph or tables, stream and
+ * events per call and frees them before returning.  No counterpart in the reference.
  */
-enum { SMX_BATCH_STREAMS = 0, SMX_BATCH_GRAPH = 1 };
+enum { SMX_BATCH_STREAMS = 0, SMX_BATCH_GRAPH = 1, SMX_BATCH_FUSED = 2 };
 int smx_plan_run_batch_mode(smx_plan *const *plans, int nplans, long long step0, long long nsteps,
                             double *sums, int mode, int *graph_launches);
 

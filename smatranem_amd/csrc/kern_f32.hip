@@ -1369,8 +1369,11 @@ __device__ inline void e_regular(const float e[3], const float b[3], const float
     }
 }
 
-template <int FX, int FY, int D, bool SPLIT, bool WIDE, bool DFT, bool BIG = false>
-__global__ __launch_bounds__(FX * FY) void two_step_f32(fused_args a)
+/* The pass of one block `id` of a plan whose arguments `a` are the kernel's own (one
+ * plan per launch) or an entry of a table in constant memory (several plans per launch,
+ * two_step_multi); scal (may be null) holds the 20 per-step scalars instead of a. */
+template <int FX, int FY, int D, bool SPLIT, bool WIDE, bool DFT, bool BIG, typename A>
+__device__ __forceinline__ void two_step_body(const A &a, int id, const double *scal)
 {
     using Ly = two_layout<FX, FY, SPLIT, WIDE>;
     constexpr int W4 = Ly::W4, N4 = Ly::N4, NSLOT = D + 1, CELLS = Ly::CELLS;
@@ -1390,7 +1393,6 @@ __global__ __launch_bounds__(FX * FY) void two_step_f32(fused_args a)
     constexpr int L_DZ = L_SC + 2 * 20;
     __shared__ __attribute__((aligned(16))) float lds[L_DZ + (DFT && !SPLIT ? Ly::kMaxFZ + 4 : 0)];
 
-    const int id = xcd_logical(blockIdx.x, a.nblocks);
     const int bxi = id % a.gx, rest = id / a.gx;
     const int byi = rest % a.gy, bzi = rest / a.gy;
     const int x0 = a.xorg + bxi * FX, y0 = a.box[2] + byi * FY, z0 = a.zc[bzi], z1 = a.zc[bzi + 1];
@@ -1399,7 +1401,10 @@ __global__ __launch_bounds__(FX * FY) void two_step_f32(fused_args a)
     /* The per-step scalars are read only on the source and monitor planes: kept in LDS
      * rather than in scalar registers across the z-march (the first barrier of the
      * march publishes them). */
-    if (t == 0) {
+    if (t == 0 && scal) {
+        double *d = reinterpret_cast<double *>(lds + L_SC);
+        for (int i = 0; i < 20; i++) d[i] = scal[i];
+    } else if (t == 0) {
         double *d = reinterpret_cast<double *>(lds + L_SC);
         d[0] = a.srcE_f2, d[1] = a.srcE_re, d[2] = a.srcE_im, d[3] = a.srcH_f2, d[4] = a.srcH_re;
         d[5] = a.srcH_im, d[6] = a.monH_re, d[7] = a.monH_im, d[8] = a.monE_re, d[9] = a.monE_im;
@@ -1842,6 +1847,27 @@ __global__ __launch_bounds__(FX * FY) void two_step_f32(fused_args a)
     }
 }
 
+template <int FX, int FY, int D, bool SPLIT, bool WIDE, bool DFT, bool BIG = false>
+__global__ __launch_bounds__(FX * FY) void two_step_f32(fused_args a)
+{
+    two_step_body<FX, FY, D, SPLIT, WIDE, DFT, BIG>(a, xcd_logical(blockIdx.x, a.nblocks), nullptr);
+}
+
+/* A two-step pass of several plans in one launch (a batch of independent runs): block i
+ * (XCD-aware order) takes entry blk[i] = plan << 20 | the block's id within its plan, the
+ * plan's arguments from args[plan] (constant memory, read through the scalar cache) and
+ * its 20 step scalars from scal[20 * plan].  The split-slot variant, which updates every
+ * cell of a grid with or without PML. */
+typedef const fused_args __attribute__((address_space(4))) const_fused_args;
+
+__global__ __launch_bounds__(256) void two_step_multi(const fused_args *args, const int *blk, const double *scal)
+{
+    const int e = blk[xcd_logical(blockIdx.x, gridDim.x)];
+    const int plan = e >> 20, id = e & ((1 << 20) - 1);
+    const const_fused_args &a = ((const_fused_args *)args)[plan];
+    two_step_body<32, 8, 1, true, false, false, false>(a, id, scal + 20 * plan);
+}
+
 /* ---------------------------------------------------------------------------------- */
 /* Region sweeps: the cells outside the staged kernel's box                           */
 /* ---------------------------------------------------------------------------------- */
@@ -2201,6 +2227,7 @@ struct two_variant {
      "two_step_f32<" #FX "," #FY "," #D "," #SP "," #WI ",true,false>"}
 constexpr two_variant kTwo[] = {SMX_TWO(32, 8, 1, false, false), SMX_TWO(32, 8, 2, false, false),
                                 SMX_TWO(32, 8, 1, true, false), SMX_TWO(32, 8, 1, false, true)};
+constexpr int kMultiVariant = 2; /* the variant two_step_multi runs: <32, 8, 1, split, narrow> */
 #undef SMX_TWO
 constexpr int kNumTwo = sizeof kTwo / sizeof kTwo[0];
 
@@ -2686,6 +2713,80 @@ hipError_t smx_f32_reserve(smx_f32_state *st, hipStream_t stream)
     if ((e = hipMalloc(&st->tmpS, sizeof(float) * 6 * st->n)) != hipSuccess) return e;
     return hipMemsetAsync(st->tmpS, 0, sizeof(float) * 6 * st->n, stream);
 }
+
+/* ---------------------------------------------------------------------------------- */
+/* Several plans per launch (batches of independent runs)                              */
+/* ---------------------------------------------------------------------------------- */
+
+size_t smx_f32_multi_args_size(void) { return sizeof(fused_args); }
+
+int smx_f32_multi_ok(const smx_f32_state *st, int fz)
+{
+    return !two_big(st) && smx_f32_two_ok(st, kMultiVariant, fz);
+}
+
+int smx_f32_multi_slots(void)
+{
+    int dev = 0, ncu = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, two_step_multi, 256, 0) != hipSuccess)
+        return 0;
+    return ncu * per_cu;
+}
+
+int smx_f32_multi_prepare(const smx_f32_state *st, const smx_f32_launch_args *la, int fz, void *out, int *nblocks)
+{
+    if (two_big(st) || !smx_f32_two_ok(st, kMultiVariant, fz)) return SMX_ERR_ARG;
+    const smx_geom &g = la->g;
+    const two_variant tv = kTwo[kMultiVariant];
+    const int64_t pl4 = 4 * (int64_t)g.L0 * g.L1;
+    if (la->Einc != la->Hinc + pl4 || la->EB != la->HB + pl4 || la->HT != la->HB + 2 * pl4 ||
+        la->ET != la->HB + 3 * pl4)
+        return SMX_ERR_ARG;
+    fused_args *args = static_cast<fused_args *>(out);
+    for (int k = 0; k < 2; k++) { /* k = 0: buffers cur -> 1 - cur; k = 1: the next pass */
+        fused_args &a = args[k];
+        std::memset(&a, 0, sizeof a);
+        a.L0 = g.L0, a.L1 = g.L1, a.L2 = g.L2;
+        const int box[6] = {0, g.L0, 0, g.L1, 0, g.L2};
+        for (int i = 0; i < 6; i++) a.box[i] = box[i];
+        a.fz = fz;
+        a.iB = g.iB, a.iT = g.iT, a.iI = g.iI;
+        two_tiles_x(a, tv.ox, tv.fx, g.L0);
+        a.gy = (g.L1 + tv.fy - 1) / tv.fy;
+        if (!two_chunks_z(a, st, fz, 0, two_layout<32, 8, true, false>::kMaxFZ)) return SMX_ERR_ARG;
+        a.nblocks = a.gx * a.gy * a.nzc;
+        a.sz = (int64_t)g.L0 * g.L1;
+        a.w[0] = (float)g.iBc0, a.w[1] = (float)g.iBc1, a.w[2] = (float)g.iBc2;
+        a.sgn = (float)g.sgn;
+        const int c = k == 0 ? st->cur : 1 - st->cur;
+        for (int i = 0; i < 6; i++) {
+            a.To[i] = st->T[c][i];
+            a.Tn[i] = st->T[1 - c][i];
+            a.So[i] = st->S[c][i];
+            a.Sn[i] = st->S[1 - c][i];
+            a.b[i] = st->b[i];
+        }
+        a.ncomp = st->n;
+        a.pml = st->pml;
+        a.pmlflag = st->pmlflag;
+        a.Einc = la->Einc;
+        a.Hinc = la->Hinc;
+        a.HB = la->HB, a.EB = la->EB, a.HT = la->HT, a.ET = la->ET;
+    }
+    *nblocks = args[0].nblocks;
+    return args[0].nblocks < (1 << 20) ? SMX_OK : SMX_ERR_ARG;
+}
+
+hipError_t smx_f32_multi_launch(const void *args, const int *blk, const double *scal, int nblocks, hipStream_t stream)
+{
+    hipLaunchKernelGGL(two_step_multi, dim3(nblocks), dim3(256), 0, stream, static_cast<const fused_args *>(args), blk,
+                       scal);
+    return hipGetLastError();
+}
+
+const char *smx_f32_multi_kernel_name(void) { return "two_step_multi"; }
 
 hipError_t smx_f32_step2(smx_f32_state *st, const smx_f32_launch_args *la, hipStream_t stream)
 {

@@ -133,3 +133,17 @@ void smx_f32_tile_shape(int tile, int *fx, int *fy);
  * the two-step kernel that also samples the DFT planes. */
 const char *smx_f32_kernel_name(const smx_f32_state *st, int dft);
 int smx_f32_tile_ok(const smx_f32_state *st, int tile, int fz);
+
+/* Several plans per launch (two_step_multi, the split-slot two-step variant 2): whether
+ * a plan can take part with z chunks of fz planes, the blocks of the kernel resident
+ * on the device at once, a plan's two argument blocks (smx_f32_multi_args_size() bytes
+ * each: the pass from buffer cur, then the next pass) and block count; the launch takes
+ * the device copy of the plans' argument blocks, the block list (plan << 20 | block id)
+ * and 20 step scalars per plan (the second step's after the first's, in the order of
+ * fused_args). */
+size_t smx_f32_multi_args_size(void);
+int smx_f32_multi_ok(const smx_f32_state *st, int fz);
+int smx_f32_multi_slots(void);
+int smx_f32_multi_prepare(const smx_f32_state *st, const smx_f32_launch_args *la, int fz, void *args2, int *nblocks);
+hipError_t smx_f32_multi_launch(const void *args, const int *blk, const double *scal, int nblocks, hipStream_t stream);
+const char *smx_f32_multi_kernel_name(void);

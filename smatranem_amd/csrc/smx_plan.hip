@@ -9,6 +9,7 @@
  */
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <cmath>
 #include <cstddef>
 #include <cstdio>
@@ -587,14 +588,12 @@ int smx_plan_set_source(smx_plan *p, const double *Hinc, const double *Einc, int
     return SMX_OK;
 }
 
-static int launch_step2(smx_plan *p, const smx_step_scalars &s, const smx_step_scalars &s2, int dft_slot,
-                        bool mon_kernel)
+/* Launch arguments of a two-step pass without its step scalars. */
+static smx_f32_launch_args step2_args(smx_plan *p, bool mon_kernel)
 {
     smx_f32_launch_args a{};
     a.g = make_geom(p);
     if (!mon_kernel) a.g.iB = a.g.iT = -1;
-    a.s = s;
-    a.s2 = s2;
     a.srcH_plane = nullptr;
     a.srcE_plane = nullptr;
     a.Hinc = p->inc;
@@ -603,6 +602,15 @@ static int launch_step2(smx_plan *p, const smx_step_scalars &s, const smx_step_s
     a.EB = p->mon + 4 * p->plane;
     a.HT = p->mon + 8 * p->plane;
     a.ET = p->mon + 12 * p->plane;
+    return a;
+}
+
+static int launch_step2(smx_plan *p, const smx_step_scalars &s, const smx_step_scalars &s2, int dft_slot,
+                        bool mon_kernel)
+{
+    smx_f32_launch_args a = step2_args(p, mon_kernel);
+    a.s = s;
+    a.s2 = s2;
     if (dft_slot >= 0) {
         a.dft = (float *)p->dft_series;
         a.dft_n = p->dft_ring;
@@ -922,6 +930,187 @@ bool segment_repeats(const smx_plan *p, long long k0, int G, long long end)
     return segment_scalars(p, k0, G, a, env) && segment_scalars(p, k0 + G, G, b, env) && same_scalars(a, b);
 }
 
+/*
+ * Mode SMX_BATCH_FUSED: the two-step passes of all plans, one launch of two_step_multi
+ * per pass.  Its grid is a block list over every plan's tiles (plan << 20 | tile), so
+ * small plans fill the device together and the host issues one launch per pass instead
+ * of one per plan and pass.  Each plan's kernel arguments sit in a device table, two
+ * entries per plan (the passes from either ping-pong buffer); the 20 step scalars per
+ * plan and pass go up a chunk of passes at a time, into one of two device buffers,
+ * while the passes of the chunk before run from the other.
+ */
+constexpr int kFusedChunk = 256; /* passes per upload of step scalars */
+constexpr int kFusedFZ[] = {8, 12, 16, 24, 32, 48, 64}; /* z chunks the batch chooses from */
+constexpr int kFusedDefaultFZ = 16;
+constexpr int kScalarsPerPass = 20;
+static_assert(sizeof(smx_step_scalars) == 10 * sizeof(double), "step scalars: 10 doubles in kernel order");
+
+struct fused_batch {
+    hipStream_t ms = nullptr;
+    hipEvent_t join_ev = nullptr;
+    std::vector<hipEvent_t> ev; /* one per plan */
+    hipEvent_t done[2] = {nullptr, nullptr};
+    void *args = nullptr;
+    int *blk = nullptr;
+    double *scal[2] = {nullptr, nullptr};
+    std::vector<double> host[2]; /* the chunks' scalars until their upload has run */
+
+    ~fused_batch()
+    {
+        if (ms) (void)hipStreamSynchronize(ms);
+        for (double *d : scal)
+            if (d) (void)hipFree(d);
+        if (blk) (void)hipFree(blk);
+        if (args) (void)hipFree(args);
+        for (hipEvent_t e : done)
+            if (e) (void)hipEventDestroy(e);
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (join_ev) (void)hipEventDestroy(join_ev);
+        if (ms) (void)hipStreamDestroy(ms);
+    }
+};
+
+bool fused_ok(smx_plan *const *plans, int nplans)
+{
+    if (nplans >= (1 << 11)) return false;
+    for (int i = 0; i < nplans; i++) {
+        const smx_plan *p = plans[i];
+        if (p->g.precision != SMX_F32 || p->dft_nplanes != 0 || mon_ring(p) ||
+            !smx_f32_multi_ok(&p->f32, kFusedDefaultFZ))
+            return false;
+    }
+    return true;
+}
+
+/*
+ * The z chunk of a fused batch.  A pass lasts about (rounds of resident blocks) x (stages
+ * a block marches, fz + 4): where the batch's blocks fill the device a few times at
+ * most (small grids), the chunk is chosen to make that product least (the measured pass
+ * times of the 24 small grids, 8 <= fz <= 64, follow it within 10%); on batches many
+ * times the device, chunks of 16 planes (measured best there: the chunks then balance
+ * the blocks of one round, and short marches keep the halo planes in the L2).
+ * SMX_BATCH_FZ overrides the choice.
+ */
+int fused_fz(smx_plan *const *plans, int nplans)
+{
+    if (const char *e = getenv("SMX_BATCH_FZ")) return atoi(e);
+    const long long slots = smx_f32_multi_slots();
+    if (slots <= 0) return kFusedDefaultFZ;
+    auto blocks = [&](int fz) {
+        long long B = 0;
+        for (int i = 0; i < nplans; i++) {
+            const smx_f32_state &st = plans[i]->f32;
+            if (!smx_f32_multi_ok(&st, fz)) return -1LL;
+            B += (long long)((st.L[0] + 31) / 32) * ((st.L[1] + 7) / 8) * ((st.L[2] + fz - 1) / fz);
+        }
+        return B;
+    };
+    if (blocks(kFusedDefaultFZ) > 4 * slots) return kFusedDefaultFZ;
+    int best = kFusedDefaultFZ;
+    long long best_cost = -1;
+    for (int fz : kFusedFZ) {
+        const long long B = blocks(fz);
+        if (B <= 0) continue;
+        const long long cost = (B + slots - 1) / slots * (fz + 4);
+        if (best_cost < 0 || cost < best_cost) best = fz, best_cost = cost;
+    }
+    return best;
+}
+
+/*
+ * Passes [0, npass) of every plan from step k0 (steps k0 .. k0 + 2 npass - 1), in
+ * order; *ran = passes launched, fewer where a step's envelope is undefined (the
+ * caller's step loop then meets that step and reports it).
+ */
+int run_fused(smx_plan *const *plans, int nplans, long long k0, long long npass, double *sums, long long *ran)
+{
+    *ran = 0;
+    const size_t asz = smx_f32_multi_args_size();
+    std::vector<unsigned char> args(2 * (size_t)nplans * asz), two(2 * asz);
+    std::vector<int> blk;
+    const int fz = fused_fz(plans, nplans);
+    for (int i = 0; i < nplans; i++) {
+        smx_plan *p = plans[i];
+        const smx_f32_launch_args la = step2_args(p, true);
+        int nb = 0;
+        if (smx_f32_multi_prepare(&p->f32, &la, fz, two.data(), &nb) != SMX_OK)
+            return arg_fail("batch: two-step tiling of a plan");
+        std::memcpy(args.data() + (size_t)i * asz, two.data(), asz);
+        std::memcpy(args.data() + ((size_t)nplans + i) * asz, two.data() + asz, asz);
+        for (int b = 0; b < nb; b++) blk.push_back(i << 20 | b);
+    }
+    if (blk.size() >= ((size_t)1 << 31)) return arg_fail("batch: too many blocks");
+    fused_batch fb;
+    SMX_HIP(hipStreamCreateWithFlags(&fb.ms, hipStreamNonBlocking));
+    SMX_HIP(hipEventCreateWithFlags(&fb.join_ev, hipEventDisableTiming));
+    for (hipEvent_t &e : fb.done) SMX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    fb.ev.assign((size_t)nplans, nullptr);
+    for (hipEvent_t &e : fb.ev) SMX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    const long long chunk = std::min<long long>(npass, kFusedChunk);
+    const size_t per_pass = (size_t)nplans * kScalarsPerPass;
+    SMX_HIP(hipMalloc(&fb.args, args.size()));
+    SMX_HIP(hipMalloc(&fb.blk, sizeof(int) * blk.size()));
+    for (double *&d : fb.scal) SMX_HIP(hipMalloc(&d, sizeof(double) * per_pass * (size_t)chunk));
+    SMX_HIP(hipMemcpy(fb.args, args.data(), args.size(), hipMemcpyHostToDevice));
+    SMX_HIP(hipMemcpy(fb.blk, blk.data(), sizeof(int) * blk.size(), hipMemcpyHostToDevice));
+    /* work queued on the plans' streams runs first */
+    for (int i = 0; i < nplans; i++) {
+        SMX_HIP(hipEventRecord(fb.ev[(size_t)i], plans[i]->stream));
+        SMX_HIP(hipStreamWaitEvent(fb.ms, fb.ev[(size_t)i], 0));
+    }
+    std::vector<double> env(4 * (size_t)nplans);
+    bool stop = false;
+    long long done = 0;
+    for (long long c = 0; done < npass && !stop; c++) {
+        const int b = (int)(c & 1);
+        if (c >= 2) SMX_HIP(hipEventSynchronize(fb.done[b])); /* chunk c - 2 has run */
+        std::vector<double> &h = fb.host[b];
+        h.resize(per_pass * (size_t)chunk);
+        long long n = 0;
+        for (; n < chunk && done + n < npass; n++) {
+            const long long k = k0 + 2 * (done + n);
+            bool ok = true;
+            for (int i = 0; i < nplans && ok; i++) {
+                const smx_plan *p = plans[i];
+                smx_step_scalars st[2];
+                for (int j = 0; j < 2 && ok; j++)
+                    ok = step_scalars(p, (int)((k + j) / p->N), (int)((k + j) % p->N), &st[j],
+                                      &env[4 * (size_t)i + 2 * j], &env[4 * (size_t)i + 2 * j + 1]) == SMX_OK;
+                std::memcpy(&h[per_pass * (size_t)n + kScalarsPerPass * (size_t)i], st, sizeof st);
+            }
+            if (!ok) {
+                stop = true;
+                break;
+            }
+            if (sums) /* per plan in step order, as the step loop adds them */
+                for (int i = 0; i < nplans; i++)
+                    for (int j = 0; j < 2; j++) {
+                        sums[2 * i] += env[4 * (size_t)i + 2 * j];
+                        sums[2 * i + 1] += env[4 * (size_t)i + 2 * j + 1];
+                    }
+        }
+        if (n == 0) break;
+        SMX_HIP(hipMemcpyAsync(fb.scal[b], h.data(), sizeof(double) * per_pass * (size_t)n, hipMemcpyHostToDevice,
+                               fb.ms));
+        for (long long j = 0; j < n; j++) {
+            const void *a = static_cast<const unsigned char *>(fb.args) + (size_t)((done + j) & 1) * nplans * asz;
+            SMX_HIP(smx_f32_multi_launch(a, fb.blk, fb.scal[b] + per_pass * (size_t)j, (int)blk.size(), fb.ms));
+        }
+        SMX_HIP(hipEventRecord(fb.done[b], fb.ms));
+        done += n;
+    }
+    /* and what is queued on them afterwards runs after the passes */
+    SMX_HIP(hipEventRecord(fb.join_ev, fb.ms));
+    for (int i = 0; i < nplans; i++) {
+        SMX_HIP(hipStreamWaitEvent(plans[i]->stream, fb.join_ev, 0));
+        if (done & 1) plans[i]->f32.cur = 1 - plans[i]->f32.cur;
+        plans[i]->next_step = k0 + 2 * done;
+    }
+    *ran = done;
+    return SMX_OK;
+}
+
 } // namespace
 
 extern "C" {
@@ -930,7 +1119,7 @@ int smx_plan_run_batch_mode(smx_plan *const *plans, int nplans, long long step0,
                             int mode, int *graph_launches)
 {
     if (graph_launches) *graph_launches = 0;
-    if (mode != SMX_BATCH_STREAMS && mode != SMX_BATCH_GRAPH) return arg_fail("batch mode");
+    if (mode != SMX_BATCH_STREAMS && mode != SMX_BATCH_GRAPH && mode != SMX_BATCH_FUSED) return arg_fail("batch mode");
     if (!plans || nplans <= 0) return arg_fail("empty batch");
     for (int i = 0; i < nplans; i++) {
         if (!plans[i]) return arg_fail("null plan in batch");
@@ -942,7 +1131,21 @@ int smx_plan_run_batch_mode(smx_plan *const *plans, int nplans, long long step0,
         }
     }
     if (step0 < 0 || nsteps < 0) return arg_fail("negative step");
-    const long long end = step0 + nsteps;
+    long long end = step0 + nsteps;
+
+    std::vector<long long> next((size_t)nplans, step0);
+    if (mode == SMX_BATCH_FUSED) {
+        if (!fused_ok(plans, nplans)) mode = SMX_BATCH_GRAPH;
+        else if (nsteps >= 2) {
+            long long ran = 0;
+            const int rc = run_fused(plans, nplans, step0, nsteps / 2, sums, &ran);
+            if (rc) return rc;
+            if (graph_launches) *graph_launches = (int)std::min<long long>(ran, INT_MAX);
+            for (long long &k : next) k = step0 + 2 * ran;
+            /* an odd last step, or the step whose envelope is undefined, as one step */
+            mode = SMX_BATCH_STREAMS;
+        }
+    }
 
     /* graph replay needs every plan's steps to be plain kernel launches (no DFT
      * samples, whose fold points depend on host counters) */
@@ -969,7 +1172,6 @@ int smx_plan_run_batch_mode(smx_plan *const *plans, int nplans, long long step0,
         }
     }
 
-    std::vector<long long> next((size_t)nplans, step0);
     bool captured = false; /* a capture was tried (one per call) */
     int wait = 0;          /* eager rounds until the next replay or capture check */
     for (;;) {

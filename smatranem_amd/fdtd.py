@@ -250,14 +250,18 @@ NOALLOC_MAX_BYTES = 64 << 20
 
 BATCH_STREAMS = 0
 BATCH_GRAPH = 1
+BATCH_FUSED = 2
 
 
 def run_batch(plans, step0, nsteps, mode=None, info=None):
     """smx_plan_run_batch: steps [step0, step0 + nsteps) of independent plans, each on its
     own stream.  ``mode`` None is the library default (BATCH_GRAPH: repeating segments
     replayed from one captured HIP graph); BATCH_STREAMS launches every step from the
-    host.  ``info`` (a dict, optional) receives ``graph_launches``.  Returns the envelope
-    sums, float64[len(plans), 2]."""
+    host; BATCH_FUSED runs every two-step pass of all plans in one launch (FP32 plans
+    without lossy/metal cells or DFT, each on the split-slot two-step kernel; others fall
+    back to BATCH_GRAPH).  ``info`` (a dict, optional) receives ``graph_launches`` (for
+    BATCH_FUSED the launches of the batch kernel).  Returns the envelope sums,
+    float64[len(plans), 2]."""
     handles = (C.c_void_p * len(plans))(*[p._h.value for p in plans])
     sums = np.zeros((len(plans), 2))
     sp = sums.ctypes.data_as(C.POINTER(C.c_double))

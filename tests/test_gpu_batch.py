@@ -185,3 +185,83 @@ def test_batch_graph_off_with_dft():
     for a, b in zip(out[0][0], out[1][0]):
         assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
     assert np.array_equal(out[0][1], out[1][1])
+
+
+FUSED_MEMBERS = [
+    (fdtd.F32, dict(seed=41, L0=12, L1=10, L2=28, metal_cells=0, lossy_cells=0, time_period=(1, 5, 7))),
+    (fdtd.F32, dict(seed=42, L0=37, L1=19, L2=41, outer=8, pml_xy=2, pml_z=3, core=(3, 3), metal_cells=0,
+                    lossy_cells=0, time_period=(1, 5, 7))),
+    (fdtd.F32, dict(seed=43, L0=20, L1=16, L2=33, outer=8, pml_xy=0, pml_z=0, iBc=(1, -1, 1), metal_cells=0,
+                    lossy_cells=0, time_period=(1, 5, 7))),
+    # several x tiles, y tiles and z chunks (z chunks of 16 planes)
+    (fdtd.F32, dict(seed=44, L0=70, L1=26, L2=75, outer=10, pml_xy=4, pml_z=4, core=(5, 4), metal_cells=0,
+                    lossy_cells=0, omega=2 * np.pi / 9.0, time_period=(1, 5, 7))),
+]
+
+
+def _fused_preps(members):
+    return [(prec, casegen.prepare(small_waveguide(**kw))) for prec, kw in members]
+
+
+@pytest.mark.parametrize("odd,fz", [(False, None), (True, None), (True, "8"), (False, "64")])
+def test_batch_fused_equals_separate_runs(odd, fz, monkeypatch):
+    """SMX_BATCH_FUSED (every plan's two-step pass in one launch) gives bit for bit the
+    fields, monitors and envelope sums of each plan run alone on the split-slot two-step
+    kernel, over two calls (the second starting on the other ping-pong buffer when the
+    first ran an odd number of steps), with the batch's own z chunk or a forced one."""
+    monkeypatch.setenv("SMX_F32_TWO", "2,16")
+    if fz:
+        monkeypatch.setenv("SMX_BATCH_FZ", fz)
+    preps = _fused_preps(FUSED_MEMBERS)
+    steps = min(pr.case.time_period[2] * pr.N for _, pr in preps) - (1 if odd else 2)
+    first = 3 if odd else 4
+    alone = []
+    for prec, pr in preps:
+        with pr.plan(prec) as p:
+            assert p.kernels == "two_step_f32<32,8,1,true,false,false,false>"
+            sums = p.run_steps(0, first) + p.run_steps(first, steps - first)
+            alone.append(_state(p) + (sums,))
+    plans = [pr.plan(prec) for prec, pr in preps]
+    try:
+        info1, info2 = {}, {}
+        s1 = fdtd.run_batch(plans, 0, first, mode=fdtd.BATCH_FUSED, info=info1)
+        s2 = fdtd.run_batch(plans, first, steps - first, mode=fdtd.BATCH_FUSED, info=info2)
+        together = [_state(p) for p in plans]
+    finally:
+        for p in plans:
+            p.close()
+    assert info1["graph_launches"] == first // 2 and info2["graph_launches"] == (steps - first) // 2
+    sums = s1 + s2
+    for i, (a, b) in enumerate(zip(alone, together)):
+        assert np.abs(a[0]).max() > 0, i
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), i
+        for ma, mb in zip(a[2], b[2]):
+            assert np.array_equal(ma, mb), i
+        assert np.array_equal(a[3], sums[i]), i
+
+
+def test_batch_fused_falls_back():
+    """A batch the fused launch cannot take (an FP64 member, a member with lossy cells)
+    runs as the graph mode, with each plan's results as alone."""
+    members = [FUSED_MEMBERS[0],
+               (fdtd.F64, dict(seed=46, L0=9, L1=14, L2=30, outer=7, pml_xy=0, pml_z=3, metal_cells=0,
+                               lossy_cells=0, time_period=(1, 5, 7))),
+               (fdtd.F32, dict(seed=47, L0=16, L1=12, L2=30, outer=7, pml_xy=2, pml_z=3, metal_cells=0,
+                               lossy_cells=4, time_period=(1, 5, 7)))]
+    for sel in ([0, 1], [0, 2]):
+        preps = _fused_preps([members[i] for i in sel])
+        steps = min(pr.case.time_period[2] * pr.N for _, pr in preps)
+        alone = []
+        for prec, pr in preps:
+            with pr.plan(prec) as p:
+                p.run_steps(0, steps)
+                alone.append(p.read_fields())
+        plans = [pr.plan(prec) for prec, pr in preps]
+        try:
+            fdtd.run_batch(plans, 0, steps, mode=fdtd.BATCH_FUSED)
+            together = [p.read_fields() for p in plans]
+        finally:
+            for p in plans:
+                p.close()
+        for a, b in zip(alone, together):
+            assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), sel
