@@ -175,6 +175,40 @@ def time_plan(plan, steps, warmup, torch):
     return (t1 - t0) * 1e3, ev0.elapsed_time(ev1)
 
 
+def box_run(args, torch, K, W):
+    """The headline grid as the survey sizes it: 10-cell PML on all six faces (the
+    reference monitors off: every plane holds PML cells) and the 16-frequency DFT on the
+    two port planes, autotuned, max(K/5, 100) steps timed."""
+    import numpy as np
+    from smatranem_amd import casegen, fdtd
+    bcase = build_workload(args.small, box=True)
+    bprep = casegen.prepare(bcase)
+    bpml = 4 if args.small else 10
+    bplanes = [] if args.no_dft else [bpml + 2, int(bcase.L[2]) - 1 - bpml - 2]
+    dft_w = [float(x) for x in 2 * np.pi / bprep.N * np.linspace(0.85, 1.15, 16)]
+    with bprep.plan(fdtd.F32) as pb:
+        if bplanes:
+            pb.set_dft(bplanes, dft_w, chunk=args.dft_chunk)
+        pb.autotune(20)
+        bkern = pb.kernels
+        kb = max(K // 5, 100)
+        wb, evb = time_plan(pb, kb, min(W, 100), torch)
+    bphi = bprep.pml_fraction
+    bb = bytes_per_cell(bphi, 4, 4)
+    box = dict(workload="same grid, 10-cell PML on all six faces, reference monitors off, "
+                        + ("16-frequency DFT on planes %s" % bplanes if bplanes else "no DFT"),
+               kernel=bkern, steps=kb, value=round(bcase.n / (wb / kb * 1e-3) / 1e9, 4),
+               pml_fraction=round(bphi, 4), bytes_per_cell=round(bb, 2),
+               kernel_ms_per_step=round(evb / kb, 5),
+               roofline_frac=round(bb * bcase.n / (evb / kb * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
+    print(f"[bench] box-PML fp32: {box['value']:.2f} Gcell/s, {box['roofline_frac']:.3f} of peak ({bkern})",
+          file=sys.stderr, flush=True)
+    return box
+
+
+SMALL_BATCH = dict(dims=((24, 48), (24, 48), (32, 64)), pml=4, seed=2026)
+
+
 def batch_run(count, steps, torch, small=False, dims=((64, 192), (64, 192), (64, 256)), pml=8, seed=2025):
     """Independent runs of seeded sizes (L0, L1, L2 uniform in `dims`, dielectric boxes,
     `pml`-cell PML on every face), FP32, each plan autotuned.  After the envelope's ramp
@@ -499,6 +533,8 @@ def main():
     ap.add_argument("--adjoint-fd-seed", type=int, default=None, metavar="SEED",
                     help="draw the --adjoint-fd cells uniformly from the design box with this seed")
     ap.add_argument("--small", action="store_true", help="32x32x48 grid (quick check)")
+    ap.add_argument("--only", choices=["box", "batch", "batch-small", "adjoint"], default=None,
+                    help="run only this leg (for profiling) and print its JSON object")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device copies of MIB MiB (scale for rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -525,6 +561,17 @@ def main():
                           "copies": 5}), flush=True)
         return
     K, W = args.steps, args.warmup
+    if args.only:
+        if args.only == "box":
+            out = box_run(args, torch, K, W)
+        elif args.only == "batch":
+            out = batch_run(args.batch_count, args.batch_steps, torch, args.small)
+        elif args.only == "batch-small":
+            out = batch_run(args.batch_count, 4 * args.batch_steps, torch, args.small, **SMALL_BATCH)
+        else:
+            out = adjoint_run(torch, args.small, args.adjoint_fd, args.adjoint_fd_seed, args.adjoint_steps)
+        print(json.dumps({"only": args.only, "result": out}), flush=True)
+        return
 
     t_setup = time.perf_counter()
     case = build_workload(args.small)
@@ -596,30 +643,7 @@ def main():
 
     box = None
     if not args.no_box:
-        # the headline grid as the survey sizes it: 10-cell PML on all six faces (the
-        # reference monitors off: every plane holds PML cells) and the 16-frequency DFT on
-        # the two port planes
-        bcase = build_workload(args.small, box=True)
-        bprep = casegen.prepare(bcase)
-        bpml = 4 if args.small else 10
-        bplanes = [] if args.no_dft else [bpml + 2, int(bcase.L[2]) - 1 - bpml - 2]
-        with bprep.plan(fdtd.F32) as pb:
-            if bplanes:
-                pb.set_dft(bplanes, dft_w, chunk=dft_chunk)
-            pb.autotune(20)
-            bkern = pb.kernels
-            kb = max(K // 5, 100)
-            wb, evb = time_plan(pb, kb, min(W, 100), torch)
-        bphi = bprep.pml_fraction
-        bb = bytes_per_cell(bphi, 4, 4)
-        box = dict(workload="same grid, 10-cell PML on all six faces, reference monitors off, "
-                            + ("16-frequency DFT on planes %s" % bplanes if bplanes else "no DFT"),
-                   kernel=bkern, steps=kb, value=round(bcase.n / (wb / kb * 1e-3) / 1e9, 4),
-                   pml_fraction=round(bphi, 4), bytes_per_cell=round(bb, 2),
-                   kernel_ms_per_step=round(evb / kb, 5),
-                   roofline_frac=round(bb * bcase.n / (evb / kb * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
-        print(f"[bench] box-PML fp32: {box['value']:.2f} Gcell/s, {box['roofline_frac']:.3f} of peak ({bkern})",
-              file=sys.stderr, flush=True)
+        box = box_run(args, torch, K, W)
 
     batch = batch_small = None
     if not args.no_batch:
@@ -629,8 +653,7 @@ def main():
               f"({batch['graph_launches']} launches), {batch['streams_mode']:.2f} streams, "
               f"{batch['one_after_another']:.2f} one after another", file=sys.stderr, flush=True)
         # small grids, where launching every step from the host is the cost
-        batch_small = batch_run(args.batch_count, 4 * args.batch_steps, torch, args.small,
-                                dims=((24, 48), (24, 48), (32, 64)), pml=4, seed=2026)
+        batch_small = batch_run(args.batch_count, 4 * args.batch_steps, torch, args.small, **SMALL_BATCH)
         print(f"[bench] batch of {args.batch_count} small grids: {batch_small['value']:.2f} Gcell/s fused, "
               f"{batch_small['graph_replay']:.2f} graph replay "
               f"({batch_small['graph_launches']} launches), {batch_small['streams_mode']:.2f} streams, "
