@@ -1668,8 +1668,11 @@ int smx_adjoint_backward(smx_plan *p, double *grad)
     if (fused && rc == SMX_OK) {
         /* blocks whose reach -- x in [x0-1, x0+FX], y in [y0-1, y0+FY], z in [z0-1, z1] --
          * holds a PML cell take the split instantiation; the list puts them first */
-        f.gx = (a.L0 + kAdjFX - 1) / kAdjFX;
-        f.gy = (a.L1 + kAdjFY - 1) / kAdjFY;
+        f.tile = 1;
+        if (const char *t = getenv("SMX_ADJ_TILE")) f.tile = std::min(std::max(atoi(t), 0), 2);
+        const int FX = kAdjTiles[f.tile][0], FY = kAdjTiles[f.tile][1];
+        f.gx = (a.L0 + FX - 1) / FX;
+        f.gy = (a.L1 + FY - 1) / FY;
         f.fz = kAdjFZ;
         const int gz = (a.L2 + kAdjFZ - 1) / kAdjFZ;
         std::vector<unsigned char> flag(a.L0 + a.L1 + a.L2);
@@ -1684,10 +1687,10 @@ int smx_adjoint_backward(smx_plan *p, double *grad)
         for (int bz = 0; bz < gz; bz++)
             for (int by = 0; by < f.gy; by++)
                 for (int bx = 0; bx < f.gx; bx++) {
-                    const int x0 = bx * kAdjFX, y0 = by * kAdjFY, z0 = bz * kAdjFZ;
+                    const int x0 = bx * FX, y0 = by * FY, z0 = bz * kAdjFZ;
                     const int z1 = std::min(z0 + kAdjFZ, a.L2);
-                    const bool near = f.has_pml && (any(0, a.L0, x0 - 1, x0 + kAdjFX) ||
-                                                    any(a.L0, a.L1, y0 - 1, y0 + kAdjFY) ||
+                    const bool near = f.has_pml && (any(0, a.L0, x0 - 1, x0 + FX) ||
+                                                    any(a.L0, a.L1, y0 - 1, y0 + FY) ||
                                                     any(a.L0 + a.L1, a.L2, z0 - 1, z1));
                     ids.push_back((bx + f.gx * (by + f.gy * bz)) | (near ? 1 << 30 : 0));
                     (near ? f.n_pml : f.n_free)++;
@@ -1697,6 +1700,43 @@ int smx_adjoint_backward(smx_plan *p, double *grad)
                                             hipMemcpyHostToDevice)) != hipSuccess))
             fail(e, "adjoint block list");
         f.blocks = blocks_dev;
+    }
+    /* two transposed steps per launch where both steps lie in one segment (SMX_ADJ_TWO=0:
+     * one step per launch throughout) */
+    const char *two_env = getenv("SMX_ADJ_TWO");
+    const bool two = fused && !(two_env && !strcmp(two_env, "0"));
+    smx_adj_two_args t2;
+    std::memset(&t2, 0, sizeof t2);
+    int *blocks2 = nullptr;
+    float *lbox = nullptr;
+    if (two && rc == SMX_OK) {
+        t2.L0 = a.L0, t2.L1 = a.L1, t2.L2 = a.L2, t2.sz = a.sz, t2.ncomp = n;
+        for (int d = 0; d < 3; d++) t2.w[d] = a.w[d];
+        t2.b = p->f32.b[0], t2.pml = p->f32.pml, t2.pmlflag = p->f32.pmlflag, t2.has_pml = p->f32.has_pml;
+        t2.D = p->adj_D, t2.zT = a.zT;
+        std::memcpy(t2.box, a.box, sizeof t2.box);
+        t2.bx = a.bx, t2.by = a.by, t2.nbox = a.nbox;
+        t2.gx = (a.L0 + 31) / 32, t2.gy = (a.L1 + 7) / 8;
+        /* z chunks: the fewest rounds of resident blocks (2 per CU) times stages per block */
+        int dev = 0, ncu = 0;
+        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            ncu = 256;
+        const long long slots = 2LL * std::max(ncu, 1), layer = (long long)t2.gx * t2.gy;
+        long long best = -1;
+        for (int nz = (a.L2 + kAdjTwoMaxFZ - 1) / kAdjTwoMaxFZ; nz <= a.L2; nz++) {
+            const int fz = (a.L2 + nz - 1) / nz, nzc = (a.L2 + fz - 1) / fz;
+            const long long cost = (layer * nzc + slots - 1) / slots * (fz + 4);
+            if (best < 0 || cost < best) best = cost, t2.fz = fz, t2.nblocks = (int)(layer * nzc);
+            if (fz <= 8) break;
+        }
+        std::vector<int> ids((size_t)t2.nblocks);
+        for (int i = 0; i < t2.nblocks; i++) ids[(size_t)i] = i;
+        if ((e = hipMalloc(&blocks2, sizeof(int) * ids.size())) != hipSuccess ||
+            (e = hipMemcpy(blocks2, ids.data(), sizeof(int) * ids.size(), hipMemcpyHostToDevice)) != hipSuccess ||
+            (e = hipMalloc(&lbox, sizeof(float) * 3 * (size_t)a.nbox)) != hipSuccess)
+            fail(e, "adjoint two-step tables");
+        t2.blocks = blocks2;
+        t2.lbox = lbox;
     }
     if (rc == SMX_OK) {
         point(cur);
@@ -1741,6 +1781,28 @@ int smx_adjoint_backward(smx_plan *p, double *grad)
         for (long long k = ke - 1; k >= ks && rc == SMX_OK; k--) {
             const float *bcur = box_state(k), *bnext = box_state(k + 1);
             const double th = p->adj_w * ((double)(k - 1) + 1.5);
+            if (two && k - 1 >= ks) {
+                /* steps k and k-1: lambda^{k+1} -> lambda^{k-1}, injecting into lambda^k inside;
+                 * then both gradient terms, and the injection into lambda^{k-1} */
+                t2.lT = cur, t2.lS = cur + 6 * (size_t)n, t2.nT = nxt, t2.nS = nxt + 6 * (size_t)n;
+                t2.cs = std::cos(th), t2.sn = std::sin(th);
+                if ((e = smx_adj_launch_two(&t2, p->stream)) != hipSuccess ||
+                    (e = smx_adj_launch_grad2(&a, cur + 3 * (size_t)n, lbox, bnext, bcur, box_state(k - 1),
+                                              p->stream)) != hipSuccess) {
+                    fail(e, "adjoint two steps");
+                    break;
+                }
+                std::swap(cur, nxt);
+                point(cur);
+                const double th2 = p->adj_w * ((double)(k - 2) + 1.5);
+                if (k - 1 >= 1 &&
+                    (e = smx_adj_launch_inject(&a, std::cos(th2), std::sin(th2), p->stream)) != hipSuccess) {
+                    fail(e, "adjoint inject");
+                    break;
+                }
+                k--; /* and the loop's own decrement passes step k-1 */
+                continue;
+            }
             if (fused) {
                 /* gradient with lambda_TE^{k+1}, then lambda^k, then the objective's sample of state k */
                 if ((e = smx_adj_launch_grad(&a, cur + 3 * (size_t)n, bcur, bnext, p->stream)) != hipSuccess) {
@@ -1784,6 +1846,8 @@ int smx_adjoint_backward(smx_plan *p, double *grad)
     if (G) (void)hipFree(G);
     if (gdev) (void)hipFree(gdev);
     if (blocks_dev) (void)hipFree(blocks_dev);
+    if (blocks2) (void)hipFree(blocks2);
+    if (lbox) (void)hipFree(lbox);
     return rc;
 }
 

@@ -144,3 +144,29 @@ def test_fused_transposed_step_equals_gather_kernels(grid, monkeypatch):
     assert np.abs(out[1]).max() > 0
     err = np.linalg.norm(out[0] - out[1]) / np.linalg.norm(out[1])
     assert err <= 1e-5, err
+
+
+TWO_GRIDS = dict(GRIDS, pml_tiles=dict(L0=40, L1=20, L2=60, outer=10, pml_xy=3, pml_z=4, iBc=(0, 0, 0),
+                                       time_period=(1, 2, 4)))
+
+
+@pytest.mark.parametrize("grid,ckpt", [(g, c) for g in TWO_GRIDS for c in (0, 7)])
+def test_two_step_transposed_pass_equals_one_step(grid, ckpt, monkeypatch):
+    """Two transposed steps per launch (adj_two_f32, with the injection into lambda^k and
+    lambda_TE^k of the design box inside the pass) give bit for bit the gradient of one
+    fused transposed step per launch, with whole-run box history and with checkpoint
+    segments of 7 steps (odd: each segment ends with a single step)."""
+    case = _case(**TWO_GRIDS[grid])
+    prep = casegen.prepare(case)
+    zT = min(case.L[2] - 5, case.iI + 50)
+    w = 0.9 * 2 * np.pi / prep.N
+    K = case.time_period[2] * prep.N
+    box = _box(case)
+    out = []
+    for two in ("0", "1"):
+        monkeypatch.setenv("SMX_ADJ_TWO", two)
+        with prep.plan(fdtd.F32) as p:
+            p.adjoint_forward(K, ckpt, zT, w, box)
+            out.append(p.adjoint_backward())
+    assert np.abs(out[0]).max() > 0
+    assert np.array_equal(out[0], out[1]), np.abs(out[0] - out[1]).max() / np.abs(out[0]).max()
