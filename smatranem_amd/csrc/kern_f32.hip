@@ -472,6 +472,13 @@ struct fused_args {
     float *dft;
     int dft_n, dft_slot;
     int dft_planes[kDftMaxPlanes];
+    /* two-step kernel (BOX instantiations): E of both steps on the design box
+     * [dbox[0], dbox[1]) x [dbox[2], dbox[3]) x [dbox[4], dbox[5]), non-PML cells, to
+     * dbox_out (after the first step) and dbox_out + 3*dnbox (after the second), each
+     * [3][dnbox] with x fastest */
+    float *dbox_out;
+    int dbox[6], dbx, dby;
+    int64_t dnbox;
 };
 
 
@@ -1372,7 +1379,7 @@ __device__ inline void e_regular(const float e[3], const float b[3], const float
 /* The pass of one block `id` of a plan whose arguments `a` are the kernel's own (one
  * plan per launch) or an entry of a table in constant memory (several plans per launch,
  * two_step_multi); scal (may be null) holds the 20 per-step scalars instead of a. */
-template <int FX, int FY, int D, bool SPLIT, bool WIDE, bool DFT, bool BIG, typename A>
+template <int FX, int FY, int D, bool SPLIT, bool WIDE, bool DFT, bool BIG, bool BOX, typename A>
 __device__ __forceinline__ void two_step_body(const A &a, int id, const double *scal)
 {
     using Ly = two_layout<FX, FY, SPLIT, WIDE>;
@@ -1448,6 +1455,18 @@ __device__ __forceinline__ void two_step_body(const A &a, int id, const double *
     auto dft_m = [&](int s_) {
         if constexpr (!DFT) return -1;
         return SPLIT ? ((int)lds[L_ZT + 5 * s_ + 4] >> 1) - 1 : reinterpret_cast<const int *>(lds + L_DZ)[s_];
+    };
+    /* E of a tile cell after step `step` (0, 1) into the design-box output */
+    auto box_save = [&](int x, int y, int z, int step, const float v[3]) {
+        if constexpr (BOX) {
+            if (x < a.dbox[0] || x >= a.dbox[1] || y < a.dbox[2] || y >= a.dbox[3] || z < a.dbox[4] || z >= a.dbox[5])
+                return;
+            const int64_t i = (x - a.dbox[0]) + (int64_t)a.dbx * ((y - a.dbox[2]) + (int64_t)a.dby * (z - a.dbox[4]));
+            float *o = a.dbox_out + step * 3 * a.dnbox;
+            o[i] = v[0];
+            o[a.dnbox + i] = v[1];
+            o[2 * a.dnbox + i] = v[2];
+        }
     };
     auto dft_sample = [&](int m, int64_t q, unsigned soff, float hx, float hy, float ex, float ey) {
         if constexpr (DFT) {
@@ -1726,6 +1745,7 @@ __device__ __forceinline__ void two_step_body(const A &a, int id, const double *
                 }
             }
             if (DFT && inT[r]) dft_sample(dm1, cq[r], 0u, hc[0], hc[1], e1n[r][0], e1n[r][1]);
+            if (BOX && inT[r] && p - 1 >= z0 && p - 1 < z1) box_save(cx[r], cy[r], zp1, 0, e1n[r]);
             if (inG2[r]) {
                 lds[L_E1 + 0 * N4 + e] = e1n[r][0];
                 lds[L_E1 + 1 * N4 + e] = e1n[r][1];
@@ -1798,6 +1818,7 @@ __device__ __forceinline__ void two_step_body(const A &a, int id, const double *
             store(false, 3, cq[r] + a.sz * zp2, en);
             if (SPLIT && pml) store(true, 3, cq[r] + a.sz * zp2, sn);
             if (DFT) dft_sample(dm2, cq[r], dstep, hc[0], hc[1], en[0], en[1]);
+            if (BOX) box_save(cx[r], cy[r], zp2, 1, en);
             if (!pml && (zp2 == a.iB || zp2 == a.iT)) { /* second step's monitors */
                 for (int m = 0; m < 2; m++) {
                     if (zp2 != (m == 0 ? a.iB : a.iT)) continue;
@@ -1847,10 +1868,10 @@ __device__ __forceinline__ void two_step_body(const A &a, int id, const double *
     }
 }
 
-template <int FX, int FY, int D, bool SPLIT, bool WIDE, bool DFT, bool BIG = false>
+template <int FX, int FY, int D, bool SPLIT, bool WIDE, bool DFT, bool BIG = false, bool BOX = false>
 __global__ __launch_bounds__(FX * FY) void two_step_f32(fused_args a)
 {
-    two_step_body<FX, FY, D, SPLIT, WIDE, DFT, BIG>(a, xcd_logical(blockIdx.x, a.nblocks), nullptr);
+    two_step_body<FX, FY, D, SPLIT, WIDE, DFT, BIG, BOX>(a, xcd_logical(blockIdx.x, a.nblocks), nullptr);
 }
 
 /* A two-step pass of several plans in one launch (a batch of independent runs): block i
@@ -1865,7 +1886,7 @@ __global__ __launch_bounds__(256) void two_step_multi(const fused_args *args, co
     const int e = blk[xcd_logical(blockIdx.x, gridDim.x)];
     const int plan = e >> 20, id = e & ((1 << 20) - 1);
     const const_fused_args &a = ((const_fused_args *)args)[plan];
-    two_step_body<32, 8, 1, true, false, false, false>(a, id, scal + 20 * plan);
+    two_step_body<32, 8, 1, true, false, false, false, false>(a, id, scal + 20 * plan);
 }
 
 /* ---------------------------------------------------------------------------------- */
@@ -2254,6 +2275,10 @@ constexpr two_kernel kTwoKernelDft[] = {
 static_assert(sizeof kTwoKernelDft / sizeof kTwoKernelDft[0] == kNumTwo, "one kernel per variant");
 /* the split-slot variant for groups of 4 GiB and more (24 n >= 2^32), without and with the DFT */
 constexpr int kTwoBigVariant = 2;
+/* the split-slot variant writing E of both steps on the design box (adjoint forward and
+ * recomputation), without and with the DFT samples */
+constexpr two_kernel kTwoKernelBox[2] = {two_step_f32<32, 8, 1, true, false, false, false, true>,
+                                         two_step_f32<32, 8, 1, true, false, true, false, true>};
 constexpr two_kernel kTwoKernelBig[2] = {two_step_f32<32, 8, 1, true, false, false, true>,
                                          two_step_f32<32, 8, 1, true, false, true, true>};
 
@@ -2788,9 +2813,18 @@ hipError_t smx_f32_multi_launch(const void *args, const int *blk, const double *
 
 const char *smx_f32_multi_kernel_name(void) { return "two_step_multi"; }
 
+int smx_f32_box_fz(const smx_f32_state *st)
+{
+    const int fz = st->two == kMultiVariant ? st->two_fz : 64;
+    return !two_big(st) && smx_f32_two_ok(st, kMultiVariant, fz) ? fz : 0;
+}
+
 hipError_t smx_f32_step2(smx_f32_state *st, const smx_f32_launch_args *la, hipStream_t stream)
 {
-    if (!smx_f32_two_ok(st, st->two, st->two_fz)) return hipErrorInvalidValue;
+    /* the design-box output runs the split-slot variant whatever the plan's choice */
+    const bool boxed = la->box_out != nullptr;
+    const int vr = boxed ? kMultiVariant : st->two, vfz = boxed ? smx_f32_box_fz(st) : st->two_fz;
+    if (boxed ? vfz == 0 : !smx_f32_two_ok(st, vr, vfz)) return hipErrorInvalidValue;
     fused_args a;
     std::memset(&a, 0, sizeof a);
     const smx_geom &g = la->g;
@@ -2799,14 +2833,21 @@ hipError_t smx_f32_step2(smx_f32_state *st, const smx_f32_launch_args *la, hipSt
     a.L2 = g.L2;
     const int box[6] = {0, g.L0, 0, g.L1, 0, g.L2};
     for (int i = 0; i < 6; i++) a.box[i] = box[i];
-    a.fz = st->two_fz;
+    a.fz = vfz;
     a.iB = g.iB; /* also read by the z chunking */
     a.iT = g.iT;
     a.iI = g.iI;
-    const two_variant tv = kTwo[st->two];
+    const two_variant tv = kTwo[vr];
+    if (boxed) {
+        for (int i = 0; i < 6; i++) a.dbox[i] = la->box[i];
+        a.dbx = la->box[1] - la->box[0];
+        a.dby = la->box[3] - la->box[2];
+        a.dnbox = (int64_t)a.dbx * a.dby * (la->box[5] - la->box[4]);
+        a.dbox_out = la->box_out;
+    }
     if (la->dft_n > 0) {
         /* the kernel samples only the cells it updates */
-        if (!smx_f32_two_full(st, st->two) || la->dft_n > kDftMaxPlanes) return hipErrorInvalidValue;
+        if (!smx_f32_two_full(st, vr) || la->dft_n > kDftMaxPlanes) return hipErrorInvalidValue;
         a.dft = la->dft;
         a.dft_n = la->dft_n;
         a.dft_slot = la->dft_slot;
@@ -2822,7 +2863,8 @@ hipError_t smx_f32_step2(smx_f32_state *st, const smx_f32_launch_args *la, hipSt
     a.nblocks = a.gx * a.gy * a.nzc;
     auto launch_two = [&](const fused_args &k) {
         const dim3 grid(k.nblocks), block(tv.fx * tv.fy);
-        const two_kernel kern = two_big(st) ? kTwoKernelBig[k.dft_n ? 1 : 0]
+        const two_kernel kern = boxed ? kTwoKernelBox[k.dft_n ? 1 : 0]
+                                : two_big(st) ? kTwoKernelBig[k.dft_n ? 1 : 0]
                                 : k.dft_n ? kTwoKernelDft[st->two] : kTwoKernel[st->two];
         hipLaunchKernelGGL(kern, grid, block, 0, stream, k);
         return hipGetLastError();

@@ -102,6 +102,11 @@ struct smx_f32_launch_args {
     float *dft;
     int dft_n, dft_slot;
     int dft_planes[8];
+    /* smx_f32_step2 only: E of the design box box[6] (non-PML cells) after each of the two
+     * steps, to box_out and box_out + 3*nbox ([3][nbox], x fastest); null: none.  Runs
+     * the split-slot variant (smx_f32_box_fz(st) != 0). */
+    float *box_out;
+    int box[6];
 };
 
 int smx_f32_create(smx_f32_state *st, const smx_geom *g, double iR, double iI, const double *mu,
@@ -120,6 +125,8 @@ int smx_f32_num_two(void);
  * (no region sweeps), so that it can also take the DFT samples of both steps. */
 int smx_f32_two_full(const smx_f32_state *st, int variant);
 int smx_f32_two_ok(const smx_f32_state *st, int variant, int fz);
+/* z chunk of the design-box two-step pass, or 0 where the grid cannot take it */
+int smx_f32_box_fz(const smx_f32_state *st);
 /* tile of two-step variant `variant`: fx x fy cells, one thread each */
 void smx_f32_two_shape(int variant, int *fx, int *fy);
 int smx_f32_write_fields(smx_f32_state *st, const double *H_t, const double *E_t, hipStream_t stream);

@@ -606,11 +606,15 @@ static smx_f32_launch_args step2_args(smx_plan *p, bool mon_kernel)
 }
 
 static int launch_step2(smx_plan *p, const smx_step_scalars &s, const smx_step_scalars &s2, int dft_slot,
-                        bool mon_kernel)
+                        bool mon_kernel, float *box_out = nullptr)
 {
     smx_f32_launch_args a = step2_args(p, mon_kernel);
     a.s = s;
     a.s2 = s2;
+    if (box_out) {
+        a.box_out = box_out;
+        std::memcpy(a.box, p->adj_box, sizeof a.box);
+    }
     if (dft_slot >= 0) {
         a.dft = (float *)p->dft_series;
         a.dft_n = p->dft_ring;
@@ -630,6 +634,13 @@ static bool pair_ok(const smx_plan *p)
     return p->dft_nplanes == 0 || (smx_f32_two_full(&p->f32, p->f32.two) && p->dft_chunk >= 2);
 }
 
+/* The same for the pass that also writes E of the design box (the split-slot variant,
+ * whatever kernel the plan selected). */
+static bool pair_box_ok(const smx_plan *p)
+{
+    return p->g.precision == SMX_F32 && smx_f32_box_fz(&p->f32) > 0 && (p->dft_nplanes == 0 || p->dft_chunk >= 2);
+}
+
 /* Ring slot of the pair (k, k+1) of DFT samples, flushing the ring first where the
  * pair does not continue its chunk or does not fit. */
 static int dft_pair_slot(smx_plan *p, long long k, int *slot)
@@ -643,16 +654,19 @@ static int dft_pair_slot(smx_plan *p, long long k, int *slot)
     return SMX_OK;
 }
 
+static int save_box(smx_plan *p, float *dst);
+
 /* Step k of the source loop, or steps k and k+1 in one pass when the plan's two-step
  * kernel is in use (pair_ok) and both steps are to run; *done = steps taken.  sums get
- * the envelope values in the same order as step-by-step. */
-static int advance(smx_plan *p, long long k, long long nleft, double *sums, int *done)
+ * the envelope values in the same order as step-by-step.  box_out (may be null): E of
+ * the design box after step k to box_out, after step k+1 to box_out + 3*nbox. */
+static int advance(smx_plan *p, long long k, long long nleft, double *sums, int *done, float *box_out = nullptr)
 {
     smx_step_scalars s;
     double fH, fE;
     int rc = step_scalars(p, (int)(k / p->N), (int)(k % p->N), &s, &fH, &fE);
     if (rc) return rc;
-    if (pair_ok(p) && nleft >= 2) {
+    if ((box_out ? pair_box_ok(p) : pair_ok(p)) && nleft >= 2) {
         smx_step_scalars s2;
         double fH2, fE2;
         if (step_scalars(p, (int)((k + 1) / p->N), (int)((k + 1) % p->N), &s2, &fH2, &fE2) == SMX_OK) {
@@ -665,7 +679,7 @@ static int advance(smx_plan *p, long long k, long long nleft, double *sums, int 
             *done = 2;
             int slot = -1;
             if (p->dft_nplanes && (rc = dft_pair_slot(p, k, &slot))) return rc;
-            if ((rc = launch_step2(p, s, s2, slot, !mon_ring(p)))) return rc;
+            if ((rc = launch_step2(p, s, s2, slot, !mon_ring(p), box_out))) return rc;
             if (p->dft_nplanes) {
                 if (mon_ring(p)) {
                     mark_mon_slot(p, slot);
@@ -683,6 +697,7 @@ static int advance(smx_plan *p, long long k, long long nleft, double *sums, int 
     }
     *done = 1;
     if ((rc = launch_step(p, s, false, !mon_ring(p)))) return rc;
+    if (box_out && (rc = save_box(p, box_out))) return rc;
     return dft_after_step(p, k, mon_ring(p));
 }
 
@@ -1421,6 +1436,15 @@ static smx_adj_args adj_args(smx_plan *p)
 
 } // extern "C"
 
+/* E of the design box (the adjoint's box) from the current fields to dst, [3][nbox]. */
+static int save_box(smx_plan *p, float *dst)
+{
+    const smx_adj_args a = adj_args(p);
+    float **T = p->f32.T[p->f32.cur];
+    SMX_HIP(smx_adj_launch_save_box(&a, T[3], T[4], T[5], dst, p->stream));
+    return SMX_OK;
+}
+
 namespace {
 
 /* The DFT state of a plan (set by smx_plan_set_dft), parked while the adjoint forward
@@ -1480,9 +1504,18 @@ bool adjoint_forward_paired(const smx_plan *p, int ckpt_every)
 int adjoint_forward_two_step(smx_plan *p, long long nsteps, int ckpt_every, int zT, double w, double *J)
 {
     const int64_t n = p->n;
+    const bool hist = ckpt_every == 0; /* E of the design box after every step, no checkpoints */
+    const smx_adj_args ab = adj_args(p);
     const dft_state saved = park_dft(p);
     int rc = smx_plan_set_dft(p, 1, &zT, 1, &w, 64);
+    if (rc == SMX_OK && hist) rc = save_box(p, p->adj_hist);
     for (long long k = 0; k < nsteps && rc == SMX_OK;) {
+        if (hist) {
+            int done = 0;
+            rc = advance(p, k, nsteps - k, nullptr, &done, p->adj_hist + (size_t)(k + 1) * 3 * (size_t)ab.nbox);
+            k += done;
+            continue;
+        }
         if (k % ckpt_every == 0) {
             float *dst = p->adj_ckpt + (size_t)(k / ckpt_every) * 12 * (size_t)n;
             hipError_t e = hipSuccess;
@@ -1577,7 +1610,8 @@ int smx_adjoint_forward(smx_plan *p, long long nsteps, int ckpt_every, int zT, d
             if (p->f32.T[k][i]) SMX_HIP(hipMemsetAsync(p->f32.T[k][i], 0, sizeof(float) * n, p->stream));
             if (p->f32.S[k][i]) SMX_HIP(hipMemsetAsync(p->f32.S[k][i], 0, sizeof(float) * n, p->stream));
         }
-    if (!hist && adjoint_forward_paired(p, ckpt_every)) {
+    const char *box2 = getenv("SMX_ADJ_BOX2"); /* "0": the history forward one step per launch */
+    if (hist ? pair_box_ok(p) && !(box2 && !strcmp(box2, "0")) : adjoint_forward_paired(p, ckpt_every)) {
         const int rc = adjoint_forward_two_step(p, nsteps, ckpt_every, zT, w, J);
         if (rc == SMX_OK) p->adj_ready = true;
         return rc;
@@ -1763,19 +1797,25 @@ int smx_adjoint_backward(smx_plan *p, double *grad)
                                         hipMemcpyDeviceToDevice, p->stream)) != hipSuccess)
                     fail(e, "restore checkpoint");
             }
-            auto save = [&](long long slot) {
-                float **T = p->f32.T[p->f32.cur];
-                hipError_t ee =
-                    smx_adj_launch_save_box(&a, T[3], T[4], T[5], seg + (size_t)slot * 3 * a.nbox, p->stream);
-                if (ee != hipSuccess) fail(ee, "save design box");
-            };
-            if (rc == SMX_OK) save(0);
-            for (long long k = ks; k < ke && rc == SMX_OK; k++) {
-                smx_step_scalars sc;
+            /* two steps per pass writing the box after both (the split-slot two-step
+             * kernel, SMX_ADJ_BOX2=0: one step per launch), monitors left alone */
+            const char *box2 = getenv("SMX_ADJ_BOX2");
+            const bool pairs = pair_box_ok(p) && !(box2 && !strcmp(box2, "0"));
+            if (rc == SMX_OK) rc = save_box(p, seg);
+            for (long long k = ks; k < ke && rc == SMX_OK;) {
+                smx_step_scalars sc, sc2;
                 double fH, fE;
                 if ((rc = step_scalars(p, (int)(k / p->N), (int)(k % p->N), &sc, &fH, &fE))) break;
-                if ((rc = launch_step(p, sc, false))) break;
-                save(k - ks + 1);
+                float *slot = seg + (size_t)(k - ks + 1) * 3 * a.nbox;
+                if (pairs && ke - k >= 2) {
+                    if ((rc = step_scalars(p, (int)((k + 1) / p->N), (int)((k + 1) % p->N), &sc2, &fH, &fE)) ||
+                        (rc = launch_step2(p, sc, sc2, -1, false, slot)))
+                        break;
+                    k += 2;
+                    continue;
+                }
+                if ((rc = launch_step(p, sc, false, false)) || (rc = save_box(p, slot))) break;
+                k++;
             }
         }
         for (long long k = ke - 1; k >= ks && rc == SMX_OK; k--) {

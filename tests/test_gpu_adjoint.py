@@ -170,3 +170,27 @@ def test_two_step_transposed_pass_equals_one_step(grid, ckpt, monkeypatch):
             out.append(p.adjoint_backward())
     assert np.abs(out[0]).max() > 0
     assert np.array_equal(out[0], out[1]), np.abs(out[0] - out[1]).max() / np.abs(out[0]).max()
+
+
+@pytest.mark.parametrize("ckpt", [0, 8, 9])
+def test_box_history_two_step_forward(ckpt, monkeypatch):
+    """The design box's E taken inside the two-step forward pass (both steps of each pass;
+    the box-history forward, and the recomputation of checkpoint segments, even and odd)
+    against one forward step per launch with a box copy after each: the same J and
+    gradient up to FP32 rounding order."""
+    monkeypatch.setenv("SMX_F32_TWO", "2,16")
+    case = _case(**TWO_GRIDS["pml_tiles"])
+    prep = casegen.prepare(case)
+    zT = min(case.L[2] - 5, case.iI + 50)
+    w = 0.9 * 2 * np.pi / prep.N
+    K = case.time_period[2] * prep.N - 1  # odd: the last step runs alone
+    box = _box(case)
+    out = []
+    for box2 in ("0", "1"):
+        monkeypatch.setenv("SMX_ADJ_BOX2", box2)
+        with prep.plan(fdtd.F32) as p:
+            J = p.adjoint_forward(K, ckpt, zT, w, box)
+            out.append((J, p.adjoint_backward()))
+    (J0, g0), (J1, g1) = out
+    assert J0 > 0 and abs(J1 - J0) <= 1e-6 * J0, (J0, J1)
+    assert np.linalg.norm(g1 - g0) <= 1e-5 * np.linalg.norm(g0), np.linalg.norm(g1 - g0) / np.linalg.norm(g0)
