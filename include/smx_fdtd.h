@@ -258,14 +258,15 @@ int smx_plan_project(smx_plan *plan, int nmodes, const double *modes, double *S_
  * over steps k = 0 .. nsteps-1 of the plan's source loop, from zero fields.
  * smx_adjoint_forward runs the forward pass, storing the full state every ckpt_every
  * steps, and returns J.  smx_adjoint_backward runs the discrete adjoint in reverse
- * time (one fused transposed step per launch), recomputing each checkpoint segment
- * forward to get E in the design box, and writes grad[3][nbox] = dJ/d eps_c (c = x, y, z) for the cells of the design box
- * box = {x0, x1, y0, y1, z0, z1} (half-open, x fastest), which must hold only 'n'
- * cells.  Both leave the plan's fields at the last recomputed state and its monitors
- * accumulated over the recomputation.  ckpt_every = 0 trades memory for the
- * recomputation: the forward pass stores E of the design box after every step
- * (12*nbox*(nsteps+1) bytes of device memory instead of 48*n per checkpoint) and the
- * backward pass recomputes nothing.
+ * time -- two transposed steps per launch, bit for bit what one step per launch gives
+ * (SMX_ADJ_TWO=0) -- recomputing each checkpoint segment forward (two steps per pass,
+ * writing E of the design box after each) and writes grad[3][nbox] = dJ/d eps_c
+ * (c = x, y, z) for the cells of the design box box = {x0, x1, y0, y1, z0, z1}
+ * (half-open, x fastest), which must hold only 'n' cells.  Both leave the plan's fields
+ * at the last recomputed state; the recomputation leaves its monitors alone.
+ * ckpt_every = 0 trades memory for the recomputation: the forward pass stores E of the
+ * design box after every step (12*nbox*(nsteps+1) bytes of device memory instead of 48*n
+ * per checkpoint) and the backward pass recomputes nothing.
  */
 int smx_adjoint_forward(smx_plan *plan, long long nsteps, int ckpt_every, int zT, double w, const int box[6],
                         double *J);
