@@ -206,6 +206,50 @@ def box_run(args, torch, K, W):
     return box
 
 
+def metal_run(args, torch, K, W):
+    import numpy as np
+    """The headline grid with an eps-metal block (eps = -20 + 2i, class 'e', 16 x 16 x 16
+    cells in the cladding beside the core, 1/2000 of the grid): the reference monitors,
+    no DFT (the two-step pass leaves the tiles around lossy/metal cells to region sweeps,
+    which take no DFT samples), autotuned, max(K/5, 100) steps timed; also the best
+    one-step kernel on the same grid."""
+    import os
+    from smatranem_amd import casegen, fdtd
+    L, pml = ((32, 32, 48), 4) if args.small else ((160, 160, 320), 10)
+    box = (12, 16, 12, 16, 20, 24) if args.small else (100, 116, 72, 88, 150, 166)
+    mcase = casegen.curvilinear_waveguide_case(L, pml=pml, metal_box=(box, -20.0 + 2.0j))
+    mprep = casegen.prepare(mcase)
+    kb = max(K // 5, 100)
+    out = {}
+    for label, two in (("two_step", None), ("one_step", "off")):
+        if two:
+            os.environ["SMX_F32_TWO"] = two  # the autotuner then times one-step kernels only
+        try:
+            with mprep.plan(fdtd.F32) as pm:
+                choice = pm.autotune(20)
+                assert (choice[3] == 3) == (two is None), (label, choice)
+                kern = pm.kernels
+                wb, evb = time_plan(pm, kb, min(W, 100), torch)
+                H, E = pm.read_fields()
+                finite = bool(np.isfinite(H).all() and np.isfinite(E).all())
+        finally:
+            os.environ.pop("SMX_F32_TWO", None)
+        out[label] = dict(kernel=kern, value=round(mcase.n / (wb / kb * 1e-3) / 1e9, 4),
+                          kernel_ms_per_step=round(evb / kb, 5), fields_finite=finite)
+    bphi = mprep.pml_fraction
+    bb = bytes_per_cell(bphi, 4, 4)
+    two = out["two_step"]
+    res = dict(workload="same grid, eps-metal block %s (eps -20+2i, class 'e'), z PML, reference monitors, "
+                        "no DFT" % list(box),
+               special_cells=int(np.count_nonzero(mprep.Cont == ord("e"))), steps=kb, kernel=two["kernel"],
+               value=two["value"], kernel_ms_per_step=two["kernel_ms_per_step"],
+               roofline_frac=round(bb * mcase.n / (two["kernel_ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+               fields_finite=two["fields_finite"], one_step=out["one_step"])
+    print(f"[bench] metal fp32: {res['value']:.2f} Gcell/s ({res['kernel']}), one step per pass "
+          f"{out['one_step']['value']:.2f} ({out['one_step']['kernel']})", file=sys.stderr, flush=True)
+    return res
+
+
 SMALL_BATCH = dict(dims=((24, 48), (24, 48), (32, 64)), pml=4, seed=2026)
 
 
@@ -516,6 +560,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 leg")
     ap.add_argument("--no-box", action="store_true", help="skip the all-faces-PML variant")
     ap.add_argument("--no-batch", action="store_true", help="skip the batch of independent runs")
+    ap.add_argument("--no-metal", action="store_true", help="skip the grid with an eps-metal block")
     ap.add_argument("--batch-count", type=int, default=24)
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint gradient workload")
@@ -533,7 +578,7 @@ def main():
     ap.add_argument("--adjoint-fd-seed", type=int, default=None, metavar="SEED",
                     help="draw the --adjoint-fd cells uniformly from the design box with this seed")
     ap.add_argument("--small", action="store_true", help="32x32x48 grid (quick check)")
-    ap.add_argument("--only", choices=["box", "batch", "batch-small", "adjoint"], default=None,
+    ap.add_argument("--only", choices=["box", "metal", "batch", "batch-small", "adjoint"], default=None,
                     help="run only this leg (for profiling) and print its JSON object")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device copies of MIB MiB (scale for rocprofv3 byte counters)")
@@ -564,6 +609,8 @@ def main():
     if args.only:
         if args.only == "box":
             out = box_run(args, torch, K, W)
+        elif args.only == "metal":
+            out = metal_run(args, torch, K, W)
         elif args.only == "batch":
             out = batch_run(args.batch_count, args.batch_steps, torch, args.small)
         elif args.only == "batch-small":
@@ -645,6 +692,8 @@ def main():
     if not args.no_box:
         box = box_run(args, torch, K, W)
 
+    metal = None if args.no_metal else metal_run(args, torch, K, W)
+
     batch = batch_small = None
     if not args.no_batch:
         batch = batch_run(args.batch_count, args.batch_steps, torch, args.small)
@@ -721,6 +770,7 @@ def main():
         "fp64": fp64,
         "err_vs_fp64": err,
         "pml_all_faces": box,
+        "metal": metal,
         "batch": batch,
         "batch_small_grids": batch_small,
         "adjoint": adjoint,

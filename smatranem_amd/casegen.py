@@ -134,7 +134,7 @@ def waveguide_case(L0, L1, L2, outer, pml_xy, pml_z, *, omega=2 * np.pi / 20.0, 
                    eps_clad=2.1, core=(8, 6), scale_amp=0.0, n_modes=8, pec_fraction=0.0,
                    seed=0, iBc=(0, 0, 0), dt_factor=1.0, time_period=(2, 1, 8), monitor_gap=2,
                    metal_cells=0, lossy_cells=0, lossy_mu_cells=0, blocks=0, design=None, layers=0, grating=False,
-                   progress=None, keep_fx=None):
+                   progress=None, keep_fx=None, metal_box=None):
     """A straight dielectric core along z in cladding, on an L0 x L1 x L2 grid with
     ``outer`` port slices at each end (B0.., T..), PML layers of ``pml_xy`` cells on the
     x/y faces and ``pml_z`` on the z faces (PMLcalc.c:188-260), optional curvilinear scale
@@ -153,7 +153,9 @@ def waveguide_case(L0, L1, L2, outer, pml_xy, pml_z, *, omega=2 * np.pi / 20.0, 
     plane just above the bottom monitor.  ``progress`` (optional) is called with a short
     text after each stage, for callers that report on long generations.  ``keep_fx``
     keeps the scale-factor ratios in the case (``fx_mu``, ``fx_eps``) for the file
-    writer; by default they are kept on grids of up to 2^25 cells."""
+    writer; by default they are kept on grids of up to 2^25 cells.  ``metal_box`` =
+    ((x0, x1, y0, y1, z0, z1), eps) fills that box with an eps-metal (Re eps < 0: class
+    'e', the stable form of F:129-139)."""
     rng = np.random.default_rng(seed)
     say = progress if progress is not None else (lambda msg: None)
     n = L0 * L1 * L2
@@ -270,6 +272,9 @@ def waveguide_case(L0, L1, L2, outer, pml_xy, pml_z, *, omega=2 * np.pi / 20.0, 
                 eps_c[zz, yy, xx, :] = -10.0 + 1.5j
             if k in (1, 2):
                 mu_c[zz, yy, xx, :] = -2.0 + 0.5j
+    if metal_box is not None:
+        (bx0, bx1, by0, by1, bz0, bz1), beps = metal_box
+        eps_c[bz0:bz1, by0:by1, bx0:bx1, :] = beps
     if lossy_cells:
         zs, ys, xs = pick(lossy_cells)
         for zz, yy, xx in zip(zs, ys, xs):
@@ -341,16 +346,19 @@ def block_in_vacuum_case():
                           time_period=(10, 80, 111), monitor_gap=1, design=((16, 32, 16, 32, 20, 44), 12.1, 12.1))
 
 
-def curvilinear_waveguide_case(L=(160, 160, 320), pml=10, pml_xy=0, time_period=(5, 100000, 100010)):
+def curvilinear_waveguide_case(L=(160, 160, 320), pml=10, pml_xy=0, time_period=(5, 100000, 100010),
+                               metal_box=None):
     """The headline workload: a curvilinear L0 x L1 x L2 grid (scale factors
     h_j = 1 + 0.3 g_j, g_j a seeded sum of 8 low-order Fourier modes), a Si-like core
     eps = 12.1 in eps = 2.1 cladding, PEC on ~5% of the scatterer's E edges as seeded slabs,
     a ``pml``-cell z PML and PEC x/y walls (``pml_xy`` > 0 puts PML on the x/y faces too,
-    with the reference monitors off), a modal sheet source."""
+    with the reference monitors off), a modal sheet source; ``metal_box`` as in
+    waveguide_case."""
     L0, L1, L2 = L
     return waveguide_case(L0, L1, L2, outer=L2 // 8, pml_xy=pml_xy, pml_z=pml, omega=2 * np.pi / 20.0,
                           eps_core=12.1, eps_clad=2.1, core=(L0 // 8, L1 // 10), scale_amp=0.3, n_modes=8,
-                          pec_fraction=0.05, seed=2024, iBc=(0, 0, 0), time_period=time_period)
+                          pec_fraction=0.05, seed=2024, iBc=(0, 0, 0), time_period=time_period,
+                          metal_box=metal_box)
 
 
 def layered_stack_case(L=(768, 768, 256), pml=12, layers=12, seed=2027, time_period=(5, 100000, 100010),

@@ -424,6 +424,7 @@ constexpr int kDefaultTile = 0, kDefaultFZ = 16;
 
 /* most z chunks of one two-step launch */
 constexpr int kMaxZChunks = 256;
+constexpr int kMaxRegions = 8; /* boxes of a region sweep */
 constexpr int kDftMaxPlanes = 8; /* as SMX_DFT_MAX_PLANES */
 
 struct fused_args {
@@ -462,10 +463,13 @@ struct fused_args {
     int zc[kMaxZChunks + 1];
     /* boxes of the region sweeps and their first thread index */
     int nreg;
-    int reg[6][6];
-    int64_t reg_start[7];
+    int reg[kMaxRegions][6];
+    int64_t reg_start[kMaxRegions + 1];
     /* region_e_f32 accumulates no monitor for cells inside this box */
     int monskip[6];
+    /* region_e_f32 with monin_on: monitors only for cells in the ranges
+     * [monin[2d], monin[2d] + monin[2d+1]) of each axis d, taken modulo its length */
+    int monin[6], monin_on;
     /* two-step kernel: ring of the multi-frequency DFT ([slot][plane][Hx, Hy, Ex, Ey][q],
      * floats); the first step's samples go to slot dft_slot, the second's to the next;
      * dft_n = 0: no DFT */
@@ -479,6 +483,8 @@ struct fused_args {
     float *dbox_out;
     int dbox[6], dbx, dby;
     int64_t dnbox;
+    /* two-step kernel: the block ids it runs (device), null = all of them */
+    const int *blist;
 };
 
 
@@ -1871,7 +1877,8 @@ __device__ __forceinline__ void two_step_body(const A &a, int id, const double *
 template <int FX, int FY, int D, bool SPLIT, bool WIDE, bool DFT, bool BIG = false, bool BOX = false>
 __global__ __launch_bounds__(FX * FY) void two_step_f32(fused_args a)
 {
-    two_step_body<FX, FY, D, SPLIT, WIDE, DFT, BIG, BOX>(a, xcd_logical(blockIdx.x, a.nblocks), nullptr);
+    const int h = xcd_logical(blockIdx.x, gridDim.x);
+    two_step_body<FX, FY, D, SPLIT, WIDE, DFT, BIG, BOX>(a, a.blist ? a.blist[h] : h, nullptr);
 }
 
 /* A two-step pass of several plans in one launch (a batch of independent runs): block i
@@ -2006,8 +2013,11 @@ __global__ __launch_bounds__(kThreads) void region_e_f32(fused_args a)
 #pragma unroll
         for (int c = 0; c < 3; c++) En[c] = e[c] + b[c] * (P[c] + M[c]);
         store3(a.Tn + 3, l, En);
-        const bool skip = x >= a.monskip[0] && x < a.monskip[1] && y >= a.monskip[2] && y < a.monskip[3] &&
-                          z >= a.monskip[4] && z < a.monskip[5];
+        auto in_mod = [](int v, int lo, int len, int L) { return ((v - lo) % L + L) % L < len; };
+        const bool skip = a.monin_on ? !(in_mod(x, a.monin[0], a.monin[1], a.L0) && in_mod(y, a.monin[2], a.monin[3], a.L1) &&
+                                         in_mod(z, a.monin[4], a.monin[5], a.L2))
+                                     : x >= a.monskip[0] && x < a.monskip[1] && y >= a.monskip[2] && y < a.monskip[3] &&
+                                           z >= a.monskip[4] && z < a.monskip[5];
         for (int m = 0; m < 2 && !skip; m++) { /* monitors: main = total here (S = 0) */
             if (z != (m == 0 ? a.iB : a.iT)) continue;
             double *HM = m == 0 ? a.HB : a.HT, *EM = m == 0 ? a.EB : a.ET;
@@ -2295,7 +2305,10 @@ int smx_f32_two_ok(const smx_f32_state *st, int variant, int fz)
      * split slots need the PML outside the kernel's box, those with them keep the
      * chunk's z tables in LDS; 16-byte copies need x groups of 4 that never straddle
      * the x wrap (the two-step box starts at a multiple of 4) */
-    if (variant < 0 || variant >= kNumTwo || fz < 1 || st->nspecial != 0) return 0;
+    if (variant < 0 || variant >= kNumTwo || fz < 1) return 0;
+    /* lossy/metal cells: the split-slot variant in its 32-bit form, the tiles near them
+     * left to region sweeps (smx_f32_step2) */
+    if (st->nspecial != 0 && (variant != kMultiVariant || two_big(st))) return 0;
     /* groups of 4 GiB and more: the split-slot variant's 64-bit form, one array below 4 GiB */
     if ((24 * st->n >= ((int64_t)1 << 32) || st->force_big) && (variant != kTwoBigVariant || 4 * st->n >= ((int64_t)1 << 32)))
         return 0;
@@ -2323,6 +2336,9 @@ const char *smx_f32_kernel_name(const smx_f32_state *st, int dft)
         if (two_big(st))
             return dft ? "two_step_f32<32,8,1,true,false,true,true>" : "two_step_f32<32,8,1,true,false,false,true>";
         if (st->has_pml && !kTwo[st->two].split) return kTwo[st->two].with_regions;
+        if (st->nspecial > 0)
+            return "two_step_f32<32,8,1,true,false,false,false> + region_h_f32 + region_e_f32 + special_h_pass + "
+                   "special_e_pass (the tiles around lossy/metal cells)";
         return dft ? kTwo[st->two].name_dft : kTwo[st->two].name;
     }
     if (!st->fused) return st->bx == 64 ? "h_sweep_f32<64> + e_sweep_f32<64>" : "h_sweep_f32<32> + e_sweep_f32<32>";
@@ -2368,6 +2384,11 @@ void smx_f32_free(smx_f32_state *st)
     st->zpml_host = nullptr;
     if (st->special) (void)hipFree(st->special);
     if (st->special_cells) (void)hipFree(st->special_cells);
+    delete[] st->special_host;
+    st->special_host = nullptr;
+    if (st->sp_blist) (void)hipFree(st->sp_blist);
+    st->sp_blist = nullptr;
+    st->sp_valid = 0;
     st->pml = nullptr;
     st->pmlflag = nullptr;
     st->special = nullptr;
@@ -2466,6 +2487,10 @@ int smx_f32_create(smx_f32_state *st, const smx_geom *g, double iR, double iI, c
         bw[5 * (size_t)n + l] = 0.f;
     }
     st->nspecial = (int)special.size();
+    if (!special_cells.empty()) {
+        st->special_host = new int64_t[special_cells.size()];
+        std::memcpy(st->special_host, special_cells.data(), sizeof(int64_t) * special_cells.size());
+    }
     st->has_pml = 0;
     for (unsigned char f : flag) st->has_pml |= f;
     plan_split(st, flag.data());
@@ -2732,7 +2757,8 @@ static void two_tiles_x(fused_args &a, int ox, int fx, int L0)
 
 hipError_t smx_f32_reserve(smx_f32_state *st, hipStream_t stream)
 {
-    if (st->two < 0 || !st->has_pml || kTwo[st->two].split || st->tmpT) return hipSuccess;
+    const bool need = st->two >= 0 && ((st->has_pml && !kTwo[st->two].split) || st->nspecial > 0);
+    if (!need || st->tmpT) return hipSuccess;
     hipError_t e;
     if ((e = hipMalloc(&st->tmpT, sizeof(float) * 6 * st->n)) != hipSuccess) return e;
     if ((e = hipMalloc(&st->tmpS, sizeof(float) * 6 * st->n)) != hipSuccess) return e;
@@ -2747,7 +2773,7 @@ size_t smx_f32_multi_args_size(void) { return sizeof(fused_args); }
 
 int smx_f32_multi_ok(const smx_f32_state *st, int fz)
 {
-    return !two_big(st) && smx_f32_two_ok(st, kMultiVariant, fz);
+    return st->nspecial == 0 && !two_big(st) && smx_f32_two_ok(st, kMultiVariant, fz);
 }
 
 int smx_f32_multi_slots(void)
@@ -2812,6 +2838,113 @@ hipError_t smx_f32_multi_launch(const void *args, const int *blk, const double *
 }
 
 const char *smx_f32_multi_kernel_name(void) { return "two_step_multi"; }
+
+/* Boxes in [0, L) covering the unwrapped ranges [lo[d], lo[d] + len[d]) of each axis
+ * (a range as long as its axis covers it once). */
+static void wrap_boxes(const int L[3], const int lo[3], const int len[3], smx_region_set *rs)
+{
+    int iv[3][2][2], ni[3];
+    for (int d = 0; d < 3; d++) {
+        if (len[d] >= L[d]) {
+            ni[d] = 1, iv[d][0][0] = 0, iv[d][0][1] = L[d];
+            continue;
+        }
+        const int a = ((lo[d] % L[d]) + L[d]) % L[d];
+        if (a + len[d] <= L[d]) {
+            ni[d] = 1, iv[d][0][0] = a, iv[d][0][1] = a + len[d];
+        } else {
+            ni[d] = 2, iv[d][0][0] = a, iv[d][0][1] = L[d], iv[d][1][0] = 0, iv[d][1][1] = a + len[d] - L[d];
+        }
+    }
+    rs->n = 0;
+    rs->start[0] = 0;
+    for (int i = 0; i < ni[0]; i++)
+        for (int j = 0; j < ni[1]; j++)
+            for (int k = 0; k < ni[2]; k++) {
+                int *b = rs->box[rs->n];
+                b[0] = iv[0][i][0], b[1] = iv[0][i][1], b[2] = iv[1][j][0], b[3] = iv[1][j][1];
+                b[4] = iv[2][k][0], b[5] = iv[2][k][1];
+                rs->start[rs->n + 1] = rs->start[rs->n] + (int64_t)(b[1] - b[0]) * (b[3] - b[2]) * (b[5] - b[4]);
+                rs->n++;
+            }
+}
+
+/*
+ * The tiles of the two-step pass `a` (its x tiles, y tiles and z chunks) on a grid with
+ * lossy/metal cells: a tile is dirty when a special cell lies in its stage region G4
+ * (x0-2 .. x0+FX+1, y0-2 .. y0+FY+1) on its stage planes (z0-2 .. z1+1).  R is the box of
+ * the dirty tiles (in tile indices, so a tile inside it that is clean belongs to it too);
+ * the tiles outside R run the pass (st->sp_blist), R runs the region sweeps.
+ */
+static hipError_t special_tiles(smx_f32_state *st, const fused_args &a, int fx, int fy)
+{
+    int key[5] = {a.gx, a.gy, a.nzc, a.xorg, 0};
+    for (int i = 0; i <= a.nzc; i++) key[4] = key[4] * 31 + a.zc[i];
+    if (st->sp_valid && std::memcmp(key, st->sp_key, sizeof key) == 0) return hipSuccess;
+    const int L[3] = {st->L[0], st->L[1], st->L[2]};
+    std::vector<char> dirty((size_t)a.gx * a.gy * a.nzc, 0);
+    auto tiles = [](int v, int Lv, int ntiles, auto lo_of, auto hi_of, std::vector<int> &out) {
+        out.clear();
+        for (int u = v - Lv; u <= v + Lv; u += Lv)
+            for (int t = 0; t < ntiles; t++)
+                if (u >= lo_of(t) && u < hi_of(t)) out.push_back(t);
+    };
+    std::vector<int> tx, ty, tz;
+    for (int i = 0; i < st->nspecial; i++) {
+        const int64_t l = st->special_host[i];
+        const int x = (int)(l % L[0]), y = (int)((l / L[0]) % L[1]), z = (int)(l / ((int64_t)L[0] * L[1]));
+        tiles(x, L[0], a.gx, [&](int t) { return a.xorg + t * fx - 2; }, [&](int t) { return a.xorg + t * fx + fx + 2; },
+              tx);
+        tiles(y, L[1], a.gy, [&](int t) { return a.box[2] + t * fy - 2; }, [&](int t) { return a.box[2] + t * fy + fy + 2; },
+              ty);
+        tiles(z, L[2], a.nzc, [&](int t) { return a.zc[t] - 2; }, [&](int t) { return a.zc[t + 1] + 2; }, tz);
+        for (int bz : tz)
+            for (int by : ty)
+                for (int bx : tx) dirty[(size_t)bx + (size_t)a.gx * (by + (size_t)a.gy * bz)] = 1;
+    }
+    int lo[3] = {a.gx, a.gy, a.nzc}, hi[3] = {-1, -1, -1};
+    for (int bz = 0; bz < a.nzc; bz++)
+        for (int by = 0; by < a.gy; by++)
+            for (int bx = 0; bx < a.gx; bx++)
+                if (dirty[(size_t)bx + (size_t)a.gx * (by + (size_t)a.gy * bz)]) {
+                    const int t[3] = {bx, by, bz};
+                    for (int d = 0; d < 3; d++) lo[d] = std::min(lo[d], t[d]), hi[d] = std::max(hi[d], t[d]);
+                }
+    std::vector<int> clean;
+    for (int bz = 0; bz < a.nzc; bz++)
+        for (int by = 0; by < a.gy; by++)
+            for (int bx = 0; bx < a.gx; bx++)
+                if (bx < lo[0] || bx > hi[0] || by < lo[1] || by > hi[1] || bz < lo[2] || bz > hi[2])
+                    clean.push_back(bx + a.gx * (by + a.gy * bz));
+    /* R in cells (unwrapped: the last x tile reaches past L0 by xorg) */
+    const int rlo[3] = {a.xorg + lo[0] * fx, a.box[2] + lo[1] * fy, a.zc[lo[2]]};
+    const int rhi[3] = {std::min(a.xorg + (hi[0] + 1) * fx, a.xhi), std::min(a.box[2] + (hi[1] + 1) * fy, a.box[3]),
+                        a.zc[hi[2] + 1]};
+    for (int k = 0; k <= 2; k++) {
+        int glo[3], glen[3];
+        for (int d = 0; d < 3; d++) {
+            glo[d] = rlo[d] - k;
+            glen[d] = rhi[d] - rlo[d] + 2 * k;
+        }
+        wrap_boxes(L, glo, glen, k == 0 ? &st->sp_R : k == 1 ? &st->sp_R1 : &st->sp_R2);
+        if (k == 0)
+            for (int d = 0; d < 3; d++) {
+                st->sp_monin[2 * d] = ((glo[d] % L[d]) + L[d]) % L[d];
+                st->sp_monin[2 * d + 1] = glen[d];
+            }
+    }
+    if (st->sp_blist) (void)hipFree(st->sp_blist);
+    st->sp_blist = nullptr;
+    st->sp_nclean = (int)clean.size();
+    hipError_t e;
+    if (!clean.empty() &&
+        ((e = hipMalloc(&st->sp_blist, sizeof(int) * clean.size())) != hipSuccess ||
+         (e = hipMemcpy(st->sp_blist, clean.data(), sizeof(int) * clean.size(), hipMemcpyHostToDevice)) != hipSuccess))
+        return e;
+    std::memcpy(st->sp_key, key, sizeof key);
+    st->sp_valid = 1;
+    return hipSuccess;
+}
 
 int smx_f32_box_fz(const smx_f32_state *st)
 {
@@ -2926,6 +3059,86 @@ hipError_t smx_f32_step2(smx_f32_state *st, const smx_f32_launch_args *la, hipSt
         /* the kernel keeps a table of the stages of its z chunk (at most kMaxFZ planes) */
         for (int i = 0; i < a.nzc; i++)
             if (a.zc[i + 1] - a.zc[i] > two_layout<32, 8, true, false>::kMaxFZ) return hipErrorInvalidValue;
+    if (st->nspecial > 0) {
+        /*
+         * Lossy/metal cells.  A tile's rings recompute the first step on its stage region
+         * with the regular update, and the metal E update's backward curl (F:310) reaches
+         * one cell further than the rings hold, so the box R of the tiles whose stage
+         * region holds a special cell goes to the region sweeps and the special passes:
+         *   1. H1 on R grown by 2, the specials' H1         old -> tmp
+         *   2. E1 on R grown by 1, the specials' E1         old, tmp H1 -> tmp
+         *   3. two steps on the tiles outside R             old -> new
+         *   4. H2 on R, the specials' H2                    tmp -> new
+         *   5. E2 on R, the specials' E2                    tmp E1, new H2 -> new
+         * Every special cell lies in R, so each region step reads only cells the steps
+         * before it wrote.  The monitors of R's first step come from step 2 (R only), of
+         * its second from step 5; the pass takes those of its own tiles.
+         */
+        if (!tv.split || boxed || la->dft_n > 0) return hipErrorInvalidValue;
+        hipError_t e;
+        if (!st->tmpT && (e = smx_f32_reserve(st, stream)) != hipSuccess) return e;
+        if (!st->tmpT) return hipErrorInvalidValue;
+        if ((e = special_tiles(st, a, tv.fx, tv.fy)) != hipSuccess) return e;
+        float *tT[6], *tS[6], *oT[6], *oS[6], *nT[6], *nS[6];
+        for (int i = 0; i < 6; i++) {
+            tT[i] = st->tmpT + (size_t)i * st->n;
+            tS[i] = st->tmpS + (size_t)i * st->n;
+            oT[i] = st->T[st->cur][i], oS[i] = st->S[st->cur][i];
+            nT[i] = st->T[1 - st->cur][i], nS[i] = st->S[1 - st->cur][i];
+        }
+        auto with = [&](float *const *inT, float *const *inS, float *const *outT, float *const *outS, int step) {
+            fused_args r = a;
+            for (int i = 0; i < 6; i++) {
+                r.To[i] = inT[i];
+                r.So[i] = inS[i];
+                r.Tn[i] = outT[i];
+                r.Sn[i] = outS[i];
+            }
+            r.special = st->special;
+            r.special_cells = st->special_cells;
+            r.nspecial = st->nspecial;
+            if (step == 2) {
+                r.srcE_f2 = a.srcE2_f2, r.srcE_re = a.srcE2_re, r.srcE_im = a.srcE2_im;
+                r.srcH_f2 = a.srcH2_f2, r.srcH_re = a.srcH2_re, r.srcH_im = a.srcH2_im;
+                r.monH_re = a.monH2_re, r.monH_im = a.monH2_im, r.monE_re = a.monE2_re, r.monE_im = a.monE2_im;
+            }
+            r.monin_on = 1;
+            std::memcpy(r.monin, st->sp_monin, sizeof r.monin);
+            return r;
+        };
+        auto sweep = [&](const smx_region_set &rs, bool e_sweep, const fused_args &r0) -> hipError_t {
+            if (rs.n == 0) return hipSuccess;
+            fused_args r = r0;
+            r.nreg = rs.n;
+            std::memcpy(r.reg, rs.box, sizeof r.reg);
+            std::memcpy(r.reg_start, rs.start, sizeof r.reg_start);
+            const unsigned blocks = (unsigned)((rs.start[rs.n] + kThreads - 1) / kThreads);
+            if (e_sweep) hipLaunchKernelGGL(region_e_f32, dim3(blocks), dim3(kThreads), 0, stream, r);
+            else hipLaunchKernelGGL(region_h_f32, dim3(blocks), dim3(kThreads), 0, stream, r);
+            return hipGetLastError();
+        };
+        auto specials = [&](bool e_pass, const fused_args &r) -> hipError_t {
+            const dim3 grid((r.nspecial + kThreads - 1) / kThreads);
+            if (e_pass) hipLaunchKernelGGL(special_e_pass, grid, dim3(kThreads), 0, stream, r);
+            else hipLaunchKernelGGL(special_h_pass, grid, dim3(kThreads), 0, stream, r);
+            return hipGetLastError();
+        };
+        const fused_args s1 = with(oT, oS, tT, tS, 1), s2 = with(tT, tS, nT, nS, 2);
+        if ((e = sweep(st->sp_R2, false, s1)) != hipSuccess || (e = specials(false, s1)) != hipSuccess ||
+            (e = sweep(st->sp_R1, true, s1)) != hipSuccess || (e = specials(true, s1)) != hipSuccess)
+            return e;
+        if (st->sp_nclean > 0) {
+            fused_args k = a;
+            k.blist = st->sp_blist;
+            k.nblocks = st->sp_nclean;
+            if ((e = launch_two(k)) != hipSuccess) return e;
+        }
+        if ((e = sweep(st->sp_R, false, s2)) != hipSuccess || (e = specials(false, s2)) != hipSuccess ||
+            (e = sweep(st->sp_R, true, s2)) != hipSuccess || (e = specials(true, s2)) != hipSuccess)
+            return e;
+        st->cur = 1 - st->cur;
+        return hipSuccess;
+    }
     if (!st->has_pml || tv.split) {
         const hipError_t e = launch_two(a);
         if (e != hipSuccess) return e;

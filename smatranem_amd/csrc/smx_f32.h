@@ -35,8 +35,8 @@ struct smx_special_f32 {
  * box, with the running cell count before each (the region sweeps' thread index). */
 struct smx_region_set {
     int n;
-    int box[6][6];
-    int64_t start[7];
+    int box[8][6];
+    int64_t start[9];
 };
 
 struct smx_f32_state {
@@ -50,6 +50,14 @@ struct smx_f32_state {
     unsigned char *pmlflag; /* L0+L1+L2 */
     smx_special_f32 *special;
     int64_t *special_cells; /* linear index of each special cell, table order */
+    int64_t *special_host;  /* the same on the host */
+    /* two-step pass on a grid with lossy/metal cells (smx_f32_step2): the box R of the
+     * tiles whose stage region holds such a cell (left to region sweeps), R grown by one
+     * and two cells, the monitor ranges of R, and the other tiles' ids (device);
+     * recomputed when the tiling (sp_key) changes */
+    int sp_valid, sp_key[5], sp_nclean, sp_monin[6];
+    int *sp_blist;
+    smx_region_set sp_R, sp_R1, sp_R2;
     int nspecial;
     int has_pml; /* any PML cell on the grid */
     /* split of a grid with PML for the staged kernel without split slots: fbox, the

@@ -347,10 +347,53 @@ def test_two_step_pass_with_pml(two, kw, monkeypatch):
             assert rel_l2(a.view(np.float64), b) < 2e-4
 
 
+@pytest.mark.parametrize("kw", [
+    dict(pml_xy=0, pml_z=3, metal_cells=6, lossy_cells=4, iBc=(1, -1, 0)),              # monitors on
+    dict(pml_xy=2, pml_z=3, metal_cells=6, lossy_cells=4, lossy_mu_cells=3, iBc=(0, 0, 0)),
+    dict(L0=40, L1=20, L2=48, outer=8, pml_xy=2, pml_z=3, metal_cells=5, lossy_cells=5, iBc=(1, 1, 0)),
+    # a compact metal box: most tiles run the pass, the box's tiles the region sweeps
+    dict(L0=70, L1=26, L2=75, outer=10, pml_xy=3, pml_z=4, core=(5, 4), metal_cells=0, lossy_cells=0,
+         iBc=(0, 0, 0), metal_box=((33, 37, 12, 15, 36, 40), -20.0 + 2.0j)),
+    dict(L0=70, L1=26, L2=75, outer=10, pml_xy=0, pml_z=4, core=(5, 4), metal_cells=0, lossy_cells=3,
+         iBc=(1, -1, 0), metal_box=((2, 6, 1, 4, 36, 40), -20.0 + 2.0j)),               # near the x/y wrap
+])
+def test_two_step_pass_with_special_cells(kw, monkeypatch):
+    """Two steps per pass on grids with lossy and metal cells: the tiles whose stage
+    region holds such a cell run the region sweeps and the special-cell passes, the rest
+    the pass.  Against the FP64 oracle (fields, monitors, envelope sums) over a run with an
+    odd number of steps, and against the one-step kernels."""
+    monkeypatch.setenv("SMX_F32_KERNEL", "fused")
+    prep = casegen.prepare(small_waveguide(seed=17, **kw))
+    assert int(np.count_nonzero(prep.Cont != ord("n")) + np.count_nonzero(prep.Cont == ord("a"))) > 0
+    steps = prep.case.time_period[2] * prep.N
+    out = []
+    for two in ("2,16", None):
+        if two:
+            monkeypatch.setenv("SMX_F32_TWO", two)
+        else:
+            monkeypatch.delenv("SMX_F32_TWO", raising=False)
+        with prep.plan(fdtd.F32) as p:
+            if two:
+                assert p.autotune(0)[3] == 3 and p.kernels.startswith("two_step_f32<32,8,1,true")
+            sums = p.run_steps(0, steps - 1)
+            p.run_steps(steps - 1, 1, sums)
+            out.append((p.read_fields(), p.read_monitors(), sums))
+    (H, E), mons, sums = out[0]
+    Ho, Eo, mo, so = _oracle_run(prep, 0, prep.case.time_period[2])
+    assert np.abs(Ho).max() > 0
+    assert rel_l2(H, Ho) < 2e-4 and rel_l2(E, Eo) < 2e-4, (rel_l2(H, Ho), rel_l2(E, Eo))
+    for a, b in zip(mons, mo):
+        if np.linalg.norm(b) > 0:
+            assert rel_l2(a.view(np.float64), b) < 2e-4
+    assert np.allclose(sums, so, rtol=1e-14, atol=0)
+    (H1, E1), _, _ = out[1]
+    assert rel_l2(H, H1) < 1e-5 and rel_l2(E, E1) < 1e-5, (rel_l2(H, H1), rel_l2(E, E1))
+
+
 def test_two_step_refused(monkeypatch):
-    """Lossy/metal cells, or a PML-free core too thin for the two-step box, keep the
-    one-step kernels; so does an x extent that is not a multiple of 4 for the 16-byte
-    copies."""
+    """The variants without split slots on a grid with lossy/metal cells, or a PML-free
+    core too thin for the two-step box, keep the one-step kernels; so does an x extent
+    that is not a multiple of 4 for the 16-byte copies."""
     monkeypatch.setenv("SMX_F32_KERNEL", "fused")
     for two, kw in (("0,16", dict(pml_xy=0, pml_z=0, metal_cells=2, lossy_cells=0)),
                     ("0,16", dict(L0=12, L1=12, pml_xy=4, pml_z=3, metal_cells=0, lossy_cells=0)),
