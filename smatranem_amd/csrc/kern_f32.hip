@@ -1275,7 +1275,8 @@ __global__ __launch_bounds__(kThreads) void fused_dma_f32(fused_args a)
  * H1, E1, H2 of the neighbours go through LDS, the -z / +z neighbours of a thread's own
  * cells stay in its registers.
  *
- * Grids with lossy/metal cells keep the one-step kernels (the host checks).  With SPLIT
+ * On grids with lossy/metal cells the tiles near them are left to region sweeps and the
+ * special-cell passes (smx_f32_step2) and the kernel runs the rest.  With SPLIT
  * the stages of PML planes and of tiles touching x/y PML also carry the old splits, and
  * PML cells take the main/split update of the one-step kernels, their splits carried
  * like the fields; without it the kernel runs only where no PML cell is reached (the
