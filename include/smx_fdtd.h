@@ -192,9 +192,11 @@ int smx_plan_run_batch(smx_plan *const *plans, int nplans, long long step0, long
  *     grid of every plan's tiles), one launch per pass; an odd last step runs as in
  *     STREAMS.  Each plan runs the split-slot two-step pass (SMX_F32_TWO variant 2)
  *     whatever kernel it had selected, so the results are bit for bit those of each
- *     plan run alone with that kernel.  Needs FP32 plans without lossy/metal cells,
- *     without a DFT set and under 4 GiB per field group; otherwise the batch runs as
- *     SMX_BATCH_GRAPH.
+ *     plan run alone with that kernel, DFT accumulators included (each plan's ring
+ *     slot of a pass goes up with its step scalars; its folds run between passes, where
+ *     the plan alone folds).  Needs FP32 plans without lossy/metal cells, with a DFT
+ *     chunk of 2 or more where a DFT is set, and under 4 GiB per field group; otherwise
+ *     the batch runs as SMX_BATCH_GRAPH.
  * graph_launches (may be NULL) receives the number of graph launches (SMX_BATCH_FUSED:
  * of two_step_multi launches).  The batch allocates its graph or tables, stream and
  * events per call and frees them before returning.  No counterpart in the reference.

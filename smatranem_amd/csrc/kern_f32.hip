@@ -1450,7 +1450,8 @@ __device__ __forceinline__ void two_step_body(const A &a, int id, const double *
         if constexpr (!SPLIT)
             for (int i = t; i < ns; i += NT) reinterpret_cast<int *>(lds + L_DZ)[i] = dft_plane_of(i);
         const int64_t slot_words = 4 * (int64_t)a.dft_n * a.sz;
-        rD = __builtin_amdgcn_make_buffer_rsrc((void *)(a.dft + a.dft_slot * slot_words), (short)0,
+        const int dslot = scal ? (int)scal[20] : a.dft_slot; /* several plans: per pass, after the scalars */
+        rD = __builtin_amdgcn_make_buffer_rsrc((void *)(a.dft + dslot * slot_words), (short)0,
                                                (int)(unsigned)(8 * slot_words), (int)kBufWord3);
         dstep = 4u * (unsigned)slot_words;
     }
@@ -1885,16 +1886,18 @@ __global__ __launch_bounds__(FX * FY) void two_step_f32(fused_args a)
 /* A two-step pass of several plans in one launch (a batch of independent runs): block i
  * (XCD-aware order) takes entry blk[i] = plan << 20 | the block's id within its plan, the
  * plan's arguments from args[plan] (constant memory, read through the scalar cache) and
- * its 20 step scalars from scal[20 * plan].  The split-slot variant, which updates every
- * cell of a grid with or without PML. */
+ * its 20 step scalars and (DFT) the ring slot of the pass's first step from
+ * scal[kMultiScalars * plan].  The split-slot variant, which updates every cell of a grid
+ * with or without PML; with DFT, plans whose dft_n is 0 take no samples. */
 typedef const fused_args __attribute__((address_space(4))) const_fused_args;
 
+template <bool DFT>
 __global__ __launch_bounds__(256) void two_step_multi(const fused_args *args, const int *blk, const double *scal)
 {
     const int e = blk[xcd_logical(blockIdx.x, gridDim.x)];
     const int plan = e >> 20, id = e & ((1 << 20) - 1);
     const const_fused_args &a = ((const_fused_args *)args)[plan];
-    two_step_body<32, 8, 1, true, false, false, false, false>(a, id, scal + 20 * plan);
+    two_step_body<32, 8, 1, true, false, DFT, false, false>(a, id, scal + SMX_MULTI_SCALARS * plan);
 }
 
 /* ---------------------------------------------------------------------------------- */
@@ -2782,7 +2785,7 @@ int smx_f32_multi_slots(void)
     int dev = 0, ncu = 0, per_cu = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, two_step_multi, 256, 0) != hipSuccess)
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, two_step_multi<true>, 256, 0) != hipSuccess)
         return 0;
     return ncu * per_cu;
 }
@@ -2826,15 +2829,26 @@ int smx_f32_multi_prepare(const smx_f32_state *st, const smx_f32_launch_args *la
         a.Einc = la->Einc;
         a.Hinc = la->Hinc;
         a.HB = la->HB, a.EB = la->EB, a.HT = la->HT, a.ET = la->ET;
+        if (la->dft_n > 0) { /* the ring; the slot of each pass comes with its scalars */
+            if (la->dft_n > kDftMaxPlanes) return SMX_ERR_ARG;
+            a.dft = la->dft;
+            a.dft_n = la->dft_n;
+            for (int m = 0; m < la->dft_n; m++) a.dft_planes[m] = la->dft_planes[m];
+        }
     }
     *nblocks = args[0].nblocks;
     return args[0].nblocks < (1 << 20) ? SMX_OK : SMX_ERR_ARG;
 }
 
-hipError_t smx_f32_multi_launch(const void *args, const int *blk, const double *scal, int nblocks, hipStream_t stream)
+hipError_t smx_f32_multi_launch(const void *args, const int *blk, const double *scal, int nblocks, int dft,
+                                 hipStream_t stream)
 {
-    hipLaunchKernelGGL(two_step_multi, dim3(nblocks), dim3(256), 0, stream, static_cast<const fused_args *>(args), blk,
-                       scal);
+    if (dft)
+        hipLaunchKernelGGL(two_step_multi<true>, dim3(nblocks), dim3(256), 0, stream,
+                           static_cast<const fused_args *>(args), blk, scal);
+    else
+        hipLaunchKernelGGL(two_step_multi<false>, dim3(nblocks), dim3(256), 0, stream,
+                           static_cast<const fused_args *>(args), blk, scal);
     return hipGetLastError();
 }
 

@@ -154,11 +154,14 @@ int smx_f32_tile_ok(const smx_f32_state *st, int tile, int fz);
  * on the device at once, a plan's two argument blocks (smx_f32_multi_args_size() bytes
  * each: the pass from buffer cur, then the next pass) and block count; the launch takes
  * the device copy of the plans' argument blocks, the block list (plan << 20 | block id)
- * and 20 step scalars per plan (the second step's after the first's, in the order of
- * fused_args). */
+ * and SMX_MULTI_SCALARS doubles per plan (the 20 step scalars, the second step's after
+ * the first's, in the order of fused_args; then the DFT ring slot of the pass's first
+ * step); dft selects the instantiation that takes DFT samples. */
+#define SMX_MULTI_SCALARS 21 /* per plan and pass: 20 step scalars, then the DFT ring slot */
 size_t smx_f32_multi_args_size(void);
 int smx_f32_multi_ok(const smx_f32_state *st, int fz);
 int smx_f32_multi_slots(void);
 int smx_f32_multi_prepare(const smx_f32_state *st, const smx_f32_launch_args *la, int fz, void *args2, int *nblocks);
-hipError_t smx_f32_multi_launch(const void *args, const int *blk, const double *scal, int nblocks, hipStream_t stream);
+hipError_t smx_f32_multi_launch(const void *args, const int *blk, const double *scal, int nblocks, int dft,
+                                 hipStream_t stream);
 const char *smx_f32_multi_kernel_name(void);

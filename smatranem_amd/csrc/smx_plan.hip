@@ -960,7 +960,7 @@ bool segment_repeats(const smx_plan *p, long long k0, int G, long long end)
 constexpr int kFusedChunk = 256; /* passes per upload of step scalars */
 constexpr int kFusedFZ[] = {8, 12, 16, 24, 32, 48, 64}; /* z chunks the batch chooses from */
 constexpr int kFusedDefaultFZ = 16;
-constexpr int kScalarsPerPass = 20;
+constexpr int kScalarsPerPass = SMX_MULTI_SCALARS; /* 20 step scalars, then the DFT ring slot */
 static_assert(sizeof(smx_step_scalars) == 10 * sizeof(double), "step scalars: 10 doubles in kernel order");
 
 struct fused_batch {
@@ -994,7 +994,8 @@ bool fused_ok(smx_plan *const *plans, int nplans)
     if (nplans >= (1 << 11)) return false;
     for (int i = 0; i < nplans; i++) {
         const smx_plan *p = plans[i];
-        if (p->g.precision != SMX_F32 || p->dft_nplanes != 0 || mon_ring(p) ||
+        /* a DFT: its ring must hold the samples of a pass (chunk >= 2) */
+        if (p->g.precision != SMX_F32 || (p->dft_nplanes != 0 && p->dft_chunk < 2) ||
             !smx_f32_multi_ok(&p->f32, kFusedDefaultFZ))
             return false;
     }
@@ -1048,9 +1049,18 @@ int run_fused(smx_plan *const *plans, int nplans, long long k0, long long npass,
     std::vector<unsigned char> args(2 * (size_t)nplans * asz), two(2 * asz);
     std::vector<int> blk;
     const int fz = fused_fz(plans, nplans);
+    bool any_dft = false;
     for (int i = 0; i < nplans; i++) {
         smx_plan *p = plans[i];
-        const smx_f32_launch_args la = step2_args(p, true);
+        /* the monitors in the pass unless the DFT ring carries them; the ring's slot of
+         * each pass goes with its scalars */
+        smx_f32_launch_args la = step2_args(p, !mon_ring(p));
+        if (p->dft_nplanes) {
+            any_dft = true;
+            la.dft = (float *)p->dft_series;
+            la.dft_n = p->dft_ring;
+            std::memcpy(la.dft_planes, p->dft_planes, sizeof(int) * (size_t)p->dft_ring);
+        }
         int nb = 0;
         if (smx_f32_multi_prepare(&p->f32, &la, fz, two.data(), &nb) != SMX_OK)
             return arg_fail("batch: two-step tiling of a plan");
@@ -1077,6 +1087,17 @@ int run_fused(smx_plan *const *plans, int nplans, long long k0, long long npass,
         SMX_HIP(hipEventRecord(fb.ev[(size_t)i], plans[i]->stream));
         SMX_HIP(hipStreamWaitEvent(fb.ms, fb.ev[(size_t)i], 0));
     }
+    /* the plans' own launches (DFT folds) go to the batch stream meanwhile */
+    std::vector<hipStream_t> own((size_t)nplans);
+    for (int i = 0; i < nplans; i++) own[(size_t)i] = plans[i]->stream, plans[i]->stream = fb.ms;
+    struct restore_streams {
+        smx_plan *const *pl;
+        std::vector<hipStream_t> &s;
+        ~restore_streams()
+        {
+            for (size_t i = 0; i < s.size(); i++) pl[i]->stream = s[i];
+        }
+    } restore{plans, own};
     std::vector<double> env(4 * (size_t)nplans);
     bool stop = false;
     long long done = 0;
@@ -1085,6 +1106,10 @@ int run_fused(smx_plan *const *plans, int nplans, long long k0, long long npass,
         if (c >= 2) SMX_HIP(hipEventSynchronize(fb.done[b])); /* chunk c - 2 has run */
         std::vector<double> &h = fb.host[b];
         h.resize(per_pass * (size_t)chunk);
+        /* the DFT rings' fill as the passes of this chunk will leave it (dft_pair_slot) */
+        std::vector<int> fill((size_t)nplans);
+        std::vector<long long> kfirst((size_t)nplans);
+        for (int i = 0; i < nplans; i++) fill[(size_t)i] = plans[i]->dft_fill, kfirst[(size_t)i] = plans[i]->dft_k0;
         long long n = 0;
         for (; n < chunk && done + n < npass; n++) {
             const long long k = k0 + 2 * (done + n);
@@ -1095,7 +1120,18 @@ int run_fused(smx_plan *const *plans, int nplans, long long k0, long long npass,
                 for (int j = 0; j < 2 && ok; j++)
                     ok = step_scalars(p, (int)((k + j) / p->N), (int)((k + j) % p->N), &st[j],
                                       &env[4 * (size_t)i + 2 * j], &env[4 * (size_t)i + 2 * j + 1]) == SMX_OK;
-                std::memcpy(&h[per_pass * (size_t)n + kScalarsPerPass * (size_t)i], st, sizeof st);
+                double *row = &h[per_pass * (size_t)n + kScalarsPerPass * (size_t)i];
+                std::memcpy(row, st, sizeof st);
+                int slot = 0;
+                if (p->dft_nplanes) {
+                    int &f = fill[(size_t)i];
+                    if (f > 0 && (k != kfirst[(size_t)i] + f || f + 2 > p->dft_chunk)) f = 0;
+                    if (f == 0) kfirst[(size_t)i] = k;
+                    slot = f;
+                    f += 2;
+                    if (f == p->dft_chunk) f = 0;
+                }
+                row[20] = (double)slot;
             }
             if (!ok) {
                 stop = true;
@@ -1112,8 +1148,32 @@ int run_fused(smx_plan *const *plans, int nplans, long long k0, long long npass,
         SMX_HIP(hipMemcpyAsync(fb.scal[b], h.data(), sizeof(double) * per_pass * (size_t)n, hipMemcpyHostToDevice,
                                fb.ms));
         for (long long j = 0; j < n; j++) {
+            const long long k = k0 + 2 * (done + j);
+            /* the rings' bookkeeping as advance() keeps it: a fold before a pass that does
+             * not continue the chunk or does not fit, after one that fills it */
+            for (int i = 0; i < nplans; i++) {
+                smx_plan *p = plans[i];
+                if (!p->dft_nplanes) continue;
+                int slot = 0, rc;
+                if ((rc = dft_pair_slot(p, k, &slot))) return rc;
+                if (slot != (int)h[per_pass * (size_t)j + kScalarsPerPass * (size_t)i + 20])
+                    return arg_fail("batch: DFT ring slot out of step");
+            }
             const void *a = static_cast<const unsigned char *>(fb.args) + (size_t)((done + j) & 1) * nplans * asz;
-            SMX_HIP(smx_f32_multi_launch(a, fb.blk, fb.scal[b] + per_pass * (size_t)j, (int)blk.size(), fb.ms));
+            SMX_HIP(smx_f32_multi_launch(a, fb.blk, fb.scal[b] + per_pass * (size_t)j, (int)blk.size(), any_dft ? 1 : 0,
+                                         fb.ms));
+            for (int i = 0; i < nplans; i++) {
+                smx_plan *p = plans[i];
+                if (!p->dft_nplanes) continue;
+                const int slot = p->dft_fill;
+                if (mon_ring(p)) {
+                    mark_mon_slot(p, slot);
+                    mark_mon_slot(p, slot + 1);
+                }
+                p->dft_fill += 2;
+                int rc;
+                if (p->dft_fill == p->dft_chunk && (rc = dft_flush(p))) return rc;
+            }
         }
         SMX_HIP(hipEventRecord(fb.done[b], fb.ms));
         done += n;
@@ -1121,7 +1181,7 @@ int run_fused(smx_plan *const *plans, int nplans, long long k0, long long npass,
     /* and what is queued on them afterwards runs after the passes */
     SMX_HIP(hipEventRecord(fb.join_ev, fb.ms));
     for (int i = 0; i < nplans; i++) {
-        SMX_HIP(hipStreamWaitEvent(plans[i]->stream, fb.join_ev, 0));
+        SMX_HIP(hipStreamWaitEvent(own[(size_t)i], fb.join_ev, 0));
         if (done & 1) plans[i]->f32.cur = 1 - plans[i]->f32.cur;
         plans[i]->next_step = k0 + 2 * done;
     }

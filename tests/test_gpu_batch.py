@@ -203,30 +203,49 @@ def _fused_preps(members):
     return [(prec, casegen.prepare(small_waveguide(**kw))) for prec, kw in members]
 
 
-@pytest.mark.parametrize("odd,fz", [(False, None), (True, None), (True, "8"), (False, "64")])
-def test_batch_fused_equals_separate_runs(odd, fz, monkeypatch):
+# multi-frequency DFT on members 2 (monitors on: the ring carries them) and 3: (planes,
+# frequencies in units of 2 pi / N_2pi, chunk); chunk 7 is odd, so a pass that would not
+# fit folds first
+FUSED_DFT = {2: ([6, 20], [0.9, 1.0, 1.1], 8), 3: ([40], [1.0, 1.05], 7)}
+
+
+@pytest.mark.parametrize("odd,fz,dft", [(False, None, False), (True, None, False), (True, "8", False),
+                                        (False, "64", False), (True, None, True), (False, "8", True)])
+def test_batch_fused_equals_separate_runs(odd, fz, dft, monkeypatch):
     """SMX_BATCH_FUSED (every plan's two-step pass in one launch) gives bit for bit the
-    fields, monitors and envelope sums of each plan run alone on the split-slot two-step
-    kernel, over two calls (the second starting on the other ping-pong buffer when the
-    first ran an odd number of steps), with the batch's own z chunk or a forced one."""
+    fields, monitors, DFT accumulators and envelope sums of each plan run alone on the
+    split-slot two-step kernel, over two calls (the second starting on the other
+    ping-pong buffer when the first ran an odd number of steps), with the batch's own z
+    chunk or a forced one, with and without DFT members."""
     monkeypatch.setenv("SMX_F32_TWO", "2,16")
     if fz:
         monkeypatch.setenv("SMX_BATCH_FZ", fz)
     preps = _fused_preps(FUSED_MEMBERS)
     steps = min(pr.case.time_period[2] * pr.N for _, pr in preps) - (1 if odd else 2)
     first = 3 if odd else 4
+
+    def setup(i, p, pr):
+        if dft and i in FUSED_DFT:
+            planes, ws, chunk = FUSED_DFT[i]
+            p.set_dft(planes, [w * 2 * np.pi / pr.N for w in ws], chunk=chunk)
+
+    def state(i, p):
+        return _state(p) + ((p.read_dft(),) if dft and i in FUSED_DFT else (None,))
     alone = []
-    for prec, pr in preps:
+    for i, (prec, pr) in enumerate(preps):
         with pr.plan(prec) as p:
-            assert p.kernels == "two_step_f32<32,8,1,true,false,false,false>"
+            setup(i, p, pr)
+            assert p.kernels.startswith("two_step_f32<32,8,1,true,false,")
             sums = p.run_steps(0, first) + p.run_steps(first, steps - first)
-            alone.append(_state(p) + (sums,))
+            alone.append(state(i, p) + (sums,))
     plans = [pr.plan(prec) for prec, pr in preps]
     try:
+        for i, (p, (_, pr)) in enumerate(zip(plans, preps)):
+            setup(i, p, pr)
         info1, info2 = {}, {}
         s1 = fdtd.run_batch(plans, 0, first, mode=fdtd.BATCH_FUSED, info=info1)
         s2 = fdtd.run_batch(plans, first, steps - first, mode=fdtd.BATCH_FUSED, info=info2)
-        together = [_state(p) for p in plans]
+        together = [state(i, p) for i, p in enumerate(plans)]
     finally:
         for p in plans:
             p.close()
@@ -237,12 +256,15 @@ def test_batch_fused_equals_separate_runs(odd, fz, monkeypatch):
         assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), i
         for ma, mb in zip(a[2], b[2]):
             assert np.array_equal(ma, mb), i
-        assert np.array_equal(a[3], sums[i]), i
+        if a[3] is not None:
+            assert np.abs(a[3]).max() > 0 and np.array_equal(a[3], b[3]), i
+        assert np.array_equal(a[4], sums[i]), i
 
 
 def test_batch_fused_falls_back():
     """A batch the fused launch cannot take (an FP64 member, a member with lossy cells)
     runs as the graph mode, with each plan's results as alone."""
+    # (the lossy member's own pass leaves its lossy tiles to region sweeps)
     members = [FUSED_MEMBERS[0],
                (fdtd.F64, dict(seed=46, L0=9, L1=14, L2=30, outer=7, pml_xy=0, pml_z=3, metal_cells=0,
                                lossy_cells=0, time_period=(1, 5, 7))),
