@@ -463,6 +463,7 @@ struct fused_args {
     int zc[kMaxZChunks + 1];
     /* boxes of the region sweeps and their first thread index */
     int nreg;
+    int reg_specials; /* region sweeps: the special cells' update after the boxes' cells */
     int reg[kMaxRegions][6];
     int64_t reg_start[kMaxRegions + 1];
     /* region_e_f32 accumulates no monitor for cells inside this box */
@@ -1903,6 +1904,99 @@ __global__ __launch_bounds__(256) void two_step_multi(const fused_args *args, co
 /* ---------------------------------------------------------------------------------- */
 /* Region sweeps: the cells outside the staged kernel's box                           */
 /* ---------------------------------------------------------------------------------- */
+/* H of the lossy/metal cells, before the fused pass (which reads the result for its
+ * E update and leaves these cells' H alone). */
+__device__ void special_h_one(const fused_args &a, int i)
+{
+    const int64_t l = a.special_cells[i];
+    const int x = (int)(l % a.L0), y = (int)((l / a.L0) % a.L1), z = (int)(l / a.sz);
+    float Hn[3], Sn[3];
+    fused_special_h(a, x, y, z, l, a.special[i], Hn, Sn);
+    for (int c = 0; c < 3; c++) {
+        a.Tn[c][l] = Hn[c];
+        a.Sn[c][l] = Sn[c];
+    }
+}
+
+/* E of the lossy/metal cells after the fused pass (the class branches of F:291-314),
+ * from the new H in buffer 1 - cur; also their monitor samples. */
+__device__ void special_e_one(const fused_args &a, int i)
+{
+    const int64_t l = a.special_cells[i];
+    const smx_special_f32 &sp = a.special[i];
+    const int x = (int)(l % a.L0), y = (int)((l / a.L0) % a.L1), z = (int)(l / a.sz);
+    float *const *H = a.Tn;
+    float *const *Hs = a.Sn;
+    float En[3], Sn[3];
+    for (int c = 0; c < 3; c++) {
+        En[c] = a.To[3 + c][l];
+        Sn[c] = a.So[3 + c][l];
+    }
+    const int k = sp.cls;
+    float P[3], M[3];
+    if (k == 'n' || k == 'a') {
+        curl_halves<true>(H, nullptr, nbrs_at(a, x, y, z, true), l, P, M);
+        if (k == 'n') {
+            float W[3] = {P[0] + M[0], P[1] + M[1], P[2] + M[2]};
+            if (z == a.iI) {
+                const int64_t q = x + (int64_t)a.L0 * y;
+                W[0] += a.sgn * (float)plane_source(a.srcH_plane, a.Hinc, a.srcH_f2, a.srcH_re, a.srcH_im, q, 1);
+                W[1] -= a.sgn * (float)plane_source(a.srcH_plane, a.Hinc, a.srcH_f2, a.srcH_re, a.srcH_im, q, 0);
+            }
+            for (int c = 0; c < 3; c++) En[c] = sp.eA[c] * En[c] + sp.eB[c] * W[c];
+        } else {
+            for (int c = 0; c < 3; c++) {
+                const float s = Sn[c], m = En[c] - s;
+                const float s2 = sp.eAs[c] * s + sp.eBs[c] * M[c];
+                En[c] = sp.eA[c] * m + sp.eB[c] * P[c] + s2;
+                Sn[c] = s2;
+            }
+        }
+    } else if (k != 'h') { /* 'e', 'b': backward curl of the new H main slots (F:310) */
+        curl_halves<false>(H, Hs, nbrs_at(a, x, y, z, false), l, P, M);
+        for (int c = 0; c < 3; c++) {
+            const float s = Sn[c];
+            float m = En[c] - s + 0.5f * s;
+            const float s2 = sp.eAs[c] * s + sp.eBs[c] * (P[c] + M[c]);
+            m += 0.5f * s2;
+            En[c] = m + s2;
+            Sn[c] = s2;
+        }
+    }
+    for (int c = 0; c < 3; c++) {
+        a.Tn[3 + c][l] = En[c];
+        a.Sn[3 + c][l] = Sn[c];
+    }
+    if (z == a.iB || z == a.iT) {
+        const int64_t q = x + (int64_t)a.L0 * y;
+        for (int m = 0; m < 2; m++) {
+            if (z != (m == 0 ? a.iB : a.iT)) continue;
+            double *HM = m == 0 ? a.HB : a.HT, *EM = m == 0 ? a.EB : a.ET;
+            for (int v = 0; v < 2; v++) {
+                const double hv = (double)H[v][l] - (double)Hs[v][l];
+                const double ev = (double)En[v] - (double)Sn[v];
+                const int64_t o = 2 * (v + 2 * q);
+                HM[o] += a.monH_re * hv;
+                HM[o + 1] += a.monH_im * hv;
+                EM[o] += a.monE_re * ev;
+                EM[o + 1] += a.monE_im * ev;
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void special_h_pass(fused_args a)
+{
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i < a.nspecial) special_h_one(a, i);
+}
+
+__global__ __launch_bounds__(kThreads) void special_e_pass(fused_args a)
+{
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i < a.nspecial) special_e_one(a, i);
+}
+
 /*
  * On a grid with PML the staged kernel without split slots covers only the box of
  * cells whose +x, +y, +z neighbours hold no PML cell either (its recomputed halo then
@@ -1944,7 +2038,12 @@ __device__ inline void cell_pml(const fused_args &a, int x, int y, int z, int ha
 __global__ __launch_bounds__(kThreads) void region_h_f32(fused_args a)
 {
     int x, y, z;
-    if (!region_cell(a, (int64_t)blockIdx.x * kThreads + threadIdx.x, x, y, z)) return;
+    const int64_t g = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (!region_cell(a, g, x, y, z)) { /* with reg_specials: the special cells follow the boxes */
+        const int64_t i = g - a.reg_start[a.nreg];
+        if (a.reg_specials && i < a.nspecial) special_h_one(a, (int)i);
+        return;
+    }
     const int64_t q = x + (int64_t)a.L0 * y, l = q + a.sz * z;
     const float b0 = a.b[0][l];
     if (__builtin_isnan(b0)) return; /* special_h_pass */
@@ -1983,7 +2082,12 @@ __global__ __launch_bounds__(kThreads) void region_h_f32(fused_args a)
 __global__ __launch_bounds__(kThreads) void region_e_f32(fused_args a)
 {
     int x, y, z;
-    if (!region_cell(a, (int64_t)blockIdx.x * kThreads + threadIdx.x, x, y, z)) return;
+    const int64_t g = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (!region_cell(a, g, x, y, z)) {
+        const int64_t i = g - a.reg_start[a.nreg];
+        if (a.reg_specials && i < a.nspecial) special_e_one(a, (int)i);
+        return;
+    }
     const int64_t q = x + (int64_t)a.L0 * y, l = q + a.sz * z;
     const float b0 = a.b[3][l];
     if (__builtin_isnan(b0)) return; /* special_e_pass */
@@ -2046,91 +2150,6 @@ __global__ __launch_bounds__(kThreads) void region_e_f32(fused_args a)
     }
     store3(a.Tn + 3, l, En);
     store3(a.Sn + 3, l, Sn);
-}
-
-/* H of the lossy/metal cells, before the fused pass (which reads the result for its
- * E update and leaves these cells' H alone). */
-__global__ __launch_bounds__(kThreads) void special_h_pass(fused_args a)
-{
-    const int i = blockIdx.x * kThreads + threadIdx.x;
-    if (i >= a.nspecial) return;
-    const int64_t l = a.special_cells[i];
-    const int x = (int)(l % a.L0), y = (int)((l / a.L0) % a.L1), z = (int)(l / a.sz);
-    float Hn[3], Sn[3];
-    fused_special_h(a, x, y, z, l, a.special[i], Hn, Sn);
-    for (int c = 0; c < 3; c++) {
-        a.Tn[c][l] = Hn[c];
-        a.Sn[c][l] = Sn[c];
-    }
-}
-
-/* E of the lossy/metal cells after the fused pass (the class branches of F:291-314),
- * from the new H in buffer 1 - cur; also their monitor samples. */
-__global__ __launch_bounds__(kThreads) void special_e_pass(fused_args a)
-{
-    const int i = blockIdx.x * kThreads + threadIdx.x;
-    if (i >= a.nspecial) return;
-    const int64_t l = a.special_cells[i];
-    const smx_special_f32 &sp = a.special[i];
-    const int x = (int)(l % a.L0), y = (int)((l / a.L0) % a.L1), z = (int)(l / a.sz);
-    float *const *H = a.Tn;
-    float *const *Hs = a.Sn;
-    float En[3], Sn[3];
-    for (int c = 0; c < 3; c++) {
-        En[c] = a.To[3 + c][l];
-        Sn[c] = a.So[3 + c][l];
-    }
-    const int k = sp.cls;
-    float P[3], M[3];
-    if (k == 'n' || k == 'a') {
-        curl_halves<true>(H, nullptr, nbrs_at(a, x, y, z, true), l, P, M);
-        if (k == 'n') {
-            float W[3] = {P[0] + M[0], P[1] + M[1], P[2] + M[2]};
-            if (z == a.iI) {
-                const int64_t q = x + (int64_t)a.L0 * y;
-                W[0] += a.sgn * (float)plane_source(a.srcH_plane, a.Hinc, a.srcH_f2, a.srcH_re, a.srcH_im, q, 1);
-                W[1] -= a.sgn * (float)plane_source(a.srcH_plane, a.Hinc, a.srcH_f2, a.srcH_re, a.srcH_im, q, 0);
-            }
-            for (int c = 0; c < 3; c++) En[c] = sp.eA[c] * En[c] + sp.eB[c] * W[c];
-        } else {
-            for (int c = 0; c < 3; c++) {
-                const float s = Sn[c], m = En[c] - s;
-                const float s2 = sp.eAs[c] * s + sp.eBs[c] * M[c];
-                En[c] = sp.eA[c] * m + sp.eB[c] * P[c] + s2;
-                Sn[c] = s2;
-            }
-        }
-    } else if (k != 'h') { /* 'e', 'b': backward curl of the new H main slots (F:310) */
-        curl_halves<false>(H, Hs, nbrs_at(a, x, y, z, false), l, P, M);
-        for (int c = 0; c < 3; c++) {
-            const float s = Sn[c];
-            float m = En[c] - s + 0.5f * s;
-            const float s2 = sp.eAs[c] * s + sp.eBs[c] * (P[c] + M[c]);
-            m += 0.5f * s2;
-            En[c] = m + s2;
-            Sn[c] = s2;
-        }
-    }
-    for (int c = 0; c < 3; c++) {
-        a.Tn[3 + c][l] = En[c];
-        a.Sn[3 + c][l] = Sn[c];
-    }
-    if (z == a.iB || z == a.iT) {
-        const int64_t q = x + (int64_t)a.L0 * y;
-        for (int m = 0; m < 2; m++) {
-            if (z != (m == 0 ? a.iB : a.iT)) continue;
-            double *HM = m == 0 ? a.HB : a.HT, *EM = m == 0 ? a.EB : a.ET;
-            for (int v = 0; v < 2; v++) {
-                const double hv = (double)H[v][l] - (double)Hs[v][l];
-                const double ev = (double)En[v] - (double)Sn[v];
-                const int64_t o = 2 * (v + 2 * q);
-                HM[o] += a.monH_re * hv;
-                HM[o + 1] += a.monH_im * hv;
-                EM[o] += a.monE_re * ev;
-                EM[o + 1] += a.monE_im * ev;
-            }
-        }
-    }
 }
 
 } // namespace
@@ -2341,8 +2360,8 @@ const char *smx_f32_kernel_name(const smx_f32_state *st, int dft)
             return dft ? "two_step_f32<32,8,1,true,false,true,true>" : "two_step_f32<32,8,1,true,false,false,true>";
         if (st->has_pml && !kTwo[st->two].split) return kTwo[st->two].with_regions;
         if (st->nspecial > 0)
-            return "two_step_f32<32,8,1,true,false,false,false> + region_h_f32 + region_e_f32 + special_h_pass + "
-                   "special_e_pass (the tiles around lossy/metal cells)";
+            return "two_step_f32<32,8,1,true,false,false,false> + region_h_f32 + region_e_f32 (the tiles around "
+                   "lossy/metal cells, and those cells)";
         return dft ? kTwo[st->two].name_dft : kTwo[st->two].name;
     }
     if (!st->fused) return st->bx == 64 ? "h_sweep_f32<64> + e_sweep_f32<64>" : "h_sweep_f32<32> + e_sweep_f32<32>";
@@ -3121,26 +3140,21 @@ hipError_t smx_f32_step2(smx_f32_state *st, const smx_f32_launch_args *la, hipSt
             std::memcpy(r.monin, st->sp_monin, sizeof r.monin);
             return r;
         };
+        /* one launch per half step: the boxes' cells, then the special cells (every special
+         * cell lies in R, so its update reads what the same half step of R reads) */
         auto sweep = [&](const smx_region_set &rs, bool e_sweep, const fused_args &r0) -> hipError_t {
-            if (rs.n == 0) return hipSuccess;
             fused_args r = r0;
             r.nreg = rs.n;
+            r.reg_specials = 1;
             std::memcpy(r.reg, rs.box, sizeof r.reg);
             std::memcpy(r.reg_start, rs.start, sizeof r.reg_start);
-            const unsigned blocks = (unsigned)((rs.start[rs.n] + kThreads - 1) / kThreads);
+            const unsigned blocks = (unsigned)((rs.start[rs.n] + r.nspecial + kThreads - 1) / kThreads);
             if (e_sweep) hipLaunchKernelGGL(region_e_f32, dim3(blocks), dim3(kThreads), 0, stream, r);
             else hipLaunchKernelGGL(region_h_f32, dim3(blocks), dim3(kThreads), 0, stream, r);
             return hipGetLastError();
         };
-        auto specials = [&](bool e_pass, const fused_args &r) -> hipError_t {
-            const dim3 grid((r.nspecial + kThreads - 1) / kThreads);
-            if (e_pass) hipLaunchKernelGGL(special_e_pass, grid, dim3(kThreads), 0, stream, r);
-            else hipLaunchKernelGGL(special_h_pass, grid, dim3(kThreads), 0, stream, r);
-            return hipGetLastError();
-        };
         const fused_args s1 = with(oT, oS, tT, tS, 1), s2 = with(tT, tS, nT, nS, 2);
-        if ((e = sweep(st->sp_R2, false, s1)) != hipSuccess || (e = specials(false, s1)) != hipSuccess ||
-            (e = sweep(st->sp_R1, true, s1)) != hipSuccess || (e = specials(true, s1)) != hipSuccess)
+        if ((e = sweep(st->sp_R2, false, s1)) != hipSuccess || (e = sweep(st->sp_R1, true, s1)) != hipSuccess)
             return e;
         if (st->sp_nclean > 0) {
             fused_args k = a;
@@ -3148,8 +3162,7 @@ hipError_t smx_f32_step2(smx_f32_state *st, const smx_f32_launch_args *la, hipSt
             k.nblocks = st->sp_nclean;
             if ((e = launch_two(k)) != hipSuccess) return e;
         }
-        if ((e = sweep(st->sp_R, false, s2)) != hipSuccess || (e = specials(false, s2)) != hipSuccess ||
-            (e = sweep(st->sp_R, true, s2)) != hipSuccess || (e = specials(true, s2)) != hipSuccess)
+        if ((e = sweep(st->sp_R, false, s2)) != hipSuccess || (e = sweep(st->sp_R, true, s2)) != hipSuccess)
             return e;
         st->cur = 1 - st->cur;
         return hipSuccess;
