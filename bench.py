@@ -209,16 +209,17 @@ def box_run(args, torch, K, W):
 def metal_run(args, torch, K, W):
     import numpy as np
     """The headline grid with an eps-metal block (eps = -20 + 2i, class 'e', 16 x 16 x 16
-    cells in the cladding beside the core, 1/2000 of the grid): the reference monitors,
-    no DFT (the two-step pass leaves the tiles around lossy/metal cells to region sweeps,
-    which take no DFT samples), autotuned, max(K/5, 100) steps timed; also the best
-    one-step kernel on the same grid."""
+    cells in the cladding beside the core, 1/2000 of the grid): the reference monitors and
+    the 16-frequency DFT on the two port planes as in the headline, autotuned, max(K/5,
+    100) steps timed; also the best one-step kernel on the same grid."""
     import os
     from smatranem_amd import casegen, fdtd
     L, pml = ((32, 32, 48), 4) if args.small else ((160, 160, 320), 10)
     box = (12, 16, 12, 16, 20, 24) if args.small else (100, 116, 72, 88, 150, 166)
     mcase = casegen.curvilinear_waveguide_case(L, pml=pml, metal_box=(box, -20.0 + 2.0j))
     mprep = casegen.prepare(mcase)
+    planes = [] if args.no_dft else [mcase.iB, mcase.iT]
+    dft_w = [float(x) for x in 2 * np.pi / mprep.N * np.linspace(0.85, 1.15, 16)]
     kb = max(K // 5, 100)
     out = {}
     for label, two in (("two_step", None), ("one_step", "off")):
@@ -226,6 +227,8 @@ def metal_run(args, torch, K, W):
             os.environ["SMX_F32_TWO"] = two  # the autotuner then times one-step kernels only
         try:
             with mprep.plan(fdtd.F32) as pm:
+                if planes:
+                    pm.set_dft(planes, dft_w, chunk=args.dft_chunk)
                 choice = pm.autotune(20)
                 assert (choice[3] == 3) == (two is None), (label, choice)
                 kern = pm.kernels
@@ -239,8 +242,8 @@ def metal_run(args, torch, K, W):
     bphi = mprep.pml_fraction
     bb = bytes_per_cell(bphi, 4, 4)
     two = out["two_step"]
-    res = dict(workload="same grid, eps-metal block %s (eps -20+2i, class 'e'), z PML, reference monitors, "
-                        "no DFT" % list(box),
+    res = dict(workload="same grid, eps-metal block %s (eps -20+2i, class 'e'), z PML, reference monitors, %s"
+                        % (list(box), "16-frequency DFT on planes %s" % planes if planes else "no DFT"),
                special_cells=int(np.count_nonzero(mprep.Cont == ord("e"))), steps=kb, kernel=two["kernel"],
                value=two["value"], kernel_ms_per_step=two["kernel_ms_per_step"],
                roofline_frac=round(bb * mcase.n / (two["kernel_ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),

@@ -40,6 +40,12 @@ __global__ __launch_bounds__(kThreads) void sample_planes_f32(smx_dft_sample_arg
     const int64_t q = i - m * per_plane;
     const int64_t l = q + per_plane * a.planes[m];
     const int x = (int)(q % a.L0), y = (int)(q / a.L0), z = a.planes[m];
+    if (a.mask_on) {
+        auto in_mod = [](int v, int lo, int len, int L) { return ((v - lo) % L + L) % L < len; };
+        if (!(in_mod(x, a.mask[0], a.mask[1], a.L0) && in_mod(y, a.mask[2], a.mask[3], a.L1) &&
+              in_mod(z, a.mask[4], a.mask[5], a.L2)))
+            return;
+    }
     /* the total field in class 'a' cells, as sample_planes_f64: the separable PML flag
      * for regular cells, the table's class for lossy and metal cells (a metal cell
      * inside a PML layer is class 'e'/'h'/'b', F:188-233, and samples its main slot) */

@@ -25,6 +25,7 @@
 #include <limits>
 #include <vector>
 
+#include "smx_dft.h"
 #include "smx_f32.h"
 #include "smx_fdtd.h"
 #include "smx_setup.h"
@@ -2360,8 +2361,10 @@ const char *smx_f32_kernel_name(const smx_f32_state *st, int dft)
             return dft ? "two_step_f32<32,8,1,true,false,true,true>" : "two_step_f32<32,8,1,true,false,false,true>";
         if (st->has_pml && !kTwo[st->two].split) return kTwo[st->two].with_regions;
         if (st->nspecial > 0)
-            return "two_step_f32<32,8,1,true,false,false,false> + region_h_f32 + region_e_f32 (the tiles around "
-                   "lossy/metal cells, and those cells)";
+            return dft ? "two_step_f32<32,8,1,true,false,true,false> + region_h_f32 + region_e_f32 + sample_planes_f32 "
+                         "(the tiles around lossy/metal cells, and those cells)"
+                       : "two_step_f32<32,8,1,true,false,false,false> + region_h_f32 + region_e_f32 (the tiles "
+                         "around lossy/metal cells, and those cells)";
         return dft ? kTwo[st->two].name_dft : kTwo[st->two].name;
     }
     if (!st->fused) return st->bx == 64 ? "h_sweep_f32<64> + e_sweep_f32<64>" : "h_sweep_f32<32> + e_sweep_f32<32>";
@@ -3108,7 +3111,7 @@ hipError_t smx_f32_step2(smx_f32_state *st, const smx_f32_launch_args *la, hipSt
          * before it wrote.  The monitors of R's first step come from step 2 (R only), of
          * its second from step 5; the pass takes those of its own tiles.
          */
-        if (!tv.split || boxed || la->dft_n > 0) return hipErrorInvalidValue;
+        if (!tv.split || boxed) return hipErrorInvalidValue;
         hipError_t e;
         if (!st->tmpT && (e = smx_f32_reserve(st, stream)) != hipSuccess) return e;
         if (!st->tmpT) return hipErrorInvalidValue;
@@ -3153,8 +3156,31 @@ hipError_t smx_f32_step2(smx_f32_state *st, const smx_f32_launch_args *la, hipSt
             else hipLaunchKernelGGL(region_h_f32, dim3(blocks), dim3(kThreads), 0, stream, r);
             return hipGetLastError();
         };
+        /* DFT samples of R's cells (the pass samples its own tiles): after the first step
+         * from the scratch buffer into the pass's first ring slot, after the second from
+         * the new buffer into the next */
+        auto sample = [&](float *const *T, float *const *S, int slot) -> hipError_t {
+            if (la->dft_n <= 0) return hipSuccess;
+            smx_dft_sample_args d;
+            std::memset(&d, 0, sizeof d);
+            d.L0 = g.L0, d.L1 = g.L1, d.L2 = g.L2;
+            d.nplanes = la->dft_n;
+            std::memcpy(d.planes, la->dft_planes, sizeof(int) * (size_t)la->dft_n);
+            d.plane_cells = (int64_t)g.L0 * g.L1;
+            for (int i = 0; i < 6; i++) d.T32[i] = T[i], d.S32[i] = S[i];
+            d.pmlflag = st->pmlflag;
+            d.bH0 = st->b[0];
+            d.bH1 = st->b[1];
+            d.special = reinterpret_cast<const int *>(st->special);
+            d.special_stride = (int)(sizeof(smx_special_f32) / sizeof(int));
+            d.series = la->dft;
+            d.mask_on = 1;
+            std::memcpy(d.mask, st->sp_monin, sizeof d.mask);
+            return smx_dft_sample(&d, slot, stream);
+        };
         const fused_args s1 = with(oT, oS, tT, tS, 1), s2 = with(tT, tS, nT, nS, 2);
-        if ((e = sweep(st->sp_R2, false, s1)) != hipSuccess || (e = sweep(st->sp_R1, true, s1)) != hipSuccess)
+        if ((e = sweep(st->sp_R2, false, s1)) != hipSuccess || (e = sweep(st->sp_R1, true, s1)) != hipSuccess ||
+            (e = sample(tT, tS, la->dft_slot)) != hipSuccess)
             return e;
         if (st->sp_nclean > 0) {
             fused_args k = a;
@@ -3162,7 +3188,8 @@ hipError_t smx_f32_step2(smx_f32_state *st, const smx_f32_launch_args *la, hipSt
             k.nblocks = st->sp_nclean;
             if ((e = launch_two(k)) != hipSuccess) return e;
         }
-        if ((e = sweep(st->sp_R, false, s2)) != hipSuccess || (e = sweep(st->sp_R, true, s2)) != hipSuccess)
+        if ((e = sweep(st->sp_R, false, s2)) != hipSuccess || (e = sweep(st->sp_R, true, s2)) != hipSuccess ||
+            (e = sample(nT, nS, la->dft_slot + 1)) != hipSuccess)
             return e;
         st->cur = 1 - st->cur;
         return hipSuccess;

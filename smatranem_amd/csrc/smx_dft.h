@@ -26,6 +26,9 @@ struct smx_dft_sample_args {
     const int *addr;
     int64_t n;
     void *series;                   /* ring: [chunk][plane][4][L0*L1], float or double */
+    /* FP32 plan, mask_on: only the cells in the ranges [mask[2d], mask[2d] + mask[2d+1]) of
+     * each axis d, taken modulo its length */
+    int mask_on, mask[6];
 };
 
 struct smx_dft_fold_args {

@@ -631,10 +631,7 @@ static int launch_step2(smx_plan *p, const smx_step_scalars &s, const smx_step_s
 static bool pair_ok(const smx_plan *p)
 {
     if (p->g.precision != SMX_F32 || p->f32.two < 0) return false;
-    /* with lossy/metal cells the pass leaves some tiles to region sweeps, which take no
-     * DFT samples */
-    return p->dft_nplanes == 0 ||
-           (smx_f32_two_full(&p->f32, p->f32.two) && p->dft_chunk >= 2 && p->f32.nspecial == 0);
+    return p->dft_nplanes == 0 || (smx_f32_two_full(&p->f32, p->f32.two) && p->dft_chunk >= 2);
 }
 
 /* The same for the pass that also writes E of the design box (the split-slot variant,
@@ -2021,7 +2018,7 @@ int smx_plan_autotune(smx_plan *p, int trial_steps, int choice[5])
                 const int fz = chunks[ci % 4], bal = ci / 4;
                 if (!smx_f32_two_ok(&st, v, fz) || (bal > 0 && !zpml)) continue;
                 /* with the DFT on, only a variant that samples both steps itself pairs steps */
-                if (p->dft_nplanes && (!smx_f32_two_full(&st, v) || p->dft_chunk < 2 || st.nspecial > 0)) continue;
+                if (p->dft_nplanes && (!smx_f32_two_full(&st, v) || p->dft_chunk < 2)) continue;
                 st.two = v;
                 st.two_fz = fz;
                 st.two_bal = bal;

@@ -357,15 +357,21 @@ def test_two_step_pass_with_pml(two, kw, monkeypatch):
     dict(L0=70, L1=26, L2=75, outer=10, pml_xy=0, pml_z=4, core=(5, 4), metal_cells=0, lossy_cells=3,
          iBc=(1, -1, 0), metal_box=((2, 6, 1, 4, 36, 40), -20.0 + 2.0j)),               # near the x/y wrap
 ])
-def test_two_step_pass_with_special_cells(kw, monkeypatch):
+@pytest.mark.parametrize("dft", [False, True])
+def test_two_step_pass_with_special_cells(kw, dft, monkeypatch):
     """Two steps per pass on grids with lossy and metal cells: the tiles whose stage
     region holds such a cell run the region sweeps and the special-cell passes, the rest
     the pass.  Against the FP64 oracle (fields, monitors, envelope sums) over a run with an
-    odd number of steps, and against the one-step kernels."""
+    odd number of steps, and against the one-step kernels; with a DFT set, the region
+    sweeps' cells sampled from the scratch and new buffers, the pass's from inside it
+    (and the monitors folded from the ring), the DFT against the one-step kernels'."""
     monkeypatch.setenv("SMX_F32_KERNEL", "fused")
     prep = casegen.prepare(small_waveguide(seed=17, **kw))
     assert int(np.count_nonzero(prep.Cont != ord("n")) + np.count_nonzero(prep.Cont == ord("a"))) > 0
     steps = prep.case.time_period[2] * prep.N
+    L2 = int(prep.case.L[2])
+    planes = [prep.case.iI + 3, L2 // 2, L2 - 9]
+    ws = [2 * np.pi / prep.N * f for f in (0.9, 1.0)]
     out = []
     for two in ("2,16", None):
         if two:
@@ -373,12 +379,17 @@ def test_two_step_pass_with_special_cells(kw, monkeypatch):
         else:
             monkeypatch.delenv("SMX_F32_TWO", raising=False)
         with prep.plan(fdtd.F32) as p:
+            if dft:
+                p.set_dft(planes, ws, chunk=8)
             if two:
                 assert p.autotune(0)[3] == 3 and p.kernels.startswith("two_step_f32<32,8,1,true")
             sums = p.run_steps(0, steps - 1)
             p.run_steps(steps - 1, 1, sums)
-            out.append((p.read_fields(), p.read_monitors(), sums))
-    (H, E), mons, sums = out[0]
+            out.append((p.read_fields(), p.read_monitors(), sums, p.read_dft() if dft else None))
+    if dft:
+        D2, D1 = out[0][3], out[1][3]
+        assert np.abs(D1).max() > 0 and rel_l2(D2, D1) < 1e-5, rel_l2(D2, D1)
+    (H, E), mons, sums, _ = out[0]
     Ho, Eo, mo, so = _oracle_run(prep, 0, prep.case.time_period[2])
     assert np.abs(Ho).max() > 0
     assert rel_l2(H, Ho) < 2e-4 and rel_l2(E, Eo) < 2e-4, (rel_l2(H, Ho), rel_l2(E, Eo))
@@ -386,7 +397,7 @@ def test_two_step_pass_with_special_cells(kw, monkeypatch):
         if np.linalg.norm(b) > 0:
             assert rel_l2(a.view(np.float64), b) < 2e-4
     assert np.allclose(sums, so, rtol=1e-14, atol=0)
-    (H1, E1), _, _ = out[1]
+    (H1, E1), _, _, _ = out[1]
     assert rel_l2(H, H1) < 1e-5 and rel_l2(E, E1) < 1e-5, (rel_l2(H, H1), rel_l2(E, E1))
 
 
