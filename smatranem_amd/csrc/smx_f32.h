@@ -55,7 +55,7 @@ struct smx_f32_state {
      * tiles whose stage region holds such a cell (left to region sweeps), R grown by one
      * and two cells, the monitor ranges of R, and the other tiles' ids (device);
      * recomputed when the tiling (sp_key) changes */
-    int sp_valid, sp_key[5], sp_nclean, sp_monin[6];
+    int sp_valid, sp_key[6], sp_nclean, sp_monin[6];
     int *sp_blist;
     smx_region_set sp_R, sp_R1, sp_R2;
     int nspecial;
@@ -91,7 +91,10 @@ struct smx_f32_state {
     int two_bal; /* z chunks of the two-step kernel: 0 uniform (two_fz planes), k > 0
                     balanced with a stage of a z-PML plane counted as 1 + k/2 */
     unsigned char *zpml_host; /* z-PML flag of every plane (host copy) */
+    unsigned char *xypml_host; /* x-PML flag of every column, then y-PML flag of every row */
     int force_big; /* the two-step pass in its 64-bit form on any grid (tests) */
+    int pml_origin; /* split-slot two-step pass: y tiles placed to put the y PML in the
+                       fewest tile rows (1, the default), or from row 0 (0) */
     /* host copies for the reference-layout field import/export */
     char *cont_host;
     int *addr_host;

@@ -456,3 +456,98 @@ def test_two_step_64bit_form(kw, monkeypatch):
     assert np.abs(out[0][0]).max() > 0
     for a, b in zip(out[0], out[1]):
         assert np.array_equal(a, b)
+
+
+def _flagged_tiles(flag, f, s):
+    """Tiles of f cells from origin s (wrapping) whose computed cells (1 before the tile, 2
+    after it) hold a flagged cell."""
+    Lv, k, t = len(flag), 0, 0
+    while t * f < Lv:
+        k += any(flag[u % Lv] for u in range(s + t * f - 1, s + t * f + f + 2))
+        t += 1
+    return k
+
+
+def _split_tile_y_origin(prep, fy=8):
+    """y origin of the split-slot two-step tiles that puts the y PML in the fewest tile rows
+    (0 on ties), with the count of PML tile rows from 0 and from it."""
+    L0, L1 = int(prep.case.L[0]), int(prep.case.L[1])
+    tol = 10 * np.finfo(np.float64).eps
+    sg = np.abs(prep.sigma)
+    fyl = [bool(sg[2 * L0 + 2 * j] > tol or sg[2 * L0 + 2 * j + 1] > tol) for j in range(L1)]
+    by, ky = 0, _flagged_tiles(fyl, fy, 0)
+    for s in range(1, fy):
+        if _flagged_tiles(fyl, fy, s) < ky:
+            by, ky = s, _flagged_tiles(fyl, fy, s)
+    return by, _flagged_tiles(fyl, fy, 0), ky
+
+
+@pytest.mark.parametrize("kw", [
+    dict(L0=96, L1=26, L2=40, outer=8, pml_xy=4, pml_z=3, iBc=(0, 0, 0), metal_cells=0, lossy_cells=0),
+    dict(L0=96, L1=26, L2=40, outer=8, pml_xy=4, pml_z=3, iBc=(1, -1, 0), metal_cells=0, lossy_cells=3,
+         metal_box=((86, 90, 5, 9, 18, 22), -20.0 + 2.0j)),
+    dict(L0=70, L1=25, L2=36, outer=8, pml_xy=3, pml_z=3, iBc=(0, 0, 0), metal_cells=0, lossy_cells=0),
+])
+def test_two_step_pml_tile_origins(kw, monkeypatch):
+    """The split-slot two-step pass starts its y tiles where the y-PML layers fall in the
+    fewest tile rows (the row across the y wrap).  The tiling changes which block computes
+    a cell, not its arithmetic: fields, monitors and DFT are bit for bit those of tiles from
+    row 0 (SMX_F32_PML_ORIGIN=0), with and without lossy/metal cells, and close to the
+    oracle."""
+    prep = casegen.prepare(small_waveguide(seed=31, core=(5, 4), **kw))
+    oy, ky0, ky = _split_tile_y_origin(prep)
+    assert oy != 0 and ky < ky0, (oy, ky0, ky)
+    L2 = int(prep.case.L[2])
+    planes = [prep.case.iI + 3, L2 // 2]
+    ws = [2 * np.pi / prep.N * f for f in (0.9, 1.0)]
+    steps = prep.case.time_period[2] * prep.N
+    out = []
+    for origin in ("0", "1"):
+        monkeypatch.setenv("SMX_F32_KERNEL", "fused")
+        monkeypatch.setenv("SMX_F32_TWO", "2,16")
+        monkeypatch.setenv("SMX_F32_PML_ORIGIN", origin)
+        with prep.plan(fdtd.F32) as p:
+            p.set_dft(planes, ws, chunk=8)
+            assert p.autotune(0)[3] == 3 and p.kernels.startswith("two_step_f32<32,8,1,true"), p.kernels
+            sums = p.run_steps(0, steps - 1)
+            p.run_steps(steps - 1, 1, sums)
+            out.append(p.read_fields() + tuple(p.read_monitors()) + (p.read_dft(),))
+    assert np.abs(out[0][0]).max() > 0 and np.abs(out[0][-1]).max() > 0
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
+    Ho, Eo, _, _ = _oracle_run(prep, 0, prep.case.time_period[2])
+    assert rel_l2(out[1][0], Ho) < 2e-4 and rel_l2(out[1][1], Eo) < 2e-4
+
+
+def test_batch_pml_tile_origins(monkeypatch):
+    """The fused batch (every plan's pass in one launch) with the y tiles placed for the PML
+    against each plan run alone with tiles from row 0: bit for bit."""
+    kws = [dict(seed=32, L0=96, L1=26, L2=40, outer=8, pml_xy=4, pml_z=3, core=(5, 4), metal_cells=0,
+                lossy_cells=0, iBc=(0, 0, 0)),
+           dict(seed=33, L0=70, L1=25, L2=36, outer=8, pml_xy=3, pml_z=3, core=(5, 4), metal_cells=0,
+                lossy_cells=0, iBc=(0, 0, 0))]
+    preps = [casegen.prepare(small_waveguide(**kw)) for kw in kws]
+    assert all(_split_tile_y_origin(pr)[0] != 0 for pr in preps)
+    steps = 2 * (min(pr.case.time_period[2] * pr.N for pr in preps) // 2)
+    monkeypatch.setenv("SMX_F32_KERNEL", "fused")
+    monkeypatch.setenv("SMX_F32_TWO", "2,16")
+    alone = []
+    monkeypatch.setenv("SMX_F32_PML_ORIGIN", "0")
+    for pr in preps:
+        with pr.plan(fdtd.F32) as p:
+            p.run_steps(0, steps)
+            alone.append(p.read_fields() + tuple(p.read_monitors()))
+    monkeypatch.delenv("SMX_F32_PML_ORIGIN")
+    plans = [pr.plan(fdtd.F32) for pr in preps]
+    try:
+        info = {}
+        fdtd.run_batch(plans, 0, steps, mode=fdtd.BATCH_FUSED, info=info)
+        assert info["graph_launches"] == steps // 2
+        together = [p.read_fields() + tuple(p.read_monitors()) for p in plans]
+    finally:
+        for p in plans:
+            p.close()
+    for a, b in zip(alone, together):
+        assert np.abs(a[0]).max() > 0
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
