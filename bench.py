@@ -253,6 +253,47 @@ def metal_run(args, torch, K, W):
     return res
 
 
+def point_run(torch, steps=2000):
+    """The broadband point-source workload: a 48 x 48 x 64 unit grid, vacuum with a centred
+    16 x 16 x 24 block of eps 12.1, 8-cell cubic PML on every face, a Gaussian-modulated
+    E_z point current source (centre 0.1 cycles per cell, 1/e spectral half-width 0.05) at
+    (24, 24, 14), `steps` steps at 0.99 of the 3-D CFL limit, a DFT at 4 frequencies on
+    plane 50.  FP64 (the plain path) and FP32 (autotuned, one step per pass), timed over
+    the whole run; the FP32 DFT against the FP64 one.  A cache-resident grid: the times
+    are launch-bound, not a roofline measurement."""
+    import dataclasses
+    import numpy as np
+    from smatranem_amd import casegen, fdtd
+    case = casegen.point_source_case()
+    prep = casegen.prepare(case)
+    iR, iI, dt, amp = casegen.point_source_drive(case, steps)
+    prep = dataclasses.replace(prep, invtau_R=iR, invtau_I=iI, dt=dt)
+    freqs = (0.085, 0.095, 0.105, 0.115)
+    ws = [2 * np.pi * f * dt for f in freqs]
+    out = {}
+    for prec in (fdtd.F64, fdtd.F32):
+        with prep.plan(prec) as p:
+            p.set_dft([50], ws, chunk=64)
+            p.set_point_source(2, (24, 24, 14), amp)
+            if prec == fdtd.F32:
+                p.autotune(10)
+            kern = p.kernels
+            wall, ev = time_plan(p, steps, 0, torch)
+            out[prec] = (wall, ev, kern, p.read_dft().reshape(4, len(ws), -1))
+    d64, d32 = out[fdtd.F64][3], out[fdtd.F32][3]
+    errs = [float(np.abs(d32[:, f] - d64[:, f]).max() / np.abs(d64[:, f]).max()) for f in range(len(ws))]
+    g = lambda prec: round(case.n * steps / (out[prec][0] * 1e-3) / 1e9, 4)  # noqa: E731
+    res = dict(workload="48x48x64 unit grid, eps 12.1 block in vacuum, 8-cell PML on every face, Gaussian-modulated "
+                        "E_z point source (f0 0.1, bw 0.05), %d steps at 0.99 CFL (Dt %.6f), 4-frequency DFT on "
+                        "plane 50" % (steps, dt),
+               cells=case.n, steps=steps, fp64_gcells=g(fdtd.F64), fp64_kernel=out[fdtd.F64][2],
+               fp32_gcells=g(fdtd.F32), fp32_kernel=out[fdtd.F32][2],
+               dft_freqs=list(freqs), dft_rel_err_fp32_vs_fp64=errs)
+    print(f"[bench] point source: FP64 {res['fp64_gcells']} Gcell/s, FP32 {res['fp32_gcells']} Gcell/s, "
+          f"FP32 DFT error {max(errs):.2e} of its largest value", file=sys.stderr, flush=True)
+    return res
+
+
 SMALL_BATCH = dict(dims=((24, 48), (24, 48), (32, 64)), pml=4, seed=2026)
 
 
@@ -581,7 +622,8 @@ def main():
     ap.add_argument("--adjoint-fd-seed", type=int, default=None, metavar="SEED",
                     help="draw the --adjoint-fd cells uniformly from the design box with this seed")
     ap.add_argument("--small", action="store_true", help="32x32x48 grid (quick check)")
-    ap.add_argument("--only", choices=["box", "metal", "batch", "batch-small", "adjoint"], default=None,
+    ap.add_argument("--no-point", action="store_true", help="skip the broadband point-source grid")
+    ap.add_argument("--only", choices=["box", "metal", "batch", "batch-small", "adjoint", "point"], default=None,
                     help="run only this leg (for profiling) and print its JSON object")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device copies of MIB MiB (scale for rocprofv3 byte counters)")
@@ -618,6 +660,8 @@ def main():
             out = batch_run(args.batch_count, args.batch_steps, torch, args.small)
         elif args.only == "batch-small":
             out = batch_run(args.batch_count, 4 * args.batch_steps, torch, args.small, **SMALL_BATCH)
+        elif args.only == "point":
+            out = point_run(torch)
         else:
             out = adjoint_run(torch, args.small, args.adjoint_fd, args.adjoint_fd_seed, args.adjoint_steps)
         print(json.dumps({"only": args.only, "result": out}), flush=True)
@@ -697,6 +741,8 @@ def main():
 
     metal = None if args.no_metal else metal_run(args, torch, K, W)
 
+    point = None if args.no_point else point_run(torch)
+
     batch = batch_small = None
     if not args.no_batch:
         batch = batch_run(args.batch_count, args.batch_steps, torch, args.small)
@@ -774,6 +820,7 @@ def main():
         "err_vs_fp64": err,
         "pml_all_faces": box,
         "metal": metal,
+        "point_source": point,
         "batch": batch,
         "batch_small_grids": batch_small,
         "adjoint": adjoint,

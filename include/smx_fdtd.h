@@ -238,6 +238,22 @@ int smx_plan_set_dft(smx_plan *plan, int nplanes, const int *planes, int nfreq, 
 int smx_plan_read_dft(smx_plan *plan, double *out);
 
 /*
+ * Extension: a point current source on E component comp (0, 1, 2 = x, y, z) of cell
+ * (cell[0], cell[1], cell[2]), which must be a lossless class-'n' cell.  After run step k
+ * (counted as for smx_plan_set_dft) and before its DFT sample, E_comp += B * amp[k] for
+ * k < nsteps, B the cell's E coefficient (Be of the reference layout, 1/(iR eps)): amp[k]
+ * is added to the curl of step k's E update, E <- A E + B (curl H + amp[k]), rounded as
+ * (A E + B curl H) + B amp[k] in FP64.  The reference's source is the modal sheet
+ * (F:261-264, F:293-296); this is the Gaussian-pulse point excitation of a broadband run,
+ * and runs beside the sheet (zero Hinc, Einc for the point source alone).  An FP32 plan
+ * with a point source takes one step per pass, and a batch runs it eagerly (no graph, no
+ * fused launch); the adjoint refuses it.  nsteps = 0 removes it.  No counterpart in the
+ * reference.
+ */
+int smx_plan_set_point_source(smx_plan *plan, int comp, const int cell[3], long long nsteps,
+                              const double *amp);
+
+/*
  * The mode projection of S_calc (F:437-445) on the device: for each column t of tilXi
  * given (modes: complex[nmodes][4*L0*L1], rows in tilXi's order, F:598-628), the sum
  * over q of conj(t[q]) Hx[q] + conj(t[q+P]) Hy[q] + conj(t[q+2P]) (-Ey[q]) +

@@ -60,6 +60,9 @@ def load_oracle():
         lib.orc_run.argtypes = [_ip, _dp, _dp, _dp, _dp, _ip, _dp, _dp, _ip, _dp, _dp, C.c_int,
                                 _ip, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _cp, _ip, _dp]
         lib.orc_run.restype = C.c_int
+        lib.orc_run_point.argtypes = lib.orc_run.argtypes + [C.c_int, C.c_int64, _dp, C.c_int64]
+        lib.orc_run_point.restype = C.c_int
+        lib.orc_point_add.argtypes = [_dp, _dp, C.c_int, C.c_int64, C.c_double]
         lib.orc_s_project.argtypes = [_ip, C.c_int, _dp, _dp, _dp, _dp]
         lib.orc_step_adjoint.argtypes = [_ip, _dp, _dp, _dp, _dp, _ip, _dp, _dp, _cp, _ip, C.c_int]
         lib.orc_step_adjoint.restype = C.c_int
@@ -188,9 +191,10 @@ def setup_from_media(L, mu, eps, omega_dt=None, iR=None, iI=None):
     return out
 
 
-def run(L, coeffs, iBc, iBTI, Hinc, Einc, N, tp, P0=0, P1=None, fields=None, monitors=None):
-    """orc_run: the time loop of F:153-167 over periods [P0, P1).  Returns
-    (H, E, (HB, EB, HT, ET), sums)."""
+def run(L, coeffs, iBc, iBTI, Hinc, Einc, N, tp, P0=0, P1=None, fields=None, monitors=None, point=None):
+    """orc_run: the time loop of F:153-167 over periods [P0, P1).  point = (c, cell, amp)
+    adds a point source on E component c of linear cell `cell` after every step k with
+    amplitude amp[k] (orc_run_point).  Returns (H, E, (HB, EB, HT, ET), sums)."""
     lib = load_oracle()
     L = i32(L)
     n = int(L[0]) * int(L[1]) * int(L[2])
@@ -207,10 +211,16 @@ def run(L, coeffs, iBc, iBTI, Hinc, Einc, N, tp, P0=0, P1=None, fields=None, mon
         mons = [np.ascontiguousarray(m, dtype=np.float64).copy() for m in monitors]
     sums = np.zeros(2)
     P1 = tp[2] if P1 is None else P1
-    rc = lib.orc_run(L, coeffs["Ah"], coeffs["Bh"], coeffs["Ae"], coeffs["Be"], i32(iBc), H, E,
-                     i32(iBTI), np.ascontiguousarray(Hinc, dtype=np.complex128).view(np.float64),
-                     np.ascontiguousarray(Einc, dtype=np.complex128).view(np.float64), N, i32(tp),
-                     P0, P1, *mons, cont_bytes(coeffs["Cont"]), i32(coeffs["addr"]), sums)
+    args = (L, coeffs["Ah"], coeffs["Bh"], coeffs["Ae"], coeffs["Be"], i32(iBc), H, E,
+            i32(iBTI), np.ascontiguousarray(Hinc, dtype=np.complex128).view(np.float64),
+            np.ascontiguousarray(Einc, dtype=np.complex128).view(np.float64), N, i32(tp),
+            P0, P1, *mons, cont_bytes(coeffs["Cont"]), i32(coeffs["addr"]), sums)
+    if point is None:
+        rc = lib.orc_run(*args)
+    else:
+        c, cell, amp = point
+        amp = np.ascontiguousarray(amp, dtype=np.float64)
+        rc = lib.orc_run_point(*args, int(c), int(cell), amp, amp.size)
     if rc:
         raise RuntimeError(f"iBTIin[{rc - 1}] error!")
     return H, E, tuple(mons), sums

@@ -37,6 +37,11 @@
  *                        of check_files/StableForm.pdf p.3 ('e', 'h', 'b') -- and
  *                        A = 1, B = 1/(iR*eps) in lossless cells.
  *   orc_run              composition of the above in the order of F:153-167.
+ *   orc_point_add,       no reference counterpart (a point current source on one E
+ *   orc_run_point        component, the reference's source being a sheet): pinned by the
+ *                        frequency-domain solve of a CW point source in a PML box
+ *                        (tests/test_oracle_freqdomain.py), which fixes the cell, the
+ *                        component, the coefficient and the step timing of the term.
  *   orc_s_project        bitwise vs reference S_calc (F:419-448) reading the same files.
  *   orc_step_adjoint,    no reference counterpart (this build's adjoint): the transpose
  *   orc_adjoint_run      is pinned by the dot-product test <M x, y> = <x, M^T y> and the
@@ -626,6 +631,53 @@ int orc_run(const int L[4], const double *Ah, const double *Bh, const double *Ae
             }
             rc = orc_main_calc(L, Ah, Bh, Ae, Be, iBc, H_t, E_t, iBTIin, Hi, Ei,
                                &ExpN[4 * ((t + 1) % N)], HB, EB, HT, ET, Cont, addr);
+        }
+    free(ExpN);
+    free(Hi);
+    free(Ei);
+    return rc;
+}
+
+/*
+ * Point current source on E component c (0, 1, 2 = x, y, z) of cell l, a class-'n' cell:
+ * after the whole step (both sweeps and the monitors), E_c(l) += Be_c(l) * s, i.e. s is
+ * added to the curl of that step's E update, E <- A E + B (curl H + s), rounded as
+ * (A E + B curl H) + B s.
+ */
+void orc_point_add(double *E_t, const double *Be, int c, int64_t l, double s)
+{
+    E_t[c + 3 * l] += Be[c + 3 * l] * s;
+}
+
+/*
+ * orc_run with a point source as well: after step k (k = P*N + t, counted from step 0 of
+ * the envelope), E_c(l) += Be_c(l) * amp[k] for k < namp.  The sheet source and the
+ * monitors run as in orc_run (zero Hinc, Einc for a point source alone).
+ */
+int orc_run_point(const int L[4], const double *Ah, const double *Bh, const double *Ae,
+                  const double *Be, const int iBc[3], double *H_t, double *E_t, const int iBTIin[4],
+                  const double *Hinc, const double *Einc, int N, const int tp[3], int P0, int P1,
+                  double *HB, double *EB, double *HT, double *ET, const char *Cont, const int *addr,
+                  double sums[2], int pc, int64_t pl, const double *amp, int64_t namp)
+{
+    const int64_t np = 2 * (int64_t)L[0] * L[1];
+    double *ExpN = malloc(sizeof(double) * 4 * N);
+    double *Hi = malloc(sizeof(double) * np);
+    double *Ei = malloc(sizeof(double) * np);
+    int rc = 0;
+    orc_expn_table(N, ExpN);
+    for (int P = P0; P < P1 && rc == 0; P++)
+        for (int t = 0; t < N && rc == 0; t++) {
+            double fH, fE;
+            orc_source_planes(np, N, tp, t, P, ExpN, Hinc, Einc, Hi, Ei, &fH, &fE);
+            if (sums) {
+                sums[0] += fH;
+                sums[1] += fE;
+            }
+            rc = orc_main_calc(L, Ah, Bh, Ae, Be, iBc, H_t, E_t, iBTIin, Hi, Ei,
+                               &ExpN[4 * ((t + 1) % N)], HB, EB, HT, ET, Cont, addr);
+            const int64_t k = (int64_t)P * N + t;
+            if (rc == 0 && k < namp) orc_point_add(E_t, Be, pc, pl, amp[k]);
         }
     free(ExpN);
     free(Hi);

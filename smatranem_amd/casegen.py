@@ -346,6 +346,42 @@ def block_in_vacuum_case():
                           time_period=(10, 80, 111), monitor_gap=1, design=((16, 32, 16, 32, 20, 44), 12.1, 12.1))
 
 
+def point_source_case(L=(48, 48, 64), pml=8, eps_block=12.1, f0=0.1):
+    """The broadband point-source workload: a uniform L0 x L1 x L2 grid (unit spacing), eps =
+    mu = 1 with a centred dielectric block of eps_block, L0/3 x L1/3 x 3 L2/8 cells (16 x 16
+    x 24 at [16, 32) x [16, 32) x [20, 44) on the 48 x 48 x 64 grid), ``pml``-cell cubic
+    PML on every face (R = 1e-8 at the
+    centre frequency f0, in cycles per cell and unit of time, c = 1), the sheet source off
+    (zero profiles) and the reference monitors off (every plane holds PML cells).  Drive it
+    with point_source_drive."""
+    L0, L1, L2 = L
+    box = []
+    for n_, w in ((L0, L0 // 3), (L1, L1 // 3), (L2, 3 * L2 // 8)):
+        box += [(n_ - w) // 2, (n_ - w) // 2 + w]
+    case = waveguide_case(L0, L1, L2, outer=max(L2 // 8, pml + 2), pml_xy=pml, pml_z=pml, omega=2 * np.pi * f0,
+                          eps_core=1.0, eps_clad=1.0, core=(1, 1), seed=1, iBc=(0, 0, 0),
+                          time_period=(1, 1000000, 1000001), design=(tuple(box), eps_block, eps_block))
+    case.Hinc = np.zeros_like(case.Hinc)
+    case.Einc = np.zeros_like(case.Einc)
+    return case
+
+
+def point_source_drive(case, steps=2000, f0=0.1, bw=0.05, cfl=0.99):
+    """Time step and source of a broadband run on ``case``: Dt = cfl x the 3-D CFL limit
+    (the "min Dt^2" value of the grid, 1/sqrt(3) on a unit vacuum grid), the plain Yee
+    time derivative and PML average (invtau_R = 1/Dt, invtau_I = omega0/2 with omega0 =
+    2 pi f0: the conductivity sigma*omega0*eps of the stretch at f0, averaged over the
+    step), and a Gaussian-modulated sinusoid s(t) = sin(2 pi f0 (t - t0))
+    exp(-((t - t0)/tau)^2), tau = 1/(pi bw) (its spectrum falls to 1/e at f0 +- bw),
+    t0 = 4 tau, sampled at t = (k + 1) Dt for run step k (the time of the sheet source's
+    Hi).  Returns (invtau_R, invtau_I, Dt, amplitudes[steps])."""
+    dt = cfl * case.min_dt
+    tau = 1.0 / (np.pi * bw)
+    t = (np.arange(steps) + 1.0) * dt - 4.0 * tau
+    amp = np.sin(2 * np.pi * f0 * t) * np.exp(-(t / tau) ** 2)
+    return 1.0 / dt, np.pi * f0, dt, amp
+
+
 def curvilinear_waveguide_case(L=(160, 160, 320), pml=10, pml_xy=0, time_period=(5, 100000, 100010),
                                metal_box=None):
     """The headline workload: a curvilinear L0 x L1 x L2 grid (scale factors

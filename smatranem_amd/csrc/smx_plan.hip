@@ -105,6 +105,13 @@ struct smx_plan {
     float *adj_ckpt = nullptr; /* [ceil(K/C)][12][n]: T[6], S[6] at steps 0, C, 2C, ... */
     float *adj_hist = nullptr; /* C = 0: [K+1][3][nbox], E of the design box after every step */
     double *adj_D = nullptr;   /* [2][L0*L1] complex */
+
+    /* point current source (smx_plan_set_point_source): E component (-1: none), cell,
+     * amplitudes of steps [0, pt_n) on the device */
+    int pt_comp = -1;
+    int64_t pt_cell = 0;
+    long long pt_n = 0;
+    double *pt_amp = nullptr;
 };
 
 namespace {
@@ -210,7 +217,8 @@ void free_plan(smx_plan *p)
     if (!p) return;
     if (p->stream) (void)hipStreamSynchronize(p->stream);
     void *bufs[] = {p->Ah, p->Bh, p->Ae, p->Be, p->H, p->E, p->Cont, p->addr, p->mon, p->inc, p->src,
-                    p->dft_series, p->dft_acc, p->dft_table, p->adj_ckpt, p->adj_D, p->expn_dev, p->adj_hist};
+                    p->dft_series, p->dft_acc, p->dft_table, p->adj_ckpt, p->adj_D, p->expn_dev, p->adj_hist,
+                    p->pt_amp};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     smx_f32_free(&p->f32);
@@ -367,6 +375,39 @@ int dft_after_step(smx_plan *p, long long k, bool mon)
     SMX_HIP(smx_dft_sample(&a, p->dft_fill, p->stream));
     if (mon) mark_mon_slot(p, p->dft_fill);
     if (++p->dft_fill == p->dft_chunk) return dft_flush(p);
+    return SMX_OK;
+}
+
+/* The point source's term after run step k: E += B * amp[k] on the source's component,
+ * one thread; a separate multiply and add (no contraction into a fused multiply-add),
+ * as the oracle rounds it. */
+__global__ void point_add_f64(double *E, const double *B, const double *amp, long long k)
+{
+#pragma clang fp contract(off)
+    const double t = B[0] * amp[k];
+    E[0] = E[0] + t;
+}
+
+__global__ void point_add_f32(float *T, const float *b, const double *amp, long long k)
+{
+#pragma clang fp contract(off)
+    const float t = b[0] * (float)amp[k];
+    T[0] = T[0] + t;
+}
+
+int point_after_step(smx_plan *p, long long k)
+{
+    if (p->pt_comp < 0 || k < 0 || k >= p->pt_n) return SMX_OK;
+    if (p->g.precision == SMX_F64) {
+        const int64_t i = p->pt_comp + 3 * p->pt_cell;
+        hipLaunchKernelGGL(point_add_f64, dim3(1), dim3(1), 0, p->stream, p->E + i, p->Be + i,
+                           (const double *)p->pt_amp, k);
+    } else {
+        const int c = 3 + p->pt_comp;
+        hipLaunchKernelGGL(point_add_f32, dim3(1), dim3(1), 0, p->stream, p->f32.T[p->f32.cur][c] + p->pt_cell,
+                           (const float *)p->f32.b[c] + p->pt_cell, (const double *)p->pt_amp, k);
+    }
+    SMX_HIP(hipGetLastError());
     return SMX_OK;
 }
 
@@ -559,7 +600,7 @@ int smx_plan_step(smx_plan *p, const double *Hi, const double *Ei, const double 
     s.monE_re = ExpN2[2];
     s.monE_im = ExpN2[3];
     int rc = launch_step(p, s, true);
-    if (rc || (rc = dft_after_step(p, p->next_step, false))) return rc;
+    if (rc || (rc = point_after_step(p, p->next_step)) || (rc = dft_after_step(p, p->next_step, false))) return rc;
     p->next_step++;
     SMX_HIP(hipStreamSynchronize(p->stream));
     return SMX_OK;
@@ -630,7 +671,7 @@ static int launch_step2(smx_plan *p, const smx_step_scalars &s, const smx_step_s
  * the ring holds two slots. */
 static bool pair_ok(const smx_plan *p)
 {
-    if (p->g.precision != SMX_F32 || p->f32.two < 0) return false;
+    if (p->g.precision != SMX_F32 || p->f32.two < 0 || p->pt_comp >= 0) return false;
     return p->dft_nplanes == 0 || (smx_f32_two_full(&p->f32, p->f32.two) && p->dft_chunk >= 2);
 }
 
@@ -638,7 +679,7 @@ static bool pair_ok(const smx_plan *p)
  * whatever kernel the plan selected). */
 static bool pair_box_ok(const smx_plan *p)
 {
-    return p->g.precision == SMX_F32 && smx_f32_box_fz(&p->f32) > 0 && (p->dft_nplanes == 0 || p->dft_chunk >= 2);
+    return p->g.precision == SMX_F32 && p->pt_comp < 0 && smx_f32_box_fz(&p->f32) > 0 && (p->dft_nplanes == 0 || p->dft_chunk >= 2);
 }
 
 /* Ring slot of the pair (k, k+1) of DFT samples, flushing the ring first where the
@@ -696,7 +737,7 @@ static int advance(smx_plan *p, long long k, long long nleft, double *sums, int 
         sums[1] += fE;
     }
     *done = 1;
-    if ((rc = launch_step(p, s, false, !mon_ring(p)))) return rc;
+    if ((rc = launch_step(p, s, false, !mon_ring(p))) || (rc = point_after_step(p, k))) return rc;
     if (box_out && (rc = save_box(p, box_out))) return rc;
     return dft_after_step(p, k, mon_ring(p));
 }
@@ -992,7 +1033,7 @@ bool fused_ok(smx_plan *const *plans, int nplans)
     for (int i = 0; i < nplans; i++) {
         const smx_plan *p = plans[i];
         /* a DFT: its ring must hold the samples of a pass (chunk >= 2) */
-        if (p->g.precision != SMX_F32 || (p->dft_nplanes != 0 && p->dft_chunk < 2) ||
+        if (p->g.precision != SMX_F32 || (p->dft_nplanes != 0 && p->dft_chunk < 2) || p->pt_comp >= 0 ||
             !smx_f32_multi_ok(&p->f32, kFusedDefaultFZ))
             return false;
     }
@@ -1223,7 +1264,8 @@ int smx_plan_run_batch_mode(smx_plan *const *plans, int nplans, long long step0,
     }
 
     /* graph replay needs every plan's steps to be plain kernel launches (no DFT
-     * samples, whose fold points depend on host counters) */
+     * samples, whose fold points depend on host counters, and no point source, whose
+     * amplitude changes every step) */
     bool graph = mode == SMX_BATCH_GRAPH;
     batch_graph bg;
     if (graph) {
@@ -1231,7 +1273,7 @@ int smx_plan_run_batch_mode(smx_plan *const *plans, int nplans, long long step0,
         for (int i = 0; i < nplans && graph; i++) {
             const long long N = plans[i]->N;
             const long long G = N / gcd_ll(N, 4) * 4;
-            graph = plans[i]->dft_nplanes == 0 && G <= kBatchMaxSegment && 2 * G <= nsteps;
+            graph = plans[i]->dft_nplanes == 0 && plans[i]->pt_comp < 0 && G <= kBatchMaxSegment && 2 * G <= nsteps;
             bg.G[(size_t)i] = (int)G;
         }
     }
@@ -1391,6 +1433,48 @@ int smx_plan_set_dft(smx_plan *p, int nplanes, const int *planes, int nfreq, con
     SMX_HIP(hipMemcpyAsync(p->dft_table, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice,
                            p->stream));
     SMX_HIP(hipStreamSynchronize(p->stream));
+    return SMX_OK;
+}
+
+int smx_plan_set_point_source(smx_plan *p, int comp, const int cell[3], long long nsteps, const double *amp)
+{
+    if (!p) return arg_fail("null pointer");
+    if (nsteps <= 0 || !amp) {
+        SMX_HIP(hipStreamSynchronize(p->stream));
+        if (p->pt_amp) SMX_HIP(hipFree(p->pt_amp));
+        p->pt_amp = nullptr;
+        p->pt_comp = -1;
+        p->pt_n = 0;
+        return SMX_OK;
+    }
+    if (!cell) return arg_fail("null pointer");
+    if (comp < 0 || comp > 2) return arg_fail("point source: component must be 0, 1 or 2 (E x, y, z)");
+    for (int d = 0; d < 3; d++)
+        if (cell[d] < 0 || cell[d] >= p->g.L[d]) return arg_fail("point source: cell outside the grid");
+    const int64_t l = cell[0] + (int64_t)p->g.L[0] * (cell[1] + (int64_t)p->g.L[1] * cell[2]);
+    /* the class of the cell, and in FP32 plans the lossy/metal flag (NaN in bE_x) */
+    char cls = 'n';
+    float flag = 0.f;
+    SMX_HIP(hipStreamSynchronize(p->stream));
+    if (p->g.precision == SMX_F64) {
+        SMX_HIP(hipMemcpy(&cls, p->Cont + l, 1, hipMemcpyDeviceToHost));
+    } else {
+        cls = p->f32.cont_host ? p->f32.cont_host[l] : 'n';
+        SMX_HIP(hipMemcpy(&flag, p->f32.b[3] + l, sizeof(float), hipMemcpyDeviceToHost));
+    }
+    if (cls != 'n' || std::isnan(flag)) return arg_fail("point source: the cell must be a lossless class-'n' cell");
+    double *dev = nullptr;
+    SMX_HIP(hipMalloc(&dev, sizeof(double) * (size_t)nsteps));
+    const hipError_t e = hipMemcpy(dev, amp, sizeof(double) * (size_t)nsteps, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(dev);
+        return hip_fail(e, "point source amplitudes");
+    }
+    if (p->pt_amp) SMX_HIP(hipFree(p->pt_amp));
+    p->pt_amp = dev;
+    p->pt_comp = comp;
+    p->pt_cell = l;
+    p->pt_n = nsteps;
     return SMX_OK;
 }
 
@@ -1629,6 +1713,7 @@ int smx_adjoint_forward(smx_plan *p, long long nsteps, int ckpt_every, int zT, d
     if (!p || !box || !J) return arg_fail("null pointer");
     if (p->g.precision != SMX_F32) return arg_fail("the adjoint runs on FP32 plans");
     if (p->f32.nspecial > 0) return arg_fail("the adjoint does not support lossy or metal cells");
+    if (p->pt_comp >= 0) return arg_fail("the adjoint does not support a point source");
     if (!p->have_source) {
         snprintf(g_last_error, sizeof g_last_error, "smx_adjoint_forward before smx_plan_set_source");
         return SMX_ERR_STATE;
@@ -2012,7 +2097,7 @@ int smx_plan_autotune(smx_plan *p, int trial_steps, int choice[5])
         bool zpml = false; /* balanced z chunks differ from uniform ones only with z PML */
         for (int z = 0; z < st.L[2]; z++) zpml |= st.zpml_host[z] != 0;
         const char *two_env = getenv("SMX_F32_TWO"); /* "off": one step per pass only */
-        const int ntwo = two_env && !strcmp(two_env, "off") ? 0 : smx_f32_num_two();
+        const int ntwo = (two_env && !strcmp(two_env, "off")) || p->pt_comp >= 0 ? 0 : smx_f32_num_two();
         for (int v = 0; v < ntwo; v++)
             for (int ci = 0; ci < 4 * 3; ci++) {
                 const int fz = chunks[ci % 4], bal = ci / 4;

@@ -93,6 +93,8 @@ def lib():
     L.smx_plan_set_dft.restype = C.c_int
     L.smx_plan_read_dft.argtypes = [C.c_void_p, _dp]
     L.smx_plan_read_dft.restype = C.c_int
+    L.smx_plan_set_point_source.argtypes = [C.c_void_p, C.c_int, _ip, C.c_longlong, C.c_void_p]
+    L.smx_plan_set_point_source.restype = C.c_int
     L.smx_adjoint_forward.argtypes = [C.c_void_p, C.c_longlong, C.c_int, C.c_int, C.c_double, _ip,
                                       C.POINTER(C.c_double)]
     L.smx_adjoint_forward.restype = C.c_int
@@ -387,6 +389,16 @@ class Plan:
         self._dft = (len(planes), len(w_per_step))
         _check(lib().smx_plan_set_dft(self._h, len(planes), _i32(planes), len(w_per_step),
                                       _f64(w_per_step), int(chunk)))
+
+    def set_point_source(self, comp, cell, amp):
+        """smx_plan_set_point_source: after run step k, E_comp of cell (j0, j1, j2) gets
+        B * amp[k] (amp[k] added to the curl of step k's E update); amp None or empty
+        removes the source."""
+        if amp is None or len(amp) == 0:
+            _check(lib().smx_plan_set_point_source(self._h, 0, _i32([0, 0, 0]), 0, None))
+            return
+        a = np.ascontiguousarray(amp, dtype=np.float64)
+        _check(lib().smx_plan_set_point_source(self._h, int(comp), _i32(cell), a.size, a.ctypes.data))
 
     def project(self, modes, monitors=True, dft=False):
         """smx_plan_project: the S_calc overlap (F:437-445) of each mode column (complex

@@ -77,8 +77,11 @@ def _curl_halves(L, iBc):
     return mats
 
 
-def frequency_domain(case, st, omega, iR, iI, N):
-    """Phasors (E_main, E_split, H_main, H_split), complex [3n] each, of the system above."""
+def frequency_domain(case, st, omega, iR, iI, N, point=None):
+    """Phasors (E_main, E_split, H_main, H_split), complex [3n] each, of the system above;
+    point = (c, l, S): also a point current source on E component c of cell l whose
+    step-k value is 2 env(0, k+1) Re(S e^{-i omega (k+1) Dt}) (the timing of Hi), added
+    to that step's curl: in the E rows, J = 2 env S z^{1/2} (a half-time phasor)."""
     L0, L1, L2 = (int(v) for v in case.L[:3])
     n = L0 * L1 * L2
     cont = np.array([chr(k) for k in st["Cont"]]).repeat(3)  # per DOF c + 3l
@@ -152,6 +155,9 @@ def frequency_domain(case, st, omega, iR, iI, N):
     # H rows: -i w mu H + Rb T_E = -J with J_x = s JE_y, J_y = -s JE_x
     b[6 * n + 0 + 3 * l[keep]] += -s * JE[1 + 2 * q[keep]]
     b[6 * n + 1 + 3 * l[keep]] += s * JE[0 + 2 * q[keep]]
+    if point is not None:
+        c, lp, S = point
+        b[c + 3 * lp] += amp * S * zh
     u = spla.spsolve(A, b)
     return u[:3 * n], u[3 * n:6 * n], u[6 * n:9 * n], u[9 * n:]
 
@@ -159,9 +165,21 @@ def frequency_domain(case, st, omega, iR, iI, N):
 POLY = 2  # degree of the drift polynomial fitted beside cos and sin
 
 
-def time_domain(case, st, N, settle_periods, fit_periods=3):
+def point_amplitudes(N, tp, S, steps):
+    """Step values of a CW point source of phasor S: 2 env(0, k+1) Re(S e^{-i Dphi (k+1)}),
+    the envelope and phase of the sheet source's Hi (F:157-158)."""
+    out = np.empty(steps)
+    for k in range(steps):
+        P, t = divmod(k, N)
+        ph = 2 * np.pi * ((t + 1) % N) / N
+        out[k] = 2.0 * oracle.set_env(N, tp, 0, t + 1, P) * (S * np.exp(-1j * ph)).real
+    return out
+
+
+def time_domain(case, st, N, settle_periods, fit_periods=3, point=None):
     """The oracle's slots after `settle_periods` periods, as phasors fitted over the next
-    `fit_periods` periods (E after step k is at (k + 1) Dt, H at (k + 1/2) Dt), with a
+    `fit_periods` periods (E after step k is at (k + 1) Dt, H at (k + 1/2) Dt); point =
+    (c, l, S) drives a CW point source (point_amplitudes) through orc_run_point; with a
     polynomial of degree POLY in time beside cos and sin.  The polynomial: the metal
     stable form takes the backward curl of H (quirk Q1), so it does not keep the discrete
     divergence of D, and adds every period to a static curl-free E field, which grows
@@ -172,8 +190,11 @@ def time_domain(case, st, N, settle_periods, fit_periods=3):
     L0, L1, L2 = (int(v) for v in case.L[:3])
     n = L0 * L1 * L2
     iBTI = (-1, -1, case.iI, case.inc_mode)
+    amp = None
+    if point is not None:
+        amp = point_amplitudes(N, case.time_period, point[2], (settle_periods + fit_periods) * N)
     H, E, mons, _ = oracle.run(case.L, st, case.iBc, iBTI, case.Hinc, case.Einc, N, case.time_period, 0,
-                               settle_periods)
+                               settle_periods, point=None if point is None else (point[0], point[1], amp))
     ExpN = oracle.expn_table(N)
     Hinc = np.ascontiguousarray(case.Hinc).view(np.float64)
     Einc = np.ascontiguousarray(case.Einc).view(np.float64)
@@ -193,6 +214,8 @@ def time_domain(case, st, N, settle_periods, fit_periods=3):
             oracle.main_calc(oracle.load_oracle().orc_main_calc, case.L, st["Ah"], st["Bh"], st["Ae"], st["Be"],
                              case.iBc, H, E, iBTI, np.ascontiguousarray(Hi), np.ascontiguousarray(Ei),
                              ExpN[4 * tn:4 * tn + 4].copy(), *mons, st["Cont"], addr)
+            if point is not None:
+                oracle.load_oracle().orc_point_add(E, st["Be"], point[0], point[1], amp[P * N + t])
             rec["Em"].append(E[:3 * n].copy())
             rec["Hm"].append(H[:3 * n].copy())
             es, hs = np.zeros(3 * n), np.zeros(3 * n)
@@ -247,3 +270,33 @@ def test_cw_steady_state_equals_frequency_domain_solution(seed, metal, iBc):
     # a plausible slip moves the solution: the split slots are not negligible, and the
     # metal cell's aux factor matters
     assert np.abs(fd["Es"]).max() > 1e-3 * scale and np.abs(fd["Hs"]).max() > 1e-3 * scale
+
+
+@pytest.mark.parametrize("c,cell", [(2, (4, 3, 9)), (0, (3, 4, 8))])
+def test_cw_point_source_equals_frequency_domain_solution(c, cell):
+    """A CW point current source (orc_run_point: after each step E_c(l) += Be_c(l) s_k)
+    alone, in the PML box of the test above with lossy cells: the steady state equals the
+    frequency-domain solution with J = 2 env S z^{1/2} at that DOF.  A source on the
+    wrong cell, component or time level, or without the B factor, moves the solution by
+    far more than the bound."""
+    case = _case(7, False)
+    case.Hinc = np.zeros_like(case.Hinc)
+    case.Einc = np.zeros_like(case.Einc)
+    N, iR, iI, dt = oracle.set_tau(case.omega, case.min_dt)
+    st = oracle.setup_from_media(case.L, case.mu, case.eps, iR=iR, iI=iI)
+    L0, L1 = int(case.L[0]), int(case.L[1])
+    lp = cell[0] + L0 * (cell[1] + L1 * cell[2])
+    assert st["Cont"][lp] == ord("n")
+    point = (c, lp, 0.7 - 0.4j)
+    fd = dict(zip(("Em", "Es", "Hm", "Hs"), frequency_domain(case, st, case.omega, iR, iI, N, point=point)))
+    td = time_domain(case, st, N, settle_periods=2000, point=point)
+    scale = max(np.abs(fd["Em"]).max(), np.abs(fd["Hm"]).max())
+    assert scale > 0
+    for k in ("Em", "Es", "Hm", "Hs"):
+        err = np.abs(td[k] - fd[k]).max() / scale
+        assert err < 1e-9, (k, err)
+    # the bound discriminates: the same source on the next component, or on the next cell
+    # along x, is far from the oracle's steady state
+    for wrong in (((c + 1) % 3, lp, point[2]), (c, lp + 1, point[2])):
+        fw = frequency_domain(case, st, case.omega, iR, iI, N, point=wrong)[0]
+        assert np.abs(td["Em"] - fw).max() / scale > 1e-3
