@@ -297,19 +297,34 @@ def point_run(torch, steps=2000):
 SMALL_BATCH = dict(dims=((24, 48), (24, 48), (32, 64)), pml=4, seed=2026)
 
 
-def batch_run(count, steps, torch, small=False, dims=((64, 192), (64, 192), (64, 256)), pml=8, seed=2025):
+def batch_run(count, steps, torch, small=False, dims=((64, 192), (64, 192), (64, 256)), pml=8, seed=2025,
+              ports=False):
     """Independent runs of seeded sizes (L0, L1, L2 uniform in `dims`, dielectric boxes,
-    `pml`-cell PML on every face), FP32, each plan autotuned.  After the envelope's ramp
-    (untimed), `steps` steps of all plans are timed four ways: the fused batch mode (one
-    launch of two_step_multi per two-step pass of all plans; `value`), smx_plan_run_batch
-    (one captured HIP graph of every plan's segment, replayed), the same batch with every
-    step launched from the host (streams mode), and the plans one after another."""
+    `pml`-cell PML on every face), FP32, each plan autotuned: with `ports`, count/12
+    structures each excited by 6 Hermite-Gaussian port modes in 2 polarisations
+    (casegen.port_batch_cases), else `count` structures of one excitation each.  After the
+    envelope's ramp (untimed), `steps` steps of all plans are timed four ways: the fused
+    batch mode (one launch of two_step_multi per two-step pass of all plans; `value`),
+    smx_plan_run_batch (one captured HIP graph of every plan's segment, replayed), the
+    same batch with every step launched from the host (streams mode), and the plans one
+    after another."""
+    import dataclasses
     from smatranem_amd import casegen, fdtd
-    cases = casegen.batch_cases(count, seed=seed, pml=pml, dims=dims)
-    if small:
-        cases = [casegen.waveguide_case(24 + 8 * i, 20, 40, outer=8, pml_xy=4, pml_z=4, seed=i, blocks=2,
-                                        time_period=(5, 100000, 100010)) for i in range(count)]
-    preps = [casegen.prepare(c) for c in cases]
+    if ports:
+        runs = casegen.port_batch_cases(max(count // 12, 1), 6, 2, seed=seed, pml=pml, dims=dims)
+        base = {}
+        preps = []
+        for si, m, pol, c in runs:
+            if si not in base:
+                base[si] = casegen.prepare(c)
+            preps.append(dataclasses.replace(base[si], case=c))
+        cases = [pr.case for pr in preps]
+    else:
+        cases = casegen.batch_cases(count, seed=seed, pml=pml, dims=dims)
+        if small:
+            cases = [casegen.waveguide_case(24 + 8 * i, 20, 40, outer=8, pml_xy=4, pml_z=4, seed=i, blocks=2,
+                                            time_period=(5, 100000, 100010)) for i in range(count)]
+        preps = [casegen.prepare(c) for c in cases]
     plans = [pr.plan(fdtd.F32) for pr in preps]
     warm = max(pr.N * (pr.case.time_period[0] + 1) for pr in preps)
 
@@ -349,10 +364,12 @@ def batch_run(count, steps, torch, small=False, dims=((64, 192), (64, 192), (64,
             p.close()
     cells = sum(c.n for c in cases)
     alg = sum(c.n * bytes_per_cell(pr.pml_fraction, 4, 4) for c, pr in zip(cases, preps))
-    return dict(workload="%d independent runs, L0 in [%d, %d], L1 in [%d, %d], L2 in [%d, %d], dielectric boxes, "
+    what = ("%d structures x 6 Hermite-Gaussian port modes x 2 polarisations" % max(count // 12, 1) if ports
+            else "%d independent runs" % count)
+    return dict(workload="%s, L0 in [%d, %d], L1 in [%d, %d], L2 in [%d, %d], dielectric boxes, "
                          "%d-cell PML on all faces, FP32, smx_plan_run_batch_mode SMX_BATCH_FUSED (one "
                          "two_step_multi launch per two-step pass of all plans)"
-                         % (count, *dims[0], *dims[1], *dims[2], pml),
+                         % (what, *dims[0], *dims[1], *dims[2], pml),
                 grids=[list(c.L[:3]) for c in cases], cells=cells, steps=steps, first_step=warm,
                 value=round(cells * steps / t_fused / 1e9, 4), unit="Gcell-updates/s",
                 roofline_frac=round(alg * steps / t_fused / 1e9 / HBM_PEAK_GBS, 4),
@@ -657,7 +674,7 @@ def main():
         elif args.only == "metal":
             out = metal_run(args, torch, K, W)
         elif args.only == "batch":
-            out = batch_run(args.batch_count, args.batch_steps, torch, args.small)
+            out = batch_run(args.batch_count, args.batch_steps, torch, args.small, ports=not args.small)
         elif args.only == "batch-small":
             out = batch_run(args.batch_count, 4 * args.batch_steps, torch, args.small, **SMALL_BATCH)
         elif args.only == "point":
@@ -745,7 +762,7 @@ def main():
 
     batch = batch_small = None
     if not args.no_batch:
-        batch = batch_run(args.batch_count, args.batch_steps, torch, args.small)
+        batch = batch_run(args.batch_count, args.batch_steps, torch, args.small, ports=not args.small)
         print(f"[bench] batch of {args.batch_count}: {batch['value']:.2f} Gcell/s fused, "
               f"{batch['graph_replay']:.2f} graph replay "
               f"({batch['graph_launches']} launches), {batch['streams_mode']:.2f} streams, "

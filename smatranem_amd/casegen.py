@@ -15,7 +15,7 @@ along d: half for mu, integer for eps (PMLcalc.c:215-232).
 """
 from __future__ import annotations
 
-from dataclasses import dataclass, field
+from dataclasses import dataclass, field, replace
 
 import numpy as np
 
@@ -331,6 +331,65 @@ def batch_cases(count, seed=0, pml=8, time_period=(5, 100000, 100010), dims=((64
         out.append(waveguide_case(L0, L1, L2, outer=max(L2 // 8, pml + 4), pml_xy=pml, pml_z=pml,
                                   omega=2 * np.pi / 20.0, core=(max(L0 // 8, 2), max(L1 // 10, 2)),
                                   seed=seed * 1000 + i, blocks=6, time_period=time_period))
+    return out
+
+
+PORT_MODES = ((0, 0), (1, 0), (0, 1), (1, 1), (2, 0), (0, 2))  # Hermite-Gaussian orders (m, n)
+
+
+def port_mode_profile(L0, L1, cx, cy, wx, wy, m, n):
+    """Hermite-Gaussian profile of order (m, n) centred on (cx, cy) with widths (wx, wy):
+    H_m(sqrt(2) X) H_n(sqrt(2) Y) exp(-X^2 - Y^2), X = (x - cx)/wx, Y = (y - cy)/wy (physicists'
+    Hermite polynomials), normalised to a largest value of 1."""
+    X = (np.arange(L0)[None, :] - cx) / wx
+    Y = (np.arange(L1)[:, None] - cy) / wy
+    hx = np.polynomial.hermite.hermval(np.sqrt(2) * X, [0] * m + [1])
+    hy = np.polynomial.hermite.hermval(np.sqrt(2) * Y, [0] * n + [1])
+    p = hx * hy * np.exp(-X ** 2 - Y ** 2)
+    return p / np.abs(p).max()
+
+
+def polarised_sheet(L0, L1, profile, pol, n_eff=1.0):
+    """A modal-like sheet of one polarisation: pol 0 has E along x and H along y (as
+    sheet_source), pol 1 E along y and H along -x; Hinc = (Xi_r, Xi_{r+L0L1}), Einc =
+    (Xi_{r+3L0L1}, -Xi_{r+2L0L1}) (F:504-509)."""
+    p = np.asarray(profile, dtype=np.float64).reshape(-1)
+    Hinc = np.zeros(2 * L0 * L1, dtype=np.complex128)
+    Einc = np.zeros(2 * L0 * L1, dtype=np.complex128)
+    if pol == 0:
+        Einc[0::2] = p
+        Hinc[1::2] = n_eff * p
+    else:
+        Einc[1::2] = p
+        Hinc[0::2] = -n_eff * p
+    return Hinc, Einc
+
+
+def port_batch_cases(structures=2, modes=6, pols=2, seed=0, pml=8, time_period=(5, 100000, 100010),
+                     dims=((64, 192), (64, 192), (64, 256))):
+    """structures x modes x pols independent runs (by default 2 x 6 x 2 = 24): each structure
+    has a seeded size (L0, L1, L2 uniform in ``dims``), a waveguide core with 6 seeded
+    dielectric boxes (eps uniform in [1, 12.1]) and ``pml``-cell PML on every face, as
+    batch_cases; each is excited by the first ``modes`` Hermite-Gaussian port modes
+    (PORT_MODES) on its core in each of ``pols`` polarisations.  Returns
+    [(structure index, mode, pol, Case)]; the runs of a structure share its media arrays
+    and differ in the sheet profiles only."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for s in range(structures):
+        L0 = int(rng.integers(dims[0][0], dims[0][1] + 1))
+        L1 = int(rng.integers(dims[1][0], dims[1][1] + 1))
+        L2 = int(rng.integers(dims[2][0], dims[2][1] + 1))
+        core = (max(L0 // 8, 2), max(L1 // 10, 2))
+        base = waveguide_case(L0, L1, L2, outer=max(L2 // 8, pml + 4), pml_xy=pml, pml_z=pml,
+                              omega=2 * np.pi / 20.0, core=core, seed=seed * 1000 + s, blocks=6,
+                              time_period=time_period)
+        cx, cy = (L0 - 1) / 2.0, (L1 - 1) / 2.0
+        for m in range(modes):
+            prof = port_mode_profile(L0, L1, cx, cy, max(core[0], 2), max(core[1], 2), *PORT_MODES[m])
+            for pol in range(pols):
+                Hinc, Einc = polarised_sheet(L0, L1, prof, pol, n_eff=np.sqrt(12.1))
+                out.append((s, m, pol, replace(base, Hinc=Hinc, Einc=Einc)))
     return out
 
 

@@ -12,7 +12,7 @@ reference monitors and of the 16-frequency DFT, normalised by the envelope sum o
 steps run, within 1e-3 absolute and 1e-3 of each |S|.
 C3: three runs drawn from the batch generator (sizes in [64, 192]^2 x [64, 256], 8-cell
 PML on all faces), advanced together with graph replay over CW periods, equal to each
-run alone.
+run alone; and one structure excited by 6 port modes in 2 polarisations, fused.
 """
 import numpy as np
 import pytest
@@ -159,6 +159,38 @@ def test_c3_batch_members_graph_equals_alone(monkeypatch):
     for a, b in zip(alone, together):
         assert np.abs(a[0]).max() > 0
         assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_c3_port_modes_fused_equals_alone(monkeypatch):
+    """The batch as the configuration states it, at a reduced size: one structure excited by
+    6 Hermite-Gaussian port modes in 2 polarisations (12 runs sharing the media), all in
+    the fused launch, each bit for bit the run alone; different modes give different
+    fields."""
+    import dataclasses
+    monkeypatch.setenv("SMX_F32_TWO", "2,16")
+    runs = casegen.port_batch_cases(1, 6, 2, seed=7, pml=4, time_period=(1, 12, 14),
+                                    dims=((40, 48), (36, 44), (40, 48)))
+    base = casegen.prepare(runs[0][3])
+    preps = [dataclasses.replace(base, case=c) for _, _, _, c in runs]
+    steps = 2 * base.N
+    alone = []
+    for pr in preps:
+        with pr.plan(fdtd.F32) as p:
+            p.run_steps(0, steps)
+            alone.append(p.read_fields())
+    plans = [pr.plan(fdtd.F32) for pr in preps]
+    try:
+        info = {}
+        fdtd.run_batch(plans, 0, steps, mode=fdtd.BATCH_FUSED, info=info)
+        together = [p.read_fields() for p in plans]
+    finally:
+        for p in plans:
+            p.close()
+    assert info["graph_launches"] == steps // 2
+    for a, b in zip(alone, together):
+        assert np.abs(a[1]).max() > 0
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert rel_l2(alone[0][1], alone[1][1]) > 0.5 and rel_l2(alone[0][1], alone[2][1]) > 0.5
 
 
 @pytest.fixture(scope="module")
