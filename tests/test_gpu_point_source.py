@@ -119,6 +119,18 @@ def test_point_source_refusals_and_removal():
     assert not H.any() and not E.any()
 
 
+def test_point_source_refused_by_the_adjoint():
+    """The adjoint's transposed step has no point-source term: a plan with one is refused."""
+    case, prep, amp, ws = _setup(L=(24, 24, 32), steps=40)
+    with prep.plan(fdtd.F32) as p:
+        p.set_point_source(2, (12, 12, 12), amp)
+        with pytest.raises(fdtd.SmxError):
+            p.adjoint_forward(40, 0, 20, ws[0], (10, 14, 10, 14, 10, 14))
+        p.set_point_source(2, (12, 12, 12), None)
+        J = p.adjoint_forward(40, 0, 20, ws[0], (10, 14, 10, 14, 10, 14))  # accepted once removed
+    assert np.isfinite(J)
+
+
 def test_point_source_plan_in_batch():
     """A plan with a point source runs eagerly inside a batch (no graph, no fused launch:
     its amplitude changes every step) and equals the same plan run alone, bit for bit,
