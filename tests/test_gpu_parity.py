@@ -491,8 +491,10 @@ def _split_tile_y_origin(prep, fy=8):
 def test_two_step_pml_tile_origins(kw, monkeypatch):
     """The split-slot two-step pass starts its y tiles where the y-PML layers fall in the
     fewest tile rows (the row across the y wrap).  The tiling changes which block computes
-    a cell, not its arithmetic: fields, monitors and DFT are bit for bit those of tiles from
-    row 0 (SMX_F32_PML_ORIGIN=0), with and without lossy/metal cells, and close to the
+    a cell, not its arithmetic: without lossy/metal cells the fields, monitors and DFT are
+    bit for bit those of tiles from row 0 (SMX_F32_PML_ORIGIN=0).  With them, the box of
+    tiles left to the region sweeps moves with the tiling, and the sweeps round some cells
+    differently from the pass: there the two agree to 1e-5.  Both are close to the
     oracle."""
     prep = casegen.prepare(small_waveguide(seed=31, core=(5, 4), **kw))
     oy, ky0, ky = _split_tile_y_origin(prep)
@@ -513,8 +515,12 @@ def test_two_step_pml_tile_origins(kw, monkeypatch):
             p.run_steps(steps - 1, 1, sums)
             out.append(p.read_fields() + tuple(p.read_monitors()) + (p.read_dft(),))
     assert np.abs(out[0][0]).max() > 0 and np.abs(out[0][-1]).max() > 0
+    special = kw.get("metal_cells", 0) or kw.get("lossy_cells", 0) or kw.get("metal_box")
     for a, b in zip(out[0], out[1]):
-        assert np.array_equal(a, b)
+        if special:
+            assert rel_l2(a, b) < 1e-5
+        else:
+            assert np.array_equal(a, b)
     Ho, Eo, _, _ = _oracle_run(prep, 0, prep.case.time_period[2])
     assert rel_l2(out[1][0], Ho) < 2e-4 and rel_l2(out[1][1], Eo) < 2e-4
 
