@@ -9,7 +9,9 @@ widths (x/y 0-3, z 2-4), metal and lossy cells, wrap weights iBc in {-1, 0, 1}, 
   oracle, and -- on grids without lossy/metal cells -- bit for bit the same with the y
   tiles from row 0;
 - a fused batch of 2-4 drawn plans (some with a DFT of a drawn chunk, odd or even step
-  counts): each plan bit for bit the plan alone."""
+  counts): each plan bit for bit the plan alone;
+- the adjoint with a drawn kernel, checkpoint interval, design box and objective: within
+  1e-4 (J) and 1e-3 (gradient) of the oracle's FP64 adjoint."""
 import os
 
 import numpy as np
@@ -148,3 +150,44 @@ def test_random_fused_batch(seed, monkeypatch):
         if dft[j]:
             assert np.array_equal(a[2], b[2])
         assert np.array_equal(a[3], sums[j])
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_adjoint(seed, monkeypatch):
+    """The FP32 adjoint on a drawn grid (no lossy/metal cells), step kernel, checkpoint
+    interval (0: design-box history), design box, objective plane and frequency: J within
+    1e-4 and dJ/d eps within 1e-3 (relative L2) of the oracle's FP64 adjoint."""
+    rng = np.random.default_rng(5000 + seed)
+    L0, L1, outer = int(rng.integers(12, 48)), int(rng.integers(12, 36)), int(rng.integers(7, 10))
+    L2 = int(rng.integers(2 * outer + 10, 2 * outer + 40))
+    pml_xy, pml_z = int(rng.integers(0, 4)), int(rng.integers(2, 5))
+    iBc = tuple(int(v) for v in rng.integers(-1, 2, 3)) if pml_xy == 0 else (0, 0, 0)
+    case = small_waveguide(seed=5000 + seed, L0=L0, L1=L1, L2=L2, outer=outer, pml_xy=pml_xy, pml_z=pml_z, iBc=iBc,
+                           metal_cells=0, lossy_cells=0, core=(max(L0 // 6, 1), max(L1 // 6, 1)),
+                           time_period=(1, int(rng.integers(1, 3)), 3))
+    prep = casegen.prepare(case)
+    kern = rng.choice(["fused", "twopass", "staged", "two-step"])
+    ckpt = int(rng.choice([0, 1, 5, 8, 10, 1000]))
+    monkeypatch.setenv("SMX_F32_KERNEL", "twopass" if kern == "twopass" else "fused")
+    if kern == "staged":
+        monkeypatch.setenv("SMX_F32_TILE", "3,16")
+    if kern == "two-step":
+        monkeypatch.setenv("SMX_F32_TWO", "2,%d" % int(rng.choice([8, 16, 64])))
+    m = max(pml_xy, 1) + 1
+    x0, y0 = int(rng.integers(m, L0 // 2)), int(rng.integers(m, L1 // 2))
+    x1, y1 = int(rng.integers(x0 + 1, L0 - m + 1)), int(rng.integers(y0 + 1, L1 - m + 1))
+    z0 = int(rng.integers(outer, L2 // 2))
+    box = (x0, x1, y0, y1, z0, int(rng.integers(z0 + 1, L2 - outer)))
+    zT = int(rng.integers(case.iI + 3, L2 - pml_z - 1))
+    w = float(rng.uniform(0.8, 1.1)) * 2 * np.pi / prep.N
+    K = case.time_period[2] * prep.N
+    with prep.plan(fdtd.F32) as p:
+        J = p.adjoint_forward(K, ckpt, zT, w, box)
+        g = p.adjoint_backward()
+    N, iR, iI, _ = oracle.set_tau(case.omega, case.min_dt)
+    st = oracle.setup_from_media(case.L, case.mu, case.eps, iR=iR, iI=iI)
+    Jo, go = oracle.adjoint_run(case.L, st, case.iBc, (-1, -1, case.iI, 1), case.Hinc, case.Einc, N,
+                                case.time_period, zT, w, iR, iI)
+    assert Jo > 0 and abs(J - Jo) <= 1e-4 * Jo
+    go3 = go.reshape(L2, L1, L0, 3)[box[4]:box[5], y0:y1, x0:x1, :].transpose(3, 0, 1, 2)
+    assert np.linalg.norm(g - go3) <= 1e-3 * np.linalg.norm(go3), (kern, ckpt, box)
