@@ -176,7 +176,7 @@ def time_plan(plan, steps, warmup, torch):
 
 
 def box_run(args, torch, K, W):
-    """The headline grid as the survey sizes it: 10-cell PML on all six faces (the
+    """The headline grid with 10-cell PML on all six faces (the
     reference monitors off: every plane holds PML cells) and the 16-frequency DFT on the
     two port planes, autotuned, max(K/5, 100) steps timed."""
     import numpy as np
@@ -647,8 +647,8 @@ def main():
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py: this build runs on one GPU only; the multi-GPU design (z-slab halo "
-              "exchange) is described in DESIGN.md and not run.", file=sys.stderr)
+        print("bench.py: runs on one GPU only; the multi-GPU path (z slabs with a halo exchange "
+              "per half step) is not built.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
