@@ -1297,15 +1297,24 @@ struct two_layout {
     static constexpr int NC = (N4 + CH - 1) / CH;          /* copies per component */
     static constexpr int ARR = 3 * NC * CH;                /* one array (3 components) of a stage */
     static constexpr int SLOT = (SPLIT ? 6 : 4) * ARR;     /* E0, H0, bH, bE [, SH0, SE0] */
-    static constexpr int NBASE = 4 * 3 * NC;               /* copies of every stage */
-    static constexpr int NSPLIT = 2 * 3 * NC;              /* extra copies of PML stages */
+    /* rows of the G4 region each array is read on: E0 all of G4; H0, bH, SH0 on G3 (the
+     * first row left out); bE, SE0 on G2 (the first and the last row left out).  An array's
+     * copies start at its first row, and its LDS words count from there */
+    static constexpr int row0(int arr) { return WIDE || arr == 0 ? 0 : 1; }
+    static constexpr int rows(int arr) { return WIDE || arr == 0 ? H4 : arr == 3 || arr == 5 ? H4 - 2 : H4 - 1; }
+    static constexpr int nca(int arr) { return WIDE ? NC : (rows(arr) * W4 + CH - 1) / CH; }
+    static constexpr int cum(int arr) { return arr == 0 ? 0 : cum(arr - 1) + 3 * nca(arr - 1); }
+    static constexpr int NBASE = cum(4);                   /* copies of every stage */
+    static constexpr int NSPLIT = cum(6) - cum(4);         /* extra copies of PML stages */
     static constexpr int kMinOps = NBASE / NW;             /* per wave, at least */
     static constexpr int CELLS = 2;                        /* a tile cell and a ring cell */
     static constexpr int kMaxFZ = 80;                      /* z tables of a chunk in LDS */
 };
 
 /* Copies of one stage dealt to wave W: arrays E0, H0, bH, bE (and SH0, SE0 when `split`),
- * each 3 components x NC runs of CH words of the G4 region. */
+ * each 3 components x nca(arr) runs of CH words of the rows of the G4 region the array is
+ * read on (vq: from the first row; vr: from the second; vlast: the last run of the arrays
+ * that stop a row short of the region). */
 /* Where a stage's copies read: one buffer resource per group of 6 component arrays
  * (component c at c*cb bytes, the plane at soff), or -- with BIG, for groups of 4 GiB and
  * more -- a resource built per copy on one array's plane (base + comp*ncomp + zoff
@@ -1332,7 +1341,8 @@ __device__ inline __amdgpu_buffer_rsrc_t two_res(const two_src &q, int group, un
 }
 
 template <int FX, int FY, bool SPLIT, bool WIDE, bool BIG, int W>
-__device__ inline void two_issue(float *slot, const two_src &q, bool split, const unsigned *vq)
+__device__ inline void two_issue(float *slot, const two_src &q, bool split, const unsigned *vq, const unsigned *vr,
+                                 unsigned vlast)
 {
     using Ly = two_layout<FX, FY, SPLIT, WIDE>;
     const unsigned so = BIG ? 0u : q.soff;
@@ -1341,23 +1351,37 @@ __device__ inline void two_issue(float *slot, const two_src &q, bool split, cons
         if constexpr (WIDE) __builtin_amdgcn_raw_ptr_buffer_load_lds(r, SMX_LDS(dst), 16, voff, so, 0, 0);
         else __builtin_amdgcn_raw_ptr_buffer_load_lds(r, SMX_LDS(dst), 4, voff, so, 0, 0);
     };
+    /* copy j of a stage: its array, component and run */
+    auto which = [](int j, int &arr, int &c, int &ch) {
+        arr = 0;
+        while (arr < 5 && j >= Ly::cum(arr + 1)) arr++;
+        const int r = j - Ly::cum(arr);
+        c = r / Ly::nca(arr);
+        ch = r % Ly::nca(arr);
+    };
+    auto offset = [&](int arr, int ch) {
+        if (Ly::row0(arr) == 0) return vq[ch];
+        return Ly::rows(arr) == Ly::H4 - 2 && ch == Ly::nca(arr) - 1 ? vlast : vr[ch];
+    };
 #pragma unroll
     for (int j = W; j < Ly::NBASE; j += Ly::NW) {
-        const int arr = j / (3 * Ly::NC), r = j % (3 * Ly::NC), c = r / Ly::NC, ch = r % Ly::NC;
+        int arr, c, ch;
+        which(j, arr, c, ch);
         /* E0 = T comp 3+c, H0 = T comp c, bH = b comp c, bE = b comp 3+c */
         const unsigned comp = (arr == 0 || arr == 3) ? 3 + c : c;
         float *dst = slot + arr * Ly::ARR + (c * Ly::NC + ch) * Ly::CH;
-        unsigned voff = vq[ch];
+        unsigned voff = offset(arr, ch);
         const __amdgpu_buffer_rsrc_t res = two_res<BIG>(q, arr < 2 ? 0 : 1, comp, voff);
         copy(res, dst, voff, so);
     }
     if (SPLIT && split) {
 #pragma unroll
-        for (int j = W; j < Ly::NSPLIT; j += Ly::NW) {
-            const int arr = j / (3 * Ly::NC), r = j % (3 * Ly::NC), c = r / Ly::NC, ch = r % Ly::NC;
-            float *dst = slot + (4 + arr) * Ly::ARR + (c * Ly::NC + ch) * Ly::CH; /* SH0 = S comp c, SE0 = 3+c */
-            unsigned voff = vq[ch];
-            const __amdgpu_buffer_rsrc_t res = two_res<BIG>(q, 2, arr * 3 + c, voff);
+        for (int j = Ly::NBASE + W; j < Ly::NBASE + Ly::NSPLIT; j += Ly::NW) {
+            int arr, c, ch;
+            which(j, arr, c, ch);
+            float *dst = slot + arr * Ly::ARR + (c * Ly::NC + ch) * Ly::CH; /* SH0 = S comp c, SE0 = 3+c */
+            unsigned voff = offset(arr, ch);
+            const __amdgpu_buffer_rsrc_t res = two_res<BIG>(q, 2, (arr - 4) * 3 + c, voff);
             copy(res, dst, voff, so);
         }
     }
@@ -1491,12 +1515,17 @@ __device__ __forceinline__ void two_step_body(const A &a, int id, const double *
 
     /* byte offset in a plane of the first word lane `lane` copies in run ch (WIDE: a group
      * of 4 x; the groups past the region repeat the last one, into the run's padding) */
-    unsigned vq[Ly::NC];
+    unsigned vq[Ly::NC], vr[Ly::NC], vlast = 0;
+    auto region_off = [&](int e) {
+        return 4u * (unsigned)(wrap_index(x0 - OX + e % W4, a.L0) + a.L0 * wrap_index(y0 - 2 + e / W4, a.L1));
+    };
 #pragma unroll
     for (int ch = 0; ch < Ly::NC; ch++) {
-        const int e = WIDE ? 4 * min(ch * 64 + lane, N4 / 4 - 1) : min(ch * 64 + lane, N4 - 1);
-        vq[ch] = 4u * (unsigned)(wrap_index(x0 - OX + e % W4, a.L0) + a.L0 * wrap_index(y0 - 2 + e / W4, a.L1));
+        vq[ch] = region_off(WIDE ? 4 * min(ch * 64 + lane, N4 / 4 - 1) : min(ch * 64 + lane, N4 - 1));
+        /* the arrays read from the second row on (from the third-last: to the last row) */
+        vr[ch] = WIDE ? vq[ch] : region_off(min(W4 + ch * 64 + lane, N4 - 1));
     }
+    if constexpr (!WIDE) vlast = region_off(min(W4 + (Ly::nca(3) - 1) * 64 + lane, N4 - W4 - 1));
 
     /* this thread's cells: r = 0 the tile cell t (every step computes it); r = 1 one cell
      * of the rings the first three updates add around the tile -- G1\T (used by c, b,
@@ -1607,26 +1636,28 @@ __device__ __forceinline__ void two_step_body(const A &a, int id, const double *
         float *slot = lds + (s % NSLOT) * Ly::SLOT;
         if constexpr (Ly::NW == 4) {
             switch (wv) {
-            case 0: two_issue<FX, FY, SPLIT, WIDE, BIG, 0>(slot, src, split, vq); break;
-            case 1: two_issue<FX, FY, SPLIT, WIDE, BIG, 1>(slot, src, split, vq); break;
-            case 2: two_issue<FX, FY, SPLIT, WIDE, BIG, 2>(slot, src, split, vq); break;
-            default: two_issue<FX, FY, SPLIT, WIDE, BIG, 3>(slot, src, split, vq); break;
+            case 0: two_issue<FX, FY, SPLIT, WIDE, BIG, 0>(slot, src, split, vq, vr, vlast); break;
+            case 1: two_issue<FX, FY, SPLIT, WIDE, BIG, 1>(slot, src, split, vq, vr, vlast); break;
+            case 2: two_issue<FX, FY, SPLIT, WIDE, BIG, 2>(slot, src, split, vq, vr, vlast); break;
+            default: two_issue<FX, FY, SPLIT, WIDE, BIG, 3>(slot, src, split, vq, vr, vlast); break;
             }
         } else {
             switch (wv) {
-            case 0: two_issue<FX, FY, SPLIT, WIDE, BIG, 0>(slot, src, split, vq); break;
-            case 1: two_issue<FX, FY, SPLIT, WIDE, BIG, 1>(slot, src, split, vq); break;
-            case 2: two_issue<FX, FY, SPLIT, WIDE, BIG, 2>(slot, src, split, vq); break;
-            case 3: two_issue<FX, FY, SPLIT, WIDE, BIG, 3>(slot, src, split, vq); break;
-            case 4: two_issue<FX, FY, SPLIT, WIDE, BIG, 4>(slot, src, split, vq); break;
-            case 5: two_issue<FX, FY, SPLIT, WIDE, BIG, 5>(slot, src, split, vq); break;
-            case 6: two_issue<FX, FY, SPLIT, WIDE, BIG, 6>(slot, src, split, vq); break;
-            default: two_issue<FX, FY, SPLIT, WIDE, BIG, 7>(slot, src, split, vq); break;
+            case 0: two_issue<FX, FY, SPLIT, WIDE, BIG, 0>(slot, src, split, vq, vr, vlast); break;
+            case 1: two_issue<FX, FY, SPLIT, WIDE, BIG, 1>(slot, src, split, vq, vr, vlast); break;
+            case 2: two_issue<FX, FY, SPLIT, WIDE, BIG, 2>(slot, src, split, vq, vr, vlast); break;
+            case 3: two_issue<FX, FY, SPLIT, WIDE, BIG, 3>(slot, src, split, vq, vr, vlast); break;
+            case 4: two_issue<FX, FY, SPLIT, WIDE, BIG, 4>(slot, src, split, vq, vr, vlast); break;
+            case 5: two_issue<FX, FY, SPLIT, WIDE, BIG, 5>(slot, src, split, vq, vr, vlast); break;
+            case 6: two_issue<FX, FY, SPLIT, WIDE, BIG, 6>(slot, src, split, vq, vr, vlast); break;
+            default: two_issue<FX, FY, SPLIT, WIDE, BIG, 7>(slot, src, split, vq, vr, vlast); break;
             }
         }
     };
     /* stage arrays: 0 = E0, 1 = H0, 2 = bH, 3 = bE, 4 = SH0, 5 = SE0 */
-    auto st = [&](const float *S, int arr, int c, int e) { return S[arr * Ly::ARR + c * Ly::NC * Ly::CH + e]; };
+    auto st = [&](const float *S, int arr, int c, int e) {
+        return S[arr * Ly::ARR + c * Ly::NC * Ly::CH + e - Ly::row0(arr) * W4];
+    };
     /* PML factors of cell r on the plane of stage s: H (integer samples) or E (half) */
     auto factors = [&](int r, int s, bool e_field, float pa[3], float pg[3]) {
         const float *tt = e_field ? tE[r] : tH[r];
