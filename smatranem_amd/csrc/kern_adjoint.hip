@@ -680,6 +680,10 @@ constexpr int L_UV1 = 2 * SLOT, L_UE1 = L_UV1 + 4 * N3, L_UV2 = L_UE1 + 4 * N2, 
 constexpr int L_FLAG = L_ZT + 5 * (kMaxFZ + 4), L_END = L_FLAG + 1;
 static_assert(4 * L_END <= 81920, "two blocks per CU");
 static_assert(2 * kMinOps < 64, "copies in flight must fit the vmcnt field");
+/* lambda_TH, bH and lambda_SH are read on G3 only: their copies start at the second row of
+ * G4 and their LDS words count from there (the same number of runs, N4 - W4 words) */
+constexpr int row0(int arr) { return arr == 1 || arr == 2 || arr == 4 ? 1 : 0; }
+static_assert((N4 - W4 + 63) / 64 == NC, "runs of the arrays that start a row down");
 /* compact arrays: the four values read one cell away */
 enum { A0 = 0, A2 = 1, B1 = 2, B2 = 3 }; /* U0|u0, U2|u2, V1|v1, V2|v2 */
 } // namespace adj2
@@ -689,7 +693,8 @@ enum { A0 = 0, A2 = 1, B1 = 2, B2 = 3 }; /* U0|u0, U2|u2, V1|v1, V2|v2 */
  * 3 components x NC runs of 64 words of G4. */
 template <int W>
 __device__ inline void adj2_issue(float *slot, __amdgpu_buffer_rsrc_t rT, __amdgpu_buffer_rsrc_t rB,
-                                  __amdgpu_buffer_rsrc_t rS, unsigned cb, unsigned soff, bool split, const unsigned *vq)
+                                  __amdgpu_buffer_rsrc_t rS, unsigned cb, unsigned soff, bool split, const unsigned *vq,
+                                  const unsigned *vr)
 {
     using namespace adj2;
 #pragma unroll
@@ -697,14 +702,16 @@ __device__ inline void adj2_issue(float *slot, __amdgpu_buffer_rsrc_t rT, __amdg
         const int arr = j / (3 * NC), r = j % (3 * NC), c = r / NC, ch = r % NC;
         const unsigned comp = (arr == 0 || arr == 3) ? 3 + c : c;
         float *dst = slot + arr * ARR + (c * NC + ch) * 64;
-        __builtin_amdgcn_raw_ptr_buffer_load_lds(arr < 2 ? rT : rB, SMX_LDS(dst), 4, vq[ch] + comp * cb, soff, 0, 0);
+        const unsigned voff = row0(arr) ? vr[ch] : vq[ch];
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(arr < 2 ? rT : rB, SMX_LDS(dst), 4, voff + comp * cb, soff, 0, 0);
     }
     if (split) {
 #pragma unroll
         for (int j = W; j < NSPLIT; j += NW) {
             const int arr = j / (3 * NC), r = j % (3 * NC), c = r / NC, ch = r % NC;
             float *dst = slot + (4 + arr) * ARR + (c * NC + ch) * 64;
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(rS, SMX_LDS(dst), 4, vq[ch] + (arr * 3 + c) * cb, soff, 0, 0);
+            const unsigned voff = row0(4 + arr) ? vr[ch] : vq[ch];
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rS, SMX_LDS(dst), 4, voff + (arr * 3 + c) * cb, soff, 0, 0);
         }
     }
 }
@@ -735,17 +742,23 @@ __global__ __launch_bounds__(adj2::NT) void adj_two_f32(smx_adj_two_args a)
     const int id = a.blocks[xcd_logical(blockIdx.x, gridDim.x)];
     const int bxi = id % a.gx, rest = id / a.gx;
     const int byi = rest % a.gy, bzi = rest / a.gy;
-    const int x0 = bxi * FX, y0 = byi * FY, z0 = bzi * a.fz, z1 = min(z0 + a.fz, a.L2);
+    /* x tiles start OX columns right of a multiple of 32, so that every row of the stage
+     * region spans two 128-byte lines; the last one wraps around (its cells from L0 + OX
+     * on are computed, not stored) */
+    const int x0 = OX + bxi * FX, y0 = byi * FY, z0 = bzi * a.fz, z1 = min(z0 + a.fz, a.L2);
     const int t = threadIdx.x, lane = t & 63;
     const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
     const int ns = z1 - z0 + 4; /* stages: planes z0-2 .. z1+1 */
     auto plane_of = [&](int s) { return wrap_index(z0 - 2 + s, a.L2); };
 
-    unsigned vq[NC];
+    unsigned vq[NC], vr[NC];
+    auto region_off = [&](int e) {
+        return 4u * (unsigned)(wrap_index(x0 - OX + e % W4, a.L0) + a.L0 * wrap_index(y0 - 2 + e / W4, a.L1));
+    };
 #pragma unroll
     for (int ch = 0; ch < NC; ch++) {
-        const int e = min(ch * 64 + lane, N4 - 1);
-        vq[ch] = 4u * (unsigned)(wrap_index(x0 - OX + e % W4, a.L0) + a.L0 * wrap_index(y0 - 2 + e / W4, a.L1));
+        vq[ch] = region_off(min(ch * 64 + lane, N4 - 1));
+        vr[ch] = region_off(min(W4 + ch * 64 + lane, N4 - 1)); /* from the second row */
     }
 
     /* this thread's cells: r = 0 the tile cell, r = 1 one cell of the rings G1\T, G2\G1,
@@ -797,7 +810,7 @@ __global__ __launch_bounds__(adj2::NT) void adj_two_f32(smx_adj_two_args a)
         cx[r] = wrap_index(x0 - OX + gx, a.L0);
         cy[r] = wrap_index(y0 - 2 + gy, a.L1);
         cq[r] = cx[r] + (int64_t)a.L0 * cy[r];
-        own[r] = r == 0 && x0 - OX + gx < a.L0 && y0 - 2 + gy < a.L1;
+        own[r] = r == 0 && x0 - OX + gx < a.L0 + OX && y0 - 2 + gy < a.L1;
         pxy[r] = ring < 4 && a.has_pml && (a.pmlflag[cx[r]] | a.pmlflag[a.L0 + cy[r]]);
         const int xm = cx[r] == 0 ? a.L0 - 1 : cx[r] - 1, ym = cy[r] == 0 ? a.L1 - 1 : cy[r] - 1;
         tH[r][0] = tx[cx[r]], tH[r][1] = tx[a.L0 + cx[r]], tH[r][2] = ty[cy[r]], tH[r][3] = ty[a.L1 + cy[r]];
@@ -839,14 +852,14 @@ __global__ __launch_bounds__(adj2::NT) void adj_two_f32(smx_adj_two_args a)
         const bool split = tile_pml || zpml(s);
         float *slot = lds + (s & 1) * SLOT;
         switch (wv) {
-        case 0: adj2_issue<0>(slot, rT, rB, rS, cb, soff, split, vq); break;
-        case 1: adj2_issue<1>(slot, rT, rB, rS, cb, soff, split, vq); break;
-        case 2: adj2_issue<2>(slot, rT, rB, rS, cb, soff, split, vq); break;
-        default: adj2_issue<3>(slot, rT, rB, rS, cb, soff, split, vq); break;
+        case 0: adj2_issue<0>(slot, rT, rB, rS, cb, soff, split, vq, vr); break;
+        case 1: adj2_issue<1>(slot, rT, rB, rS, cb, soff, split, vq, vr); break;
+        case 2: adj2_issue<2>(slot, rT, rB, rS, cb, soff, split, vq, vr); break;
+        default: adj2_issue<3>(slot, rT, rB, rS, cb, soff, split, vq, vr); break;
         }
     };
     /* stage arrays: 0 lambda_TE, 1 lambda_TH, 2 bH, 3 bE, 4 lambda_SH, 5 lambda_SE */
-    auto st = [&](const float *S, int arr, int c, int e) { return S[arr * ARR + c * NC * 64 + e]; };
+    auto st = [&](const float *S, int arr, int c, int e) { return S[arr * ARR + c * NC * 64 + e - row0(arr) * W4]; };
     float *const UV1 = lds + L_UV1, *const UE1 = lds + L_UE1, *const UV2 = lds + L_UV2;
 
     issue(0, plane_of(0));
