@@ -674,46 +674,53 @@ constexpr int W3 = FX + 3, N3 = W3 * (FY + 3);        /* G3 = [1, W4) x [1, H4) 
 constexpr int W2 = FX + 2, N2 = W2 * (FY + 2);        /* G2 = [1, W4-1) x [1, H4-1) */
 constexpr int W1 = FX + 1, N1 = W1 * (FY + 1);        /* G1 = [2, W4-1) x [2, H4-1) */
 constexpr int NC = (N4 + 63) / 64, ARR = 3 * NC * 64, SLOT = 6 * ARR;
-constexpr int NBASE = 4 * 3 * NC, NSPLIT = 2 * 3 * NC, kMinOps = NBASE / NW;
+/* rows of G4 each array component k = 3 arr + c is read on (arrays 0 lambda_TE, 1
+ * lambda_TH, 2 bH, 3 bE, 4 lambda_SH, 5 lambda_SE): the E side on all of G4, but its y
+ * components (read one row up only for the x and z gathers) from the second row; the H
+ * side on G3, from the second row, its components that the last row's U0 and V2 do not
+ * need (lambda_TH and bH y, lambda_SH x and y) a row short.  A component's copies start
+ * at its first row, and its LDS words count from there. */
+constexpr int row0(int k) { return k / 3 == 0 || k / 3 == 3 || k / 3 == 5 ? (k % 3 == 1 ? 1 : 0) : 1; }
+constexpr int rows(int k)
+{
+    return k / 3 == 0 || k / 3 == 3 || k / 3 == 5 ? (k % 3 == 1 ? H4 - 1 : H4)
+           : k == 4 || k == 7 || k == 12 || k == 13 ? H4 - 2
+                                                    : H4 - 1;
+}
+constexpr int nca(int k) { return (rows(k) * W4 + 63) / 64; }
+constexpr int cum(int k) { return k == 0 ? 0 : cum(k - 1) + nca(k - 1); }
+constexpr int kof_from(int j, int k) { return k >= 17 || j < cum(k + 1) ? 0 : 1 + kof_from(j, k + 1); }
+constexpr int kof(int j) { return j < cum(1) ? 0 : 1 + kof_from(j, 1); } /* the component of copy j */
+constexpr int NBASE = cum(12), NSPLIT = cum(18) - cum(12), kMinOps = NBASE / NW;
 constexpr int kMaxFZ = 48;
 constexpr int L_UV1 = 2 * SLOT, L_UE1 = L_UV1 + 4 * N3, L_UV2 = L_UE1 + 4 * N2, L_ZT = L_UV2 + 4 * N1;
 constexpr int L_FLAG = L_ZT + 5 * (kMaxFZ + 4), L_END = L_FLAG + 1;
 static_assert(4 * L_END <= 81920, "two blocks per CU");
 static_assert(2 * kMinOps < 64, "copies in flight must fit the vmcnt field");
-/* lambda_TH, bH and lambda_SH are read on G3 only: their copies start at the second row of
- * G4 and their LDS words count from there (the same number of runs, N4 - W4 words) */
-constexpr int row0(int arr) { return arr == 1 || arr == 2 || arr == 4 ? 1 : 0; }
-static_assert((N4 - W4 + 63) / 64 == NC, "runs of the arrays that start a row down");
 /* compact arrays: the four values read one cell away */
 enum { A0 = 0, A2 = 1, B1 = 2, B2 = 3 }; /* U0|u0, U2|u2, V1|v1, V2|v2 */
 } // namespace adj2
 
 /* Copies of one stage dealt to wave W: lambda_TE (T comps 3..5), lambda_TH (T 0..2), bH
  * (b 0..2), bE (b 3..5), and in PML stages lambda_SH (S 0..2), lambda_SE (S 3..5), each
- * 3 components x NC runs of 64 words of G4. */
+ * component in nca(k) runs of 64 words of its rows of G4 (vq: from the first row; vr:
+ * from the second; vlast: the last run of the components that stop a row short). */
 template <int W>
 __device__ inline void adj2_issue(float *slot, __amdgpu_buffer_rsrc_t rT, __amdgpu_buffer_rsrc_t rB,
                                   __amdgpu_buffer_rsrc_t rS, unsigned cb, unsigned soff, bool split, const unsigned *vq,
-                                  const unsigned *vr)
+                                  const unsigned *vr, unsigned vlast)
 {
-    using namespace adj2;
-#pragma unroll
-    for (int j = W; j < NBASE; j += NW) {
-        const int arr = j / (3 * NC), r = j % (3 * NC), c = r / NC, ch = r % NC;
-        const unsigned comp = (arr == 0 || arr == 3) ? 3 + c : c;
+    auto one = [&](auto jc) {
+        using namespace adj2;
+        constexpr int j = decltype(jc)::value, k = kof(j), arr = k / 3, c = k % 3, ch = j - cum(k);
+        constexpr unsigned comp = arr >= 4 ? (arr - 4) * 3 + c : (arr == 0 || arr == 3) ? 3 + c : c;
         float *dst = slot + arr * ARR + (c * NC + ch) * 64;
-        const unsigned voff = row0(arr) ? vr[ch] : vq[ch];
-        __builtin_amdgcn_raw_ptr_buffer_load_lds(arr < 2 ? rT : rB, SMX_LDS(dst), 4, voff + comp * cb, soff, 0, 0);
-    }
-    if (split) {
-#pragma unroll
-        for (int j = W; j < NSPLIT; j += NW) {
-            const int arr = j / (3 * NC), r = j % (3 * NC), c = r / NC, ch = r % NC;
-            float *dst = slot + (4 + arr) * ARR + (c * NC + ch) * 64;
-            const unsigned voff = row0(4 + arr) ? vr[ch] : vq[ch];
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(rS, SMX_LDS(dst), 4, voff + (arr * 3 + c) * cb, soff, 0, 0);
-        }
-    }
+        const unsigned voff = row0(k) == 0 ? vq[ch] : rows(k) == H4 - 2 && ch == nca(k) - 1 ? vlast : vr[ch];
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(arr < 2 ? rT : arr < 4 ? rB : rS, SMX_LDS(dst), 4, voff + comp * cb,
+                                                 soff, 0, 0);
+    };
+    static_for<W, adj2::NBASE, adj2::NW>(one);
+    if (split) static_for<adj2::NBASE + W, adj2::NBASE + adj2::NSPLIT, adj2::NW>(one);
 }
 
 /* Own-cell transform of a PML cell: main aM x, split (aS - aM) x + aS s. */
@@ -751,7 +758,7 @@ __global__ __launch_bounds__(adj2::NT) void adj_two_f32(smx_adj_two_args a)
     const int ns = z1 - z0 + 4; /* stages: planes z0-2 .. z1+1 */
     auto plane_of = [&](int s) { return wrap_index(z0 - 2 + s, a.L2); };
 
-    unsigned vq[NC], vr[NC];
+    unsigned vq[NC], vr[NC], vlast;
     auto region_off = [&](int e) {
         return 4u * (unsigned)(wrap_index(x0 - OX + e % W4, a.L0) + a.L0 * wrap_index(y0 - 2 + e / W4, a.L1));
     };
@@ -760,6 +767,7 @@ __global__ __launch_bounds__(adj2::NT) void adj_two_f32(smx_adj_two_args a)
         vq[ch] = region_off(min(ch * 64 + lane, N4 - 1));
         vr[ch] = region_off(min(W4 + ch * 64 + lane, N4 - 1)); /* from the second row */
     }
+    vlast = region_off(min(W4 + (nca(4) - 1) * 64 + lane, N4 - W4 - 1)); /* ... to the second-last */
 
     /* this thread's cells: r = 0 the tile cell, r = 1 one cell of the rings G1\T, G2\G1,
      * G3\G2 (ring cell h on thread (h % 4)*64 + h/4).  Local coordinates count from the
@@ -852,14 +860,14 @@ __global__ __launch_bounds__(adj2::NT) void adj_two_f32(smx_adj_two_args a)
         const bool split = tile_pml || zpml(s);
         float *slot = lds + (s & 1) * SLOT;
         switch (wv) {
-        case 0: adj2_issue<0>(slot, rT, rB, rS, cb, soff, split, vq, vr); break;
-        case 1: adj2_issue<1>(slot, rT, rB, rS, cb, soff, split, vq, vr); break;
-        case 2: adj2_issue<2>(slot, rT, rB, rS, cb, soff, split, vq, vr); break;
-        default: adj2_issue<3>(slot, rT, rB, rS, cb, soff, split, vq, vr); break;
+        case 0: adj2_issue<0>(slot, rT, rB, rS, cb, soff, split, vq, vr, vlast); break;
+        case 1: adj2_issue<1>(slot, rT, rB, rS, cb, soff, split, vq, vr, vlast); break;
+        case 2: adj2_issue<2>(slot, rT, rB, rS, cb, soff, split, vq, vr, vlast); break;
+        default: adj2_issue<3>(slot, rT, rB, rS, cb, soff, split, vq, vr, vlast); break;
         }
     };
     /* stage arrays: 0 lambda_TE, 1 lambda_TH, 2 bH, 3 bE, 4 lambda_SH, 5 lambda_SE */
-    auto st = [&](const float *S, int arr, int c, int e) { return S[arr * ARR + c * NC * 64 + e - row0(arr) * W4]; };
+    auto st = [&](const float *S, int arr, int c, int e) { return S[arr * ARR + c * NC * 64 + e - row0(3 * arr + c) * W4]; };
     float *const UV1 = lds + L_UV1, *const UE1 = lds + L_UE1, *const UV2 = lds + L_UV2;
 
     issue(0, plane_of(0));

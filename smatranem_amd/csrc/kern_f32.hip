@@ -1297,15 +1297,29 @@ struct two_layout {
     static constexpr int NC = (N4 + CH - 1) / CH;          /* copies per component */
     static constexpr int ARR = 3 * NC * CH;                /* one array (3 components) of a stage */
     static constexpr int SLOT = (SPLIT ? 6 : 4) * ARR;     /* E0, H0, bH, bE [, SH0, SE0] */
-    /* rows of the G4 region each array is read on: E0 all of G4; H0, bH, SH0 on G3 (the
-     * first row left out); bE, SE0 on G2 (the first and the last row left out).  An array's
-     * copies start at its first row, and its LDS words count from there */
-    static constexpr int row0(int arr) { return WIDE || arr == 0 ? 0 : 1; }
-    static constexpr int rows(int arr) { return WIDE || arr == 0 ? H4 : arr == 3 || arr == 5 ? H4 - 2 : H4 - 1; }
-    static constexpr int nca(int arr) { return WIDE ? NC : (rows(arr) * W4 + CH - 1) / CH; }
-    static constexpr int cum(int arr) { return arr == 0 ? 0 : cum(arr - 1) + 3 * nca(arr - 1); }
-    static constexpr int NBASE = cum(4);                   /* copies of every stage */
-    static constexpr int NSPLIT = cum(6) - cum(4);         /* extra copies of PML stages */
+    /* rows of the G4 region each array component k = 3 arr + c is read on: E0 on all of
+     * G4, but its y component (read one row up only for Ex and Ez) from the second row;
+     * H0, bH, SH0 on G3, from the second row, their y components (whose H1 the last row
+     * does not need: that row is read only for Hx and Hz) a row short; bE, SE0 on G2, the
+     * second row to the second-last.  A component's copies start at its first row, and
+     * its LDS words count from there */
+    static constexpr int row0(int k) { return WIDE || k == 0 || k == 2 ? 0 : 1; }
+    static constexpr int rows(int k)
+    {
+        return WIDE || k == 0 || k == 2 ? H4
+               : k == 1                 ? H4 - 1
+               : k >= 9 && k < 12       ? H4 - 2 /* bE */
+               : k >= 15                ? H4 - 2 /* SE0 */
+               : k % 3 == 1             ? H4 - 2 /* H0, bH, SH0: y */
+                                        : H4 - 1;
+    }
+    static constexpr int nca(int k) { return WIDE ? NC : (rows(k) * W4 + CH - 1) / CH; }
+    static constexpr int cum(int k) { return k == 0 ? 0 : cum(k - 1) + nca(k - 1); }
+    /* the component copy j of a stage belongs to */
+    static constexpr int kof(int j) { return j < cum(1) ? 0 : 1 + kof_from(j, 1); }
+    static constexpr int kof_from(int j, int k) { return k >= 17 || j < cum(k + 1) ? 0 : 1 + kof_from(j, k + 1); }
+    static constexpr int NBASE = cum(12);                  /* copies of every stage */
+    static constexpr int NSPLIT = cum(18) - cum(12);       /* extra copies of PML stages */
     static constexpr int kMinOps = NBASE / NW;             /* per wave, at least */
     static constexpr int CELLS = 2;                        /* a tile cell and a ring cell */
     static constexpr int kMaxFZ = 80;                      /* z tables of a chunk in LDS */
@@ -1345,46 +1359,25 @@ __device__ inline void two_issue(float *slot, const two_src &q, bool split, cons
                                  unsigned vlast)
 {
     using Ly = two_layout<FX, FY, SPLIT, WIDE>;
-    const unsigned so = BIG ? 0u : q.soff;
-    /* the copy's size must be a literal */
-    auto copy = [](__amdgpu_buffer_rsrc_t r, float *dst, unsigned voff, unsigned so) {
-        if constexpr (WIDE) __builtin_amdgcn_raw_ptr_buffer_load_lds(r, SMX_LDS(dst), 16, voff, so, 0, 0);
-        else __builtin_amdgcn_raw_ptr_buffer_load_lds(r, SMX_LDS(dst), 4, voff, so, 0, 0);
-    };
-    /* copy j of a stage: its array, component and run */
-    auto which = [](int j, int &arr, int &c, int &ch) {
-        arr = 0;
-        while (arr < 5 && j >= Ly::cum(arr + 1)) arr++;
-        const int r = j - Ly::cum(arr);
-        c = r / Ly::nca(arr);
-        ch = r % Ly::nca(arr);
-    };
-    auto offset = [&](int arr, int ch) {
-        if (Ly::row0(arr) == 0) return vq[ch];
-        return Ly::rows(arr) == Ly::H4 - 2 && ch == Ly::nca(arr) - 1 ? vlast : vr[ch];
-    };
-#pragma unroll
-    for (int j = W; j < Ly::NBASE; j += Ly::NW) {
-        int arr, c, ch;
-        which(j, arr, c, ch);
-        /* E0 = T comp 3+c, H0 = T comp c, bH = b comp c, bE = b comp 3+c */
-        const unsigned comp = (arr == 0 || arr == 3) ? 3 + c : c;
+    /* copy j of a stage: component k = 3 arr + c, run ch (all known at compile time) */
+    auto one = [&](auto jc) {
+        const unsigned so = BIG ? 0u : q.soff;
+        constexpr int j = decltype(jc)::value, k = Ly::kof(j), arr = k / 3, c = k % 3, ch = j - Ly::cum(k);
+        /* E0 = T comp 3+c, H0 = T comp c, bH = b comp c, bE = b comp 3+c, SH0 = S comp c,
+         * SE0 = S comp 3+c */
+        constexpr unsigned comp = arr >= 4 ? (arr - 4) * 3 + c : (arr == 0 || arr == 3) ? 3 + c : c;
+        constexpr int group = arr < 2 ? 0 : arr < 4 ? 1 : 2;
         float *dst = slot + arr * Ly::ARR + (c * Ly::NC + ch) * Ly::CH;
-        unsigned voff = offset(arr, ch);
-        const __amdgpu_buffer_rsrc_t res = two_res<BIG>(q, arr < 2 ? 0 : 1, comp, voff);
-        copy(res, dst, voff, so);
-    }
-    if (SPLIT && split) {
-#pragma unroll
-        for (int j = Ly::NBASE + W; j < Ly::NBASE + Ly::NSPLIT; j += Ly::NW) {
-            int arr, c, ch;
-            which(j, arr, c, ch);
-            float *dst = slot + arr * Ly::ARR + (c * Ly::NC + ch) * Ly::CH; /* SH0 = S comp c, SE0 = 3+c */
-            unsigned voff = offset(arr, ch);
-            const __amdgpu_buffer_rsrc_t res = two_res<BIG>(q, 2, (arr - 4) * 3 + c, voff);
-            copy(res, dst, voff, so);
-        }
-    }
+        unsigned voff = Ly::row0(k) == 0                                      ? vq[ch]
+                        : Ly::rows(k) == Ly::H4 - 2 && ch == Ly::nca(k) - 1 ? vlast
+                                                                               : vr[ch];
+        const __amdgpu_buffer_rsrc_t res = two_res<BIG>(q, group, comp, voff);
+        /* the copy's size must be a literal */
+        if constexpr (WIDE) __builtin_amdgcn_raw_ptr_buffer_load_lds(res, SMX_LDS(dst), 16, voff, so, 0, 0);
+        else __builtin_amdgcn_raw_ptr_buffer_load_lds(res, SMX_LDS(dst), 4, voff, so, 0, 0);
+    };
+    static_for<W, Ly::NBASE, Ly::NW>(one);
+    if (SPLIT && split) static_for<Ly::NBASE + W, Ly::NBASE + Ly::NSPLIT, Ly::NW>(one);
 }
 
 /* New E of one regular cell: lossless X += b*(P + M), or the PML main/split pair. */
@@ -1525,7 +1518,7 @@ __device__ __forceinline__ void two_step_body(const A &a, int id, const double *
         /* the arrays read from the second row on (from the third-last: to the last row) */
         vr[ch] = WIDE ? vq[ch] : region_off(min(W4 + ch * 64 + lane, N4 - 1));
     }
-    if constexpr (!WIDE) vlast = region_off(min(W4 + (Ly::nca(3) - 1) * 64 + lane, N4 - W4 - 1));
+    if constexpr (!WIDE) vlast = region_off(min(W4 + (Ly::nca(9) - 1) * 64 + lane, N4 - W4 - 1));
 
     /* this thread's cells: r = 0 the tile cell t (every step computes it); r = 1 one cell
      * of the rings the first three updates add around the tile -- G1\T (used by c, b,
@@ -1656,7 +1649,7 @@ __device__ __forceinline__ void two_step_body(const A &a, int id, const double *
     };
     /* stage arrays: 0 = E0, 1 = H0, 2 = bH, 3 = bE, 4 = SH0, 5 = SE0 */
     auto st = [&](const float *S, int arr, int c, int e) {
-        return S[arr * Ly::ARR + c * Ly::NC * Ly::CH + e - Ly::row0(arr) * W4];
+        return S[arr * Ly::ARR + c * Ly::NC * Ly::CH + e - Ly::row0(3 * arr + c) * W4];
     };
     /* PML factors of cell r on the plane of stage s: H (integer samples) or E (half) */
     auto factors = [&](int r, int s, bool e_field, float pa[3], float pg[3]) {

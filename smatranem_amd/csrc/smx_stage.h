@@ -1,12 +1,15 @@
 /*
  * smx_stage.h -- device helpers shared by the LDS-staged z-march kernels (kern_f32.hip,
  * kern_adjoint.hip): XCD-aware block numbering, index wrap, buffer resources, the
- * asynchronous HBM -> LDS copies and the counted waits that order them.
+ * asynchronous HBM -> LDS copies, a compile-time loop over them and the counted waits
+ * that order them.
  */
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <type_traits>
 
 namespace {
 
@@ -49,6 +52,18 @@ __device__ inline void wait_vm_barrier()
     __builtin_amdgcn_s_waitcnt((VM & 0xF) | ((VM >> 4) << 14) | (0x7 << 4));
     __builtin_amdgcn_s_barrier();
     asm volatile("" ::: "memory");
+}
+
+/* f(integral_constant<int, J>) for J = J0, J0 + STEP, ... below END, unrolled at compile
+ * time (the copies of a stage: which array, component and run a copy takes is then a
+ * constant, and the run offsets stay in registers) */
+template <int J, int END, int STEP, typename F>
+__device__ __forceinline__ void static_for(F &f)
+{
+    if constexpr (J < END) {
+        f(std::integral_constant<int, J>{});
+        static_for<J + STEP, END, STEP>(f);
+    }
 }
 
 /* Wait for every outstanding vector memory operation of this wave.  Used right after
