@@ -299,6 +299,28 @@ def test_two_step_equals_one_step_kernels(variant, monkeypatch):
     assert rel_l2(out[0][0], out[1][0]) < 1e-5 and rel_l2(out[0][1], out[1][1]) < 1e-5
 
 
+@pytest.mark.parametrize("L0,L1", [(40, 20), (24, 36), (64, 13)])
+def test_two_step_stage_rows(L0, L1, monkeypatch):
+    """The 4-byte-copy variants stage each array component only on the rows of the stage
+    region its values are read on; the 16-byte-copy variant stages every row.  On a
+    PML-free grid all of them run the same per-cell arithmetic, so their fields agree bit
+    for bit, with x tiles wrapping (L0 not a multiple of 32, or below one tile) and y
+    tiles past the grid (L1 not a multiple of 8)."""
+    prep = casegen.prepare(small_waveguide(seed=21, L0=L0, L1=L1, L2=30, outer=8, pml_xy=0, pml_z=0,
+                                           iBc=(1, -1, 1), metal_cells=0, lossy_cells=0))
+    out = {}
+    for two in ("3,16", "0,16", "2,16", "2,5"):
+        monkeypatch.setenv("SMX_F32_TWO", two)
+        with prep.plan(fdtd.F32) as p:
+            assert "two_step_f32<32,8,1," + ("false,true" if two[0] == "3" else
+                                              "true,false" if two[0] == "2" else "false,false") in p.kernels
+            p.run(0, prep.case.time_period[2])
+            out[two] = p.read_fields()
+    assert np.abs(out["3,16"][1]).max() > 0
+    for two in ("0,16", "2,16", "2,5"):
+        assert _eq(out[two][0], out["3,16"][0]) and _eq(out[two][1], out["3,16"][1]), two
+
+
 @pytest.mark.parametrize("two,kw", [
     ("0,16", dict(pml_xy=0, pml_z=3, iBc=(1, -1, 0))),                 # z PML, both monitors in the box
     ("1,8", dict(pml_xy=0, pml_z=3, monitor_gap=0, iBc=(0, 0, 0))),    # monitor planes in the region sweeps
