@@ -1326,9 +1326,9 @@ struct two_layout {
 };
 
 /* Copies of one stage dealt to wave W: arrays E0, H0, bH, bE (and SH0, SE0 when `split`),
- * each 3 components x nca(arr) runs of CH words of the rows of the G4 region the array is
- * read on (vq: from the first row; vr: from the second; vlast: the last run of the arrays
- * that stop a row short of the region). */
+ * each component k in nca(k) runs of CH words of the rows of the G4 region it is read on
+ * (vq: from the first row; vr: from the second; vlast: the last run of the components that
+ * also end a row before the region's last). */
 /* Where a stage's copies read: one buffer resource per group of 6 component arrays
  * (component c at c*cb bytes, the plane at soff), or -- with BIG, for groups of 4 GiB and
  * more -- a resource built per copy on one array's plane (base + comp*ncomp + zoff
