@@ -676,7 +676,8 @@ constexpr int W1 = FX + 1, N1 = W1 * (FY + 1);        /* G1 = [2, W4-1) x [2, H4
 constexpr int NC = (N4 + 63) / 64, ARR = 3 * NC * 64, SLOT = 6 * ARR;
 /* rows of G4 each array component k = 3 arr + c is read on (arrays 0 lambda_TE, 1
  * lambda_TH, 2 bH, 3 bE, 4 lambda_SH, 5 lambda_SE): the E side on all of G4, but its y
- * components (read one row up only for the x and z gathers) from the second row; the H
+ * components (a cell's -y neighbour is read only for its x and z components) from the
+ * second row; the H
  * side on G3, from the second row, its components that the last row's U0 and V2 do not
  * need (lambda_TH and bH y, lambda_SH x and y) a row short.  A component's copies start
  * at its first row, and its LDS words count from there. */
