@@ -29,6 +29,25 @@ namespace {
 
 constexpr int kThreads = 256;
 
+/* The point source's term of the step (smx_dft_sample_args::pt_amp), rounded as the
+ * oracle rounds it: no contraction into a fused multiply-add. */
+__device__ inline void point_term(const smx_dft_sample_args &a)
+{
+#pragma clang fp contract(off)
+    if (a.pt_dst64) {
+        const double t = a.pt_coef64[0] * a.pt_amp[a.pt_k];
+        a.pt_dst64[0] = a.pt_dst64[0] + t;
+    } else {
+        const float t = a.pt_coef32[0] * (float)a.pt_amp[a.pt_k];
+        a.pt_dst32[0] = a.pt_dst32[0] + t;
+    }
+}
+
+__device__ inline bool point_thread(const smx_dft_sample_args &a, int64_t i, int m, int64_t q)
+{
+    return a.pt_amp && (a.pt_m < 0 ? i == 0 : m == a.pt_m && q == a.pt_q);
+}
+
 /* series layout: [t][plane][comp][q], comp = Hx, Hy, Ex, Ey */
 template <typename T>
 __global__ __launch_bounds__(kThreads) void sample_planes_f32(smx_dft_sample_args a, int slot)
@@ -40,6 +59,7 @@ __global__ __launch_bounds__(kThreads) void sample_planes_f32(smx_dft_sample_arg
     const int64_t q = i - m * per_plane;
     const int64_t l = q + per_plane * a.planes[m];
     const int x = (int)(q % a.L0), y = (int)(q / a.L0), z = a.planes[m];
+    if (point_thread(a, i, m, q)) point_term(a); /* before this thread's own samples */
     if (a.mask_on) {
         auto in_mod = [](int v, int lo, int len, int L) { return ((v - lo) % L + L) % L < len; };
         if (!(in_mod(x, a.mask[0], a.mask[1], a.L0) && in_mod(y, a.mask[2], a.mask[3], a.L1) &&
@@ -69,6 +89,7 @@ __global__ __launch_bounds__(kThreads) void sample_planes_f64(smx_dft_sample_arg
     const int m = (int)(i / per_plane);
     const int64_t q = i - m * per_plane;
     const int64_t l = q + per_plane * a.planes[m];
+    if (point_thread(a, i, m, q)) point_term(a); /* before this thread's own samples */
     double *dst = (double *)a.series + ((int64_t)slot * a.nplanes + m) * 4 * per_plane + q;
     double v[4] = {a.H64[0 + 3 * l], a.H64[1 + 3 * l], a.E64[0 + 3 * l], a.E64[1 + 3 * l]};
     if (a.cont[l] == 'a') { /* PML: main + split = total field */

@@ -29,6 +29,19 @@ struct smx_dft_sample_args {
     /* FP32 plan, mask_on: only the cells in the ranges [mask[2d], mask[2d] + mask[2d+1]) of
      * each axis d, taken modulo its length */
     int mask_on, mask[6];
+    /* a point source's term of this step, applied before the samples when pt_amp is set:
+     * dst += coef * pt_amp[pt_k] (FP64: pt_dst64, pt_coef64; FP32: pt_dst32, pt_coef32, the
+     * amplitude rounded to FP32), a separate multiply and add, in the thread that samples
+     * cell pt_q of ring plane pt_m, or in thread 0 when pt_m < 0 (no ring plane holds the
+     * source's cell) */
+    const double *pt_amp;
+    long long pt_k;
+    int pt_m;
+    int64_t pt_q;
+    double *pt_dst64;
+    const double *pt_coef64;
+    float *pt_dst32;
+    const float *pt_coef32;
 };
 
 struct smx_dft_fold_args {

@@ -338,6 +338,24 @@ void mark_mon_slot(smx_plan *p, int slot) { p->dft_mon_mask[slot / 64] |= (uint6
 
 /* Sample the ring's planes after run step k (steps must come in order within a chunk);
  * mon: the ring carries this step's monitors. */
+/* The ring plane that holds the point source's cell (-1: none), or -2 when several do
+ * (the sample kernel then cannot apply the source's term before every sample of it). */
+int point_ring_plane(const smx_plan *p)
+{
+    const int z = (int)(p->pt_cell / p->plane);
+    int m = -1;
+    for (int i = 0; i < p->dft_ring; i++)
+        if (p->dft_planes[i] == z) m = m == -1 ? i : -2;
+    return m;
+}
+
+/* Whether the DFT sample kernel applies the point source's term of each step (one launch
+ * less per step): with a DFT set and the source's cell on at most one ring plane. */
+bool point_in_sample(const smx_plan *p)
+{
+    return p->pt_comp >= 0 && p->dft_nplanes > 0 && point_ring_plane(p) != -2;
+}
+
 int dft_after_step(smx_plan *p, long long k, bool mon)
 {
     if (!p->dft_nplanes) return SMX_OK;
@@ -372,6 +390,21 @@ int dft_after_step(smx_plan *p, long long k, bool mon)
         a.special_stride = (int)(sizeof(smx_special_f32) / sizeof(int));
     }
     a.series = p->dft_series;
+    if (point_in_sample(p) && k >= 0 && k < p->pt_n) {
+        a.pt_amp = (const double *)p->pt_amp;
+        a.pt_k = k;
+        a.pt_m = point_ring_plane(p);
+        a.pt_q = p->pt_cell % p->plane;
+        if (p->g.precision == SMX_F64) {
+            const int64_t i = p->pt_comp + 3 * p->pt_cell;
+            a.pt_dst64 = p->E + i;
+            a.pt_coef64 = p->Be + i;
+        } else {
+            const int c = 3 + p->pt_comp;
+            a.pt_dst32 = p->f32.T[p->f32.cur][c] + p->pt_cell;
+            a.pt_coef32 = p->f32.b[c] + p->pt_cell;
+        }
+    }
     SMX_HIP(smx_dft_sample(&a, p->dft_fill, p->stream));
     if (mon) mark_mon_slot(p, p->dft_fill);
     if (++p->dft_fill == p->dft_chunk) return dft_flush(p);
@@ -397,7 +430,7 @@ __global__ void point_add_f32(float *T, const float *b, const double *amp, long 
 
 int point_after_step(smx_plan *p, long long k)
 {
-    if (p->pt_comp < 0 || k < 0 || k >= p->pt_n) return SMX_OK;
+    if (p->pt_comp < 0 || k < 0 || k >= p->pt_n || point_in_sample(p)) return SMX_OK;
     if (p->g.precision == SMX_F64) {
         const int64_t i = p->pt_comp + 3 * p->pt_cell;
         hipLaunchKernelGGL(point_add_f64, dim3(1), dim3(1), 0, p->stream, p->E + i, p->Be + i,

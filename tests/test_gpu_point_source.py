@@ -29,11 +29,11 @@ def _setup(L=(48, 48, 64), steps=2000):
     return case, prep, amp, ws
 
 
-def _oracle(case, prep, amp, ws, plane):
-    """The oracle step by step with the point source, and a DFT of the plane's total
-    tangential fields with the library's phases (H of step k at e^{iw(k+1)}, E at
-    e^{iw(k+1.5)}).  Returns (H, E, dft[4][nf][L0*L1], the largest |H| and |E| of the
-    run)."""
+def _oracle(case, prep, amp, ws, plane, src=SRC, comp=2):
+    """The oracle step by step with the point source (E component `comp` at cell `src`),
+    and a DFT of the plane's total tangential fields with the library's phases (H of step
+    k at e^{iw(k+1)}, E at e^{iw(k+1.5)}).  Returns (H, E, dft[4][nf][L0*L1], the largest
+    |H| and |E| of the run)."""
     st = oracle.setup_from_media(case.L, case.mu, case.eps, iR=prep.invtau_R, iI=prep.invtau_I)
     L0, L1, L2 = (int(v) for v in case.L[:3])
     n, P = L0 * L1 * L2, L0 * L1
@@ -46,7 +46,7 @@ def _oracle(case, prep, amp, ws, plane):
     cells = np.arange(P) + P * plane
     a = st["Cont"][cells] == ord("a")
     sl = 3 * n + 3 * st["addr"][cells[a]]
-    lp = SRC[0] + L0 * (SRC[1] + L1 * SRC[2])
+    lp = src[0] + L0 * (src[1] + L1 * src[2])
     dft = np.zeros((4, len(ws), P), dtype=np.complex128)
     w = np.asarray(ws)[:, None]
     peak = np.zeros(2)
@@ -55,7 +55,7 @@ def _oracle(case, prep, amp, ws, plane):
         oracle.main_calc(lib.orc_main_calc, case.L, st["Ah"], st["Bh"], st["Ae"], st["Be"], case.iBc, H, E,
                          (-1, -1, case.iI, case.inc_mode), zero, zero, ExpN[4 * tn:4 * tn + 4].copy(), *mons,
                          st["Cont"], st["addr"])
-        lib.orc_point_add(E, st["Be"], 2, lp, float(amp[k]))
+        lib.orc_point_add(E, st["Be"], comp, lp, float(amp[k]))
         peak = np.maximum(peak, (np.abs(H).max(), np.abs(E).max()))
         for v, (F, c) in enumerate(((H, 0), (H, 1), (E, 0), (E, 1))):
             x = F[c + 3 * cells].copy()
@@ -104,6 +104,32 @@ def test_point_source_f32(c1):
         assert err < 1e-4, (f, err)
     eH, eE = np.abs(H - Ho).max() / peak[0], np.abs(E - Eo).max() / peak[1]
     assert eH < 1e-5 and eE < 1e-5, (eH, eE)
+
+
+@pytest.mark.parametrize("planes", [[9], [9, 20], [9, 9]])
+def test_point_source_on_a_dft_plane(planes):
+    """An E_x source on a DFT plane: the sample kernel applies the step's source term in
+    the thread that samples the source's cell, before its samples ([9], [9, 20]), or, with
+    two ring planes on that cell, the term keeps a launch of its own ([9, 9]).  FP64
+    fields bit for bit and the DFT within 1e-12 of its largest value against the oracle;
+    FP32 DFT within 1e-4."""
+    case, prep, amp, ws = _setup(L=(24, 24, 32), steps=300)
+    src = (11, 13, 9)
+    Ho, Eo, Do, _ = _oracle(case, prep, amp, ws, 9, src=src, comp=0)
+    for precision in (fdtd.F64, fdtd.F32):
+        with prep.plan(precision) as p:
+            p.set_dft(planes, ws, chunk=32)
+            p.set_point_source(0, src, amp)
+            p.run_steps(0, amp.size)
+            H, E = p.read_fields()
+            D = p.read_dft()[0].reshape(4, len(ws), -1)
+        assert np.abs(Do).max() > 0
+        err = np.abs(D - Do).max() / np.abs(Do).max()
+        if precision == fdtd.F64:
+            assert np.array_equal(H, Ho) and np.array_equal(E, Eo)
+            assert err <= 1e-12, err
+        else:
+            assert err < 1e-4, err
 
 
 def test_point_source_refusals_and_removal():
