@@ -12,8 +12,8 @@ timed region.
 
 Prints one JSON line: FP32 Gcell-updates/s (value; the step kernel chosen by the
 plan's autotuner), the roofline of that kernel against MI355X HBM peak, the plain FP64
-path on the same grid, the FP32-vs-FP64 |S| error (a separate run of the ramp plus 3000
-CW steps from zero fields, whatever --steps is), and the reference's own main_calc
+path on the same grid, the FP32-vs-FP64 |S| error (a separate run of 10,000 steps from
+zero fields, the envelope's ramp then CW, whatever --steps is), and the reference's own main_calc
 timed on this host's cores (oracle/_ref; the restatement in oracle/ where that is not
 built).
 """
@@ -97,7 +97,7 @@ def overlap(prep, H, E):
     return complex(np.vdot(w, v) / np.vdot(w, w).real)
 
 
-ACC_CW_STEPS = 3000  # CW steps of the accuracy run after the envelope's ramp
+ACC_STEPS = 10000  # steps of the accuracy run (the 160x160x320 configuration's run length), ramp included
 
 
 def run_from_zero(plan, dft_planes, dft_w, chunk, steps):
@@ -699,9 +699,9 @@ def main():
     print(f"[bench] grid {case.L[:3]} n={n} N_2pi={prep.N} phi={phi:.4f} setup {setup_s:.1f}s",
           file=sys.stderr, flush=True)
 
-    # the accuracy run: the envelope's ramp, then ACC_CW_STEPS CW steps, from zero fields,
+    # the accuracy run: ACC_STEPS steps from zero fields (the envelope's ramp, then CW),
     # whatever --steps is (the wave crosses the grid; a few steps would compare zeros)
-    acc_steps = case.time_period[0] * prep.N + ACC_CW_STEPS
+    acc_steps = max(ACC_STEPS, case.time_period[0] * prep.N + 1000)
     # FP32 path: the autotuner picks the step kernel (fused tile shape or the two sweeps)
     with prep.plan(fdtd.F32) as p32:
         if dft_planes:
